@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Headline benchmark: D3Q19 fused stream-collide on a 256^3 periodic TGV.
+
+One step = one lbc_fluid_step launch over the whole 256^3 domain (the
+Re = 1600 Taylor-Green configuration with a seeded divergence-free
+perturbation), populations resident in HBM.  The headline `value` is the TRT
+collision with FP32 storage; the same loop is also timed with FP16 storage,
+with BGK and MRT, and reported under "variants".  MLUPS = cells * steps / t
+(LB-Cube's definition, src/main.cpp:620-625).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+Only one GPU is supported: --gpus > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lb-cube_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E spec peak
+BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16"): 76, ("D3Q19", "f64"): 304,
+                  ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108}
+
+
+def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None):
+    from lbcube import init as lbinit
+
+    a = lbinit.equilibrium(lattice, None, u, storage)
+    b = torch.empty_like(a)
+    for _ in range(warmup):
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block)
+        a, b = b, a
+    stream = torch.cuda.current_stream()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block)
+        a, b = b, a
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    cells = u[0].numel()
+    mlups = cells / (ms * 1e-3) / 1e6
+    gbs = BYTES_PER_CELL[(lattice, storage)] * cells / (ms * 1e-3) / 1e9
+    diag = lbcube.fluid_diagnostics(lattice, a)
+    del a, b
+    torch.cuda.empty_cache()
+    return {"ms_per_step": ms, "mlups": mlups, "achieved_gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+            "bytes_per_cell": BYTES_PER_CELL[(lattice, storage)], "final_mass": diag[0], "final_ke": diag[1]}
+
+
+def cpu_baseline(u_host, omega, n):
+    """The oracle (CPU restatement of step_cpu) timed on this host: one step of
+    the same 256^3 D3Q19 TRT FP32 workload on all host threads, and one on a
+    single thread."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+
+    O.build()
+    rho = np.ones(u_host.shape[1:])
+    f = O.feq_field("D3Q19", "f32", rho, u_host)
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    O.fluid_step("D3Q19", "TRT", "f32", f, omega, threads=threads)
+    t_all = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.fluid_step("D3Q19", "TRT", "f32", f, omega, threads=1)
+    t_one = time.perf_counter() - t0
+    cells = n ** 3
+    return {"value": cells / t_all / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"1 step of the D3Q19 TRT FP32 {n}^3 TGV, oracle, {threads} OpenMP threads",
+            "single_thread_value": cells / t_one / 1e6}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", default="trt_f32,trt_f16,bgk_f32,bgk_f16,mrt_f32,mrt_f16,trt_bf16")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: the lattice Boltzmann path here is single-device "
+              "(multi-GPU slab decomposition is designed, not built). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import lbcube
+    from lbcube import init as lbinit
+
+    n = args.n
+    tau = lbinit.tgv_re1600_tau(n)
+    omega = 1.0 / tau
+    u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
+
+    variants = {}
+    for name in args.variants.split(","):
+        coll, storage = name.split("_")
+        variants[name] = time_variant(lbcube, torch, "D3Q19", coll.upper(), storage, u, omega, args.steps,
+                                      args.warmup)
+        print(f"# {name}: {variants[name]['mlups']:.0f} MLUPS, {variants[name]['achieved_gbs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    head = variants["trt_f32"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(u.cpu().numpy(), omega, n)
+    out = {
+        "metric": "D3Q19 MLUPS & %HBM roofline @256^3, fp32 vs fp16-store; adjoint/forward time",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"D3Q19 TRT periodic TGV {n}^3, Re=1600, u0=0.05, tau={tau:.6f}, FP32 storage, "
+                               "seeded divergence-free perturbation rms 0.01*u0",
+                   "cells": n ** 3, "lattice": "D3Q19", "collision": "TRT", "storage": "f32"},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": None,
+                     "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "bytes_per_cell": 152},
+        "cpu_baseline": cpu,
+        "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                     for k, v in variants.items()},
+    }
+    if "trt_f16" in variants:
+        out["fp16_over_fp32"] = round(variants["trt_f16"]["mlups"] / head["mlups"], 3)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

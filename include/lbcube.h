@@ -1,0 +1,113 @@
+/*
+ * lbcube.h — C ABI of the MI355X (gfx950) lattice Boltzmann kernels.
+ *
+ * This is the drop-in boundary for LB-Cube's GPU launchers
+ * (include/lattice_cuda.cuh of LB-Cube).  Every entry point:
+ *   - takes device pointers to SoA population buffers laid out [Q][Z][Y][X]
+ *     with x fastest (include/lattice_memory.hpp:67-272, src/lattice_cuda.cu:69-78),
+ *   - validates in the reference's order (src/lattice_cuda.cu:488-503):
+ *       null pointer            -> hipErrorInvalidDevicePointer (17)
+ *       any zero block extent   -> hipErrorInvalidConfiguration (9)
+ *       2D lattice              -> z extent and block z forced to 1
+ *       any zero domain extent  -> hipErrorInvalidValue (1)
+ *     then enqueues on `stream` (NULL = the default stream) and returns
+ *     hipGetLastError() without synchronizing,
+ *   - never allocates, copies or frees caller memory, and never swaps
+ *     buffers: the caller swaps current/next after each call
+ *     (src/main.cpp:587, src/simulate_reaction.cpp:527,542-543).
+ * Checks the reference does not make, all reported as hipErrorInvalidValue:
+ * an unsupported lattice/collision/storage code, current == next (the
+ * ping-pong buffers must not alias), and a population plane of 2^32 or more
+ * cells.  Return values are hipError_t codes as int (0 = hipSuccess).
+ *
+ * Streams are passed as void* (a hipStream_t) so this header needs no HIP
+ * headers.  Thread safety: every entry point is stateless and re-entrant,
+ * except the statistics entry points, which share a small per-device
+ * workspace and must not run concurrently on the same device.
+ */
+#ifndef LBCUBE_H
+#define LBCUBE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Lattices (include/lattice_traits.hpp:62-305 of LB-Cube). */
+enum lbc_lattice { LBC_D2Q9 = 0, LBC_D3Q19 = 1, LBC_D3Q27 = 2, LBC_D2Q5 = 3, LBC_D3Q7 = 4 };
+/* Collision operators (CollisionType, include/lattice_core.hpp:33-42). */
+enum lbc_collision { LBC_BGK = 0, LBC_TRT = 1, LBC_MRT = 2, LBC_RLBM = 3 };
+/* Population storage.  F32/F64 store populations as float/double with
+ * arithmetic in the same type (the reference's Real).  F16/BF16 store
+ * (f_i - w_i) for fluid populations and g_i for scalar populations as IEEE
+ * half / bfloat16 bit patterns (uint16_t), rounded to nearest-even, with
+ * float arithmetic. */
+enum lbc_storage { LBC_F32 = 0, LBC_F64 = 1, LBC_F16 = 2, LBC_BF16 = 3 };
+
+/* Fused pull-stream + collide step of one fluid lattice.
+ * Replaces launch_step_gpu<Lattice, Real, CT> (include/lattice_cuda.cuh:179-190,
+ * src/lattice_cuda.cu:476-520) and its LatticeMemory overload (:310-319,
+ * src/lattice_cuda.cu:680-698).  omega is 1/tau (TRT: omega_plus, with
+ * omega_minus from the magic parameter 1/4; MRT: the shear rate s_nu). */
+int lbc_fluid_step(int lattice, int collision, int storage, const void* current, void* next,
+                   uint64_t nx, uint64_t ny, uint64_t nz, double omega,
+                   uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
+
+/* Passive-scalar advection-diffusion step with a uniform source.
+ * Replaces launch_scalar_step_gpu<F, S, Real> (include/lattice_cuda.cuh:209-222,
+ * src/lattice_cuda.cu:539-589).  Pairs: D2Q9/D2Q5, D3Q19/D3Q7, D3Q27/D3Q7. */
+int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
+                    void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
+                    double omega_c, double source_term, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                    void* stream);
+
+/* Fused A + B -> C advection-diffusion-reaction step of two species.
+ * Replaces launch_reaction_AB_gpu<F, S, Real> (include/lattice_cuda.cuh:276-291,
+ * src/lattice_cuda.cu:610-666). */
+int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_a, void* next_a,
+                         const void* current_b, void* next_b, const void* current_fluid, uint64_t nx, uint64_t ny,
+                         uint64_t nz, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
+                         uint32_t block_z, void* stream);
+
+/* One pass that performs lbc_fluid_step followed by lbc_reaction_ab_step on
+ * the freshly written fluid state (the order of src/simulate_reaction.cpp:516-546),
+ * reading and writing every population once.  Results are identical to the
+ * two separate calls.  No counterpart in the reference: it fuses the two
+ * launches the reference's reactive drivers make per step. */
+int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
+                            const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
+                            const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                            double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
+                            uint32_t block_z, void* stream);
+
+/* Fluid integrals of log_diagnostics (include/lattice_io.hpp:182-234):
+ * out[0] = sum rho, out[1] = sum rho |u|^2 / 2, out[2] = max |u|.
+ * Accumulated in double on the device; synchronizes `stream`. */
+int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
+                          double* out3, void* stream);
+
+/* Reactive statistics of compute_reactive_stats (include/lattice_io.hpp:256-338):
+ * out = {mean_A, mean_B, var_A, var_B, covariance, segregation_intensity,
+ *        true_reaction_rate, mixed_reaction_rate}.
+ * Accumulated in double on the device; synchronizes `stream`. */
+int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b, uint64_t nx,
+                       uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream);
+
+/* Block shape the library picks for a domain (the C++ launcher mirror uses it
+ * as its default argument; the reference's default is {16, 8, 1},
+ * include/lattice_cuda.cuh:52-54). */
+void lbc_default_block(int lattice, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t* block3);
+
+/* Human-readable name of a returned status code. */
+const char* lbc_status_string(int status);
+
+/* ABI version: bumped when an entry point changes. */
+int lbc_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBCUBE_H */

@@ -1,0 +1,266 @@
+// Advection-diffusion(-reaction) kernels for D2Q5 / D3Q7 scalars carried by a
+// D2Q9 / D3Q19 / D3Q27 fluid — the gfx950 replacements of LB-Cube's
+// kernel_scalar_step (src/lattice_cuda.cu:234-327) and kernel_reaction_AB
+// (src/lattice_cuda.cu:349-457), plus a single-pass kernel that fuses the fluid
+// step with the reaction step.
+//
+// Same mapping as the fluid kernel: one work-item per cell, x along the lanes
+// of a wave64, unit-stride loads and stores of every population plane.
+#include <hip/hip_runtime.h>
+
+#include "../../include/lbcube.h"
+#include "lbc_launch.hpp"
+#include "lbc_physics.hpp"
+#include "lbc_storage.hpp"
+
+namespace lbc {
+
+// Scalar step (REACT = false: one species with a uniform source) and the
+// A + B -> C step (REACT = true: two species, source from the exact one-step
+// reaction increment).  The advecting velocity is the fluid's at the
+// destination cell, unshifted (include/lattice_core.hpp:259-265, :381-387).
+template <class F, class S, int ST, bool REACT, int MAXT>
+__global__ __launch_bounds__(MAXT) void scalar_step_kernel(
+    const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
+    const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b,
+    const typename Storage<ST>::S* __restrict__ cur_f, int nx, int ny, int nz, typename Storage<ST>::R omega_c,
+    typename Storage<ST>::R param) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= nx || y >= ny || z >= nz) return;
+    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
+                                             static_cast<uint32_t>(nz));
+    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    R fl[F::Q];
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i)
+        fl[i] = St::load(plane_ptr(cur_f, i, plane) + cell, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+    const Macro<R> m = macro_state<F, R>(fl);
+    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    R ga[S::Q], gb[S::Q];
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
+                             ax.pull(S::cx[i]);
+        ga[i] = St::load(plane_ptr(cur_a, i, plane) + src, R(0));
+        if constexpr (REACT) gb[i] = St::load(plane_ptr(cur_b, i, plane) + src, R(0));
+    }
+    if constexpr (REACT) {
+        const R ca = concentration<S, R>(ga);
+        const R cb = concentration<S, R>(gb);
+        const R src = reaction_source<R>(ca, cb, param);
+        collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, src);
+        collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, src);
+    } else {
+        collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, param);
+    }
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        St::store(plane_ptr(nxt_a, i, plane) + cell, ga[i], R(0));
+        if constexpr (REACT) St::store(plane_ptr(nxt_b, i, plane) + cell, gb[i], R(0));
+    }
+}
+
+// Fluid step + A + B -> C step in one pass.  The reaction step of the
+// reference reads the fluid populations that the fluid step has just written
+// at the same cell (src/simulate_reaction.cpp:527-541); here they are still in
+// registers.  With reduced-precision storage they are passed through the
+// storage rounding first, so the velocity is the one the unfused pair sees.
+template <class F, int CT, class S, int ST, int MAXT>
+__global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
+    const typename Storage<ST>::S* __restrict__ cur_f, typename Storage<ST>::S* __restrict__ nxt_f,
+    const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
+    const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, int nx, int ny,
+    int nz, typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
+    typename Storage<ST>::R k_react) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= nx || y >= ny || z >= nz) return;
+    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
+                                             static_cast<uint32_t>(nz));
+    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    R f[F::Q];
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(F::cz[i])) * ny + ay.pull(F::cy[i])) * nx +
+                             ax.pull(F::cx[i]);
+        f[i] = St::load(plane_ptr(cur_f, i, plane) + src, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+    }
+    collide_cell<F, CT, R>(f, omega, omega_minus);
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const R off = kOffsetStorage<ST> ? R(F::w[i]) : R(0);
+        St::store(plane_ptr(nxt_f, i, plane) + cell, f[i], off);
+        if constexpr (kOffsetStorage<ST>) f[i] = St::roundtrip(f[i], off);
+    }
+    const Macro<R> m = macro_state<F, R>(f);
+    R ga[S::Q], gb[S::Q];
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
+                             ax.pull(S::cx[i]);
+        ga[i] = St::load(plane_ptr(cur_a, i, plane) + src, R(0));
+        gb[i] = St::load(plane_ptr(cur_b, i, plane) + src, R(0));
+    }
+    const R ca = concentration<S, R>(ga);
+    const R cb = concentration<S, R>(gb);
+    const R source = reaction_source<R>(ca, cb, k_react);
+    collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, source);
+    collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        St::store(plane_ptr(nxt_a, i, plane) + cell, ga[i], R(0));
+        St::store(plane_ptr(nxt_b, i, plane) + cell, gb[i], R(0));
+    }
+}
+
+template <class F, class S, int ST, bool REACT>
+hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, const void* cf, const LaunchShape& s,
+                         double omega_c, double param, hipStream_t stream) {
+    using St = Storage<ST>;
+    using T = typename St::S;
+    using R = typename St::R;
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(ca), static_cast<T*>(na),
+                           static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.nx, s.ny,
+                           s.nz, static_cast<R>(omega_c), static_cast<R>(param));
+    };
+    if (s.threads() <= 256) args(scalar_step_kernel<F, S, ST, REACT, 256>);
+    else args(scalar_step_kernel<F, S, ST, REACT, 1024>);
+    return hipGetLastError();
+}
+
+template <class F, class S, bool REACT>
+hipError_t scalar_by_storage(int storage, const void* ca, void* na, const void* cb, void* nb, const void* cf,
+                             const LaunchShape& s, double omega_c, double param, hipStream_t st) {
+    switch (storage) {
+        case kF32: return launch_scalar<F, S, kF32, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
+        case kF64: return launch_scalar<F, S, kF64, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
+        case kF16: return launch_scalar<F, S, kF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
+        case kBF16: return launch_scalar<F, S, kBF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool REACT>
+hipError_t scalar_dispatch(int fl, int sl, int storage, const void* ca, void* na, const void* cb, void* nb,
+                           const void* cf, const LaunchShape& s, double omega_c, double param, hipStream_t st) {
+    if (fl == kD2Q9 && sl == kD2Q5)
+        return scalar_by_storage<D2Q9, D2Q5, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+    if (fl == kD3Q19 && sl == kD3Q7)
+        return scalar_by_storage<D3Q19, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+    if (fl == kD3Q27 && sl == kD3Q7)
+        return scalar_by_storage<D3Q27, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+    return hipErrorInvalidValue;
+}
+
+template <class F, int CT, class S, int ST>
+hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
+                        const LaunchShape& s, double omega, double omega_c, double k, hipStream_t stream) {
+    using St = Storage<ST>;
+    using T = typename St::S;
+    using R = typename St::R;
+    const R w = static_cast<R>(omega);
+    const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
+                           static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
+                           static_cast<T*>(nb), s.nx, s.ny, s.nz, w, wm, static_cast<R>(omega_c),
+                           static_cast<R>(k));
+    };
+    if (s.threads() <= 256) args(reactive_fused_kernel<F, CT, S, ST, 256>);
+    else args(reactive_fused_kernel<F, CT, S, ST, 1024>);
+    return hipGetLastError();
+}
+
+template <class F, class S, int CT>
+hipError_t fused_by_storage(int storage, const void* cf, void* nf, const void* ca, void* na, const void* cb,
+                            void* nb, const LaunchShape& s, double omega, double omega_c, double k, hipStream_t st) {
+    switch (storage) {
+        case kF32: return launch_fused<F, CT, S, kF32>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF64: return launch_fused<F, CT, S, kF64>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF16: return launch_fused<F, CT, S, kF16>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kBF16: return launch_fused<F, CT, S, kBF16>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class F, class S>
+hipError_t fused_by_collision(int collision, int storage, const void* cf, void* nf, const void* ca, void* na,
+                              const void* cb, void* nb, const LaunchShape& s, double omega, double omega_c, double k,
+                              hipStream_t st) {
+    switch (collision) {
+        case kBGK: return fused_by_storage<F, S, kBGK>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kTRT: return fused_by_storage<F, S, kTRT>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kMRT: return fused_by_storage<F, S, kMRT>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kRLBM: return fused_by_storage<F, S, kRLBM>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
+                               void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
+                               double omega_c, double source_term, uint32_t block_x, uint32_t block_y,
+                               uint32_t block_z, void* stream) {
+    using namespace lbc;
+    if (current_scalar == nullptr || next_scalar == nullptr || current_fluid == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_scalar == next_scalar || next_scalar == current_fluid) return hipErrorInvalidValue;
+    return scalar_dispatch<false>(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, nullptr,
+                                  nullptr, current_fluid, s, omega_c, source_term, as_stream(stream));
+}
+
+extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_a,
+                                    void* next_a, const void* current_b, void* next_b, const void* current_fluid,
+                                    uint64_t nx, uint64_t ny, uint64_t nz, double omega_c, double k_react,
+                                    uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
+    using namespace lbc;
+    if (current_a == nullptr || next_a == nullptr || current_b == nullptr || next_b == nullptr ||
+        current_fluid == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_a == next_a || current_b == next_b || next_a == next_b) return hipErrorInvalidValue;
+    return scalar_dispatch<true>(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
+                                 current_fluid, s, omega_c, k_react, as_stream(stream));
+}
+
+extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
+                                       const void* current_fluid, void* next_fluid, const void* current_a,
+                                       void* next_a, const void* current_b, void* next_b, uint64_t nx, uint64_t ny,
+                                       uint64_t nz, double omega, double omega_c, double k_react, uint32_t block_x,
+                                       uint32_t block_y, uint32_t block_z, void* stream) {
+    using namespace lbc;
+    if (current_fluid == nullptr || next_fluid == nullptr || current_a == nullptr || next_a == nullptr ||
+        current_b == nullptr || next_b == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_fluid == next_fluid || current_a == next_a || current_b == next_b || next_a == next_b)
+        return hipErrorInvalidValue;
+    const hipStream_t st = as_stream(stream);
+    if (fluid_lattice == kD2Q9 && scalar_lattice == kD2Q5)
+        return fused_by_collision<D2Q9, D2Q5>(collision, storage, current_fluid, next_fluid, current_a, next_a,
+                                              current_b, next_b, s, omega, omega_c, k_react, st);
+    if (fluid_lattice == kD3Q19 && scalar_lattice == kD3Q7)
+        return fused_by_collision<D3Q19, D3Q7>(collision, storage, current_fluid, next_fluid, current_a, next_a,
+                                               current_b, next_b, s, omega, omega_c, k_react, st);
+    if (fluid_lattice == kD3Q27 && scalar_lattice == kD3Q7)
+        return fused_by_collision<D3Q27, D3Q7>(collision, storage, current_fluid, next_fluid, current_a, next_a,
+                                               current_b, next_b, s, omega, omega_c, k_react, st);
+    return hipErrorInvalidValue;
+}

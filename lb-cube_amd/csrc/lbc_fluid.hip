@@ -1,0 +1,109 @@
+// Fused pull-stream + collide kernels for D2Q9 / D3Q19 / D3Q27 — the gfx950
+// replacement of LB-Cube's kernel_step (src/lattice_cuda.cu:149-213) behind
+// lbc_fluid_step (launch_step_gpu, src/lattice_cuda.cu:476-520).
+//
+// One work-item owns one destination cell, work-items along x are
+// consecutive lanes of a wave64, so each of the Q population planes is read
+// and written with unit stride.  A population is pulled from x - c_i
+// (periodic), collided in registers and stored once: Q loads + Q stores per
+// cell, the whole algorithmic traffic.  Periodic wrap uses selects on 32-bit
+// coordinates; the plane offset of population i is a wave-uniform 64-bit base.
+#include <hip/hip_runtime.h>
+
+#include "../../include/lbcube.h"
+#include "lbc_launch.hpp"
+#include "lbc_physics.hpp"
+#include "lbc_storage.hpp"
+
+namespace lbc {
+
+template <class L, int CT, int ST, int MAXT>
+__global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
+                                                          typename Storage<ST>::S* __restrict__ nxt, int nx, int ny,
+                                                          int nz, typename Storage<ST>::R omega,
+                                                          typename Storage<ST>::R omega_minus) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= nx || y >= ny || z >= nz) return;
+    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
+                                             static_cast<uint32_t>(nz));
+    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    R f[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
+                             ax.pull(L::cx[i]);
+        const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
+        f[i] = St::load(plane_ptr(cur, i, plane) + src, off);
+    }
+    collide_cell<L, CT, R>(f, omega, omega_minus);
+    const uint32_t dst = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
+        St::store(plane_ptr(nxt, i, plane) + dst, f[i], off);
+    }
+}
+
+template <class L, int CT, int ST>
+hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double omega, hipStream_t stream) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const R w = static_cast<R>(omega);
+    const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
+    if (s.threads() <= 256)
+        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, 256>), s.grid, s.block, 0, stream,
+                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
+                           s.nz, w, wm);
+    else
+        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, 1024>), s.grid, s.block, 0, stream,
+                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
+                           s.nz, w, wm);
+    return hipGetLastError();
+}
+
+template <class L, int CT>
+hipError_t dispatch_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, double omega,
+                            hipStream_t st) {
+    switch (storage) {
+        case kF32: return launch_fluid<L, CT, kF32>(cur, nxt, s, omega, st);
+        case kF64: return launch_fluid<L, CT, kF64>(cur, nxt, s, omega, st);
+        case kF16: return launch_fluid<L, CT, kF16>(cur, nxt, s, omega, st);
+        case kBF16: return launch_fluid<L, CT, kBF16>(cur, nxt, s, omega, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class L>
+hipError_t dispatch_collision(int collision, int storage, const void* cur, void* nxt, const LaunchShape& s,
+                              double omega, hipStream_t st) {
+    switch (collision) {
+        case kBGK: return dispatch_storage<L, kBGK>(storage, cur, nxt, s, omega, st);
+        case kTRT: return dispatch_storage<L, kTRT>(storage, cur, nxt, s, omega, st);
+        case kMRT: return dispatch_storage<L, kMRT>(storage, cur, nxt, s, omega, st);
+        case kRLBM: return dispatch_storage<L, kRLBM>(storage, cur, nxt, s, omega, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const void* current, void* next, uint64_t nx,
+                              uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
+                              uint32_t block_z, void* stream) {
+    using namespace lbc;
+    if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current == next) return hipErrorInvalidValue;
+    switch (lattice) {
+        case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, current, next, s, omega, as_stream(stream));
+        case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, current, next, s, omega, as_stream(stream));
+        case kD3Q27: return dispatch_collision<D3Q27>(collision, storage, current, next, s, omega, as_stream(stream));
+        default: return hipErrorInvalidValue;
+    }
+}

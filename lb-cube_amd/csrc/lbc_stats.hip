@@ -1,0 +1,286 @@
+// Domain statistics on the device: the fluid integrals of log_diagnostics
+// (include/lattice_io.hpp:182-234) and the reactive statistics of
+// compute_reactive_stats (include/lattice_io.hpp:256-338).
+//
+// The reference sums serially on the host in Real after copying the whole
+// population buffers back.  Here every work-item reduces a grid-stride slice
+// of cells in double, waves combine with cross-lane shuffles, blocks through
+// LDS, and one finishing block sums the per-block partials in a fixed order,
+// so the result is deterministic for a given domain size.  Per-cell terms
+// (density, |u|^2, concentrations) are formed in the storage's arithmetic type
+// exactly as the reference forms them; only the accumulation is double.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+
+#include "../../include/lbcube.h"
+#include "lbc_launch.hpp"
+#include "lbc_physics.hpp"
+#include "lbc_storage.hpp"
+
+namespace lbc {
+
+constexpr int kStatBlocks = 1024;
+constexpr int kStatThreads = 256;
+constexpr int kStatSlots = 8;
+
+// op: 0 = sum, 1 = max
+template <int N>
+__device__ __forceinline__ void block_reduce(double (&v)[N], const int (&op)[N], double* out) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double o = __shfl_xor(v[k], off, 64);
+            v[k] = op[k] ? fmax(v[k], o) : v[k] + o;
+        }
+    }
+    __shared__ double part[kStatThreads / 64][N];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) part[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double acc = part[0][k];
+            for (int w = 1; w < kStatThreads / 64; ++w) acc = op[k] ? fmax(acc, part[w][k]) : acc + part[w][k];
+            out[k] = acc;
+        }
+    }
+}
+
+template <class L, int ST>
+__global__ __launch_bounds__(kStatThreads) void fluid_diag_kernel(const typename Storage<ST>::S* __restrict__ f,
+                                                                  uint32_t cells, double* partial) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    double v[3] = {0.0, 0.0, 0.0};
+    constexpr int op[3] = {0, 0, 1};
+    const size_t plane = cells;
+    for (uint32_t c = blockIdx.x * kStatThreads + threadIdx.x; c < cells; c += kStatBlocks * kStatThreads) {
+        R loc[L::Q];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) loc[i] = St::load(plane_ptr(f, i, plane) + c, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+        const Macro<R> m = macro_state<L, R>(loc);
+        const R sp2 = vel_sq<L, R>(m);
+        v[0] += static_cast<double>(m.rho);
+        v[1] += static_cast<double>(R(0.5) * m.rho * sp2);
+        v[2] = fmax(v[2], static_cast<double>(sqrt(sp2)));
+    }
+    block_reduce<3>(v, op, partial + blockIdx.x * kStatSlots);
+}
+
+template <class S, int ST>
+__global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const typename Storage<ST>::S* __restrict__ a,
+                                                                      const typename Storage<ST>::S* __restrict__ b,
+                                                                      uint32_t cells, double* partial) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    constexpr int op[5] = {0, 0, 0, 0, 0};
+    const size_t plane = cells;
+    for (uint32_t c = blockIdx.x * kStatThreads + threadIdx.x; c < cells; c += kStatBlocks * kStatThreads) {
+        R ga[S::Q], gb[S::Q];
+#pragma unroll
+        for (int i = 0; i < S::Q; ++i) {
+            ga[i] = St::load(plane_ptr(a, i, plane) + c, R(0));
+            gb[i] = St::load(plane_ptr(b, i, plane) + c, R(0));
+        }
+        const double ca = static_cast<double>(concentration<S, R>(ga));
+        const double cb = static_cast<double>(concentration<S, R>(gb));
+        v[0] += ca;
+        v[1] += cb;
+        v[2] += ca * ca;
+        v[3] += cb * cb;
+        v[4] += ca * cb;
+    }
+    block_reduce<5>(v, op, partial + blockIdx.x * kStatSlots);
+}
+
+// Sums the per-block partials in a fixed order and derives the reported
+// quantities: N = 3 fluid diagnostics, N = 5 reactive statistics
+// (formulas of include/lattice_io.hpp:309-337 evaluated in double; eps is the
+// epsilon of the storage's arithmetic type, as the reference's
+// numeric_limits<Real>::epsilon()).
+template <int N>
+__global__ __launch_bounds__(kStatThreads) void finish_kernel(const double* partial, double* result, uint32_t cells,
+                                                              double k_react, double eps) {
+    constexpr int mode = N == 3 ? 0 : 1;
+    double v[N];
+    int op[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        v[k] = 0.0;
+        op[k] = (mode == 0 && k == 2) ? 1 : 0;
+    }
+    for (int blk = threadIdx.x; blk < kStatBlocks; blk += kStatThreads) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double p = partial[blk * kStatSlots + k];
+            v[k] = op[k] ? fmax(v[k], p) : v[k] + p;
+        }
+    }
+    __shared__ double tot[N];
+    int opc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) opc[k] = op[k];
+    block_reduce<N>(v, opc, tot);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if constexpr (mode == 0) {
+        result[0] = tot[0];
+        result[1] = tot[1];
+        result[2] = tot[2];
+        return;
+    } else {
+    const double inv = 1.0 / static_cast<double>(cells);
+    const double ma = tot[0] * inv, mb = tot[1] * inv, maa = tot[2] * inv, mbb = tot[3] * inv, mab = tot[4] * inv;
+    const double cov = mab - ma * mb;
+    const double mp = ma * mb;
+    result[0] = ma;
+    result[1] = mb;
+    result[2] = maa - ma * ma;
+    result[3] = mbb - mb * mb;
+    result[4] = cov;
+    result[5] = fabs(mp) > eps ? cov / mp : 0.0;
+    result[6] = k_react * mab;
+    result[7] = k_react * ma * mb;
+    }
+}
+
+// Per-device workspace: kStatBlocks partial slots plus the result slots.
+double* workspace(hipError_t& err) {
+    static std::mutex mu;
+    static double* ws[64] = {nullptr};
+    int dev = 0;
+    err = hipGetDevice(&dev);
+    if (err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= 64) {
+        err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (ws[dev] == nullptr) {
+        err = hipMalloc(reinterpret_cast<void**>(&ws[dev]), (kStatBlocks + 1) * kStatSlots * sizeof(double));
+        if (err != hipSuccess) {
+            ws[dev] = nullptr;
+            return nullptr;
+        }
+    }
+    return ws[dev];
+}
+
+hipError_t read_back(const double* dev_result, double* host, int n, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(host, dev_result, n * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
+template <class L, int ST>
+hipError_t run_fluid_diag(const void* f, uint32_t cells, double* out, hipStream_t st) {
+    hipError_t e;
+    double* ws = workspace(e);
+    if (!ws) return e;
+    hipLaunchKernelGGL((fluid_diag_kernel<L, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
+                       static_cast<const typename Storage<ST>::S*>(f), cells, ws);
+    hipLaunchKernelGGL((finish_kernel<3>), dim3(1), dim3(kStatThreads), 0, st, ws, ws + kStatBlocks * kStatSlots,
+                       cells, 0.0, 0.0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return read_back(ws + kStatBlocks * kStatSlots, out, 3, st);
+}
+
+template <class S, int ST>
+hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double* out, hipStream_t st) {
+    hipError_t e;
+    double* ws = workspace(e);
+    if (!ws) return e;
+    using R = typename Storage<ST>::R;
+    hipLaunchKernelGGL((reactive_stats_kernel<S, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
+                       static_cast<const typename Storage<ST>::S*>(a), static_cast<const typename Storage<ST>::S*>(b),
+                       cells, ws);
+    hipLaunchKernelGGL((finish_kernel<5>), dim3(1), dim3(kStatThreads), 0, st, ws, ws + kStatBlocks * kStatSlots,
+                       cells, k, static_cast<double>(Eps<R>::v));
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return read_back(ws + kStatBlocks * kStatSlots, out, 8, st);
+}
+
+template <class L>
+hipError_t fluid_diag_storage(int storage, const void* f, uint32_t cells, double* out, hipStream_t st) {
+    switch (storage) {
+        case kF32: return run_fluid_diag<L, kF32>(f, cells, out, st);
+        case kF64: return run_fluid_diag<L, kF64>(f, cells, out, st);
+        case kF16: return run_fluid_diag<L, kF16>(f, cells, out, st);
+        case kBF16: return run_fluid_diag<L, kBF16>(f, cells, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class S>
+hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, double k, double* out,
+                         hipStream_t st) {
+    switch (storage) {
+        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, out, st);
+        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, out, st);
+        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, out, st);
+        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,
+                                     uint64_t nz, double* out3, void* stream) {
+    using namespace lbc;
+    if (populations == nullptr || out3 == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (lattice) {
+        case kD2Q9: return fluid_diag_storage<D2Q9>(storage, populations, cells, out3, st);
+        case kD3Q19: return fluid_diag_storage<D3Q19>(storage, populations, cells, out3, st);
+        case kD3Q27: return fluid_diag_storage<D3Q27>(storage, populations, cells, out3, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b,
+                                  uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream) {
+    using namespace lbc;
+    if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, 1, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (scalar_lattice) {
+        case kD2Q5: return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, out8, st);
+        case kD3Q7: return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, out8, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" void lbc_default_block(int lattice, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t* block3) {
+    (void)ny;
+    (void)nz;
+    (void)lattice;
+    // Whole rows along x for 256 work-items per block: x-shifted loads of a
+    // wave then stay inside lines the same block reads anyway.
+    uint32_t bx = 256;
+    if (nx < 256) bx = nx <= 64 ? 64 : (nx <= 128 ? 128 : 256);
+    block3[0] = bx;
+    block3[1] = 256 / bx;
+    block3[2] = 1;
+}
+
+extern "C" const char* lbc_status_string(int status) { return hipGetErrorString(static_cast<hipError_t>(status)); }
+
+extern "C" int lbc_abi_version(void) { return 1; }

@@ -1,0 +1,80 @@
+// Population storage formats and the shared launch plumbing of the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lbc_lattice.hpp"
+
+namespace lbc {
+
+// kF32 / kF64: the population itself (LatticeMemory<L, Real> of
+// include/lattice_memory.hpp).  kF16 / kBF16: (f_i - offset) rounded to
+// nearest-even, offset = w_i for fluid populations and 0 for scalar ones,
+// arithmetic in float.
+template <int ST> struct Storage;
+
+template <> struct Storage<kF32> {
+    using S = float;
+    using R = float;
+    __device__ static R load(const S* p, R) { return *p; }
+    __device__ static void store(S* p, R v, R) { *p = v; }
+    __device__ static R roundtrip(R v, R) { return v; }
+};
+
+template <> struct Storage<kF64> {
+    using S = double;
+    using R = double;
+    __device__ static R load(const S* p, R) { return *p; }
+    __device__ static void store(S* p, R v, R) { *p = v; }
+    __device__ static R roundtrip(R v, R) { return v; }
+};
+
+template <> struct Storage<kF16> {
+    using S = uint16_t;
+    using R = float;
+    __device__ static R load(const S* p, R off) { return off + static_cast<float>(__builtin_bit_cast(_Float16, *p)); }
+    __device__ static void store(S* p, R v, R off) {
+        *p = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - off));
+    }
+    // The value a later load of a stored v returns.
+    __device__ static R roundtrip(R v, R off) { return off + static_cast<float>(static_cast<_Float16>(v - off)); }
+};
+
+template <> struct Storage<kBF16> {
+    using S = uint16_t;
+    using R = float;
+    __device__ static R load(const S* p, R off) {
+        return off + __builtin_bit_cast(float, static_cast<uint32_t>(*p) << 16);
+    }
+    __device__ static uint16_t encode(float d) {
+        uint32_t x = __builtin_bit_cast(uint32_t, d);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((x >> 16) | 0x40u);
+        x += 0x7fffu + ((x >> 16) & 1u);
+        return static_cast<uint16_t>(x >> 16);
+    }
+    __device__ static void store(S* p, R v, R off) { *p = encode(v - off); }
+    __device__ static R roundtrip(R v, R off) {
+        return off + __builtin_bit_cast(float, static_cast<uint32_t>(encode(v - off)) << 16);
+    }
+};
+
+// Base of population plane i: a wave-uniform pointer, so the per-lane part of
+// every address stays a 32-bit cell offset.
+template <class T> __device__ __forceinline__ T* plane_ptr(T* base, int i, size_t plane) {
+    return base + static_cast<size_t>(i) * plane;
+}
+
+template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
+
+// Periodic neighbour coordinates along one axis for pull streaming: the
+// source of a population with velocity component c at coordinate x is x - c
+// (include/lattice_core.hpp:70-78), so c = +1 reads x - 1 and c = -1 reads x + 1.
+struct Axis {
+    int lo, mid, hi;  // x - 1, x, x + 1 with periodic wrap
+    __device__ Axis(int x, int n) : lo(x == 0 ? n - 1 : x - 1), mid(x), hi(x + 1 == n ? 0 : x + 1) {}
+    __device__ int pull(int c) const { return c > 0 ? lo : (c < 0 ? hi : mid); }
+};
+
+}  // namespace lbc

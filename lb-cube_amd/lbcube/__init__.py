@@ -1,0 +1,184 @@
+"""Python view of the MI355X lattice Boltzmann kernels (liblbcube_hip.so).
+
+Populations live in torch CUDA tensors laid out [Q][Z][Y][X] (x fastest; 2D:
+[Q][Y][X]), the SoA layout of LB-Cube's LatticeMemory
+(include/lattice_memory.hpp:67-272).  The storage format follows the tensor
+dtype: float32 / float64 hold populations; float16 / bfloat16 hold
+(f_i - w_i) for fluid lattices and g_i for scalar lattices (see
+include/lbcube.h).  Every call is asynchronous on the current torch stream and
+raises LbcError on a non-zero status; no call allocates device memory except
+the statistics calls (one small per-device workspace, allocated once).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import LbcError, check, lib  # noqa: F401
+
+LATTICES = {"D2Q9": 0, "D3Q19": 1, "D3Q27": 2, "D2Q5": 3, "D3Q7": 4}
+COLLISIONS = {"BGK": 0, "TRT": 1, "MRT": 2, "RLBM": 3}
+Q = {"D2Q9": 9, "D3Q19": 19, "D3Q27": 27, "D2Q5": 5, "D3Q7": 7}
+DIM = {"D2Q9": 2, "D3Q19": 3, "D3Q27": 3, "D2Q5": 2, "D3Q7": 3}
+STORAGE = {torch.float32: 0, torch.float64: 1, torch.float16: 2, torch.bfloat16: 3}
+STORAGE_NAMES = {"f32": torch.float32, "f64": torch.float64, "f16": torch.float16, "bf16": torch.bfloat16}
+WEIGHTS = {
+    "D2Q9": [4 / 9] + [1 / 9] * 4 + [1 / 36] * 4,
+    "D3Q19": [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12,
+    "D3Q27": [8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8,
+    "D2Q5": [1 / 3] + [1 / 6] * 4,
+    "D3Q7": [1 / 4] + [1 / 8] * 6,
+}
+# Lattice velocities, direction order of include/lattice_traits.hpp:62-305.
+VELOCITIES = {
+    "D2Q9": [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)],
+    "D3Q19": [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, 1, 0),
+              (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 1), (0, -1, 1),
+              (0, 1, -1), (0, -1, -1)],
+    "D2Q5": [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1)],
+    "D3Q7": [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)],
+}
+VELOCITIES["D3Q27"] = VELOCITIES["D3Q19"] + [(sx, sy, sz) for sz in (1, -1) for sy in (1, -1) for sx in (1, -1)]
+SCALAR_PAIRS = {("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")}
+
+
+def _stream(stream):
+    if stream is None:
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+def _extents(lattice, t):
+    if DIM[lattice] == 3:
+        _, nz, ny, nx = t.shape
+    else:
+        _, ny, nx = t.shape
+        nz = 1
+    return nx, ny, nz
+
+
+def _check_pop(lattice, t, name):
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous CUDA tensor")
+    if t.dim() != DIM[lattice] + 1 or t.shape[0] != Q[lattice]:
+        raise ValueError(f"{name} must have shape (Q={Q[lattice]}, ...) with {DIM[lattice]} spatial axes")
+    if t.dtype not in STORAGE:
+        raise ValueError(f"{name}: unsupported dtype {t.dtype}")
+
+
+def default_block(lattice, nx, ny, nz=1):
+    out = (ctypes.c_uint32 * 3)()
+    lib().lbc_default_block(LATTICES[lattice], nx, ny, nz, out)
+    return tuple(out)
+
+
+def _block(lattice, block, nx, ny, nz):
+    return default_block(lattice, nx, ny, nz) if block is None else tuple(block)
+
+
+def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None):
+    """Fused pull-stream + collide: nxt <- step(cur). Replaces launch_step_gpu."""
+    _check_pop(lattice, cur, "cur")
+    _check_pop(lattice, nxt, "nxt")
+    if cur.shape != nxt.shape or cur.dtype != nxt.dtype:
+        raise ValueError("cur and nxt must have the same shape and dtype")
+    nx, ny, nz = _extents(lattice, cur)
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_fluid_step(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype], cur.data_ptr(),
+                               nxt.data_ptr(), nx, ny, nz, float(omega), bx, by, bz, _stream(stream)),
+          "lbc_fluid_step")
+
+
+def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None):
+    """Passive-scalar ADR step. Replaces launch_scalar_step_gpu."""
+    _check_pop(scalar_lattice, cur_s, "cur_s")
+    _check_pop(scalar_lattice, nxt_s, "nxt_s")
+    _check_pop(fluid_lattice, cur_f, "cur_f")
+    nx, ny, nz = _extents(scalar_lattice, cur_s)
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_scalar_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_s.dtype],
+                                cur_s.data_ptr(), nxt_s.data_ptr(), cur_f.data_ptr(), nx, ny, nz, float(omega_c),
+                                float(source), bx, by, bz, _stream(stream)), "lbc_scalar_step")
+
+
+def reaction_step(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur_f, omega_c, k_react, block=None,
+                  stream=None):
+    """Fused A + B -> C ADR step. Replaces launch_reaction_AB_gpu."""
+    for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
+        _check_pop(scalar_lattice, t, n)
+    _check_pop(fluid_lattice, cur_f, "cur_f")
+    nx, ny, nz = _extents(scalar_lattice, cur_a)
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_reaction_ab_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_a.dtype],
+                                     cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(),
+                                     cur_f.data_ptr(), nx, ny, nz, float(omega_c), float(k_react), bx, by, bz,
+                                     _stream(stream)), "lbc_reaction_ab_step")
+
+
+def reactive_fused_step(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, omega,
+                        omega_c, k_react, block=None, stream=None):
+    """Fluid step + A + B -> C step in one pass (same results as the two calls)."""
+    _check_pop(fluid_lattice, cur_f, "cur_f")
+    _check_pop(fluid_lattice, nxt_f, "nxt_f")
+    for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
+        _check_pop(scalar_lattice, t, n)
+    nx, ny, nz = _extents(scalar_lattice, cur_a)
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_reactive_fused_step(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
+                                        STORAGE[cur_f.dtype], cur_f.data_ptr(), nxt_f.data_ptr(), cur_a.data_ptr(),
+                                        nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(), nx, ny, nz,
+                                        float(omega), float(omega_c), float(k_react), bx, by, bz, _stream(stream)),
+          "lbc_reactive_fused_step")
+
+
+def fluid_diagnostics(lattice, f, stream=None):
+    """(total mass, total kinetic energy, max |u|), double accumulation; synchronizes."""
+    _check_pop(lattice, f, "f")
+    nx, ny, nz = _extents(lattice, f)
+    out = (ctypes.c_double * 3)()
+    check(lib().lbc_fluid_diagnostics(LATTICES[lattice], STORAGE[f.dtype], f.data_ptr(), nx, ny, nz, out,
+                                      _stream(stream)), "lbc_fluid_diagnostics")
+    return tuple(out)
+
+
+STAT_NAMES = ("mean_A", "mean_B", "var_A", "var_B", "covariance", "segregation_intensity", "true_reaction_rate",
+              "mixed_reaction_rate")
+
+
+def reactive_stats(scalar_lattice, a, b, k_react, stream=None):
+    """compute_reactive_stats on the device; returns a dict; synchronizes."""
+    _check_pop(scalar_lattice, a, "a")
+    _check_pop(scalar_lattice, b, "b")
+    nx, ny, nz = _extents(scalar_lattice, a)
+    out = (ctypes.c_double * 8)()
+    check(lib().lbc_reactive_stats(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx, ny,
+                                   nz, float(k_react), out, _stream(stream)), "lbc_reactive_stats")
+    return dict(zip(STAT_NAMES, out))
+
+
+def weights_tensor(lattice, dtype=torch.float32, device="cuda"):
+    return torch.tensor(WEIGHTS[lattice], dtype=torch.float64).to(dtype).to(device)
+
+
+def encode(lattice, f, storage):
+    """Populations (float tensor) -> storage tensor (f16/bf16 hold f - w for fluid lattices)."""
+    dt = STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+    if dt in (torch.float32, torch.float64):
+        return f.to(dt).contiguous()
+    f32 = f.to(torch.float32)
+    if lattice in ("D2Q5", "D3Q7"):
+        return f32.to(dt).contiguous()
+    w = weights_tensor(lattice, torch.float32, f.device).view((-1,) + (1,) * (f.dim() - 1))
+    return (f32 - w).to(dt).contiguous()
+
+
+def decode(lattice, t):
+    """Storage tensor -> float64 populations."""
+    if t.dtype in (torch.float32, torch.float64):
+        return t.to(torch.float64)
+    dev = t.to(torch.float32)
+    if lattice in ("D2Q5", "D3Q7"):
+        return dev.to(torch.float64)
+    w = weights_tensor(lattice, torch.float32, t.device).view((-1,) + (1,) * (t.dim() - 1))
+    return (w + dev).to(torch.float64)

@@ -1,0 +1,64 @@
+"""Loader for the gfx950 kernel library (liblbcube_hip.so, C ABI in include/lbcube.h).
+
+There is no fallback: if the library is missing or cannot be loaded, every
+call raises.  Build it with ``make -C lb-cube_amd`` (or
+``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG_DIR), "liblbcube_hip.so")
+
+_lib = None
+
+c_int, c_u32, c_u64, c_double, c_vp = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+# name -> (argtypes, restype); mirrors include/lbcube.h
+SIGNATURES = {
+    "lbc_fluid_step": ([c_int, c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_u32, c_u32, c_u32, c_vp],
+                       c_int),
+    "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_u32, c_u32,
+                         c_u32, c_vp], c_int),
+    "lbc_reaction_ab_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double,
+                              c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_reactive_fused_step": ([c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64,
+                                 c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
+    "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, _dp, c_vp], c_int),
+    "lbc_default_block": ([c_int, c_u64, c_u64, c_u64, _u32p], None),
+    "lbc_status_string": ([c_int], ctypes.c_char_p),
+    "lbc_abi_version": ([], c_int),
+}
+
+
+class LbcError(RuntimeError):
+    """A C entry point returned a non-zero hipError_t."""
+
+    def __init__(self, status: int, what: str):
+        self.status = status
+        msg = lib().lbc_status_string(status)
+        super().__init__(f"{what} failed: hipError {status} ({msg.decode() if msg else '?'})")
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C lb-cube_amd` first")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = handle
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        raise LbcError(status, what)

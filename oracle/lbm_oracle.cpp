@@ -1,0 +1,1168 @@
+// ============================================================================
+// THE ORACLE — test infrastructure only.
+//
+// A CPU restatement of LB-Cube's fused pull-stream/collide loop, passive-scalar
+// loop, fused A+B->C reaction loop and host statistics, written to follow the
+// reference's operation order so that the HIP kernels can be checked against
+// it bit-for-bit where both sides evaluate the same IEEE operations.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library, and only as the checker or the timed CPU baseline.  Nothing in
+// the product (lb-cube_amd/) links, loads or calls it.
+//
+// Reference files restated (paths relative to the LB-Cube source tree):
+//   include/lattice_traits.hpp   lattice tables, lines 62-305
+//   include/lattice_physics.hpp  macro/feq/BGK/TRT/RLBM/scalar/reaction, 116-743
+//   include/lattice_mrt.hpp      D2Q9/D3Q19/D3Q27 MRT, 38-1305
+//   include/lattice_core.hpp     step_cpu 146-216, step_scalar_cpu 238-337,
+//                                step_reaction_AB 357-474, pull index 70-78
+//   include/lattice_io.hpp       log_diagnostics 182-234,
+//                                compute_reactive_stats 256-338
+//
+// Pinning (what checks each part; the checks live in tests/test_oracle_*.py):
+//   tables, pull index, opposite map ... involution / c[opp]=-c / omega=0
+//                                        permutation KAT with unique ids
+//   macro + feq ........................ isotropy sums (sum feq = rho,
+//                                        sum feq c = rho u), rest fixed point
+//   BGK/TRT/MRT/RLBM ................... the reference's analytic decay tests
+//                                        (tests/test_validation.cpp) restated,
+//                                        TRT(w-=w+) == BGK, D2Q9/D3Q27 MRT with
+//                                        all rates = w == BGK, conservation
+//   MRT transforms ..................... the reference's literal identity
+//                                        vectors (test_validation.cpp:1532-1582)
+//                                        and M M^-1 = I by brute force
+//   scalar ADR ......................... advected decaying sine closed form
+//                                        (test_validation.cpp:1590-1646)
+//   reaction source .................... batch closed form C0/(1+C0 k t),
+//                                        high-precision ODE integration
+//   statistics ......................... closed forms (uniform, complementary
+//                                        half planes) and long double sums
+//   Eigen's summation grouping in dot() is not observable here; dot products
+//   are taken left to right (c_x u_x + c_y u_y) + c_z u_z.
+// ============================================================================
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+using i64 = long long;
+
+// ---------------------------------------------------------------------------
+// Lattice tables — include/lattice_traits.hpp:62-305 (same direction order).
+// ---------------------------------------------------------------------------
+struct D2Q9 {
+    static constexpr int D = 2, Q = 9;
+    static constexpr double cs2 = 1.0 / 3.0;
+    static constexpr double w[Q] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                                    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+    static constexpr int c[Q * D] = {0, 0, 1, 0, 0, 1, -1, 0, 0, -1, 1, 1, -1, 1, -1, -1, 1, -1};
+    static constexpr int opp[Q] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+};
+struct D3Q19 {
+    static constexpr int D = 3, Q = 19;
+    static constexpr double cs2 = 1.0 / 3.0;
+    static constexpr double w[Q] = {1.0 / 3.0,
+                                    1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                                    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                                    1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+    static constexpr int c[Q * D] = {0, 0, 0,   1, 0, 0,   -1, 0, 0,  0, 1, 0,   0, -1, 0,
+                                     0, 0, 1,   0, 0, -1,  1, 1, 0,   -1, 1, 0,  1, -1, 0,
+                                     -1, -1, 0, 1, 0, 1,   -1, 0, 1,  1, 0, -1,  -1, 0, -1,
+                                     0, 1, 1,   0, -1, 1,  0, 1, -1,  0, -1, -1};
+    static constexpr int opp[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+};
+struct D3Q27 {
+    static constexpr int D = 3, Q = 27;
+    static constexpr double cs2 = 1.0 / 3.0;
+    static constexpr double w[Q] = {8.0 / 27.0,
+                                    2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0, 2.0 / 27.0,
+                                    1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0,
+                                    1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0, 1.0 / 54.0,
+                                    1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0,
+                                    1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0, 1.0 / 216.0};
+    static constexpr int c[Q * D] = {0, 0, 0,   1, 0, 0,   -1, 0, 0,  0, 1, 0,   0, -1, 0,
+                                     0, 0, 1,   0, 0, -1,  1, 1, 0,   -1, 1, 0,  1, -1, 0,
+                                     -1, -1, 0, 1, 0, 1,   -1, 0, 1,  1, 0, -1,  -1, 0, -1,
+                                     0, 1, 1,   0, -1, 1,  0, 1, -1,  0, -1, -1,
+                                     1, 1, 1,   -1, 1, 1,  1, -1, 1,  -1, -1, 1,
+                                     1, 1, -1,  -1, 1, -1, 1, -1, -1, -1, -1, -1};
+    static constexpr int opp[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15,
+                                   26, 25, 24, 23, 22, 21, 20, 19};
+};
+struct D2Q5 {
+    static constexpr int D = 2, Q = 5;
+    static constexpr double cs2 = 1.0 / 3.0;
+    static constexpr double w[Q] = {1.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
+    static constexpr int c[Q * D] = {0, 0, 1, 0, 0, 1, -1, 0, 0, -1};
+    static constexpr int opp[Q] = {0, 3, 4, 1, 2};
+};
+struct D3Q7 {
+    static constexpr int D = 3, Q = 7;
+    static constexpr double cs2 = 1.0 / 4.0;  // lattice_traits.hpp:279
+    static constexpr double w[Q] = {1.0 / 4.0, 1.0 / 8.0, 1.0 / 8.0, 1.0 / 8.0, 1.0 / 8.0, 1.0 / 8.0, 1.0 / 8.0};
+    static constexpr int c[Q * D] = {0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1};
+    static constexpr int opp[Q] = {0, 2, 1, 4, 3, 6, 5};
+};
+
+enum Lat { L_D2Q9 = 0, L_D3Q19 = 1, L_D3Q27 = 2, L_D2Q5 = 3, L_D3Q7 = 4 };
+enum Coll { C_BGK = 0, C_TRT = 1, C_MRT = 2, C_RLBM = 3 };
+enum Dt { T_F32 = 0, T_F64 = 1, T_F16 = 2, T_BF16 = 3 };
+
+// ---------------------------------------------------------------------------
+// Storage codecs.  F32/F64 store populations as-is (the reference's
+// LatticeMemory<L, Real>).  F16/BF16 are the project's reduced-precision
+// storage: the stored value is (f - offset) rounded to nearest-even, with
+// offset = w_i for fluid populations and 0 for scalar populations; arithmetic
+// runs in float.  Conversions are IEEE 754 round-to-nearest-even, written
+// from the format definitions.
+// ---------------------------------------------------------------------------
+inline uint16_t f32_to_f16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t a = x & 0x7fffffffu;
+    if (a >= 0x7f800000u) return static_cast<uint16_t>(sign | 0x7c00u | (a > 0x7f800000u ? 0x200u : 0u));
+    if (a >= 0x477ff000u) return static_cast<uint16_t>(sign | 0x7c00u);  // rounds to +-inf
+    if (a < 0x38800000u) {                                               // half subnormal range
+        if (a < 0x33000000u) return static_cast<uint16_t>(sign);
+        const uint32_t e = a >> 23;
+        const uint32_t m = (a & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - e;
+        uint32_t q = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        return static_cast<uint16_t>(sign | q);
+    }
+    const uint32_t r = a - (112u << 23);
+    uint32_t q = r >> 13;
+    const uint32_t rem = r & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++q;
+    return static_cast<uint16_t>(sign | q);
+}
+inline float f16_to_f32(uint16_t h) {
+    const uint32_t sign = (static_cast<uint32_t>(h) & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu;
+    const uint32_t m = h & 0x3ffu;
+    float out;
+    if (e == 0) {
+        out = std::ldexp(static_cast<float>(m), -24);
+        return sign ? -out : out;
+    }
+    uint32_t bits;
+    if (e == 31) bits = sign | 0x7f800000u | (m << 13);
+    else bits = sign | ((e + 112u) << 23) | (m << 13);
+    std::memcpy(&out, &bits, 4);
+    return out;
+}
+inline uint16_t f32_to_bf16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((x >> 16) | 0x40u);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return static_cast<uint16_t>(x >> 16);
+}
+inline float bf16_to_f32(uint16_t h) {
+    const uint32_t bits = static_cast<uint32_t>(h) << 16;
+    float out;
+    std::memcpy(&out, &bits, 4);
+    return out;
+}
+
+template <int DT> struct Codec;
+template <> struct Codec<T_F32> {
+    using S = float; using R = float;
+    static R load(S v, R) { return v; }
+    static S store(R f, R) { return f; }
+};
+template <> struct Codec<T_F64> {
+    using S = double; using R = double;
+    static R load(S v, R) { return v; }
+    static S store(R f, R) { return f; }
+};
+template <> struct Codec<T_F16> {
+    using S = uint16_t; using R = float;
+    static R load(S v, R off) { return off + f16_to_f32(v); }
+    static S store(R f, R off) { return f32_to_f16(f - off); }
+};
+template <> struct Codec<T_BF16> {
+    using S = uint16_t; using R = float;
+    static R load(S v, R off) { return off + bf16_to_f32(v); }
+    static S store(R f, R off) { return f32_to_bf16(f - off); }
+};
+
+// ---------------------------------------------------------------------------
+// Stateless per-cell physics — include/lattice_physics.hpp.
+// ---------------------------------------------------------------------------
+template <class L, class R> struct Macro {
+    R rho{};
+    R u[3]{};
+};
+
+// compute_macro_state, lattice_physics.hpp:227-249
+template <class L, class R> Macro<L, R> macro_state(const R* f) {
+    R density{};
+    R mom[3]{};
+    for (int i = 0; i < L::Q; ++i) {
+        density += f[i];
+        for (int d = 0; d < L::D; ++d) mom[d] += f[i] * static_cast<R>(L::c[i * L::D + d]);
+    }
+    Macro<L, R> m;
+    m.rho = density;
+    for (int d = 0; d < L::D; ++d) m.u[d] = mom[d] / density;
+    return m;
+}
+
+// Left-to-right dot product of a lattice direction with a velocity.
+template <class L, class R> R dir_dot(int i, const R* u) {
+    R s = static_cast<R>(L::c[i * L::D]) * u[0];
+    for (int d = 1; d < L::D; ++d) s += static_cast<R>(L::c[i * L::D + d]) * u[d];
+    return s;
+}
+template <class L, class R> R vel_dot(const R* u) {
+    R s = u[0] * u[0];
+    for (int d = 1; d < L::D; ++d) s += u[d] * u[d];
+    return s;
+}
+
+// compute_equilibrium, lattice_physics.hpp:267-290
+template <class L, class R> R feq(int i, const Macro<L, R>& m) {
+    const R weight = static_cast<R>(L::w[i]);
+    const R cs2 = static_cast<R>(L::cs2);
+    const R inv_cs2 = static_cast<R>(1.0) / cs2;
+    const R cu = dir_dot<L, R>(i, m.u);
+    const R uu = vel_dot<L, R>(m.u);
+    return weight * m.rho *
+           (static_cast<R>(1.0) + inv_cs2 * cu + static_cast<R>(0.5) * inv_cs2 * inv_cs2 * cu * cu -
+            static_cast<R>(0.5) * inv_cs2 * uu);
+}
+
+// collide_bgk, lattice_physics.hpp:307-318
+template <class L, class R> void collide_bgk(R* f, R omega) {
+    const Macro<L, R> m = macro_state<L, R>(f);
+    for (int i = 0; i < L::Q; ++i) f[i] += omega * (feq<L, R>(i, m) - f[i]);
+}
+
+// compute_omega_minus, lattice_physics.hpp:336-344 (Lambda = 1/4)
+template <class R> R omega_minus(R omega_plus) {
+    const R magic = R{1} / R{4};
+    const R tau_plus = R{1} / omega_plus;
+    const R tau_minus = magic / (tau_plus - R{1} / R{2}) + R{1} / R{2};
+    return R{1} / tau_minus;
+}
+
+// collide_trt, lattice_physics.hpp:366-398
+template <class L, class R> void collide_trt(R* f, const Macro<L, R>& m, R wp, R wm) {
+    R eq[L::Q], post[L::Q];
+    for (int i = 0; i < L::Q; ++i) eq[i] = feq<L, R>(i, m);
+    for (int i = 0; i < L::Q; ++i) {
+        const int j = L::opp[i];
+        const R even = R{0.5} * (f[i] + f[j]);
+        const R eq_even = R{0.5} * (eq[i] + eq[j]);
+        const R odd = R{0.5} * (f[i] - f[j]);
+        const R eq_odd = R{0.5} * (eq[i] - eq[j]);
+        post[i] = f[i] - wp * (even - eq_even) - wm * (odd - eq_odd);
+    }
+    for (int i = 0; i < L::Q; ++i) f[i] = post[i];
+}
+
+// ---------------------------------------------------------------------------
+// RLBM — lattice_physics.hpp:116-211 (Pi_neq) and 422-605 (reconstruction).
+// ---------------------------------------------------------------------------
+template <class L, class R> struct Pi { R xx{}, yy{}, zz{}, xy{}, xz{}, yz{}; };
+
+template <class L, class R> Pi<L, R> pi_neq(const R* f, const R* eq) {
+    Pi<L, R> p;
+    for (int i = 0; i < L::Q; ++i) {
+        const R fn = f[i] - eq[i];
+        const R cx = static_cast<R>(L::c[i * L::D]);
+        const R cy = static_cast<R>(L::c[i * L::D + 1]);
+        if constexpr (L::D == 2) {
+            p.xx += fn * cx * cx;
+            p.yy += fn * cy * cy;
+            p.xy += fn * cx * cy;
+        } else {
+            const R cz = static_cast<R>(L::c[i * L::D + 2]);
+            p.xx += fn * cx * cx;
+            p.yy += fn * cy * cy;
+            p.zz += fn * cz * cz;
+            p.xy += fn * cx * cy;
+            p.xz += fn * cx * cz;
+            p.yz += fn * cy * cz;
+        }
+    }
+    return p;
+}
+
+template <class L, class R> void collide_rlbm(R* f, const Macro<L, R>& m, R omega) {
+    R eq[L::Q];
+    for (int i = 0; i < L::Q; ++i) eq[i] = feq<L, R>(i, m);
+    const Pi<L, R> p = pi_neq<L, R>(f, eq);
+    const R cs2 = static_cast<R>(L::cs2);
+    const R pref = R{9} / R{2};
+    const R ux = m.u[0], uy = m.u[1];
+    if constexpr (L::Q == 9) {
+        const R a_xxx = R{3} * ux * p.xx;
+        const R a_yyy = R{3} * uy * p.yy;
+        const R a_xxy = R{2} * ux * p.xy + uy * p.xx;
+        const R a_xyy = R{2} * uy * p.xy + ux * p.yy;
+        for (int i = 0; i < L::Q; ++i) {
+            const R cx = static_cast<R>(L::c[i * 2]);
+            const R cy = static_cast<R>(L::c[i * 2 + 1]);
+            const R q_xx = cx * cx - cs2;
+            const R q_yy = cy * cy - cs2;
+            const R q_xy = cx * cy;
+            const R second = q_xx * p.xx + q_yy * p.yy + R{2} * q_xy * p.xy;
+            const R h_xxx = cx * cx * cx - R{3} * cs2 * cx;
+            const R h_yyy = cy * cy * cy - R{3} * cs2 * cy;
+            const R h_xxy = cx * cx * cy - cs2 * cy;
+            const R h_xyy = cx * cy * cy - cs2 * cx;
+            const R third = a_xxx * h_xxx + a_yyy * h_yyy + R{3} * (a_xxy * h_xxy + a_xyy * h_xyy);
+            const R rec = static_cast<R>(L::w[i]) * pref * (second + third);
+            f[i] = eq[i] + (R{1} - omega) * rec;
+        }
+    } else if constexpr (L::Q == 19) {
+        const R uz = m.u[2];
+        const R a_xxy = R{2} * ux * p.xy + uy * p.xx;
+        const R a_xxz = R{2} * ux * p.xz + uz * p.xx;
+        const R a_xyy = R{2} * uy * p.xy + ux * p.yy;
+        const R a_yyz = R{2} * uy * p.yz + uz * p.yy;
+        const R a_xzz = R{2} * uz * p.xz + ux * p.zz;
+        const R a_yzz = R{2} * uz * p.yz + uy * p.zz;
+        for (int i = 0; i < L::Q; ++i) {
+            const R cx = static_cast<R>(L::c[i * 3]);
+            const R cy = static_cast<R>(L::c[i * 3 + 1]);
+            const R cz = static_cast<R>(L::c[i * 3 + 2]);
+            const R q_xx = cx * cx - cs2;
+            const R q_yy = cy * cy - cs2;
+            const R q_zz = cz * cz - cs2;
+            const R q_xy = cx * cy;
+            const R q_xz = cx * cz;
+            const R q_yz = cy * cz;
+            const R second = q_xx * p.xx + q_yy * p.yy + q_zz * p.zz +
+                             R{2} * (q_xy * p.xy + q_xz * p.xz + q_yz * p.yz);
+            const R h_xxy = cx * cx * cy - cs2 * cy;
+            const R h_xxz = cx * cx * cz - cs2 * cz;
+            const R h_xyy = cx * cy * cy - cs2 * cx;
+            const R h_yyz = cy * cy * cz - cs2 * cz;
+            const R h_xzz = cx * cz * cz - cs2 * cx;
+            const R h_yzz = cy * cz * cz - cs2 * cy;
+            // paired third-order projection, lattice_physics.hpp:523-529
+            const R third = R{3} * (h_xxy + h_yzz) * (a_xxy + a_yzz) + (h_xxy - h_yzz) * (a_xxy - a_yzz) +
+                            R{3} * (h_xzz + h_xyy) * (a_xzz + a_xyy) + (h_xzz - h_xyy) * (a_xzz - a_xyy) +
+                            R{3} * (h_yyz + h_xxz) * (a_yyz + a_xxz) + (h_yyz - h_xxz) * (a_yyz - a_xxz);
+            const R rec = static_cast<R>(L::w[i]) * pref * (second + third);
+            f[i] = eq[i] + (R{1} - omega) * rec;
+        }
+    } else {
+        const R uz = m.u[2];
+        const R a_xxx = R{3} * ux * p.xx;
+        const R a_yyy = R{3} * uy * p.yy;
+        const R a_zzz = R{3} * uz * p.zz;
+        const R a_xxy = R{2} * ux * p.xy + uy * p.xx;
+        const R a_xxz = R{2} * ux * p.xz + uz * p.xx;
+        const R a_xyy = ux * p.yy + R{2} * uy * p.xy;
+        const R a_yyz = R{2} * uy * p.yz + uz * p.yy;
+        const R a_xzz = ux * p.zz + R{2} * uz * p.xz;
+        const R a_yzz = uy * p.zz + R{2} * uz * p.yz;
+        const R a_xyz = ux * p.yz + uy * p.xz + uz * p.xy;
+        for (int i = 0; i < L::Q; ++i) {
+            const R cx = static_cast<R>(L::c[i * 3]);
+            const R cy = static_cast<R>(L::c[i * 3 + 1]);
+            const R cz = static_cast<R>(L::c[i * 3 + 2]);
+            const R q_xx = cx * cx - cs2;
+            const R q_yy = cy * cy - cs2;
+            const R q_zz = cz * cz - cs2;
+            const R q_xy = cx * cy;
+            const R q_xz = cx * cz;
+            const R q_yz = cy * cz;
+            const R second = q_xx * p.xx + q_yy * p.yy + q_zz * p.zz +
+                             R{2} * (q_xy * p.xy + q_xz * p.xz + q_yz * p.yz);
+            const R h_xxx = cx * cx * cx - R{3} * cs2 * cx;
+            const R h_yyy = cy * cy * cy - R{3} * cs2 * cy;
+            const R h_zzz = cz * cz * cz - R{3} * cs2 * cz;
+            const R h_xxy = cx * cx * cy - cs2 * cy;
+            const R h_xxz = cx * cx * cz - cs2 * cz;
+            const R h_xyy = cx * cy * cy - cs2 * cx;
+            const R h_yyz = cy * cy * cz - cs2 * cz;
+            const R h_xzz = cx * cz * cz - cs2 * cx;
+            const R h_yzz = cy * cz * cz - cs2 * cy;
+            const R h_xyz = cx * cy * cz;
+            const R third = a_xxx * h_xxx + a_yyy * h_yyy + a_zzz * h_zzz +
+                            R{3} * (a_xxy * h_xxy + a_xxz * h_xxz + a_xyy * h_xyy + a_yyz * h_yyz +
+                                    a_xzz * h_xzz + a_yzz * h_yzz) +
+                            R{6} * a_xyz * h_xyz;
+            const R rec = static_cast<R>(L::w[i]) * pref * (second + third);
+            f[i] = eq[i] + (R{1} - omega) * rec;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MRT — include/lattice_mrt.hpp.
+// The inverse transforms of D2Q9 and D3Q19 are M^T D^-1 (D = squared row
+// norms) summed in the reference's term order: natural order for D2Q9
+// (lattice_mrt.hpp:470-534), and m0..m9, m14, m10, m15, m11..m13, m16..m18 for
+// D3Q19 (lattice_mrt.hpp:674-892).  Each coefficient is the correctly rounded
+// quotient M[r][i] / D[r], which equals the reference's reduced constants
+// (e.g. -30/2394 = -5/399).
+// ---------------------------------------------------------------------------
+// Moment basis rows as integer polynomials of the velocity.
+static constexpr int MD2Q9[9][9] = {
+    {1, 1, 1, 1, 1, 1, 1, 1, 1},
+    {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+    {4, -2, -2, -2, -2, 1, 1, 1, 1},
+    {0, 1, 0, -1, 0, 1, -1, -1, 1},
+    {0, -2, 0, 2, 0, 1, -1, -1, 1},
+    {0, 0, 1, 0, -1, 1, 1, -1, -1},
+    {0, 0, -2, 0, 2, 1, 1, -1, -1},
+    {0, 1, -1, 1, -1, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+
+// D3Q19 rows from the moment expressions in lattice_mrt.hpp:617-661.
+static int MD3Q19[19][19];
+static bool md3q19_ready = false;
+static void build_md3q19() {
+    if (md3q19_ready) return;
+    for (int i = 0; i < 19; ++i) {
+        const int cx = D3Q19::c[i * 3], cy = D3Q19::c[i * 3 + 1], cz = D3Q19::c[i * 3 + 2];
+        const int c2 = cx * cx + cy * cy + cz * cz;
+        MD3Q19[0][i] = 1;
+        MD3Q19[1][i] = 19 * c2 - 30;
+        MD3Q19[2][i] = (21 * c2 * c2 - 53 * c2 + 24) / 2;
+        MD3Q19[3][i] = cx;
+        MD3Q19[4][i] = (5 * c2 - 9) * cx;
+        MD3Q19[5][i] = cy;
+        MD3Q19[6][i] = (5 * c2 - 9) * cy;
+        MD3Q19[7][i] = cz;
+        MD3Q19[8][i] = (5 * c2 - 9) * cz;
+        MD3Q19[9][i] = 3 * cx * cx - c2;
+        MD3Q19[10][i] = cy * cy - cz * cz;
+        MD3Q19[11][i] = cx * cy;
+        MD3Q19[12][i] = cy * cz;
+        MD3Q19[13][i] = cx * cz;
+        MD3Q19[14][i] = (3 * c2 - 5) * (3 * cx * cx - c2);
+        MD3Q19[15][i] = (3 * c2 - 5) * (cy * cy - cz * cz);
+        MD3Q19[16][i] = (cy * cy - cz * cz) * cx;
+        MD3Q19[17][i] = (cz * cz - cx * cx) * cy;
+        MD3Q19[18][i] = (cx * cx - cy * cy) * cz;
+    }
+    md3q19_ready = true;
+}
+static constexpr int D3Q19_INV_ORDER[19] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 14, 10, 15, 11, 12, 13, 16, 17, 18};
+
+template <class R, int Q>
+void inverse_by_rows(const int (*M)[Q], const int* order, const R* m, R* f) {
+    int norm[Q];
+    for (int r = 0; r < Q; ++r) {
+        norm[r] = 0;
+        for (int i = 0; i < Q; ++i) norm[r] += M[r][i] * M[r][i];
+    }
+    for (int i = 0; i < Q; ++i) {
+        bool first = true;
+        R acc{};
+        for (int k = 0; k < Q; ++k) {
+            const int r = order[k];
+            if (M[r][i] == 0) continue;
+            const R coef = static_cast<R>(M[r][i]) / static_cast<R>(norm[r]);
+            if (first) { acc = coef * m[r]; first = false; }
+            else acc = acc + coef * m[r];
+        }
+        f[i] = acc;
+    }
+}
+
+// compute_moments_d2q9, lattice_mrt.hpp:439-455
+template <class R> void moments_d2q9(const R* f, R* m) {
+    m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    m[1] = -R{4} * f[0] - f[1] - f[2] - f[3] - f[4] + R{2} * (f[5] + f[6] + f[7] + f[8]);
+    m[2] = R{4} * f[0] - R{2} * (f[1] + f[2] + f[3] + f[4]) + f[5] + f[6] + f[7] + f[8];
+    m[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+    m[4] = -R{2} * f[1] + R{2} * f[3] + f[5] - f[6] - f[7] + f[8];
+    m[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+    m[6] = -R{2} * f[2] + R{2} * f[4] + f[5] + f[6] - f[7] - f[8];
+    m[7] = f[1] - f[2] + f[3] - f[4];
+    m[8] = f[5] - f[6] + f[7] - f[8];
+}
+static constexpr int NATURAL9[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+template <class R> void populations_d2q9(const R* m, R* f) { inverse_by_rows<R, 9>(MD2Q9, NATURAL9, m, f); }
+
+// compute_moments_d3q19, lattice_mrt.hpp:617-661
+template <class R> void moments_d3q19(const R* f, R* m) {
+    m[0] = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
+           f[14] + f[15] + f[16] + f[17] + f[18];
+    m[1] = -R{30} * f[0] - R{11} * (f[1] + f[2] + f[3] + f[4] + f[5] + f[6]) +
+           R{8} * (f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18]);
+    m[2] = R{12} * f[0] - R{4} * (f[1] + f[2] + f[3] + f[4] + f[5] + f[6]) + f[7] + f[8] + f[9] + f[10] + f[11] +
+           f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+    m[3] = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
+    m[4] = -R{4} * f[1] + R{4} * f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
+    m[5] = f[3] - f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18];
+    m[6] = -R{4} * f[3] + R{4} * f[4] + f[7] + f[8] - f[9] - f[10] + f[15] - f[16] + f[17] - f[18];
+    m[7] = f[5] - f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18];
+    m[8] = -R{4} * f[5] + R{4} * f[6] + f[11] + f[12] - f[13] - f[14] + f[15] + f[16] - f[17] - f[18];
+    m[9] = R{2} * (f[1] + f[2]) - f[3] - f[4] - f[5] - f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
+           f[14] - R{2} * (f[15] + f[16] + f[17] + f[18]);
+    m[10] = f[3] + f[4] - f[5] - f[6] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14];
+    m[11] = f[7] - f[8] - f[9] + f[10];
+    m[12] = f[15] - f[16] - f[17] + f[18];
+    m[13] = f[11] - f[12] - f[13] + f[14];
+    m[14] = -R{4} * (f[1] + f[2]) + R{2} * (f[3] + f[4] + f[5] + f[6]) + f[7] + f[8] + f[9] + f[10] + f[11] +
+            f[12] + f[13] + f[14] - R{2} * (f[15] + f[16] + f[17] + f[18]);
+    m[15] = -R{2} * (f[3] + f[4]) + R{2} * (f[5] + f[6]) + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14];
+    m[16] = f[7] - f[8] + f[9] - f[10] - f[11] + f[12] - f[13] + f[14];
+    m[17] = -f[7] - f[8] + f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
+    m[18] = f[11] + f[12] - f[13] - f[14] - f[15] - f[16] + f[17] + f[18];
+}
+template <class R> void populations_d3q19(const R* m, R* f) {
+    build_md3q19();
+    inverse_by_rows<R, 19>(MD3Q19, D3Q19_INV_ORDER, m, f);
+}
+
+// collide_mrt_d2q9, lattice_mrt.hpp:548-603 (rates :38-65)
+template <class R> void collide_mrt_d2q9(R* f, const Macro<D2Q9, R>& mac, R s_nu) {
+    R m[9];
+    moments_d2q9<R>(f, m);
+    const R rho = mac.rho, ux = mac.u[0], uy = mac.u[1];
+    const R jx = rho * ux, jy = rho * uy;
+    const R usq = ux * ux + uy * uy;
+    const R eq[9] = {rho, -R{2} * rho + R{3} * rho * usq, rho - R{3} * rho * usq, jx, -jx, jy, -jy,
+                     rho * (ux * ux - uy * uy), rho * ux * uy};
+    const R s_e = R{1.63}, s_eps = R{1.14}, s_q = R{1.92};
+    m[1] -= s_e * (m[1] - eq[1]);
+    m[2] -= s_eps * (m[2] - eq[2]);
+    m[4] -= s_q * (m[4] - eq[4]);
+    m[6] -= s_q * (m[6] - eq[6]);
+    m[7] -= s_nu * (m[7] - eq[7]);
+    m[8] -= s_nu * (m[8] - eq[8]);
+    populations_d2q9<R>(m, f);
+}
+
+// collide_mrt_d3q19, lattice_mrt.hpp:908-1009; ghost rates fixed to 1 (:980-1001)
+template <class R> void collide_mrt_d3q19(R* f, const Macro<D3Q19, R>& mac, R omega) {
+    R m[19];
+    moments_d3q19<R>(f, m);
+    const R rho = mac.rho, ux = mac.u[0], uy = mac.u[1], uz = mac.u[2];
+    const R jx = rho * ux, jy = rho * uy, jz = rho * uz;
+    const R ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    const R usq = ux2 + uy2 + uz2;
+    const R m9 = R{3} * rho * (R{2} * ux2 - uy2 - uz2);
+    const R m10 = rho * (uy2 - uz2);
+    const R eq[19] = {rho,
+                      -R{11} * rho + R{19} * rho * usq,
+                      R{3} * rho - (R{11} / R{2}) * rho * usq,
+                      jx, -(R{2} / R{3}) * jx,
+                      jy, -(R{2} / R{3}) * jy,
+                      jz, -(R{2} / R{3}) * jz,
+                      m9, m10, rho * ux * uy, rho * uy * uz, rho * ux * uz,
+                      R{0}, R{0}, R{0}, R{0}, R{0}};
+    const R rates[19] = {R{0}, R{1}, R{1}, R{0}, R{1}, R{0}, R{1}, R{0}, R{1}, omega,
+                         omega, omega, omega, omega, R{1}, R{1}, R{1}, R{1}, R{1}};
+    for (int i = 0; i < 19; ++i) m[i] -= rates[i] * (m[i] - eq[i]);
+    populations_d3q19<R>(m, f);
+}
+
+// D3Q27 separable tensor-product transforms, lattice_mrt.hpp:206-426, 1023-1225.
+template <class R> R axis_basis(int order, int c) {
+    const R v = static_cast<R>(c);
+    if (order == 0) return R{1};
+    if (order == 1) return v;
+    return R{3} * v * v - R{2};
+}
+template <class R> R axis_norm(int order) { return order == 0 ? R{3} : (order == 1 ? R{2} : R{6}); }
+inline int tix(int a, int b, int c) { return (a * 3 + b) * 3 + c; }
+
+template <class R> void moments_d3q27(const R* f, R* mo) {
+    R p[27], xt[27], xyt[27], raw[27];
+    for (int i = 0; i < 27; ++i)
+        p[tix(D3Q27::c[i * 3] + 1, D3Q27::c[i * 3 + 1] + 1, D3Q27::c[i * 3 + 2] + 1)] = f[i];
+    for (int ax = 0; ax < 3; ++ax)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) {
+                R v{};
+                for (int ix = 0; ix < 3; ++ix) v += axis_basis<R>(ax, ix - 1) * p[tix(ix, iy, iz)];
+                xt[tix(ax, iy, iz)] = v;
+            }
+    for (int ax = 0; ax < 3; ++ax)
+        for (int ay = 0; ay < 3; ++ay)
+            for (int iz = 0; iz < 3; ++iz) {
+                R v{};
+                for (int iy = 0; iy < 3; ++iy) v += axis_basis<R>(ay, iy - 1) * xt[tix(ax, iy, iz)];
+                xyt[tix(ax, ay, iz)] = v;
+            }
+    for (int ax = 0; ax < 3; ++ax)
+        for (int ay = 0; ay < 3; ++ay)
+            for (int az = 0; az < 3; ++az) {
+                R v{};
+                for (int iz = 0; iz < 3; ++iz) v += axis_basis<R>(az, iz - 1) * xyt[tix(ax, ay, iz)];
+                raw[tix(ax, ay, az)] = v;
+            }
+    const R r200 = raw[tix(2, 0, 0)], r020 = raw[tix(0, 2, 0)], r002 = raw[tix(0, 0, 2)];
+    mo[0] = raw[tix(0, 0, 0)];
+    mo[1] = raw[tix(1, 0, 0)];
+    mo[2] = raw[tix(0, 1, 0)];
+    mo[3] = raw[tix(0, 0, 1)];
+    mo[4] = r200 + r020 + r002;
+    mo[5] = R{2} * r200 - r020 - r002;
+    mo[6] = r020 - r002;
+    static constexpr int rest[21][3] = {{1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1},
+                                        {1, 0, 2}, {0, 1, 2}, {1, 1, 1}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {2, 1, 1},
+                                        {1, 2, 1}, {1, 1, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2}, {2, 2, 2}, {0, 0, 0}};
+    for (int k = 0; k < 20; ++k) mo[7 + k] = raw[tix(rest[k][0], rest[k][1], rest[k][2])];
+}
+
+template <class R> void populations_d3q27(const R* mo, R* f) {
+    R raw[27], zi[27], yzi[27], pt[27];
+    raw[tix(0, 0, 0)] = mo[0];
+    raw[tix(1, 0, 0)] = mo[1];
+    raw[tix(0, 1, 0)] = mo[2];
+    raw[tix(0, 0, 1)] = mo[3];
+    raw[tix(2, 0, 0)] = (mo[4] + mo[5]) / R{3};
+    raw[tix(0, 2, 0)] = (R{2} * mo[4] - mo[5] + R{3} * mo[6]) / R{6};
+    raw[tix(0, 0, 2)] = (R{2} * mo[4] - mo[5] - R{3} * mo[6]) / R{6};
+    static constexpr int rest[20][3] = {{1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1},
+                                        {1, 0, 2}, {0, 1, 2}, {1, 1, 1}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {2, 1, 1},
+                                        {1, 2, 1}, {1, 1, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    for (int k = 0; k < 20; ++k) raw[tix(rest[k][0], rest[k][1], rest[k][2])] = mo[7 + k];
+    for (int ax = 0; ax < 3; ++ax)
+        for (int ay = 0; ay < 3; ++ay)
+            for (int iz = 0; iz < 3; ++iz) {
+                R v{};
+                for (int az = 0; az < 3; ++az)
+                    v += raw[tix(ax, ay, az)] * axis_basis<R>(az, iz - 1) / axis_norm<R>(az);
+                zi[tix(ax, ay, iz)] = v;
+            }
+    for (int ax = 0; ax < 3; ++ax)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) {
+                R v{};
+                for (int ay = 0; ay < 3; ++ay)
+                    v += zi[tix(ax, ay, iz)] * axis_basis<R>(ay, iy - 1) / axis_norm<R>(ay);
+                yzi[tix(ax, iy, iz)] = v;
+            }
+    for (int ix = 0; ix < 3; ++ix)
+        for (int iy = 0; iy < 3; ++iy)
+            for (int iz = 0; iz < 3; ++iz) {
+                R v{};
+                for (int ax = 0; ax < 3; ++ax)
+                    v += yzi[tix(ax, iy, iz)] * axis_basis<R>(ax, ix - 1) / axis_norm<R>(ax);
+                pt[tix(ix, iy, iz)] = v;
+            }
+    for (int i = 0; i < 27; ++i)
+        f[i] = pt[tix(D3Q27::c[i * 3] + 1, D3Q27::c[i * 3 + 1] + 1, D3Q27::c[i * 3 + 2] + 1)];
+}
+
+// collide_mrt_d3q27, lattice_mrt.hpp:1242-1305 (rates :161-204)
+template <class R> void collide_mrt_d3q27(R* f, const Macro<D3Q27, R>& mac, R s_nu) {
+    R m[27], feqp[27], eq[27];
+    moments_d3q27<R>(f, m);
+    const R rho = mac.rho, ux = mac.u[0], uy = mac.u[1], uz = mac.u[2];
+    const R usq = ux * ux + uy * uy + uz * uz;
+    for (int i = 0; i < 27; ++i) {
+        const R cx = static_cast<R>(D3Q27::c[i * 3]);
+        const R cy = static_cast<R>(D3Q27::c[i * 3 + 1]);
+        const R cz = static_cast<R>(D3Q27::c[i * 3 + 2]);
+        const R cu = cx * ux + cy * uy + cz * uz;
+        feqp[i] = static_cast<R>(D3Q27::w[i]) * rho * (R{1} + R{3} * cu + R{4.5} * cu * cu - R{1.5} * usq);
+    }
+    moments_d3q27<R>(feqp, eq);
+    const R s_b = R{1.19}, s_e = R{1.1}, s_eps = R{1.2}, s_q = R{1.2};
+    const R rates[27] = {R{0}, R{0}, R{0}, R{0}, s_b, s_nu, s_nu, s_nu, s_nu, s_nu, s_e, s_e, s_e, s_e,
+                         s_e, s_e, s_eps, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q};
+    for (int i = 0; i < 27; ++i) m[i] -= rates[i] * (m[i] - eq[i]);
+    populations_d3q27<R>(m, f);
+}
+
+// detail::collide_cell, lattice_core.hpp:90-126
+template <class L, int CT, class R> void collide_cell(R* f, R omega) {
+    if constexpr (CT == C_BGK) {
+        collide_bgk<L, R>(f, omega);
+    } else if constexpr (CT == C_TRT) {
+        const Macro<L, R> m = macro_state<L, R>(f);
+        collide_trt<L, R>(f, m, omega, omega_minus<R>(omega));
+    } else if constexpr (CT == C_MRT) {
+        const Macro<L, R> m = macro_state<L, R>(f);
+        if constexpr (L::Q == 9) collide_mrt_d2q9<R>(f, m, omega);
+        else if constexpr (L::Q == 19) collide_mrt_d3q19<R>(f, m, omega);
+        else collide_mrt_d3q27<R>(f, m, omega);
+    } else {
+        const Macro<L, R> m = macro_state<L, R>(f);
+        collide_rlbm<L, R>(f, m, omega);
+    }
+}
+
+// Scalar physics, lattice_physics.hpp:618-694
+template <class S, class R> R concentration(const R* g) {
+    R c{};
+    for (int i = 0; i < S::Q; ++i) c += g[i];
+    return c;
+}
+template <class S, class R> R scalar_feq(int i, R conc, const R* u) {
+    R cu{};
+    for (int d = 0; d < S::D; ++d) cu += static_cast<R>(S::c[i * S::D + d]) * u[d];
+    const R weight = static_cast<R>(S::w[i]);
+    const R cs2 = static_cast<R>(S::cs2);
+    return weight * conc * (static_cast<R>(1.0) + cu / cs2);
+}
+template <class S, class R> void collide_scalar(R* g, const R* u, R omega_c, R source) {
+    const R conc = concentration<S, R>(g);
+    for (int i = 0; i < S::Q; ++i) {
+        const R weight = static_cast<R>(S::w[i]);
+        const R eq = scalar_feq<S, R>(i, conc, u);
+        g[i] += omega_c * (eq - g[i]) + weight * source;
+    }
+}
+
+// compute_reaction_ab_source, lattice_physics.hpp:712-743
+template <class R> R reaction_source(R a, R b, R k) {
+    const R abs_a = a < R{} ? -a : a;
+    const R abs_b = b < R{} ? -b : b;
+    const R diff = b - a;
+    const R abs_diff = diff < R{} ? -diff : diff;
+    const R tol = static_cast<R>(64.0) * std::numeric_limits<R>::epsilon() * (static_cast<R>(1.0) + abs_a + abs_b);
+    R after{};
+    if (abs_diff <= tol) {
+        after = a / (static_cast<R>(1.0) + k * a);
+    } else {
+        using std::exp;
+        const R att = exp(-k * diff);
+        after = a * diff * att / (b - a * att);
+    }
+    return after - a;
+}
+
+// periodic_pull_index, lattice_core.hpp:70-78
+inline i64 pull(i64 x, int c, i64 n) { return (x - c + n) % n; }
+
+struct Grid {
+    i64 nx, ny, nz;
+    i64 cells() const { return nx * ny * nz; }
+};
+
+// step_cpu, lattice_core.hpp:146-216 (no swap: the caller swaps buffers).
+template <class L, int CT, int DT>
+void fluid_step(const void* curv, void* nxtv, Grid g, double omega_d, int threads) {
+    using C = Codec<DT>;
+    using S = typename C::S;
+    using R = typename C::R;
+    const S* cur = static_cast<const S*>(curv);
+    S* nxt = static_cast<S*>(nxtv);
+    const R omega = static_cast<R>(omega_d);
+    const i64 N = g.cells();
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    (void)threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (i64 zy = 0; zy < g.nz * g.ny; ++zy) {
+        const i64 z = zy / g.ny, y = zy % g.ny;
+        for (i64 x = 0; x < g.nx; ++x) {
+            R f[L::Q];
+            for (int i = 0; i < L::Q; ++i) {
+                const i64 sx = pull(x, L::c[i * L::D], g.nx);
+                const i64 sy = pull(y, L::c[i * L::D + 1], g.ny);
+                const i64 sz = L::D == 3 ? pull(z, L::c[i * L::D + 2], g.nz) : 0;
+                const R o = off ? static_cast<R>(L::w[i]) : R{0};
+                f[i] = C::load(cur[i * N + (sz * g.ny + sy) * g.nx + sx], o);
+            }
+            collide_cell<L, CT, R>(f, omega);
+            for (int i = 0; i < L::Q; ++i) {
+                const R o = off ? static_cast<R>(L::w[i]) : R{0};
+                nxt[i * N + (z * g.ny + y) * g.nx + x] = C::store(f[i], o);
+            }
+        }
+    }
+}
+
+// step_scalar_cpu (lattice_core.hpp:238-337) and step_reaction_AB
+// (lattice_core.hpp:357-474), one cell at a time; `reaction` selects the
+// fused A+B->C form.  The fluid velocity comes from the fluid buffer at the
+// destination cell, unshifted.
+template <class F, class S, int DT>
+void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void* nxt_bv, const void* cur_fv, Grid g,
+                      double omega_cd, double param, bool reaction, int threads) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* ca = static_cast<const St*>(cur_av);
+    St* na = static_cast<St*>(nxt_av);
+    const St* cb = static_cast<const St*>(cur_bv);
+    St* nb = static_cast<St*>(nxt_bv);
+    const St* cf = static_cast<const St*>(cur_fv);
+    const R omega_c = static_cast<R>(omega_cd);
+    const R prm = static_cast<R>(param);
+    const i64 N = g.cells();
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    (void)threads;
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (i64 zy = 0; zy < g.nz * g.ny; ++zy) {
+        const i64 z = zy / g.ny, y = zy % g.ny;
+        for (i64 x = 0; x < g.nx; ++x) {
+            const i64 cell = (z * g.ny + y) * g.nx + x;
+            R fl[F::Q];
+            for (int i = 0; i < F::Q; ++i)
+                fl[i] = C::load(cf[i * N + cell], off ? static_cast<R>(F::w[i]) : R{0});
+            const Macro<F, R> m = macro_state<F, R>(fl);
+            R ga[S::Q], gb[S::Q];
+            for (int i = 0; i < S::Q; ++i) {
+                const i64 sx = pull(x, S::c[i * S::D], g.nx);
+                const i64 sy = pull(y, S::c[i * S::D + 1], g.ny);
+                const i64 sz = S::D == 3 ? pull(z, S::c[i * S::D + 2], g.nz) : 0;
+                const i64 src = i * N + (sz * g.ny + sy) * g.nx + sx;
+                ga[i] = C::load(ca[src], R{0});
+                if (reaction) gb[i] = C::load(cb[src], R{0});
+            }
+            if (reaction) {
+                const R conc_a = concentration<S, R>(ga);
+                const R conc_b = concentration<S, R>(gb);
+                const R src = reaction_source<R>(conc_a, conc_b, prm);
+                collide_scalar<S, R>(ga, m.u, omega_c, src);
+                collide_scalar<S, R>(gb, m.u, omega_c, src);
+            } else {
+                collide_scalar<S, R>(ga, m.u, omega_c, prm);
+            }
+            for (int i = 0; i < S::Q; ++i) {
+                na[i * N + cell] = C::store(ga[i], R{0});
+                if (reaction) nb[i * N + cell] = C::store(gb[i], R{0});
+            }
+        }
+    }
+}
+
+// compute_reactive_stats, lattice_io.hpp:256-338 (accumulation in Real as the
+// reference does) plus a long double accumulation used as the brute-force sum.
+template <class S, int DT>
+void reactive_stats(const void* av, const void* bv, Grid g, double k_d, double* out, bool wide) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* a = static_cast<const St*>(av);
+    const St* b = static_cast<const St*>(bv);
+    const i64 N = g.cells();
+    if (!wide) {
+        R sa{}, sb{}, saa{}, sbb{}, sab{};
+        for (i64 cell = 0; cell < N; ++cell) {
+            R ca{}, cb{};
+            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], R{0});
+            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], R{0});
+            sa += ca; sb += cb; saa += ca * ca; sbb += cb * cb; sab += ca * cb;
+        }
+        const R k = static_cast<R>(k_d);
+        const R inv = R{1} / static_cast<R>(N);
+        const R ma = sa * inv, mb = sb * inv, maa = saa * inv, mbb = sbb * inv, mab = sab * inv;
+        const R mixed = k * ma * mb;
+        const R cov = mab - ma * mb;
+        const R mp = ma * mb;
+        const R seg = std::abs(mp) > std::numeric_limits<R>::epsilon() ? cov / mp : R{};
+        const R res[8] = {ma, mb, maa - ma * ma, mbb - mb * mb, cov, seg, k * mab, mixed};
+        for (int j = 0; j < 8; ++j) out[j] = static_cast<double>(res[j]);
+    } else {
+        long double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
+        for (i64 cell = 0; cell < N; ++cell) {
+            R ca{}, cb{};
+            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], R{0});
+            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], R{0});
+            const long double la = ca, lb = cb;
+            sa += la; sb += lb; saa += la * la; sbb += lb * lb; sab += la * lb;
+        }
+        const long double k = k_d;
+        const long double inv = 1.0L / static_cast<long double>(N);
+        const long double ma = sa * inv, mb = sb * inv, maa = saa * inv, mbb = sbb * inv, mab = sab * inv;
+        const long double cov = mab - ma * mb;
+        const long double mp = ma * mb;
+        const long double seg = std::fabs(mp) > std::numeric_limits<R>::epsilon() ? cov / mp : 0.0L;
+        const long double res[8] = {ma, mb, maa - ma * ma, mbb - mb * mb, cov, seg, k * mab, k * ma * mb};
+        for (int j = 0; j < 8; ++j) out[j] = static_cast<double>(res[j]);
+    }
+}
+
+// log_diagnostics, lattice_io.hpp:182-234: total mass, total kinetic energy
+// (sum of rho |u|^2 / 2) and max |u|.  `wide` accumulates in long double.
+template <class L, int DT> void fluid_diagnostics(const void* fv, Grid g, double* out, bool wide) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* f = static_cast<const St*>(fv);
+    const i64 N = g.cells();
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    R mass{}, ke{}, umax{};
+    long double lmass = 0, lke = 0;
+    for (i64 cell = 0; cell < N; ++cell) {
+        R loc[L::Q];
+        for (int i = 0; i < L::Q; ++i) loc[i] = C::load(f[i * N + cell], off ? static_cast<R>(L::w[i]) : R{0});
+        const Macro<L, R> m = macro_state<L, R>(loc);
+        const R sp2 = vel_dot<L, R>(m.u);
+        if (wide) {
+            lmass += m.rho;
+            lke += 0.5L * static_cast<long double>(m.rho) * static_cast<long double>(sp2);
+        } else {
+            mass += m.rho;
+            ke += static_cast<R>(0.5) * m.rho * sp2;
+        }
+        umax = std::max(umax, std::sqrt(sp2));
+    }
+    out[0] = wide ? static_cast<double>(lmass) : static_cast<double>(mass);
+    out[1] = wide ? static_cast<double>(lke) : static_cast<double>(ke);
+    out[2] = static_cast<double>(umax);
+}
+
+// Equilibrium fields from macroscopic fields (initialisers of
+// src/main.cpp:368-411 and src/simulate_reaction.cpp:197-232 go through
+// compute_equilibrium / compute_scalar_equilibrium per cell).
+template <class L, int DT> void feq_field(const double* rho, const double* u, void* outv, i64 N) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    St* out = static_cast<St*>(outv);
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    for (i64 cell = 0; cell < N; ++cell) {
+        Macro<L, R> m;
+        m.rho = static_cast<R>(rho[cell]);
+        for (int d = 0; d < L::D; ++d) m.u[d] = static_cast<R>(u[d * N + cell]);
+        for (int i = 0; i < L::Q; ++i)
+            out[i * N + cell] = C::store(feq<L, R>(i, m), off ? static_cast<R>(L::w[i]) : R{0});
+    }
+}
+template <class S, int DT> void scalar_feq_field(const double* conc, const double* u, void* outv, i64 N) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    St* out = static_cast<St*>(outv);
+    for (i64 cell = 0; cell < N; ++cell) {
+        R uu[3];
+        for (int d = 0; d < S::D; ++d) uu[d] = static_cast<R>(u[d * N + cell]);
+        for (int i = 0; i < S::Q; ++i)
+            out[i * N + cell] = C::store(scalar_feq<S, R>(i, static_cast<R>(conc[cell]), uu), R{0});
+    }
+}
+
+}  // namespace orc
+
+// ============================================================================
+// C entry points (ctypes).  Return 0 on success, -1 for an unsupported
+// combination.
+// ============================================================================
+using namespace orc;
+
+#define ORC_DT_SWITCH(DTV, ...)                      \
+    switch (DTV) {                                    \
+        case T_F32: { constexpr int DT = T_F32; __VA_ARGS__; break; }   \
+        case T_F64: { constexpr int DT = T_F64; __VA_ARGS__; break; }   \
+        case T_F16: { constexpr int DT = T_F16; __VA_ARGS__; break; }   \
+        case T_BF16: { constexpr int DT = T_BF16; __VA_ARGS__; break; } \
+        default: return -1;                           \
+    }
+
+#define ORC_CT_SWITCH(CTV, ...)                      \
+    switch (CTV) {                                    \
+        case C_BGK: { constexpr int CT = C_BGK; __VA_ARGS__; break; }   \
+        case C_TRT: { constexpr int CT = C_TRT; __VA_ARGS__; break; }   \
+        case C_MRT: { constexpr int CT = C_MRT; __VA_ARGS__; break; }   \
+        case C_RLBM: { constexpr int CT = C_RLBM; __VA_ARGS__; break; } \
+        default: return -1;                           \
+    }
+
+#define ORC_FLUID_SWITCH(LV, ...)                    \
+    switch (LV) {                                     \
+        case L_D2Q9: { using L = D2Q9; __VA_ARGS__; break; }  \
+        case L_D3Q19: { using L = D3Q19; __VA_ARGS__; break; } \
+        case L_D3Q27: { using L = D3Q27; __VA_ARGS__; break; } \
+        default: return -1;                           \
+    }
+
+static Grid make_grid(int lat, long long nx, long long ny, long long nz) {
+    Grid g{nx, ny, nz};
+    if (lat == L_D2Q9 || lat == L_D2Q5) g.nz = 1;
+    return g;
+}
+
+extern "C" {
+
+int orc_fluid_step(int lat, int ct, int dt, const void* cur, void* nxt, long long nx, long long ny, long long nz,
+                   double omega, int threads) {
+    const Grid g = make_grid(lat, nx, ny, nz);
+    ORC_FLUID_SWITCH(lat, ORC_CT_SWITCH(ct, ORC_DT_SWITCH(dt, (fluid_step<L, CT, DT>(cur, nxt, g, omega, threads)))));
+    return 0;
+}
+
+// Runs `steps` fluid steps ping-ponging between buf and tmp; the final state is
+// copied back into buf.
+int orc_fluid_run(int lat, int ct, int dt, void* buf, void* tmp, long long nx, long long ny, long long nz,
+                  double omega, int steps, int threads) {
+    const Grid g = make_grid(lat, nx, ny, nz);
+    int q = lat == L_D2Q9 ? 9 : (lat == L_D3Q19 ? 19 : 27);
+    const size_t es = dt == T_F64 ? 8 : (dt == T_F32 ? 4 : 2);
+    void* a = buf;
+    void* b = tmp;
+    for (int s = 0; s < steps; ++s) {
+        const int rc = orc_fluid_step(lat, ct, dt, a, b, nx, ny, nz, omega, threads);
+        if (rc) return rc;
+        std::swap(a, b);
+    }
+    if (a != buf) std::memcpy(buf, a, es * q * g.cells());
+    return 0;
+}
+
+static int scalar_dispatch(int flat, int slat, int dt, const void* ca, void* na, const void* cb, void* nb,
+                           const void* cf, long long nx, long long ny, long long nz, double omega_c, double param,
+                           bool reaction, int threads) {
+    const Grid g = make_grid(flat, nx, ny, nz);
+    if (flat == L_D2Q9 && slat == L_D2Q5) {
+        ORC_DT_SWITCH(dt, (scalar_like_step<D2Q9, D2Q5, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+    } else if (flat == L_D3Q19 && slat == L_D3Q7) {
+        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q19, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+    } else if (flat == L_D3Q27 && slat == L_D3Q7) {
+        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q27, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_scalar_step(int flat, int slat, int dt, const void* cur_s, void* nxt_s, const void* cur_f, long long nx,
+                    long long ny, long long nz, double omega_c, double source, int threads) {
+    return scalar_dispatch(flat, slat, dt, cur_s, nxt_s, nullptr, nullptr, cur_f, nx, ny, nz, omega_c, source, false,
+                           threads);
+}
+
+int orc_reaction_step(int flat, int slat, int dt, const void* cur_a, void* nxt_a, const void* cur_b, void* nxt_b,
+                      const void* cur_f, long long nx, long long ny, long long nz, double omega_c, double k_react,
+                      int threads) {
+    return scalar_dispatch(flat, slat, dt, cur_a, nxt_a, cur_b, nxt_b, cur_f, nx, ny, nz, omega_c, k_react, true,
+                           threads);
+}
+
+// The reactive loop of src/simulate_reaction.cpp:548-554: a fluid step, then
+// the reaction step reading the freshly written fluid state.  Final states are
+// copied back into f, a and b.
+int orc_reactive_run(int flat, int ct, int slat, int dt, void* f, void* ft, void* a, void* at, void* b, void* bt,
+                     long long nx, long long ny, long long nz, double omega, double omega_c, double k_react,
+                     int steps, int threads) {
+    const Grid g = make_grid(flat, nx, ny, nz);
+    const int qf = flat == L_D2Q9 ? 9 : (flat == L_D3Q19 ? 19 : 27);
+    const int qs = slat == L_D2Q5 ? 5 : 7;
+    const size_t es = dt == T_F64 ? 8 : (dt == T_F32 ? 4 : 2);
+    void *f0 = f, *f1 = ft, *a0 = a, *a1 = at, *b0 = b, *b1 = bt;
+    for (int s = 0; s < steps; ++s) {
+        int rc = orc_fluid_step(flat, ct, dt, f0, f1, nx, ny, nz, omega, threads);
+        if (rc) return rc;
+        std::swap(f0, f1);
+        rc = orc_reaction_step(flat, slat, dt, a0, a1, b0, b1, f0, nx, ny, nz, omega_c, k_react, threads);
+        if (rc) return rc;
+        std::swap(a0, a1);
+        std::swap(b0, b1);
+    }
+    if (f0 != f) std::memcpy(f, f0, es * qf * g.cells());
+    if (a0 != a) std::memcpy(a, a0, es * qs * g.cells());
+    if (b0 != b) std::memcpy(b, b0, es * qs * g.cells());
+    return 0;
+}
+
+// Single-cell physics probes (dt: 0 = float, 1 = double; arrays in that type).
+int orc_collide_cell(int lat, int ct, int dt, void* f, double omega) {
+    if (dt == T_F32) {
+        ORC_FLUID_SWITCH(lat, ORC_CT_SWITCH(ct, (collide_cell<L, CT, float>(static_cast<float*>(f), (float)omega))));
+    } else if (dt == T_F64) {
+        ORC_FLUID_SWITCH(lat, ORC_CT_SWITCH(ct, (collide_cell<L, CT, double>(static_cast<double*>(f), omega))));
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_macro_feq(int lat, const double* f, double* rho_u, double* feq_out) {
+    ORC_FLUID_SWITCH(lat, {
+        const Macro<L, double> m = macro_state<L, double>(f);
+        rho_u[0] = m.rho;
+        for (int d = 0; d < L::D; ++d) rho_u[1 + d] = m.u[d];
+        for (int i = 0; i < L::Q; ++i) feq_out[i] = feq<L, double>(i, m);
+    });
+    return 0;
+}
+
+int orc_mrt_forward(int lat, const double* f, double* m) {
+    if (lat == L_D2Q9) moments_d2q9<double>(f, m);
+    else if (lat == L_D3Q19) moments_d3q19<double>(f, m);
+    else if (lat == L_D3Q27) moments_d3q27<double>(f, m);
+    else return -1;
+    return 0;
+}
+
+int orc_mrt_inverse(int lat, const double* m, double* f) {
+    if (lat == L_D2Q9) populations_d2q9<double>(m, f);
+    else if (lat == L_D3Q19) populations_d3q19<double>(m, f);
+    else if (lat == L_D3Q27) populations_d3q27<double>(m, f);
+    else return -1;
+    return 0;
+}
+
+int orc_reaction_source(int dt, const double* a, const double* b, double k, double* out, long long n) {
+    for (long long j = 0; j < n; ++j) {
+        if (dt == T_F64) out[j] = reaction_source<double>(a[j], b[j], k);
+        else if (dt == T_F32) out[j] = reaction_source<float>((float)a[j], (float)b[j], (float)k);
+        else return -1;
+    }
+    return 0;
+}
+
+int orc_reactive_stats(int slat, int dt, const void* a, const void* b, long long nx, long long ny, long long nz,
+                       double k_react, int wide, double* out) {
+    const Grid g = make_grid(slat, nx, ny, nz);
+    if (slat == L_D2Q5) {
+        ORC_DT_SWITCH(dt, (reactive_stats<D2Q5, DT>(a, b, g, k_react, out, wide != 0)));
+    } else if (slat == L_D3Q7) {
+        ORC_DT_SWITCH(dt, (reactive_stats<D3Q7, DT>(a, b, g, k_react, out, wide != 0)));
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_fluid_diagnostics(int lat, int dt, const void* f, long long nx, long long ny, long long nz, int wide,
+                          double* out) {
+    const Grid g = make_grid(lat, nx, ny, nz);
+    ORC_FLUID_SWITCH(lat, ORC_DT_SWITCH(dt, (fluid_diagnostics<L, DT>(f, g, out, wide != 0))));
+    return 0;
+}
+
+int orc_feq_field(int lat, int dt, const double* rho, const double* u, void* out, long long ncells) {
+    ORC_FLUID_SWITCH(lat, ORC_DT_SWITCH(dt, (feq_field<L, DT>(rho, u, out, ncells))));
+    return 0;
+}
+
+int orc_scalar_feq_field(int slat, int dt, const double* conc, const double* u, void* out, long long ncells) {
+    if (slat == L_D2Q5) {
+        ORC_DT_SWITCH(dt, (scalar_feq_field<D2Q5, DT>(conc, u, out, ncells)));
+    } else if (slat == L_D3Q7) {
+        ORC_DT_SWITCH(dt, (scalar_feq_field<D3Q7, DT>(conc, u, out, ncells)));
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_f16_roundtrip(const float* in, uint16_t* h, float* back, long long n, int bf) {
+    for (long long j = 0; j < n; ++j) {
+        h[j] = bf ? f32_to_bf16(in[j]) : f32_to_f16(in[j]);
+        back[j] = bf ? bf16_to_f32(h[j]) : f16_to_f32(h[j]);
+    }
+    return 0;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the oracle.
+
+The kernels evaluate the reference's per-cell operation sequence with
+-ffp-contract=off, and the oracle does the same on the CPU, so fluid and
+passive-scalar steps must agree BIT FOR BIT in every storage format.  The
+reaction step calls exp(), whose device and host implementations may differ
+in the last ulp; it is held to a relative tolerance instead (stated per test).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import cases
+import lbcube
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FLUIDS = ["D2Q9", "D3Q19", "D3Q27"]
+COLLS = ["BGK", "TRT", "MRT", "RLBM"]
+STORES = ["f64", "f32", "f16", "bf16"]
+SHAPE = {"D2Q9": (11, 13), "D3Q19": (5, 7, 9), "D3Q27": (5, 7, 9)}  # odd extents: every wrap path
+TORCH_DT = {"f64": torch.float64, "f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}
+
+
+def to_dev(a, storage, device):
+    a = np.ascontiguousarray(a)
+    if storage in ("f16", "bf16"):
+        return torch.from_numpy(a.view(np.int16)).view(TORCH_DT[storage]).to(device)
+    return torch.from_numpy(a).to(device)
+
+
+def to_host(t, storage):
+    if storage in ("f16", "bf16"):
+        return t.cpu().view(torch.int16).numpy().view(np.uint16)
+    return t.cpu().numpy()
+
+
+def gpu_run(lattice, coll, storage, f0, omega, steps, device, block=None):
+    a = to_dev(f0, storage, device)
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block)
+        a, b = b, a
+    torch.cuda.synchronize()
+    return to_host(a, storage)
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("coll", COLLS)
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_fluid_step_bit_exact(gpu, lattice, coll, storage):
+    print(f"case fluid {lattice} {coll} {storage}", flush=True)
+    f0 = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=11)
+    omega = 1.0 / 0.62
+    got = gpu_run(lattice, coll, storage, f0, omega, 6, gpu)
+    want = O.fluid_run(lattice, coll, storage, f0, omega, 6)
+    assert got.dtype == want.dtype
+    assert np.array_equal(got, want), f"max |diff| {np.max(np.abs(got.astype(np.float64) - want))}"
+
+
+@pytest.mark.parametrize("block", [(16, 8, 1), (32, 32, 1), (64, 1, 1), (7, 3, 2), (1, 1, 1)])
+def test_block_shape_does_not_change_results(gpu, block):
+    print(f"case block {block}", flush=True)
+    f0 = cases.random_perturbed("D3Q19", (6, 10, 70), "f32", seed=3)
+    want = O.fluid_run("D3Q19", "TRT", "f32", f0, 1.7, 3)
+    got = gpu_run("D3Q19", "TRT", "f32", f0, 1.7, 3, gpu, block=block)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_streaming_permutation_kat(gpu, lattice, storage):
+    """BGK with omega = 0 is the pure pull permutation next[i][x] = cur[i][x - c_i]."""
+    print(f"case kat {lattice} {storage}", flush=True)
+    shape = SHAPE[lattice]
+    q = O.Q_OF[lattice]
+    n = int(np.prod(shape))
+    ids = (1.0 + np.arange(q * n, dtype=np.float64)).reshape((q,) + shape)
+    if storage == "f32":
+        ids = ids.astype(np.float32)
+    got = gpu_run(lattice, "BGK", storage, ids, 0.0, 1, gpu)
+    vel = lbcube.VELOCITIES[lattice]
+    want = np.empty_like(ids)
+    for i, c in enumerate(vel):
+        axes = tuple(range(len(shape) - 1, -1, -1))  # x is the last array axis
+        want[i] = np.roll(ids[i], shift=tuple(c), axis=axes)
+    assert np.array_equal(got, want)
+
+
+def test_config1_tgv_64cubed_1000_steps(gpu):
+    """D3Q19 BGK, 64^3 periodic TGV, FP64, tau = 0.6, u0 = 0.05, 1000 steps."""
+    print("case config1 D3Q19 BGK f64 64^3 x 1000", flush=True)
+    f0 = cases.taylor_green_3d(64, 0.05)
+    omega = 1.0 / 0.6
+    got = gpu_run("D3Q19", "BGK", "f64", f0, omega, 1000, gpu)
+    want = O.fluid_run("D3Q19", "BGK", "f64", f0, omega, 1000, threads=O.max_threads())
+    rel = np.max(np.abs(got - want) / np.abs(want))
+    assert rel <= 1e-12, rel  # the stated tolerance; the kernels are in fact bit-exact here
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
+def test_scalar_step_bit_exact(gpu, pair, storage):
+    fl, sl = pair
+    print(f"case scalar {fl} {sl} {storage}", flush=True)
+    shape = SHAPE[fl]
+    f = cases.random_perturbed(fl, shape, storage, seed=5)
+    g = cases.random_perturbed(sl, shape, storage, seed=9, eps=0.05)
+    want = O.scalar_step(fl, sl, storage, g, f, 1.25, 3e-4)
+    gd, fd = to_dev(g, storage, gpu), to_dev(f, storage, gpu)
+    out = torch.empty_like(gd)
+    lbcube.scalar_step(fl, sl, gd, out, fd, 1.25, 3e-4)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(out, storage), want)
+
+
+def _species(sl, shape, storage, seed):
+    g = cases.rng(seed)
+    d = O.D_OF[sl]
+    ca = g.uniform(0.0, 1.0, size=shape)
+    cb = 1.0 - ca + g.uniform(-0.05, 0.05, size=shape)
+    cb[..., 0] = ca[..., 0]  # exercise the |A - B| <= tol branch
+    u = g.uniform(-0.02, 0.02, size=(d,) + tuple(shape))
+    fa = O.scalar_feq_field(sl, storage if storage in ("f32", "f64") else "f64", ca, u)
+    fb = O.scalar_feq_field(sl, storage if storage in ("f32", "f64") else "f64", cb, u)
+    if storage in ("f16", "bf16"):
+        fa, fb = cases.encode(sl, storage, fa), cases.encode(sl, storage, fb)
+    return fa, fb
+
+
+# Reaction tolerances: exp() of the device library may differ from the host's
+# by an ulp; relative error of the populations after the steps below.
+REACT_TOL = {"f64": 1e-13, "f32": 2e-6, "f16": 2e-3, "bf16": 1.6e-2}
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
+def test_reaction_step_parity(gpu, pair, storage):
+    fl, sl = pair
+    print(f"case reaction {fl} {sl} {storage}", flush=True)
+    shape = SHAPE[fl]
+    f = cases.random_perturbed(fl, shape, storage, seed=21)
+    a, b = _species(sl, shape, storage, seed=4)
+    k = 0.05
+    wa, wb = O.reaction_step(fl, sl, storage, a, b, f, 1.1, k)
+    ad, bd, fd = to_dev(a, storage, gpu), to_dev(b, storage, gpu), to_dev(f, storage, gpu)
+    na, nb = torch.empty_like(ad), torch.empty_like(bd)
+    lbcube.reaction_step(fl, sl, ad, na, bd, nb, fd, 1.1, k)
+    torch.cuda.synchronize()
+    ga, gb = to_host(na, storage), to_host(nb, storage)
+    for got, want in ((ga, wa), (gb, wb)):
+        gd_, wd_ = O.decode(sl, storage, got), O.decode(sl, storage, want)
+        err = np.max(np.abs(gd_ - wd_) / np.maximum(np.abs(wd_), 1e-3))
+        assert err <= REACT_TOL[storage], err
+        # nearly everything is bit-identical; the exceptions are exp() ulps
+        assert np.mean(got == want) > 0.5
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("coll", COLLS)
+def test_fused_reactive_equals_two_launches(gpu, coll, storage):
+    print(f"case fused D2Q9 {coll} {storage}", flush=True)
+    shape = SHAPE["D2Q9"]
+    f = to_dev(cases.random_perturbed("D2Q9", shape, storage, seed=2), storage, gpu)
+    a0, b0 = _species("D2Q5", shape, storage, seed=8)
+    a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+    omega, omega_c, k = 1.3, 1.05, 0.02
+    # two launches, reference order
+    f1, a1, b1 = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    lbcube.fluid_step("D2Q9", coll, f, f1, omega)
+    lbcube.reaction_step("D2Q9", "D2Q5", a, a1, b, b1, f1, omega_c, k)
+    # one pass
+    f2, a2, b2 = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, f2, a, a2, b, b2, omega, omega_c, k)
+    torch.cuda.synchronize()
+    for x, y in ((f1, f2), (a1, a2), (b1, b2)):
+        assert torch.equal(x.view(torch.uint8), y.view(torch.uint8))
+
+
+def test_reactive_run_against_oracle(gpu):
+    """16 steps of the 2D reactive loop (fluid BGK + A+B->C), FP64."""
+    print("case reactive run D2Q9/D2Q5 f64", flush=True)
+    n = 32
+    f0, u = cases.taylor_green_2d(n, 0.02)
+    a0, b0 = cases.split_species(n, n, u)
+    omega, omega_c, k = 1.0 / 0.8, 1.0 / 0.8, 0.05
+    wf, wa, wb = O.reactive_run("D2Q9", "BGK", "D2Q5", "f64", f0, a0, b0, omega, omega_c, k, 16)
+    f, a, b = (to_dev(x, "f64", gpu) for x in (f0, a0, b0))
+    ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    for _ in range(16):
+        lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k)
+        f, ft, a, at, b, bt = ft, f, at, a, bt, b
+    torch.cuda.synchronize()
+    assert np.array_equal(f.cpu().numpy(), wf)  # the fluid never sees exp()
+    for got, want in ((a, wa), (b, wb)):
+        g = got.cpu().numpy()
+        assert np.max(np.abs(g - want)) <= 1e-13
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_fluid_diagnostics(gpu, lattice, storage):
+    f = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=13)
+    want = O.fluid_diagnostics(lattice, storage, f, wide=True)
+    got = lbcube.fluid_diagnostics(lattice, to_dev(f, storage, gpu))
+    tol = 1e-13 if storage == "f64" else 1e-6
+    assert abs(got[0] - want[0]) <= tol * abs(want[0])
+    assert abs(got[1] - want[1]) <= tol * abs(want[1]) + 1e-300
+    assert got[2] == want[2]  # max of correctly rounded sqrt: exact
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("sl", ["D2Q5", "D3Q7"])
+def test_reactive_stats(gpu, sl, storage):
+    shape = SHAPE["D2Q9"] if sl == "D2Q5" else SHAPE["D3Q19"]
+    a, b = _species(sl, shape, storage, seed=17)
+    want = O.reactive_stats(sl, storage, a, b, 0.01, wide=True)
+    got = lbcube.reactive_stats(sl, to_dev(a, storage, gpu), to_dev(b, storage, gpu), 0.01)
+    got = np.array([got[k] for k in lbcube.STAT_NAMES])
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-14)
+
+
+def test_stats_closed_forms(gpu):
+    """Complementary half planes: var = 1/4, cov = -1/4, segregation = -1 (rest fluid)."""
+    n = 64
+    u = np.zeros((2, n, n))
+    a, b = cases.split_species(n, n, u)
+    s = lbcube.reactive_stats("D2Q5", to_dev(a, "f64", gpu), to_dev(b, "f64", gpu), 0.5)
+    assert abs(s["mean_A"] - 0.5) < 1e-15 and abs(s["mean_B"] - 0.5) < 1e-15
+    assert abs(s["var_A"] - 0.25) < 1e-15 and abs(s["covariance"] + 0.25) < 1e-15
+    assert abs(s["segregation_intensity"] + 1.0) < 1e-14
+    assert abs(s["true_reaction_rate"]) < 1e-15 and abs(s["mixed_reaction_rate"] - 0.125) < 1e-15
