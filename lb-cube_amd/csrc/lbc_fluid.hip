@@ -37,14 +37,14 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
         const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
                              ax.pull(L::cx[i]);
         const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-        f[i] = St::load(plane_ptr(cur, i, plane) + src, off);
+        f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
     }
     collide_cell<L, CT, R>(f, omega, omega_minus);
     const uint32_t dst = (static_cast<uint32_t>(z) * ny + y) * nx + x;
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-        St::store(plane_ptr(nxt, i, plane) + dst, f[i], off);
+        St::store(at(plane_ptr(nxt, i, plane), dst), f[i], off);
     }
 }
 

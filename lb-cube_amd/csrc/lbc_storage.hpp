@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "lbc_lattice.hpp"
 
@@ -64,6 +65,15 @@ template <> struct Storage<kBF16> {
 // every address stays a 32-bit cell offset.
 template <class T> __device__ __forceinline__ T* plane_ptr(T* base, int i, size_t plane) {
     return base + static_cast<size_t>(i) * plane;
+}
+
+// Element `cell` of a plane, addressed as uniform base + zero-extended 32-bit
+// byte offset: the form the global-load/store "saddr" addressing mode takes
+// directly (no per-lane 64-bit address arithmetic).  Requires the plane to be
+// smaller than 4 GiB, which the launchers check.
+template <class T> __device__ __forceinline__ T* at(T* plane_base, uint32_t cell) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(plane_base) + cell * static_cast<uint32_t>(sizeof(T)));
 }
 
 template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
