@@ -27,20 +27,20 @@ BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16")
                   ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108}
 
 
-def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None):
+def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False):
     from lbcube import init as lbinit
 
     a = lbinit.equilibrium(lattice, None, u, storage)
     b = torch.empty_like(a)
     for _ in range(warmup):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block)
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast)
         a, b = b, a
     stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     for _ in range(steps):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block)
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast)
         a, b = b, a
     end.record(stream)
     torch.cuda.synchronize()
@@ -53,6 +53,24 @@ def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup,
     torch.cuda.empty_cache()
     return {"ms_per_step": ms, "mlups": mlups, "achieved_gbs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
             "bytes_per_cell": BYTES_PER_CELL[(lattice, storage)], "final_mass": diag[0], "final_ke": diag[1]}
+
+
+def copy_ceiling(lbcube, torch, nbytes, reps=20):
+    """Measured HBM ceiling: streaming copy of the same bytes one D3Q19 FP32 step moves."""
+    src = torch.empty(nbytes // 2 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    lbcube.stream_copy(src, dst)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        lbcube.stream_copy(src, dst)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(u_host, omega, n):
@@ -87,7 +105,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--variants", default="trt_f32,trt_f16,bgk_f32,bgk_f16,mrt_f32,mrt_f16,trt_bf16")
+    ap.add_argument("--variants", default="trt_f32,trt_f16fast,trt_f32fast,trt_f16,bgk_f32,bgk_f16fast,mrt_f32,"
+                                          "mrt_f16,trt_bf16fast")
+    ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -105,14 +125,18 @@ def main():
     omega = 1.0 / tau
     u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
 
+    block = tuple(int(v) for v in args.block.split(",")) if args.block else None
     variants = {}
     for name in args.variants.split(","):
         coll, storage = name.split("_")
+        fast = storage.endswith("fast")
+        storage = storage[:-4] if fast else storage
         variants[name] = time_variant(lbcube, torch, "D3Q19", coll.upper(), storage, u, omega, args.steps,
-                                      args.warmup)
+                                      args.warmup, block=block, fast=fast)
         print(f"# {name}: {variants[name]['mlups']:.0f} MLUPS, {variants[name]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
     head = variants["trt_f32"]
+    copy_gbs = copy_ceiling(lbcube, torch, 152 * n ** 3)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(u.cpu().numpy(), omega, n)
@@ -134,13 +158,14 @@ def main():
                    "cells": n ** 3, "lattice": "D3Q19", "collision": "TRT", "storage": "f32"},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": None,
-                     "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "bytes_per_cell": 152},
+                     "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "bytes_per_cell": 152,
+                     "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
-    if "trt_f16" in variants:
-        out["fp16_over_fp32"] = round(variants["trt_f16"]["mlups"] / head["mlups"], 3)
+    if "trt_f16fast" in variants:
+        out["fp16_over_fp32"] = round(variants["trt_f16fast"]["mlups"] / head["mlups"], 3)
     print(json.dumps(out))
 
 

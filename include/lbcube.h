@@ -55,6 +55,22 @@ int lbc_fluid_step(int lattice, int collision, int storage, const void* current,
                    uint64_t nx, uint64_t ny, uint64_t nz, double omega,
                    uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
 
+/* Flags of lbc_fluid_step_ex. */
+enum lbc_flags {
+    /* BGK and TRT only (ignored by MRT and RLBM): deviation-form arithmetic
+     * (f_i - w_i carried through the collision) with fused multiply-adds and a
+     * hardware reciprocal of the density.  About half the VALU work of the
+     * reference-order form; results differ from it by rounding only (bounded
+     * by the tests).  Intended for f16/bf16 storage. */
+    LBC_FLAG_FAST = 1u
+};
+
+/* lbc_fluid_step with flags (0 = lbc_fluid_step exactly).  Unknown flag bits
+ * -> hipErrorInvalidValue.  No counterpart in the reference. */
+int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
+                      uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
+                      uint32_t block_z, void* stream);
+
 /* Passive-scalar advection-diffusion step with a uniform source.
  * Replaces launch_scalar_step_gpu<F, S, Real> (include/lattice_cuda.cuh:209-222,
  * src/lattice_cuda.cu:539-589).  Pairs: D2Q9/D2Q5, D3Q19/D3Q7, D3Q27/D3Q7. */
@@ -94,6 +110,11 @@ int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uin
  * Accumulated in double on the device; synchronizes `stream`. */
 int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b, uint64_t nx,
                        uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream);
+
+/* Streaming device copy of `bytes` (a multiple of 16) from src to dst: the
+ * measured HBM ceiling the benchmark reports next to the spec peak.  No
+ * counterpart in the reference. */
+int lbc_stream_copy(const void* src, void* dst, uint64_t bytes, void* stream);
 
 /* Block shape the library picks for a domain (the C++ launcher mirror uses it
  * as its default argument; the reference's default is {16, 8, 1},

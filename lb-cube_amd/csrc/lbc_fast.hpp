@@ -1,0 +1,115 @@
+// Deviation-form BGK and TRT for reduced-precision storage (the opt-in
+// LBC_FLAG_FAST arithmetic of lbc_fluid_step_ex).
+//
+// Populations are carried as deviations d_i = f_i - w_i from the rest state,
+// the quantity the f16/bf16 formats store.  With sum w_i = 1 and
+// sum w_i c_i = 0:
+//   rho = 1 + sum d_i,   j = sum c_i d_i,   u = j / rho,
+//   feq_i - w_i = w_i (drho + rho (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)),
+// split for each pair (i, opp(i)) into an even part w_i A and an odd part
+// w_i B with A = drho + rho (4.5 (c.u)^2 - 1.5 u.u), B = 3 rho c.u.  Written
+// with explicit fused multiply-adds and a hardware reciprocal of rho, this is
+// roughly half the VALU work of the reference-order form and keeps the
+// rounding relative to the deviations rather than to f_i ~ w_i.  Results
+// differ from the reference order by rounding; the tests bound the difference.
+#pragma once
+
+#include "lbc_physics.hpp"
+
+namespace lbc {
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class L, class R> struct DevMacro {
+    R drho, rho, ux, uy, uz, m15uu;
+};
+
+template <class L, class R> __device__ __forceinline__ DevMacro<L, R> dev_macro(const R* d) {
+    // pairwise sums over the opposite pairs keep the adds short and balanced
+    R s = d[0], jx = R(0), jy = R(0), jz = R(0);
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        if (L::opp[i] < i) continue;
+        const int j = L::opp[i];
+        s += d[i] + d[j];
+        const R diff = d[i] - d[j];
+        if (L::cx[i] > 0) jx += diff;
+        if (L::cx[i] < 0) jx -= diff;
+        if (L::cy[i] > 0) jy += diff;
+        if (L::cy[i] < 0) jy -= diff;
+        if (L::cz[i] > 0) jz += diff;
+        if (L::cz[i] < 0) jz -= diff;
+    }
+    DevMacro<L, R> m;
+    m.drho = s;
+    m.rho = R(1) + s;
+    const R inv = fast_rcp(m.rho);
+    m.ux = jx * inv;
+    m.uy = jy * inv;
+    m.uz = L::D == 3 ? jz * inv : R(0);
+    const R uu = L::D == 3 ? fmadd(m.uz, m.uz, fmadd(m.uy, m.uy, m.ux * m.ux)) : fmadd(m.uy, m.uy, m.ux * m.ux);
+    m.m15uu = R(-1.5) * uu;
+    return m;
+}
+
+// c_i . u with the signs folded (same helper as the reference-order form).
+template <class L, class R> __device__ __forceinline__ R dev_cu(int i, const DevMacro<L, R>& m) {
+    return dir_dot<L, R>(i, m.ux, m.uy, m.uz);
+}
+
+// BGK on deviations: d_i <- d_i + omega (deq_i - d_i).
+template <class L, class R> __device__ __forceinline__ void collide_bgk_dev(R* d, R omega) {
+    const DevMacro<L, R> m = dev_macro<L, R>(d);
+    const R r3 = R(3) * m.rho;
+    {
+        const R eq0 = R(L::w[0]) * fmadd(m.rho, m.m15uu, m.drho);
+        d[0] = fmadd(omega, eq0 - d[0], d[0]);
+    }
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        if (L::opp[i] < i) continue;
+        const int j = L::opp[i];
+        const R w = R(L::w[i]);
+        const R cu = dev_cu<L, R>(i, m);
+        const R t = fmadd(R(4.5) * cu, cu, m.m15uu);
+        const R a = fmadd(m.rho, t, m.drho);
+        const R wb = (w * r3) * cu;
+        const R eqi = fmadd(w, a, wb);
+        const R eqj = fmadd(w, a, -wb);
+        d[i] = fmadd(omega, eqi - d[i], d[i]);
+        d[j] = fmadd(omega, eqj - d[j], d[j]);
+    }
+}
+
+// TRT on deviations, magic parameter through omega_minus (computed by the
+// launcher exactly as compute_omega_minus, lattice_physics.hpp:336-344).
+template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d, R wp, R wm) {
+    const DevMacro<L, R> m = dev_macro<L, R>(d);
+    const R r3 = R(3) * m.rho;
+    {
+        // the rest population is its own opposite: only the even part relaxes
+        const R eq0 = R(L::w[0]) * fmadd(m.rho, m.m15uu, m.drho);
+        d[0] = fmadd(wp, eq0 - d[0], d[0]);
+    }
+#pragma unroll
+    for (int i = 1; i < L::Q; ++i) {
+        if (L::opp[i] < i) continue;
+        const int j = L::opp[i];
+        const R w = R(L::w[i]);
+        const R cu = dev_cu<L, R>(i, m);
+        const R t = fmadd(R(4.5) * cu, cu, m.m15uu);
+        const R wa = w * fmadd(m.rho, t, m.drho);
+        const R wb = (w * r3) * cu;
+        const R pe = fmadd(R(0.5), d[i] + d[j], -wa);
+        const R po = fmadd(R(0.5), d[i] - d[j], -wb);
+        const R ce = wp * pe;
+        const R co = wm * po;
+        d[i] = d[i] - ce - co;
+        d[j] = d[j] - ce + co;
+    }
+}
+
+}  // namespace lbc

@@ -11,13 +11,16 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/lbcube.h"
+#include "lbc_fast.hpp"
 #include "lbc_launch.hpp"
 #include "lbc_physics.hpp"
 #include "lbc_storage.hpp"
 
 namespace lbc {
 
-template <class L, int CT, int ST, int MAXT>
+// FAST selects the deviation-form BGK/TRT arithmetic (lbc_fast.hpp) behind
+// LBC_FLAG_FAST; every other combination evaluates the reference's order.
+template <class L, int CT, int ST, bool FAST, int MAXT>
 __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                           typename Storage<ST>::S* __restrict__ nxt, int nx, int ny,
                                                           int nz, typename Storage<ST>::R omega,
@@ -31,79 +34,106 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
                                              static_cast<uint32_t>(nz));
     const Axis ax(x, nx), ay(y, ny), az(z, nz);
-    R f[L::Q];
-#pragma unroll
-    for (int i = 0; i < L::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
-                             ax.pull(L::cx[i]);
-        const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-        f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
-    }
-    collide_cell<L, CT, R>(f, omega, omega_minus);
     const uint32_t dst = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    R f[L::Q];
+    if constexpr (FAST) {
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) {
-        const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-        St::store(at(plane_ptr(nxt, i, plane), dst), f[i], off);
+        for (int i = 0; i < L::Q; ++i) {
+            const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
+                                 ax.pull(L::cx[i]);
+            f[i] = St::load_dev(at(plane_ptr(cur, i, plane), src), R(L::w[i]));
+        }
+        if constexpr (CT == kBGK) collide_bgk_dev<L, R>(f, omega);
+        else collide_trt_dev<L, R>(f, omega, omega_minus);
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) St::store_dev(at(plane_ptr(nxt, i, plane), dst), f[i], R(L::w[i]));
+    } else {
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
+                                 ax.pull(L::cx[i]);
+            const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
+            f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
+        }
+        collide_cell<L, CT, R>(f, omega, omega_minus);
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
+            St::store(at(plane_ptr(nxt, i, plane), dst), f[i], off);
+        }
     }
 }
 
-template <class L, int CT, int ST>
+template <class L, int CT, int ST, bool FAST>
 hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double omega, hipStream_t stream) {
     using St = Storage<ST>;
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
     if (s.threads() <= 256)
-        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, 256>), s.grid, s.block, 0, stream,
+        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 256>), s.grid, s.block, 0, stream,
                            static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
                            s.nz, w, wm);
     else
-        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, 1024>), s.grid, s.block, 0, stream,
+        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 1024>), s.grid, s.block, 0, stream,
                            static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
                            s.nz, w, wm);
     return hipGetLastError();
 }
 
-template <class L, int CT>
+template <class L, int CT, bool FAST>
 hipError_t dispatch_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, double omega,
                             hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_fluid<L, CT, kF32>(cur, nxt, s, omega, st);
-        case kF64: return launch_fluid<L, CT, kF64>(cur, nxt, s, omega, st);
-        case kF16: return launch_fluid<L, CT, kF16>(cur, nxt, s, omega, st);
-        case kBF16: return launch_fluid<L, CT, kBF16>(cur, nxt, s, omega, st);
+        case kF32: return launch_fluid<L, CT, kF32, FAST>(cur, nxt, s, omega, st);
+        case kF64: return launch_fluid<L, CT, kF64, FAST>(cur, nxt, s, omega, st);
+        case kF16: return launch_fluid<L, CT, kF16, FAST>(cur, nxt, s, omega, st);
+        case kBF16: return launch_fluid<L, CT, kBF16, FAST>(cur, nxt, s, omega, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class L>
-hipError_t dispatch_collision(int collision, int storage, const void* cur, void* nxt, const LaunchShape& s,
-                              double omega, hipStream_t st) {
+hipError_t dispatch_collision(int collision, int storage, bool fast, const void* cur, void* nxt,
+                              const LaunchShape& s, double omega, hipStream_t st) {
     switch (collision) {
-        case kBGK: return dispatch_storage<L, kBGK>(storage, cur, nxt, s, omega, st);
-        case kTRT: return dispatch_storage<L, kTRT>(storage, cur, nxt, s, omega, st);
-        case kMRT: return dispatch_storage<L, kMRT>(storage, cur, nxt, s, omega, st);
-        case kRLBM: return dispatch_storage<L, kRLBM>(storage, cur, nxt, s, omega, st);
+        case kBGK:
+            return fast ? dispatch_storage<L, kBGK, true>(storage, cur, nxt, s, omega, st)
+                        : dispatch_storage<L, kBGK, false>(storage, cur, nxt, s, omega, st);
+        case kTRT:
+            return fast ? dispatch_storage<L, kTRT, true>(storage, cur, nxt, s, omega, st)
+                        : dispatch_storage<L, kTRT, false>(storage, cur, nxt, s, omega, st);
+        case kMRT: return dispatch_storage<L, kMRT, false>(storage, cur, nxt, s, omega, st);
+        case kRLBM: return dispatch_storage<L, kRLBM, false>(storage, cur, nxt, s, omega, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace lbc
 
-extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const void* current, void* next, uint64_t nx,
-                              uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
-                              uint32_t block_z, void* stream) {
+extern "C" int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, const void* current,
+                                 void* next, uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x,
+                                 uint32_t block_y, uint32_t block_z, void* stream) {
     using namespace lbc;
     if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
     const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (current == next) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    const bool fast = (flags & LBC_FLAG_FAST) != 0;
+    const hipStream_t st = as_stream(stream);
     switch (lattice) {
-        case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, current, next, s, omega, as_stream(stream));
-        case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, current, next, s, omega, as_stream(stream));
-        case kD3Q27: return dispatch_collision<D3Q27>(collision, storage, current, next, s, omega, as_stream(stream));
+        case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, fast, current, next, s, omega, st);
+        case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, fast, current, next, s, omega, st);
+        case kD3Q27: return dispatch_collision<D3Q27>(collision, storage, fast, current, next, s, omega, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const void* current, void* next, uint64_t nx,
+                              uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
+                              uint32_t block_z, void* stream) {
+    return lbc_fluid_step_ex(lattice, collision, storage, 0u, current, next, nx, ny, nz, omega, block_x, block_y,
+                             block_z, stream);
 }

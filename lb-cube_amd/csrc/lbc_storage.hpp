@@ -16,11 +16,19 @@ namespace lbc {
 // arithmetic in float.
 template <int ST> struct Storage;
 
+// Every population store is written once and read back only by the next
+// launch, so it is issued as a streaming (non-temporal) store: it does not
+// displace the lines the same launch still reads.
+template <class T> __device__ __forceinline__ void put(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 template <> struct Storage<kF32> {
     using S = float;
     using R = float;
     __device__ static R load(const S* p, R) { return *p; }
-    __device__ static void store(S* p, R v, R) { *p = v; }
+    __device__ static void store(S* p, R v, R) { put(p, v); }
+    // deviation form: d = f - w on load, f = w + d on store
+    __device__ static R load_dev(const S* p, R w) { return *p - w; }
+    __device__ static void store_dev(S* p, R d, R w) { put(p, static_cast<S>(w + d)); }
     __device__ static R roundtrip(R v, R) { return v; }
 };
 
@@ -28,7 +36,9 @@ template <> struct Storage<kF64> {
     using S = double;
     using R = double;
     __device__ static R load(const S* p, R) { return *p; }
-    __device__ static void store(S* p, R v, R) { *p = v; }
+    __device__ static void store(S* p, R v, R) { put(p, v); }
+    __device__ static R load_dev(const S* p, R w) { return *p - w; }
+    __device__ static void store_dev(S* p, R d, R w) { put(p, static_cast<S>(w + d)); }
     __device__ static R roundtrip(R v, R) { return v; }
 };
 
@@ -37,10 +47,13 @@ template <> struct Storage<kF16> {
     using R = float;
     __device__ static R load(const S* p, R off) { return off + static_cast<float>(__builtin_bit_cast(_Float16, *p)); }
     __device__ static void store(S* p, R v, R off) {
-        *p = __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - off));
+        put(p, __builtin_bit_cast(uint16_t, static_cast<_Float16>(v - off)));
     }
     // The value a later load of a stored v returns.
     __device__ static R roundtrip(R v, R off) { return off + static_cast<float>(static_cast<_Float16>(v - off)); }
+    // fluid deviations are stored as they are
+    __device__ static R load_dev(const S* p, R) { return static_cast<float>(__builtin_bit_cast(_Float16, *p)); }
+    __device__ static void store_dev(S* p, R d, R) { put(p, __builtin_bit_cast(uint16_t, static_cast<_Float16>(d))); }
 };
 
 template <> struct Storage<kBF16> {
@@ -55,10 +68,12 @@ template <> struct Storage<kBF16> {
         x += 0x7fffu + ((x >> 16) & 1u);
         return static_cast<uint16_t>(x >> 16);
     }
-    __device__ static void store(S* p, R v, R off) { *p = encode(v - off); }
+    __device__ static void store(S* p, R v, R off) { put(p, encode(v - off)); }
     __device__ static R roundtrip(R v, R off) {
         return off + __builtin_bit_cast(float, static_cast<uint32_t>(encode(v - off)) << 16);
     }
+    __device__ static R load_dev(const S* p, R) { return __builtin_bit_cast(float, static_cast<uint32_t>(*p) << 16); }
+    __device__ static void store_dev(S* p, R d, R) { put(p, encode(d)); }
 };
 
 // Base of population plane i: a wave-uniform pointer, so the per-lane part of

@@ -77,17 +77,23 @@ def _block(lattice, block, nx, ny, nz):
     return default_block(lattice, nx, ny, nz) if block is None else tuple(block)
 
 
-def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None):
-    """Fused pull-stream + collide: nxt <- step(cur). Replaces launch_step_gpu."""
+FLAG_FAST = 1
+
+
+def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False):
+    """Fused pull-stream + collide: nxt <- step(cur). Replaces launch_step_gpu.
+
+    fast=True selects the deviation-form FMA arithmetic for BGK/TRT
+    (LBC_FLAG_FAST); the default evaluates the reference's operation order."""
     _check_pop(lattice, cur, "cur")
     _check_pop(lattice, nxt, "nxt")
     if cur.shape != nxt.shape or cur.dtype != nxt.dtype:
         raise ValueError("cur and nxt must have the same shape and dtype")
     nx, ny, nz = _extents(lattice, cur)
     bx, by, bz = _block(lattice, block, nx, ny, nz)
-    check(lib().lbc_fluid_step(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype], cur.data_ptr(),
-                               nxt.data_ptr(), nx, ny, nz, float(omega), bx, by, bz, _stream(stream)),
-          "lbc_fluid_step")
+    check(lib().lbc_fluid_step_ex(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
+                                  FLAG_FAST if fast else 0, cur.data_ptr(), nxt.data_ptr(), nx, ny, nz, float(omega),
+                                  bx, by, bz, _stream(stream)), "lbc_fluid_step_ex")
 
 
 def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None):
@@ -182,3 +188,13 @@ def decode(lattice, t):
         return dev.to(torch.float64)
     w = weights_tensor(lattice, torch.float32, t.device).view((-1,) + (1,) * (t.dim() - 1))
     return (w + dev).to(torch.float64)
+
+
+def stream_copy(src, dst, stream=None):
+    """Streaming copy of src into dst (same byte size, multiple of 16 bytes)."""
+    if not (src.is_cuda and dst.is_cuda and src.is_contiguous() and dst.is_contiguous()):
+        raise ValueError("src and dst must be contiguous CUDA tensors")
+    nbytes = src.numel() * src.element_size()
+    if nbytes != dst.numel() * dst.element_size():
+        raise ValueError("src and dst must have the same size in bytes")
+    check(lib().lbc_stream_copy(src.data_ptr(), dst.data_ptr(), nbytes, _stream(stream)), "lbc_stream_copy")

@@ -22,6 +22,8 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 SIGNATURES = {
     "lbc_fluid_step": ([c_int, c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_u32, c_u32, c_u32, c_vp],
                        c_int),
+    "lbc_fluid_step_ex": ([c_int, c_int, c_int, c_u32, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_u32, c_u32,
+                           c_u32, c_vp], c_int),
     "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_u32, c_u32,
                          c_u32, c_vp], c_int),
     "lbc_reaction_ab_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double,
@@ -30,6 +32,7 @@ SIGNATURES = {
                                  c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, _dp, c_vp], c_int),
+    "lbc_stream_copy": ([c_vp, c_vp, c_u64, c_vp], c_int),
     "lbc_default_block": ([c_int, c_u64, c_u64, c_u64, _u32p], None),
     "lbc_status_string": ([c_int], ctypes.c_char_p),
     "lbc_abi_version": ([], c_int),
