@@ -47,6 +47,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -944,6 +945,197 @@ template <class S, int DT> void scalar_feq_field(const double* conc, const doubl
     }
 }
 
+
+// ===========================================================================
+// Discrete adjoint of the 2D reactive loop (project-written; no counterpart in
+// the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK or TRT),
+// then (a_{n+1}, b_{n+1}) = A+B->C step of (a_n, b_n) advected by u(f_{n+1})
+// at the same cell (the order of src/simulate_reaction.cpp:548-554).
+// Adjoint buffers are kept "at the collision cell": nu[i][x] holds the
+// derivative with respect to the population i gathered by cell x, so the
+// adjoint of the stored state is Lambda[i][y] = nu[i][y + c_i] (reverse pull).
+// Pinned by central finite differences of the forward (tests).
+// ===========================================================================
+
+// Derivatives of the exact one-step A+B->C map.  With D = B - A (invariant)
+// and z = k D, A' = A / (1 + k A phi(z) + expm1(z)), phi(z) = expm1(z) / z; the
+// form has no cancellation at A = B.  Returns dS/dA and dS/dB of S = A' - A.
+template <class R> void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
+    const R d = b - a;
+    const R z = k * d;
+    R phi, dphi, em1;
+    if (std::fabs(z) < R(1e-3)) {
+        // series: phi = 1 + z/2 + z^2/6 + z^3/24, phi' = 1/2 + z/3 + z^2/8 + z^3/30
+        phi = R(1) + z * (R(0.5) + z * (R(1) / R(6) + z * (R(1) / R(24))));
+        dphi = R(0.5) + z * (R(1) / R(3) + z * (R(0.125) + z * (R(1) / R(30))));
+        em1 = z * phi;
+    } else {
+        em1 = std::expm1(z);
+        phi = em1 / z;
+        dphi = (std::exp(z) * (z - R(1)) + R(1)) / (z * z);
+    }
+    const R e = R(1) + em1;
+    const R den = R(1) + k * a * phi + em1;
+    const R dA_dA_fixedD = e / (den * den);                     // d A' / d A at fixed D
+    const R dA_dD = -a * k * (k * a * dphi + e) / (den * den);  // d A' / d D at fixed A
+    dsa = dA_dA_fixedD - dA_dD - R(1);
+    dsb = dA_dD;
+}
+
+template <class R, int CT>
+void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, const R* nua1, const R* nub1, R* nuf0,
+                           R* nua0, R* nub0, Grid g, R omega, R omega_c, R k, long double* djdw) {
+    using F = D2Q9;
+    using S = D2Q5;
+    const i64 N = g.cells();
+    const R wm = CT == C_TRT ? omega_minus<R>(omega) : R(0);
+    const R cs2 = static_cast<R>(S::cs2);
+    long double acc = 0;
+    for (i64 y = 0; y < g.ny; ++y) {
+        for (i64 x = 0; x < g.nx; ++x) {
+            const i64 cell = y * g.nx + x;
+            R fl[9], fp[9], lf[9];
+            for (int i = 0; i < 9; ++i) {
+                fl[i] = f[i * N + pull(y, F::c[2 * i + 1], g.ny) * g.nx + pull(x, F::c[2 * i], g.nx)];
+                // reverse pull: the adjoint of the population stored at this cell
+                lf[i] = nuf1[i * N + pull(y, -F::c[2 * i + 1], g.ny) * g.nx + pull(x, -F::c[2 * i], g.nx)];
+                fp[i] = fl[i];
+            }
+            collide_cell<F, CT, R>(fp, omega);
+            const Macro<F, R> mp = macro_state<F, R>(fp);  // what the scalar step sees
+            R ga[5], gb[5], la[5], lb[5];
+            for (int j = 0; j < 5; ++j) {
+                const i64 src = j * N + pull(y, S::c[2 * j + 1], g.ny) * g.nx + pull(x, S::c[2 * j], g.nx);
+                const i64 rsrc = j * N + pull(y, -S::c[2 * j + 1], g.ny) * g.nx + pull(x, -S::c[2 * j], g.nx);
+                ga[j] = a[src];
+                gb[j] = b[src];
+                la[j] = nua1[rsrc];
+                lb[j] = nub1[rsrc];
+            }
+            const R ca = concentration<S, R>(ga), cb = concentration<S, R>(gb);
+            R dsa, dsb;
+            reaction_grad<R>(ca, cb, k, dsa, dsb);
+            R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0;
+            for (int j = 0; j < 5; ++j) {
+                const R w = static_cast<R>(S::w[j]);
+                const R cx = static_cast<R>(S::c[2 * j]), cy = static_cast<R>(S::c[2 * j + 1]);
+                const R e = R(1) + (cx * mp.u[0] + cy * mp.u[1]) / cs2;
+                ea += la[j] * w * e;
+                eb += lb[j] * w * e;
+                wa += la[j] * w;
+                wb += lb[j] * w;
+                pax += la[j] * w * cx;
+                pay += la[j] * w * cy;
+                pbx += lb[j] * w * cx;
+                pby += lb[j] * w * cy;
+                acc += static_cast<long double>(la[j]) * (scalar_feq<S, R>(j, ca, mp.u) - ga[j]) +
+                       static_cast<long double>(lb[j]) * (scalar_feq<S, R>(j, cb, mp.u) - gb[j]);
+            }
+            const R sw = wa + wb;
+            for (int j = 0; j < 5; ++j) {
+                nua0[j * N + cell] = (R(1) - omega_c) * la[j] + omega_c * ea + sw * dsa;
+                nub0[j * N + cell] = (R(1) - omega_c) * lb[j] + omega_c * eb + sw * dsb;
+            }
+            const R lux = omega_c / cs2 * (ca * pax + cb * pbx);
+            const R luy = omega_c / cs2 * (ca * pay + cb * pby);
+            // velocity coupling: d u' / d fp_i = (c_i - u') / rho'
+            for (int i = 0; i < 9; ++i) {
+                const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
+                lf[i] += (lux * (cx - mp.u[0]) + luy * (cy - mp.u[1])) / mp.rho;
+            }
+            // transposed collision Jacobian at the pre-collision state
+            const Macro<F, R> m = macro_state<F, R>(fl);
+            const R uu = m.u[0] * m.u[0] + m.u[1] * m.u[1];
+            auto moments = [&](const R* lam, R& al, R& bx, R& by) {
+                al = bx = by = 0;
+                for (int q = 0; q < 9; ++q) {
+                    const R w = static_cast<R>(F::w[q]);
+                    const R cx = static_cast<R>(F::c[2 * q]), cy = static_cast<R>(F::c[2 * q + 1]);
+                    const R cu = cx * m.u[0] + cy * m.u[1];
+                    const R P = R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu;
+                    al += lam[q] * w * P;
+                    bx += lam[q] * w * (R(3) * cx + R(9) * cu * cx - R(3) * m.u[0]);
+                    by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * m.u[1]);
+                }
+            };
+            if constexpr (CT == C_BGK) {
+                R al, bx, by;
+                moments(lf, al, bx, by);
+                for (int i = 0; i < 9; ++i) {
+                    const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
+                    nuf0[i * N + cell] =
+                        (R(1) - omega) * lf[i] + omega * (al + bx * (cx - m.u[0]) + by * (cy - m.u[1]));
+                }
+            } else {
+                R lp[9], lm[9];
+                for (int i = 0; i < 9; ++i) {
+                    lp[i] = R(0.5) * (lf[i] + lf[F::opp[i]]);
+                    lm[i] = R(0.5) * (lf[i] - lf[F::opp[i]]);
+                }
+                R ap, bxp, byp, am, bxm, bym;
+                moments(lp, ap, bxp, byp);
+                moments(lm, am, bxm, bym);
+                for (int i = 0; i < 9; ++i) {
+                    const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
+                    const R dx = cx - m.u[0], dy = cy - m.u[1];
+                    nuf0[i * N + cell] = lf[i] - omega * lp[i] - wm * lm[i] +
+                                         omega * (ap + bxp * dx + byp * dy) + wm * (am + bxm * dx + bym * dy);
+                }
+            }
+        }
+    }
+    *djdw += acc;
+}
+
+template <class R> R mean_conc(const R* g, i64 N) {
+    long double s = 0;
+    for (i64 j = 0; j < 5 * N; ++j) s += g[j];
+    return static_cast<R>(s / N);
+}
+
+// J = <C>(T) = <A>(0) - <A>(T).  Runs the forward (all states kept), then the
+// adjoint; returns J, dJ/d(f0, a0, b0) (as adjoints of the stored initial
+// populations) and dJ/d(omega_c).
+template <class R, int CT>
+void reactive_gradient(const R* f0, const R* a0, const R* b0, Grid g, R omega, R omega_c, R k, int steps, R* J,
+                       R* gf, R* ga, R* gb, R* djdwc) {
+    const i64 N = g.cells();
+    const size_t nf = 9 * N, ns = 5 * N;
+    std::vector<R> F((steps + 1) * nf), A((steps + 1) * ns), B((steps + 1) * ns);
+    std::copy(f0, f0 + nf, F.begin());
+    std::copy(a0, a0 + ns, A.begin());
+    std::copy(b0, b0 + ns, B.begin());
+    constexpr int DT = std::is_same<R, double>::value ? T_F64 : T_F32;
+    for (int n = 0; n < steps; ++n) {
+        fluid_step<D2Q9, CT, DT>(&F[n * nf], &F[(n + 1) * nf], g, omega, 1);
+        scalar_like_step<D2Q9, D2Q5, DT>(&A[n * ns], &A[(n + 1) * ns], &B[n * ns], &B[(n + 1) * ns],
+                                         &F[(n + 1) * nf], g, omega_c, k, true, 1);
+    }
+    *J = mean_conc<R>(&A[0], N) - mean_conc<R>(&A[steps * ns], N);
+    std::vector<R> nf1(nf, R(0)), na1(ns, R(-1) / static_cast<R>(N)), nb1(ns, R(0));
+    std::vector<R> nf0(nf), na0(ns), nb0(ns);
+    long double dj = 0;
+    for (int n = steps - 1; n >= 0; --n) {
+        reactive_adjoint_step<R, CT>(&F[n * nf], &A[n * ns], &B[n * ns], nf1.data(), na1.data(), nb1.data(),
+                                     nf0.data(), na0.data(), nb0.data(), g, omega, omega_c, k, &dj);
+        nf1.swap(nf0);
+        na1.swap(na0);
+        nb1.swap(nb0);
+    }
+    for (i64 y = 0; y < g.ny; ++y)
+        for (i64 x = 0; x < g.nx; ++x) {
+            for (int i = 0; i < 9; ++i)
+                gf[i * N + y * g.nx + x] =
+                    nf1[i * N + pull(y, -D2Q9::c[2 * i + 1], g.ny) * g.nx + pull(x, -D2Q9::c[2 * i], g.nx)];
+            for (int j = 0; j < 5; ++j) {
+                const i64 r = j * N + pull(y, -D2Q5::c[2 * j + 1], g.ny) * g.nx + pull(x, -D2Q5::c[2 * j], g.nx);
+                ga[j * N + y * g.nx + x] = na1[r] + R(1) / static_cast<R>(N);
+                gb[j * N + y * g.nx + x] = nb1[r];
+            }
+        }
+    *djdwc = static_cast<R>(dj);
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -1154,6 +1346,46 @@ int orc_f16_roundtrip(const float* in, uint16_t* h, float* back, long long n, in
         h[j] = bf ? f32_to_bf16(in[j]) : f32_to_f16(in[j]);
         back[j] = bf ? bf16_to_f32(h[j]) : f16_to_f32(h[j]);
     }
+    return 0;
+}
+
+// Reactive loop gradient (D2Q9 BGK/TRT + two D2Q5 species); dt 0 = float, 1 = double.
+int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const void* b0, long long nx, long long ny,
+                          double omega, double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb,
+                          double* djdwc) {
+    const Grid g{nx, ny, 1};
+    if (dt == T_F64) {
+        double j, w;
+        if (ct == C_BGK)
+            reactive_gradient<double, C_BGK>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
+                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
+        else if (ct == C_TRT)
+            reactive_gradient<double, C_TRT>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
+                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
+        else
+            return -1;
+        *J = j;
+        *djdwc = w;
+    } else if (dt == T_F32) {
+        float j, w;
+        if (ct == C_BGK)
+            reactive_gradient<float, C_BGK>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
+                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
+        else if (ct == C_TRT)
+            reactive_gradient<float, C_TRT>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
+                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
+        else
+            return -1;
+        *J = j;
+        *djdwc = w;
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_reaction_grad(const double* a, const double* b, double k, double* dsa, double* dsb, long long n) {
+    for (long long j = 0; j < n; ++j) reaction_grad<double>(a[j], b[j], k, dsa[j], dsb[j]);
     return 0;
 }
 

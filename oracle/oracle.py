@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
             "orc_feq_field": [i, i, vp, vp, vp, ll],
             "orc_scalar_feq_field": [i, i, vp, vp, vp, ll],
             "orc_f16_roundtrip": [vp, vp, vp, ll, i],
+            "orc_reactive_gradient": [i, i, vp, vp, vp, ll, ll, d, d, d, i, dp, vp, vp, vp, dp],
+            "orc_reaction_grad": [vp, vp, d, vp, vp, ll],
             "orc_max_threads": [],
         }
         for name, args in sig.items():
@@ -248,3 +250,26 @@ WEIGHTS = {
     "D2Q5": [1 / 3] + [1 / 6] * 4,
     "D3Q7": [1 / 4] + [1 / 8] * 6,
 }
+
+
+def reactive_gradient(collision, dtype, f0, a0, b0, omega, omega_c, k, steps):
+    """J = <A>(0) - <A>(T) of the 2D reactive loop and its discrete adjoint.
+
+    Returns (J, dJ/df0, dJ/da0, dJ/db0, dJ/domega_c).  Project-written CPU adjoint
+    (test infrastructure): the reference has no adjoint."""
+    f0, a0, b0 = (np.ascontiguousarray(x) for x in (f0, a0, b0))
+    gf, ga, gb = np.empty_like(f0), np.empty_like(a0), np.empty_like(b0)
+    J, dw = ctypes.c_double(), ctypes.c_double()
+    ny, nx = f0.shape[1:]
+    _check(lib().orc_reactive_gradient(COLLISIONS[collision], DTYPES[dtype], _p(f0), _p(a0), _p(b0), nx, ny,
+                                       float(omega), float(omega_c), float(k), int(steps), ctypes.byref(J), _p(gf),
+                                       _p(ga), _p(gb), ctypes.byref(dw)), "reactive_gradient")
+    return J.value, gf, ga, gb, dw.value
+
+
+def reaction_grad(a, b, k):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    da, db = np.empty_like(a), np.empty_like(b)
+    lib().orc_reaction_grad(_p(a), _p(b), float(k), _p(da), _p(db), a.size)
+    return da, db

@@ -1,0 +1,63 @@
+"""The oracle's FP64 discrete adjoint of the 2D reactive loop (D2Q9 BGK/TRT +
+two D2Q5 species, A + B -> C), pinned by central finite differences of the
+oracle's own forward loop.  The reference has no adjoint; this CPU adjoint is
+the checker for the HIP adjoint kernel."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as O
+
+
+def small_state(nx=10, ny=8, seed=3):
+    g = cases.rng(seed)
+    u = g.uniform(-0.03, 0.03, size=(2, ny, nx))
+    rho = 1.0 + g.uniform(-0.01, 0.01, size=(ny, nx))
+    f = O.feq_field("D2Q9", "f64", rho, u)
+    ca = 0.5 * (1 + np.tanh((np.arange(nx) - nx / 2) / 1.5)) * np.ones((ny, 1))
+    a = O.scalar_feq_field("D2Q5", "f64", ca, u)
+    b = O.scalar_feq_field("D2Q5", "f64", 1.0 - ca + 0.02 * g.uniform(size=(ny, nx)), u)
+    return f, a, b
+
+
+def J_of(coll, f, a, b, omega, omega_c, k, steps):
+    _, wa, _ = O.reactive_run("D2Q9", coll, "D2Q5", "f64", f, a, b, omega, omega_c, k, steps)
+    ny, nx = f.shape[1:]
+    return a.sum() / (nx * ny) - wa.sum() / (nx * ny)
+
+
+def test_reaction_map_derivatives_against_fd():
+    g = cases.rng(11)
+    a = np.concatenate([g.uniform(0.01, 1.2, 50), [0.4, 0.7]])
+    b = np.concatenate([g.uniform(0.01, 1.2, 50), [0.4 + 1e-7, 0.7 - 3e-4]])
+    k = 0.3
+    da, db = O.reaction_grad(a, b, k)
+    # the reference's closed form cancels near A = B, so the differences use a
+    # step far above |A - B| (truncation error ~1e-10 here)
+    h = 1e-4
+    sa = (O.reaction_source("f64", a + h, b, k) - O.reaction_source("f64", a - h, b, k)) / (2 * h)
+    sb = (O.reaction_source("f64", a, b + h, k) - O.reaction_source("f64", a, b - h, k)) / (2 * h)
+    assert np.max(np.abs(da - sa)) < 1e-8 and np.max(np.abs(db - sb)) < 1e-8
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+def test_adjoint_matches_central_differences(coll):
+    f, a, b = small_state()
+    omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 12
+    J, gf, ga, gb, dw = O.reactive_gradient(coll, "f64", f, a, b, omega, omega_c, k, steps)
+    assert abs(J - J_of(coll, f, a, b, omega, omega_c, k, steps)) < 1e-15
+    g = cases.rng(5)
+    for trial in range(4):
+        vf, va, vb = (g.standard_normal(x.shape) * 1e-3 for x in (f, a, b))
+        h = 1e-4
+        jp = J_of(coll, f + h * vf, a + h * va, b + h * vb, omega, omega_c, k, steps)
+        jm = J_of(coll, f - h * vf, a - h * va, b - h * vb, omega, omega_c, k, steps)
+        fd = (jp - jm) / (2 * h)
+        ad = np.sum(gf * vf) + np.sum(ga * va) + np.sum(gb * vb)
+        assert abs(fd - ad) <= 1e-7 * max(abs(fd), 1e-12), (trial, fd, ad)
+    hw = 1e-6
+    fd = (J_of(coll, f, a, b, omega, omega_c + hw, k, steps) - J_of(coll, f, a, b, omega, omega_c - hw, k, steps)) / (
+        2 * hw)
+    assert abs(fd - dw) <= 1e-6 * abs(fd), (fd, dw)
