@@ -73,6 +73,106 @@ def copy_ceiling(lbcube, torch, nbytes, reps=20):
     return nbytes / (ms * 1e-3) / 1e9
 
 
+def _time(torch, fn, reps=1):
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def adjoint_ratio(torch, n=512, steps=4096, interval=64):
+    """Differentiated 2D reactive run (D2Q9 BGK + two D2Q5, A+B->C), FP32:
+    time of the forward loop vs time of the full gradient (forward with
+    checkpoints, segment recompute, adjoint sweep), per step."""
+    from lbcube import adjoint as ADJ
+
+    du = ADJ.gaussian_random_field((n, n), 32.0, 0.01, seed=0x5EED1234)
+    f0, a0, b0 = ADJ.initial_state(du, dtype=torch.float32)
+    omega, omega_c, k = 1 / 0.6, 1 / 0.8, 0.05
+    ADJ.reactive_run("BGK", f0, a0, b0, 8, omega, omega_c, k)
+    t_fwd = _time(torch, lambda: ADJ.reactive_run("BGK", f0, a0, b0, steps, omega, omega_c, k))
+    import lbcube
+    block = lbcube.default_block("D2Q9", n, n)
+    ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block), dtype=torch.uint8,
+                     device="cuda")
+    ADJ.reactive_gradient("BGK", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws)
+    out = {}
+    t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("BGK", f0, a0, b0, steps, omega, omega_c, k,
+                                                                     interval, workspace=ws)))
+    loss = out["r"][0]
+    return {"workload": f"D2Q9 BGK + 2xD2Q5 A+B->C, {n}^2, {steps} steps, FP32, checkpoint every {interval}",
+            "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
+            "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
+            "loss": loss}
+
+
+def ensemble_rate(torch, steps=1000):
+    """512 independent 2D reactive instances (D2Q9 TRT + 2xD2Q5), edges
+    log-uniform in 64..1024, FP16 storage, autotuned tiling, hipGraph replay."""
+    from lbcube import init as lbinit
+    from lbcube.ensemble import Ensemble, reactive_mixing_ensemble
+
+    n, omega, omega_c, k = reactive_mixing_ensemble()
+    ens = Ensemble("TRT", "f16", [(int(x), int(x)) for x in n], omega, omega_c, k)
+    for i, ni in enumerate(n):
+        rho, u = lbinit.taylor_green_2d(int(ni), int(ni), 0.04)
+        ca, cb = lbinit.tanh_half_planes(int(ni), delta=max(2.0, ni / 32))
+        ens.fluid(i).copy_(lbinit.equilibrium("D2Q9", rho, u, "f16"))
+        a, b = ens.species(i)
+        a.copy_(lbinit.scalar_equilibrium("D2Q5", ca, u, "f16"))
+        b.copy_(lbinit.scalar_equilibrium("D2Q5", cb, u, "f16"))
+    chosen, timings = ens.autotune(trial_steps=3)
+    ens.run(2, graph=True)
+    ms = _time(torch, lambda: ens.run(steps, graph=True))
+    cells = ens.cells
+    ens.close()
+    gbs = 76 * cells * steps / (ms * 1e-3) / 1e9
+    return {"workload": "512 instances D2Q9 TRT + 2xD2Q5 A+B->C, N log-uniform 64..1024, FP16 storage, "
+                        f"{steps} steps, one launch per step, hipGraph replay",
+            "cells": cells, "mlups": cells * steps / (ms * 1e-3) / 1e6, "achieved_gbs": gbs,
+            "bytes_per_cell": 76, "tiling": {"tile_cap": chosen[0], "threads": chosen[1]},
+            "autotune_ms_per_step": {f"{c}x{t}": round(v, 4) for (c, t), v in timings.items()}}
+
+
+def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
+    """2048^2 D2Q9 MRT double shear layer + two D2Q5 species, fused reactive
+    step, FP32 and FP64, with reactive statistics every 50 steps."""
+    from lbcube import init as lbinit
+
+    out = {}
+    u = lbinit.double_shear_layer(n, u0=0.04, delta=80.0)
+    ca, cb = lbinit.tanh_half_planes(n, delta=80.0)
+    k = 10.0 * 0.04 / 80.0
+    for storage, bpc in (("f32", 152), ("f64", 304)):
+        f = lbinit.equilibrium("D2Q9", None, u, storage)
+        a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage)
+        b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage)
+        ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+        state = [f, ft, a, at, b, bt]
+
+        def step(state=state):
+            f, ft, a, at, b, bt = state
+            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k)
+            state[:] = [ft, f, at, a, bt, b]
+
+        for _ in range(4):
+            step()
+        ms = _time(torch, step, reps=steps)
+        stats = lbcube.reactive_stats("D2Q5", state[2], state[4], k)
+        ms_stats = _time(torch, lambda: lbcube.reactive_stats("D2Q5", state[2], state[4], k), reps=5)
+        out[storage] = {"mlups": n * n / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * n * n / (ms * 1e-3) / 1e9,
+                        "bytes_per_cell": bpc, "stats_ms": ms_stats, "segregation_intensity":
+                        stats["segregation_intensity"]}
+        del f, ft, a, at, b, bt, state
+        torch.cuda.empty_cache()
+    out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C fused, {n}^2 double shear layer, Da=10"
+    return out
+
+
 def cpu_baseline(u_host, omega, n):
     """The oracle (CPU restatement of step_cpu) timed on this host: one step of
     the same 256^3 D3Q19 TRT FP32 workload on all host threads, and one on a
@@ -108,6 +208,7 @@ def main():
     ap.add_argument("--variants", default="trt_f32,trt_f16fast,trt_f32fast,trt_f16,bgk_f32,bgk_f16fast,mrt_f32,"
                                           "mrt_f16,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -137,6 +238,18 @@ def main():
               file=sys.stderr, flush=True)
     head = variants["trt_f32"]
     copy_gbs = copy_ceiling(lbcube, torch, 152 * n ** 3)
+    extras = {}
+    if not args.no_extras:
+        del u
+        torch.cuda.empty_cache()
+        extras["adjoint"] = adjoint_ratio(torch)
+        print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
+        extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)
+        print(f"# reactive 2048^2 MRT f32: {extras['reactive_2d']['f32']['mlups']:.0f} MLUPS", file=sys.stderr,
+              flush=True)
+        extras["ensemble"] = ensemble_rate(torch)
+        print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(u.cpu().numpy(), omega, n)
@@ -166,6 +279,9 @@ def main():
     }
     if "trt_f16fast" in variants:
         out["fp16_over_fp32"] = round(variants["trt_f16fast"]["mlups"] / head["mlups"], 3)
+    if extras:
+        out["adjoint_over_forward"] = round(extras["adjoint"]["adjoint_over_forward"], 3)
+        out["configs"] = extras
     print(json.dumps(out))
 
 

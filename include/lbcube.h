@@ -17,8 +17,8 @@
  *     (src/main.cpp:587, src/simulate_reaction.cpp:527,542-543).
  * Checks the reference does not make, all reported as hipErrorInvalidValue:
  * an unsupported lattice/collision/storage code, current == next (the
- * ping-pong buffers must not alias), and a population plane of 2^32 or more
- * cells.  Return values are hipError_t codes as int (0 = hipSuccess).
+ * ping-pong buffers must not alias), and a population plane of more than 2^29
+ * cells (in-plane byte offsets are 32-bit).  Return values are hipError_t codes as int (0 = hipSuccess).
  *
  * Streams are passed as void* (a hipStream_t) so this header needs no HIP
  * headers.  Thread safety: every entry point is stateless and re-entrant,
@@ -97,6 +97,53 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
                             const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
                             double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
                             uint32_t block_z, void* stream);
+
+/* Gradient of the 2D reactive loop (D2Q9 BGK or TRT + two D2Q5 species,
+ * A + B -> C; `steps` calls of lbc_reactive_fused_step from f0, a0, b0) by the
+ * discrete adjoint.  loss = <C>(T) = <A>(0) - <A>(T) (domain means).  Outputs:
+ * *loss, the adjoints dloss/d(f0, a0, b0) of the initial populations (same
+ * layout and type as the inputs) and dloss/d(omega_c).  States are
+ * checkpointed every `interval` steps and each segment is recomputed before
+ * it is walked backward.  storage: LBC_F32 or LBC_F64.  The caller passes a
+ * device workspace of at least lbc_reactive_gradient_workspace() bytes;
+ * nothing is allocated.  Synchronizes `stream`.  No counterpart in the
+ * reference. */
+int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
+                                    uint32_t block_x, uint32_t block_y, uint64_t* bytes);
+int lbc_reactive_gradient(int collision, int storage, const void* f0, const void* a0, const void* b0, uint64_t nx,
+                          uint64_t ny, int steps, int interval, double omega, double omega_c, double k_react,
+                          double* loss, void* grad_f0, void* grad_a0, void* grad_b0, double* dloss_domega_c,
+                          void* workspace, uint64_t workspace_bytes, uint32_t block_x, uint32_t block_y,
+                          void* stream);
+
+/* Ensembles of independent 2D reactive instances (D2Q9 BGK/TRT + two D2Q5
+ * species, A + B -> C), each with its own extents, omega, omega_c and k, in
+ * three packed buffers per time level (fluid, A, B; instance i at the element
+ * offsets reported by lbc_ensemble_layout, each a [Q][ny_i][nx_i] block) and
+ * advanced by one launch per step.  Each instance advances bit-identically to
+ * lbc_reactive_fused_step on it alone.  create allocates the instance table
+ * and a capture stream; step/run/autotune allocate nothing.  No counterpart
+ * in the reference. */
+typedef struct lbc_ensemble lbc_ensemble;
+int lbc_ensemble_create(int collision, int storage, int count, const uint32_t* nx, const uint32_t* ny,
+                        const double* omega, const double* omega_c, const double* k_react, lbc_ensemble** out);
+int lbc_ensemble_layout(const lbc_ensemble* ensemble, uint64_t* fluid_elems, uint64_t* species_elems,
+                        uint64_t* fluid_offsets, uint64_t* species_offsets);
+/* tile_cap in {64, 128, 256, 512} (work-items along x per tile), threads in {256, 512}. */
+int lbc_ensemble_set_tiling(lbc_ensemble* ensemble, uint32_t tile_cap, uint32_t threads);
+int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* next_fluid, const void* current_a,
+                      void* next_a, const void* current_b, void* next_b, void* stream);
+/* `steps` steps between buffer sets 0 and 1 (result in set steps % 2); with
+ * use_graph, even chunks of up to 64 steps are captured once into a hipGraph
+ * and replayed. */
+int lbc_ensemble_run(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
+                     int steps, int use_graph, void* stream);
+/* Times the 8 tiling candidates over 2*trial_steps steps each (advancing the
+ * buffers), keeps the fastest; chosen[2] = {tile_cap, threads}; ms[8] (optional)
+ * = ms per step of each candidate (-1 when not applicable). */
+int lbc_ensemble_autotune(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
+                          int trial_steps, uint32_t* chosen, float* ms, void* stream);
+int lbc_ensemble_destroy(lbc_ensemble* ensemble);
 
 /* Fluid integrals of log_diagnostics (include/lattice_io.hpp:182-234):
  * out[0] = sum rho, out[1] = sum rho |u|^2 / 2, out[2] = max |u|.

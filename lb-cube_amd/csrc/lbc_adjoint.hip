@@ -1,0 +1,416 @@
+// Discrete adjoint of the 2D reactive loop: D2Q9 BGK or TRT fluid plus two
+// D2Q5 species reacting A + B -> C (the loop of src/simulate_reaction.cpp:516-546,
+// run by lbc_reactive_fused_step).  No counterpart in the reference.
+//
+// Forward step n: f_{n+1} = collide(pull f_n); (a, b)_{n+1} = reaction step of
+// pulled (a, b)_n advected by u(f_{n+1}) at the same cell.  The adjoint keeps
+// its state "at the collision cell": nu[i][x] is the derivative with respect
+// to the population i that cell x gathered, so the adjoint of a stored
+// population is Lambda[i][y] = nu[i][y + c_i], a pull along -c_i (reverse
+// streaming).  One adjoint step per cell:
+//   1. reverse-pull nu_f, nu_a, nu_b of step n+1 (the adjoints of the stored
+//      f_{n+1}, a_{n+1}, b_{n+1} at this cell);
+//   2. pull f_n and re-collide it (same code as the forward) to get the state
+//      and velocity u' the forward reaction step used;
+//   3. reaction step transpose: scalar BGK + source Jacobians, the reaction
+//      Jacobian of the exact one-step map, and the velocity coupling into the
+//      fluid adjoint through du'/df' = (c - u') / rho';
+//   4. fluid collision transpose at the pre-collision state (BGK, or TRT with
+//      its even/odd split), written to nu_f of step n.
+// Long runs use checkpoints every `interval` steps and recompute each
+// segment forward before walking it backward (lbc_reactive_gradient).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/lbcube.h"
+#include "lbc_launch.hpp"
+#include "lbc_physics.hpp"
+#include "lbc_storage.hpp"
+
+namespace lbc {
+
+__device__ __forceinline__ float expm1_dev(float z) { return expm1f(z); }
+__device__ __forceinline__ double expm1_dev(double z) { return expm1(z); }
+
+// dS/dA, dS/dB of S = A' - A for the exact one-step map of dA/dt = dB/dt =
+// -k A B.  With D = B - A and z = k D: A' = A / (1 + k A phi(z) + expm1(z)),
+// phi(z) = expm1(z) / z, which has no cancellation at A = B.
+template <class R> __device__ __forceinline__ void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
+    const R d = b - a;
+    const R z = k * d;
+    R phi, dphi, em1;
+    if (fabs(z) < R(1e-3)) {
+        phi = R(1) + z * (R(0.5) + z * (R(1) / R(6) + z * (R(1) / R(24))));
+        dphi = R(0.5) + z * (R(1) / R(3) + z * (R(0.125) + z * (R(1) / R(30))));
+        em1 = z * phi;
+    } else {
+        em1 = expm1_dev(z);
+        phi = em1 / z;
+        dphi = ((R(1) + em1) * (z - R(1)) + R(1)) / (z * z);
+    }
+    const R e = R(1) + em1;
+    const R den = R(1) + k * a * phi + em1;
+    const R inv2 = R(1) / (den * den);
+    const R d_fixed = e * inv2;
+    const R d_dd = -a * k * (k * a * dphi + e) * inv2;
+    dsa = d_fixed - d_dd - R(1);
+    dsb = d_dd;
+}
+
+// alpha = sum lam_q w_q P_q, beta = sum lam_q w_q grad_u P_q: the pieces of the
+// transposed equilibrium Jacobian (d feq_q / d f_i = w_q [P_q + G_q . (c_i - u)]).
+template <class R>
+__device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R& al, R& bx, R& by) {
+    const R uu = ux * ux + uy * uy;
+    al = bx = by = R(0);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const R w = R(D2Q9::w[q]);
+        const R cx = R(D2Q9::cx[q]), cy = R(D2Q9::cy[q]);
+        const R cu = cx * ux + cy * uy;
+        const R P = R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu;
+        al += lam[q] * w * P;
+        bx += lam[q] * w * (R(3) * cx + R(9) * cu * cx - R(3) * ux);
+        by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * uy);
+    }
+}
+
+template <class R, int CT, int MAXT>
+__global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
+    const R* __restrict__ f, const R* __restrict__ a, const R* __restrict__ b, const R* __restrict__ nuf1,
+    const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
+    R* __restrict__ nub0, int nx, int ny, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const bool live = x < nx && y < ny;
+    double acc = 0.0;
+    if (live) {
+        const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny));
+        const Axis ax(x, nx), ay(y, ny);
+        const uint32_t cell = static_cast<uint32_t>(y) * nx + x;
+        R fl[9], fp[9], lf[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const uint32_t src = static_cast<uint32_t>(ay.pull(D2Q9::cy[i])) * nx + ax.pull(D2Q9::cx[i]);
+            const uint32_t rev = static_cast<uint32_t>(ay.pull(-D2Q9::cy[i])) * nx + ax.pull(-D2Q9::cx[i]);
+            fl[i] = plane_ptr(f, i, plane)[src];
+            lf[i] = plane_ptr(nuf1, i, plane)[rev];
+            fp[i] = fl[i];
+        }
+        collide_cell<D2Q9, CT, R>(fp, omega, omega_minus);
+        const Macro<R> mp = macro_state<D2Q9, R>(fp);
+        R ga[5], gb[5], la[5], lb[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const uint32_t src = static_cast<uint32_t>(ay.pull(D2Q5::cy[j])) * nx + ax.pull(D2Q5::cx[j]);
+            const uint32_t rev = static_cast<uint32_t>(ay.pull(-D2Q5::cy[j])) * nx + ax.pull(-D2Q5::cx[j]);
+            ga[j] = plane_ptr(a, j, plane)[src];
+            gb[j] = plane_ptr(b, j, plane)[src];
+            la[j] = plane_ptr(nua1, j, plane)[rev];
+            lb[j] = plane_ptr(nub1, j, plane)[rev];
+        }
+        const R ca = concentration<D2Q5, R>(ga), cb = concentration<D2Q5, R>(gb);
+        R dsa, dsb;
+        reaction_grad<R>(ca, cb, k, dsa, dsb);
+        const R inv_cs2 = R(3);
+        R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const R w = R(D2Q5::w[j]);
+            const R cx = R(D2Q5::cx[j]), cy = R(D2Q5::cy[j]);
+            const R e = R(1) + (cx * mp.ux + cy * mp.uy) * inv_cs2;
+            ea += la[j] * w * e;
+            eb += lb[j] * w * e;
+            wa += la[j] * w;
+            wb += lb[j] * w;
+            pax += la[j] * w * cx;
+            pay += la[j] * w * cy;
+            pbx += lb[j] * w * cx;
+            pby += lb[j] * w * cy;
+            acc += static_cast<double>(la[j] * (w * ca * e - ga[j]) + lb[j] * (w * cb * e - gb[j]));
+        }
+        const R sw = wa + wb;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            put(plane_ptr(nua0, j, plane) + cell, (R(1) - omega_c) * la[j] + omega_c * ea + sw * dsa);
+            put(plane_ptr(nub0, j, plane) + cell, (R(1) - omega_c) * lb[j] + omega_c * eb + sw * dsb);
+        }
+        const R lux = omega_c * inv_cs2 * (ca * pax + cb * pbx);
+        const R luy = omega_c * inv_cs2 * (ca * pay + cb * pby);
+        const R irho = R(1) / mp.rho;
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            lf[i] += (lux * (R(D2Q9::cx[i]) - mp.ux) + luy * (R(D2Q9::cy[i]) - mp.uy)) * irho;
+        const Macro<R> m = macro_state<D2Q9, R>(fl);
+        if constexpr (CT == kBGK) {
+            R al, bx, by;
+            feq_transpose<R>(lf, m.ux, m.uy, al, bx, by);
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                put(plane_ptr(nuf0, i, plane) + cell,
+                    (R(1) - omega) * lf[i] +
+                        omega * (al + bx * (R(D2Q9::cx[i]) - m.ux) + by * (R(D2Q9::cy[i]) - m.uy)));
+        } else {
+            R lp[9], lm[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                lp[i] = R(0.5) * (lf[i] + lf[D2Q9::opp[i]]);
+                lm[i] = R(0.5) * (lf[i] - lf[D2Q9::opp[i]]);
+            }
+            R ap, bxp, byp, am, bxm, bym;
+            feq_transpose<R>(lp, m.ux, m.uy, ap, bxp, byp);
+            feq_transpose<R>(lm, m.ux, m.uy, am, bxm, bym);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const R dx = R(D2Q9::cx[i]) - m.ux, dy = R(D2Q9::cy[i]) - m.uy;
+                put(plane_ptr(nuf0, i, plane) + cell, lf[i] - omega * lp[i] - omega_minus * lm[i] +
+                                                          omega * (ap + bxp * dx + byp * dy) +
+                                                          omega_minus * (am + bxm * dx + bym * dy));
+            }
+        }
+    }
+    // d J / d omega_c: per-block partial added to this block's own slot, so
+    // successive launches accumulate without atomics (deterministic).
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double wsum[MAXT / 64];
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((t & 63) == 0) wsum[t >> 6] = acc;
+    __syncthreads();
+    if (t == 0) {
+        double s = 0.0;
+        const int nw = (blockDim.x * blockDim.y + 63) / 64;
+        for (int w = 0; w < nw; ++w) s += wsum[w];
+        djdw[blockIdx.y * gridDim.x + blockIdx.x] += s;
+    }
+}
+
+// Lambda[i][y] = nu[i][y + c_i] (+ shift): the adjoint of stored populations.
+template <class R, class L>
+__global__ __launch_bounds__(256) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, int nx,
+                                                           int ny, R shift) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nx || y >= ny) return;
+    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny));
+    const Axis ax(x, nx), ay(y, ny);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const uint32_t rev = static_cast<uint32_t>(ay.pull(-L::cy[i])) * nx + ax.pull(-L::cx[i]);
+        lam[i * plane + static_cast<uint32_t>(y) * nx + x] = plane_ptr(nu, i, plane)[rev] + shift;
+    }
+}
+
+template <class R> __global__ __launch_bounds__(256) void fill_kernel(R* p, size_t n, R v) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256) p[i] = v;
+}
+
+template <class R> __global__ __launch_bounds__(256) void zero_double_kernel(double* p, size_t n) {
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256) p[i] = 0.0;
+}
+
+// Sum of n values into out[0] (double), one block, fixed order.
+template <class R> __global__ __launch_bounds__(256) void sum_kernel(const R* p, size_t n, double* out) {
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += 256) s += static_cast<double>(p[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+// Per-block partial sums of a field, for large fields (grid-stride).
+template <class R> __global__ __launch_bounds__(256) void partial_sum_kernel(const R* p, size_t n, double* part) {
+    double s = 0.0;
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+        s += static_cast<double>(p[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+struct GradLayout {
+    size_t cells, state_elems, es, nseg, seg_slots, blocks;
+    size_t off_ckpt, off_seg, off_nu, off_part, off_red, total;
+};
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, int interval, dim3 grid) {
+    GradLayout L{};
+    L.cells = nx * ny;
+    L.state_elems = 19 * L.cells;
+    L.es = es;
+    L.nseg = (static_cast<size_t>(steps) + interval - 1) / interval;
+    L.seg_slots = static_cast<size_t>(interval);
+    L.blocks = static_cast<size_t>(grid.x) * grid.y;
+    size_t off = 0;
+    L.off_ckpt = off;
+    off = align_up(off + L.nseg * L.state_elems * es);  // states at the segment starts
+    L.off_seg = off;
+    off = align_up(off + (L.seg_slots + 1) * L.state_elems * es);  // one recomputed segment
+    L.off_nu = off;
+    off = align_up(off + 2 * L.state_elems * es);  // adjoint ping-pong
+    L.off_part = off;
+    off = align_up(off + L.blocks * sizeof(double));
+    L.off_red = off;
+    off = align_up(off + 1024 * sizeof(double));
+    L.total = off;
+    return L;
+}
+
+template <class R> struct State {
+    R *f, *a, *b;
+    State(R* base, size_t cells) : f(base), a(base + 9 * cells), b(base + 14 * cells) {}
+};
+
+template <class R, int CT>
+hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape& s, int steps, int interval,
+                        double omega_d, double omega_c_d, double k_d, double* J_out, R* gf, R* ga, R* gb,
+                        double* djdwc_out, char* ws, hipStream_t st) {
+    const int storage = std::is_same<R, double>::value ? kF64 : kF32;
+    const GradLayout L = grad_layout(sizeof(R), s.nx, s.ny, steps, interval, s.grid);
+    const size_t cells = L.cells, se = L.state_elems;
+    R* ckpt = reinterpret_cast<R*>(ws + L.off_ckpt);
+    R* seg = reinterpret_cast<R*>(ws + L.off_seg);
+    R* nu0 = reinterpret_cast<R*>(ws + L.off_nu);
+    R* nu1 = nu0 + se;
+    double* part = reinterpret_cast<double*>(ws + L.off_part);
+    double* red = reinterpret_cast<double*>(ws + L.off_red);
+    const R omega = static_cast<R>(omega_d), omega_c = static_cast<R>(omega_c_d), k = static_cast<R>(k_d);
+    const R wm = CT == kTRT ? trt_omega_minus<R>(omega) : R(0);
+    hipError_t e;
+    auto step = [&](State<R> in, State<R> out) {
+        return static_cast<hipError_t>(lbc_reactive_fused_step(kD2Q9, CT, kD2Q5, storage, in.f, out.f, in.a, out.a,
+                                                               in.b, out.b, s.nx, s.ny, 1, omega_d, omega_c_d, k_d,
+                                                               s.block.x, s.block.y, 1, st));
+    };
+    // forward with checkpoints at the segment starts
+    if ((e = hipMemcpyAsync(ckpt, f0, 9 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ckpt + 9 * cells, a0, 5 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(ckpt + 14 * cells, b0, 5 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) !=
+        hipSuccess)
+        return e;
+    R* last = nullptr;
+    for (size_t sg = 0; sg < L.nseg; ++sg) {
+        const int len = static_cast<int>(std::min<size_t>(interval, steps - sg * interval));
+        State<R> cur(ckpt + sg * se, cells);
+        for (int t = 0; t < len; ++t) {
+            const bool to_ckpt = (t == len - 1) && (sg + 1 < L.nseg);
+            R* dst = to_ckpt ? ckpt + (sg + 1) * se : seg + ((t & 1) ? se : 0);
+            State<R> out(dst, cells);
+            if ((e = step(cur, out)) != hipSuccess) return e;
+            cur = out;
+            last = dst;
+        }
+    }
+    // J = <A>(0) - <A>(T)
+    hipLaunchKernelGGL((sum_kernel<R>), dim3(1), dim3(256), 0, st, a0, 5 * cells, red);
+    hipLaunchKernelGGL((sum_kernel<R>), dim3(1), dim3(256), 0, st, last + 9 * cells, 5 * cells, red + 1);
+    // adjoint initial condition: dJ/da_T = -1/N everywhere
+    const R inv_n = R(1) / static_cast<R>(cells);
+    const unsigned fb = static_cast<unsigned>(std::min<size_t>(4096, (se + 255) / 256));
+    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1, 9 * cells, R(0));
+    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1 + 9 * cells, 5 * cells, -inv_n);
+    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1 + 14 * cells, 5 * cells, R(0));
+    hipLaunchKernelGGL((zero_double_kernel<R>), dim3(4), dim3(256), 0, st, part, L.blocks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (size_t sgp = L.nseg; sgp-- > 0;) {
+        const int len = static_cast<int>(std::min<size_t>(interval, steps - sgp * interval));
+        // recompute the segment's states 0..len-1 into seg slots (slot 0 = checkpoint)
+        if ((e = hipMemcpyAsync(seg, ckpt + sgp * se, se * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+            return e;
+        for (int t = 0; t + 1 < len; ++t) {
+            if ((e = step(State<R>(seg + t * se, cells), State<R>(seg + (t + 1) * se, cells))) != hipSuccess)
+                return e;
+        }
+        for (int t = len - 1; t >= 0; --t) {
+            State<R> fw(seg + t * se, cells), in(nu1, cells), out(nu0, cells);
+            if (s.threads() <= 256)
+                hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a, fw.b,
+                                   in.f, in.a, in.b, out.f, out.a, out.b, s.nx, s.ny, omega, wm, omega_c, k, part);
+            else
+                hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 1024>), s.grid, s.block, 0, st, fw.f, fw.a,
+                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.nx, s.ny, omega, wm, omega_c, k,
+                                   part);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            std::swap(nu0, nu1);
+        }
+    }
+    // adjoints of the stored initial populations
+    State<R> fin(nu1, cells);
+    const dim3 rg(static_cast<unsigned>((s.nx + 255) / 256), static_cast<unsigned>(s.ny));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), rg, dim3(256), 0, st, fin.f, gf, s.nx, s.ny, R(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), rg, dim3(256), 0, st, fin.a, ga, s.nx, s.ny, inv_n);
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), rg, dim3(256), 0, st, fin.b, gb, s.nx, s.ny, R(0));
+    hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, part, L.blocks, red + 2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    double h[3];
+    if ((e = hipMemcpyAsync(h, red, 3 * sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    *J_out = (h[0] - h[1]) / static_cast<double>(cells);
+    *djdwc_out = h[2];
+    return hipSuccess;
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
+                                               uint32_t block_x, uint32_t block_y, uint64_t* bytes) {
+    using namespace lbc;
+    if (bytes == nullptr) return hipErrorInvalidValue;
+    if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
+    if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
+    LaunchShape s;
+    const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
+    if (e != hipSuccess) return e;
+    *bytes = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid).total;
+    return hipSuccess;
+}
+
+extern "C" int lbc_reactive_gradient(int collision, int storage, const void* f0, const void* a0, const void* b0,
+                                     uint64_t nx, uint64_t ny, int steps, int interval, double omega, double omega_c,
+                                     double k_react, double* loss, void* grad_f0, void* grad_a0, void* grad_b0,
+                                     double* dloss_domega_c, void* workspace, uint64_t workspace_bytes,
+                                     uint32_t block_x, uint32_t block_y, void* stream) {
+    using namespace lbc;
+    if (f0 == nullptr || a0 == nullptr || b0 == nullptr || grad_f0 == nullptr || grad_a0 == nullptr ||
+        grad_b0 == nullptr || workspace == nullptr)
+        return hipErrorInvalidDevicePointer;
+    if (loss == nullptr || dloss_domega_c == nullptr) return hipErrorInvalidValue;
+    LaunchShape s;
+    const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
+    if (e != hipSuccess) return e;
+    if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
+    if (collision != kBGK && collision != kTRT) return hipErrorInvalidValue;
+    if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
+    const GradLayout L = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid);
+    if (workspace_bytes < L.total) return hipErrorInvalidValue;
+    char* ws = static_cast<char*>(workspace);
+    const hipStream_t st = as_stream(stream);
+    if (storage == kF64) {
+        auto P = [](const void* p) { return static_cast<const double*>(p); };
+        auto Q = [](void* p) { return static_cast<double*>(p); };
+        return collision == kBGK
+                   ? run_gradient<double, kBGK>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react,
+                                                loss, Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st)
+                   : run_gradient<double, kTRT>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react,
+                                                loss, Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st);
+    }
+    auto P = [](const void* p) { return static_cast<const float*>(p); };
+    auto Q = [](void* p) { return static_cast<float*>(p); };
+    return collision == kBGK
+               ? run_gradient<float, kBGK>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react, loss,
+                                           Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st)
+               : run_gradient<float, kTRT>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react, loss,
+                                           Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st);
+}

@@ -11,6 +11,7 @@
 #include "../../include/lbcube.h"
 #include "lbc_launch.hpp"
 #include "lbc_physics.hpp"
+#include "lbc_reactive.hpp"
 #include "lbc_storage.hpp"
 
 namespace lbc {
@@ -37,7 +38,7 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     R fl[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i)
-        fl[i] = St::load(plane_ptr(cur_f, i, plane) + cell, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+        fl[i] = St::load(at(plane_ptr(cur_f, i, plane), cell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     const Macro<R> m = macro_state<F, R>(fl);
     const Axis ax(x, nx), ay(y, ny), az(z, nz);
     R ga[S::Q], gb[S::Q];
@@ -45,8 +46,8 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
                              ax.pull(S::cx[i]);
-        ga[i] = St::load(plane_ptr(cur_a, i, plane) + src, R(0));
-        if constexpr (REACT) gb[i] = St::load(plane_ptr(cur_b, i, plane) + src, R(0));
+        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
+        if constexpr (REACT) gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
     }
     if constexpr (REACT) {
         const R ca = concentration<S, R>(ga);
@@ -59,16 +60,11 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     }
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(plane_ptr(nxt_a, i, plane) + cell, ga[i], R(0));
-        if constexpr (REACT) St::store(plane_ptr(nxt_b, i, plane) + cell, gb[i], R(0));
+        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));
+        if constexpr (REACT) St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(0));
     }
 }
 
-// Fluid step + A + B -> C step in one pass.  The reaction step of the
-// reference reads the fluid populations that the fluid step has just written
-// at the same cell (src/simulate_reaction.cpp:527-541); here they are still in
-// registers.  With reduced-precision storage they are passed through the
-// storage rounding first, so the velocity is the one the unfused pair sees.
 template <class F, int CT, class S, int ST, int MAXT>
 __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_f, typename Storage<ST>::S* __restrict__ nxt_f,
@@ -76,49 +72,12 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, int nx, int ny,
     int nz, typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
     typename Storage<ST>::R k_react) {
-    using St = Storage<ST>;
-    using R = typename St::R;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     const int z = blockIdx.z * blockDim.z + threadIdx.z;
     if (x >= nx || y >= ny || z >= nz) return;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
-                                             static_cast<uint32_t>(nz));
-    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
-    const Axis ax(x, nx), ay(y, ny), az(z, nz);
-    R f[F::Q];
-#pragma unroll
-    for (int i = 0; i < F::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(F::cz[i])) * ny + ay.pull(F::cy[i])) * nx +
-                             ax.pull(F::cx[i]);
-        f[i] = St::load(plane_ptr(cur_f, i, plane) + src, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
-    }
-    collide_cell<F, CT, R>(f, omega, omega_minus);
-#pragma unroll
-    for (int i = 0; i < F::Q; ++i) {
-        const R off = kOffsetStorage<ST> ? R(F::w[i]) : R(0);
-        St::store(plane_ptr(nxt_f, i, plane) + cell, f[i], off);
-        if constexpr (kOffsetStorage<ST>) f[i] = St::roundtrip(f[i], off);
-    }
-    const Macro<R> m = macro_state<F, R>(f);
-    R ga[S::Q], gb[S::Q];
-#pragma unroll
-    for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
-                             ax.pull(S::cx[i]);
-        ga[i] = St::load(plane_ptr(cur_a, i, plane) + src, R(0));
-        gb[i] = St::load(plane_ptr(cur_b, i, plane) + src, R(0));
-    }
-    const R ca = concentration<S, R>(ga);
-    const R cb = concentration<S, R>(gb);
-    const R source = reaction_source<R>(ca, cb, k_react);
-    collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, source);
-    collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
-#pragma unroll
-    for (int i = 0; i < S::Q; ++i) {
-        St::store(plane_ptr(nxt_a, i, plane) + cell, ga[i], R(0));
-        St::store(plane_ptr(nxt_b, i, plane) + cell, gb[i], R(0));
-    }
+    reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, x, y, z, nx, ny, nz, omega, omega_minus,
+                                omega_c, k_react);
 }
 
 template <class F, class S, int ST, bool REACT>

@@ -1,0 +1,355 @@
+// Batched ensembles of independent 2D reactive-mixing instances (D2Q9 BGK or
+// TRT + two D2Q5 species, A + B -> C), each with its own extents and
+// parameters, advanced by ONE launch per step over a packed index space.
+// No counterpart in the reference (its drivers run one domain per process).
+//
+// Layout: three packed buffers per time level (fluid, species A, species B);
+// instance i owns a [Q][ny_i][nx_i] block of each at element offset off_i.
+// Work decomposition: instance i is cut into tiles of `tile_w` cells along x by
+// `threads / tile_w` rows; a block-start prefix table maps blockIdx.x to its
+// instance (binary search over a wave-uniform table), so small and large
+// instances share one grid with no idle padding beyond each tile's edge.
+// Every cell runs the same code as lbc_reactive_fused_step, so an instance
+// advances bit-identically to a standalone run.  The tile width per size class
+// is chosen by timing (lbc_ensemble_autotune), and long runs replay a
+// captured hipGraph of consecutive steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <new>
+#include <vector>
+
+#include "../../include/lbcube.h"
+#include "lbc_launch.hpp"
+#include "lbc_reactive.hpp"
+
+namespace lbc {
+
+struct EnsembleInstance {
+    uint64_t f_off, s_off;  // element offsets of this instance in the fluid / species buffers
+    uint32_t nx, ny;
+    uint32_t block_start, tiles_x;
+    uint32_t tile_w, rows;
+    float omega, omega_minus, omega_c, k;
+    double omega_d, omega_minus_d, omega_c_d, k_d;
+};
+
+template <int CT, int ST, int THREADS>
+__global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storage<ST>::S* __restrict__ cf,
+                                                           typename Storage<ST>::S* __restrict__ nf,
+                                                           const typename Storage<ST>::S* __restrict__ ca,
+                                                           typename Storage<ST>::S* __restrict__ na,
+                                                           const typename Storage<ST>::S* __restrict__ cb,
+                                                           typename Storage<ST>::S* __restrict__ nb,
+                                                           const EnsembleInstance* __restrict__ table, int count) {
+    using R = typename Storage<ST>::R;
+    const uint32_t blk = blockIdx.x;
+    int lo = 0, hi = count - 1;  // last instance with block_start <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (table[mid].block_start <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const EnsembleInstance& in = table[lo];
+    const uint32_t lb = blk - in.block_start;
+    const uint32_t ty = lb / in.tiles_x, tx = lb - ty * in.tiles_x;
+    const int x = static_cast<int>(tx * in.tile_w + threadIdx.x % in.tile_w);
+    const int y = static_cast<int>(ty * in.rows + threadIdx.x / in.tile_w);
+    if (x >= static_cast<int>(in.nx) || y >= static_cast<int>(in.ny)) return;
+    const R omega = std::is_same<R, double>::value ? R(in.omega_d) : R(in.omega);
+    const R wm = std::is_same<R, double>::value ? R(in.omega_minus_d) : R(in.omega_minus);
+    const R wc = std::is_same<R, double>::value ? R(in.omega_c_d) : R(in.omega_c);
+    const R k = std::is_same<R, double>::value ? R(in.k_d) : R(in.k);
+    reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
+                                      nb + in.s_off, x, y, 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1,
+                                      omega, wm, wc, k);
+}
+
+}  // namespace lbc
+
+struct lbc_ensemble {
+    int collision, storage, count;
+    std::vector<lbc::EnsembleInstance> host;
+    lbc::EnsembleInstance* table = nullptr;
+    uint64_t f_elems = 0, s_elems = 0, blocks = 0;
+    uint32_t cap = 256, threads = 256;
+    hipStream_t capture = nullptr;
+    hipGraphExec_t graph = nullptr;
+    const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int graph_steps = 0;
+};
+
+namespace lbc {
+
+static uint32_t pow2_at_least(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Tile policy: tile width = min(cap, threads, next power of two >= nx), at
+// least one wave.
+static hipError_t build_table(lbc_ensemble* e, uint32_t cap, uint32_t threads) {
+    uint64_t blocks = 0;
+    for (auto& in : e->host) {
+        uint32_t tw = std::min(std::min(cap, threads), std::max<uint32_t>(64, pow2_at_least(in.nx)));
+        in.tile_w = tw;
+        in.rows = threads / tw;
+        in.tiles_x = (in.nx + tw - 1) / tw;
+        in.block_start = static_cast<uint32_t>(blocks);
+        blocks += static_cast<uint64_t>(in.tiles_x) * ((in.ny + in.rows - 1) / in.rows);
+    }
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    e->blocks = blocks;
+    e->cap = cap;
+    e->threads = threads;
+    hipError_t err = hipMemcpy(e->table, e->host.data(), e->host.size() * sizeof(EnsembleInstance),
+                               hipMemcpyHostToDevice);
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    return err;
+}
+
+template <int CT, int ST>
+static void launch_ensemble(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na, const void* cb,
+                            void* nb, hipStream_t st) {
+    using T = typename Storage<ST>::S;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(e->blocks)), dim3(e->threads), 0, st,
+                           static_cast<const T*>(cf), static_cast<T*>(nf), static_cast<const T*>(ca),
+                           static_cast<T*>(na), static_cast<const T*>(cb), static_cast<T*>(nb), e->table, e->count);
+    };
+    if (e->threads == 256) go(ensemble_kernel<CT, ST, 256>);
+    else go(ensemble_kernel<CT, ST, 512>);
+}
+
+template <int CT>
+static hipError_t step_ct(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na, const void* cb,
+                          void* nb, hipStream_t st) {
+    switch (e->storage) {
+        case kF32: launch_ensemble<CT, kF32>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kF64: launch_ensemble<CT, kF64>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kF16: launch_ensemble<CT, kF16>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kBF16: launch_ensemble<CT, kBF16>(e, cf, nf, ca, na, cb, nb, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t ensemble_step(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na,
+                                const void* cb, void* nb, hipStream_t st) {
+    return e->collision == kBGK ? step_ct<kBGK>(e, cf, nf, ca, na, cb, nb, st)
+                                : step_ct<kTRT>(e, cf, nf, ca, na, cb, nb, st);
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_ensemble_create(int collision, int storage, int count, const uint32_t* nx, const uint32_t* ny,
+                                   const double* omega, const double* omega_c, const double* k_react,
+                                   lbc_ensemble** out) {
+    using namespace lbc;
+    if (out == nullptr || nx == nullptr || ny == nullptr || omega == nullptr || omega_c == nullptr ||
+        k_react == nullptr)
+        return hipErrorInvalidValue;
+    *out = nullptr;
+    if (count <= 0 || (collision != kBGK && collision != kTRT)) return hipErrorInvalidValue;
+    if (storage < kF32 || storage > kBF16) return hipErrorInvalidValue;
+    auto* e = new (std::nothrow) lbc_ensemble();
+    if (e == nullptr) return hipErrorOutOfMemory;
+    e->collision = collision;
+    e->storage = storage;
+    e->count = count;
+    e->host.resize(count);
+    uint64_t fo = 0, so = 0;
+    for (int i = 0; i < count; ++i) {
+        if (nx[i] == 0 || ny[i] == 0 || static_cast<uint64_t>(nx[i]) * ny[i] > 0x7fffffffull) {
+            delete e;
+            return hipErrorInvalidValue;
+        }
+        EnsembleInstance& in = e->host[i];
+        in.nx = nx[i];
+        in.ny = ny[i];
+        in.f_off = fo;
+        in.s_off = so;
+        const uint64_t cells = static_cast<uint64_t>(nx[i]) * ny[i];
+        fo += 9 * cells;
+        so += 5 * cells;
+        in.omega_d = omega[i];
+        in.omega_c_d = omega_c[i];
+        in.k_d = k_react[i];
+        in.omega_minus_d = collision == kTRT ? trt_omega_minus<double>(omega[i]) : 0.0;
+        in.omega = static_cast<float>(omega[i]);
+        in.omega_c = static_cast<float>(omega_c[i]);
+        in.k = static_cast<float>(k_react[i]);
+        in.omega_minus = collision == kTRT ? trt_omega_minus<float>(static_cast<float>(omega[i])) : 0.0f;
+    }
+    e->f_elems = fo;
+    e->s_elems = so;
+    hipError_t err = hipMalloc(reinterpret_cast<void**>(&e->table), count * sizeof(EnsembleInstance));
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->capture, hipStreamNonBlocking);
+    if (err == hipSuccess) err = build_table(e, 256, 256);
+    if (err != hipSuccess) {
+        if (e->table) (void)hipFree(e->table);
+        if (e->capture) (void)hipStreamDestroy(e->capture);
+        delete e;
+        return err;
+    }
+    *out = e;
+    return hipSuccess;
+}
+
+extern "C" int lbc_ensemble_layout(const lbc_ensemble* e, uint64_t* fluid_elems, uint64_t* species_elems,
+                                   uint64_t* fluid_offsets, uint64_t* species_offsets) {
+    if (e == nullptr) return hipErrorInvalidValue;
+    if (fluid_elems) *fluid_elems = e->f_elems;
+    if (species_elems) *species_elems = e->s_elems;
+    for (int i = 0; i < e->count; ++i) {
+        if (fluid_offsets) fluid_offsets[i] = e->host[i].f_off;
+        if (species_offsets) species_offsets[i] = e->host[i].s_off;
+    }
+    return hipSuccess;
+}
+
+extern "C" int lbc_ensemble_set_tiling(lbc_ensemble* e, uint32_t tile_cap, uint32_t threads) {
+    if (e == nullptr) return hipErrorInvalidValue;
+    if ((threads != 256 && threads != 512) || (tile_cap != 64 && tile_cap != 128 && tile_cap != 256 &&
+                                               tile_cap != 512))
+        return hipErrorInvalidConfiguration;
+    return lbc::build_table(e, tile_cap, threads);
+}
+
+extern "C" int lbc_ensemble_step(lbc_ensemble* e, const void* cur_f, void* next_f, const void* cur_a, void* next_a,
+                                 const void* cur_b, void* next_b, void* stream) {
+    if (e == nullptr) return hipErrorInvalidValue;
+    if (!cur_f || !next_f || !cur_a || !next_a || !cur_b || !next_b) return hipErrorInvalidDevicePointer;
+    if (cur_f == next_f || cur_a == next_a || cur_b == next_b) return hipErrorInvalidValue;
+    return lbc::ensemble_step(e, cur_f, next_f, cur_a, next_a, cur_b, next_b, lbc::as_stream(stream));
+}
+
+// Advances `steps` steps ping-ponging between buffer sets 0 and 1; the result
+// is in set (steps % 2).  With use_graph, chunks of up to 64 steps (an even
+// number) are captured once into a hipGraph on a private stream and replayed;
+// the caller's stream is ordered before and after through events.
+extern "C" int lbc_ensemble_run(lbc_ensemble* e, void* f0, void* f1, void* a0, void* a1, void* b0, void* b1,
+                                int steps, int use_graph, void* stream) {
+    using namespace lbc;
+    if (e == nullptr || steps < 0) return hipErrorInvalidValue;
+    if (!f0 || !f1 || !a0 || !a1 || !b0 || !b1) return hipErrorInvalidDevicePointer;
+    if (f0 == f1 || a0 == a1 || b0 == b1) return hipErrorInvalidValue;
+    const hipStream_t st = as_stream(stream);
+    void* F[2] = {f0, f1};
+    void* A[2] = {a0, a1};
+    void* B[2] = {b0, b1};
+    int done = 0;
+    hipError_t err = hipSuccess;
+    if (use_graph && steps >= 2) {
+        const int chunk = std::min(64, steps - steps % 2);
+        const void* key[6] = {f0, f1, a0, a1, b0, b1};
+        bool same = e->graph != nullptr && e->graph_steps == chunk;
+        for (int i = 0; i < 6 && same; ++i) same = e->graph_key[i] == key[i];
+        if (!same) {
+            if (e->graph) {
+                (void)hipGraphExecDestroy(e->graph);
+                e->graph = nullptr;
+            }
+            hipGraph_t g;
+            err = hipStreamBeginCapture(e->capture, hipStreamCaptureModeThreadLocal);
+            if (err != hipSuccess) return err;
+            for (int s = 0; s < chunk && err == hipSuccess; ++s)
+                err = ensemble_step(e, F[s & 1], F[(s + 1) & 1], A[s & 1], A[(s + 1) & 1], B[s & 1], B[(s + 1) & 1],
+                                    e->capture);
+            hipError_t end = hipStreamEndCapture(e->capture, &g);
+            if (err != hipSuccess) return err;
+            if (end != hipSuccess) return end;
+            err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (err != hipSuccess) return err;
+            e->graph_steps = chunk;
+            for (int i = 0; i < 6; ++i) e->graph_key[i] = key[i];
+        }
+        hipEvent_t before, after;
+        if ((err = hipEventCreateWithFlags(&before, hipEventDisableTiming)) != hipSuccess) return err;
+        if ((err = hipEventCreateWithFlags(&after, hipEventDisableTiming)) != hipSuccess) return err;
+        err = hipEventRecord(before, st);
+        if (err == hipSuccess) err = hipStreamWaitEvent(e->capture, before, 0);
+        while (err == hipSuccess && steps - done >= chunk) {
+            err = hipGraphLaunch(e->graph, e->capture);
+            done += chunk;
+        }
+        if (err == hipSuccess) err = hipEventRecord(after, e->capture);
+        if (err == hipSuccess) err = hipStreamWaitEvent(st, after, 0);
+        (void)hipEventDestroy(before);
+        (void)hipEventDestroy(after);
+        if (err != hipSuccess) return err;
+    }
+    for (; done < steps && err == hipSuccess; ++done)
+        err = ensemble_step(e, F[done & 1], F[(done + 1) & 1], A[done & 1], A[(done + 1) & 1], B[done & 1],
+                            B[(done + 1) & 1], st);
+    return err;
+}
+
+// Times every tiling candidate for `trial_steps` steps on the given buffers
+// (their contents are advanced) and keeps the fastest.  Candidates: tile
+// width caps {64, 128, 256, 512} x block sizes {256, 512}.  Writes the chosen
+// (cap, threads) and, if ms is non-null, the 8 measured times in ms per step.
+extern "C" int lbc_ensemble_autotune(lbc_ensemble* e, void* f0, void* f1, void* a0, void* a1, void* b0, void* b1,
+                                     int trial_steps, uint32_t* chosen, float* ms, void* stream) {
+    using namespace lbc;
+    if (e == nullptr || trial_steps <= 0) return hipErrorInvalidValue;
+    const hipStream_t st = as_stream(stream);
+    const uint32_t caps[4] = {64, 128, 256, 512};
+    const uint32_t thr[2] = {256, 512};
+    hipEvent_t t0, t1;
+    hipError_t err;
+    if ((err = hipEventCreate(&t0)) != hipSuccess) return err;
+    if ((err = hipEventCreate(&t1)) != hipSuccess) return err;
+    float best = 1e30f;
+    uint32_t bc = 256, bt = 256;
+    int idx = 0;
+    for (uint32_t t : thr) {
+        for (uint32_t c : caps) {
+            float m = -1.0f;
+            if (c <= t) {
+                if ((err = build_table(e, c, t)) != hipSuccess) break;
+                err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2, 0, stream));
+                if (err == hipSuccess) err = hipEventRecord(t0, st);
+                if (err == hipSuccess)
+                    err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2 * trial_steps, 0,
+                                                                   stream));
+                if (err == hipSuccess) err = hipEventRecord(t1, st);
+                if (err == hipSuccess) err = hipEventSynchronize(t1);
+                if (err == hipSuccess) err = hipEventElapsedTime(&m, t0, t1);
+                if (err != hipSuccess) break;
+                m /= 2.0f * trial_steps;
+                if (m < best) {
+                    best = m;
+                    bc = c;
+                    bt = t;
+                }
+            }
+            if (ms) ms[idx] = m;
+            ++idx;
+        }
+        if (err != hipSuccess) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (err != hipSuccess) return err;
+    if (chosen) {
+        chosen[0] = bc;
+        chosen[1] = bt;
+    }
+    return build_table(e, bc, bt);
+}
+
+extern "C" int lbc_ensemble_destroy(lbc_ensemble* e) {
+    if (e == nullptr) return hipSuccess;
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    if (e->capture) (void)hipStreamDestroy(e->capture);
+    if (e->table) (void)hipFree(e->table);
+    delete e;
+    return hipSuccess;
+}

@@ -32,9 +32,10 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
     }
     if (nx == 0 || ny == 0 || nz == 0) return hipErrorInvalidValue;
     if (static_cast<uint64_t>(bx) * by * bz > 1024) return hipErrorInvalidConfiguration;
-    // Cell offsets inside one population plane are 32-bit.
+    // Byte offsets inside one population plane are 32-bit (the global-load
+    // saddr form), so a plane holds at most 2^29 cells (4 GiB of doubles).
     if (nx > 0x7fffffffull || ny > 0x7fffffffull || nz > 0x7fffffffull) return hipErrorInvalidValue;
-    if (nx * ny * nz > 0xffffffffull) return hipErrorInvalidValue;
+    if (nx * ny * nz > (1ull << 29)) return hipErrorInvalidValue;
     const uint64_t gx = (nx + bx - 1) / bx, gy = (ny + by - 1) / by, gz = (nz + bz - 1) / bz;
     if (gx > 0x7fffffffull || gy > 65535 || gz > 65535) return hipErrorInvalidConfiguration;
     s.grid = dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy), static_cast<unsigned>(gz));

@@ -47,7 +47,7 @@ def test_validation_order_fluid():
     assert fluid(FAKE_A, FAKE_B, ny=0) == HIP_INVALID_VALUE
     assert fluid(FAKE_A, FAKE_B, block=(64, 32, 1)) == HIP_INVALID_CONFIGURATION  # > 1024 work-items
     assert fluid(FAKE_A, FAKE_A) == HIP_INVALID_VALUE  # ping-pong buffers must not alias
-    assert fluid(FAKE_A, FAKE_B, nx=1 << 20, ny=1 << 12, nz=1) == HIP_INVALID_VALUE  # plane >= 2^32 cells
+    assert fluid(FAKE_A, FAKE_B, nx=1 << 20, ny=1 << 10, nz=1) == HIP_INVALID_VALUE  # plane > 2^29 cells
 
 
 def test_two_d_lattice_ignores_z():

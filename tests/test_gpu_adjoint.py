@@ -1,0 +1,118 @@
+"""GPU: the discrete adjoint of the 2D reactive loop (lbc_reactive_gradient).
+
+  * against the oracle's FP64 CPU adjoint (tight tolerance, several checkpoint
+    intervals so the segment recompute is exercised);
+  * checkpoint-interval invariance: recomputed segments are bit-identical, so
+    every interval gives the same gradient bit for bit;
+  * torch.autograd.gradcheck of the autograd wrapper in FP64 on a tiny grid;
+  * the 512^2 x 4096-step configuration: FP64 central differences along 32
+    random directions of the initial velocity perturbation (and of tau_s).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import test_oracle_adjoint as TA
+from lbcube import adjoint as ADJ
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, gpu, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(gpu).to(dtype)
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+@pytest.mark.parametrize("interval", [1, 5, 64])
+def test_adjoint_matches_oracle_fp64(gpu, coll, interval):
+    print(f"case adjoint {coll} interval={interval}", flush=True)
+    f, a, b = TA.small_state(nx=12, ny=10, seed=4)
+    omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 17
+    J, gf, ga, gb, dw = O.reactive_gradient(coll, "f64", f, a, b, omega, omega_c, k, steps)
+    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient(coll, dev(f, gpu), dev(a, gpu), dev(b, gpu), steps, omega, omega_c,
+                                                   k, interval=interval)
+    assert abs(Jg - J) <= 1e-15
+    for got, want in ((gfg, gf), (gag, ga), (gbg, gb)):
+        g = got.cpu().numpy()
+        assert np.max(np.abs(g - want)) <= 1e-12 * np.max(np.abs(want))
+    assert abs(dwg - dw) <= 1e-11 * abs(dw)
+
+
+def test_adjoint_fp32_against_oracle_fp32(gpu):
+    f, a, b = TA.small_state(nx=12, ny=10, seed=6)
+    f, a, b = (x.astype(np.float32) for x in (f, a, b))
+    omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 17
+    J, gf, ga, gb, dw = O.reactive_gradient("BGK", "f32", f, a, b, omega, omega_c, k, steps)
+    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient("BGK", dev(f, gpu, torch.float32), dev(a, gpu, torch.float32),
+                                                   dev(b, gpu, torch.float32), steps, omega, omega_c, k, interval=4)
+    assert abs(Jg - J) <= 1e-6
+    for got, want in ((gfg, gf), (gag, ga), (gbg, gb)):
+        assert np.max(np.abs(got.cpu().numpy() - want)) <= 1e-5 * np.max(np.abs(want))
+    assert abs(dwg - dw) <= 1e-4 * abs(dw)
+
+
+def test_checkpoint_interval_invariance(gpu):
+    f, a, b = TA.small_state(nx=64, ny=48, seed=2)
+    args = (dev(f, gpu), dev(a, gpu), dev(b, gpu), 50, 1 / 0.6, 1 / 0.8, 0.05)
+    ref = ADJ.reactive_gradient("BGK", *args, interval=50)
+    for interval in (1, 7, 64):
+        got = ADJ.reactive_gradient("BGK", *args, interval=interval)
+        assert got[0] == ref[0] and got[4] == ref[4]
+        for x, y in zip(got[1:4], ref[1:4]):
+            assert torch.equal(x, y)
+
+
+def test_autograd_gradcheck(gpu):
+    f, a, b = TA.small_state(nx=6, ny=5, seed=8)
+    f0, a0, b0 = (dev(x, gpu).requires_grad_(True) for x in (f, a, b))
+    tau = torch.tensor(0.9, dtype=torch.float64, device=gpu, requires_grad=True)
+
+    def fn(f0, a0, b0, tau):
+        return ADJ.reactive_loss(f0, a0, b0, tau, "BGK", steps=4, omega=1 / 0.7, k_react=0.4, interval=2)
+
+    assert torch.autograd.gradcheck(fn, (f0, a0, b0, tau), eps=1e-6, atol=1e-9, rtol=1e-6)
+
+
+def test_config4_gradients_against_central_differences(gpu):
+    """512^2, 4096 steps, checkpoints every 64: dJ/d(du) and dJ/dtau_s, FP64,
+    against central differences along 32 random directions; the FP32 adjoint
+    is compared with the same differences."""
+    n, steps, tau_s, omega, k = 512, 4096, 0.8, 1 / 0.6, 0.05
+    du = ADJ.gaussian_random_field((n, n), 32.0, 0.01, seed=0x5EED1234, device=gpu)
+    results = {}
+    for dtype in (torch.float64, torch.float32):
+        d = du.clone().requires_grad_(True)
+        tau = torch.tensor(tau_s, dtype=torch.float64, device=gpu, requires_grad=True)
+        f0, a0, b0 = ADJ.initial_state(d, dtype=dtype)
+        loss = ADJ.reactive_loss(f0, a0, b0, tau, "BGK", steps=steps, omega=omega, k_react=k, interval=64)
+        loss.backward()
+        results[dtype] = (loss.item(), d.grad.detach().clone(), tau.grad.item())
+        print(f"dtype {dtype}: loss {loss.item():.10e}, dJ/dtau_s {tau.grad.item():.10e}", flush=True)
+    loss64, g64, gt64 = results[torch.float64]
+    _, g32, gt32 = results[torch.float32]
+
+    def J(du_, tau_=tau_s):
+        f0, a0, b0 = ADJ.initial_state(du_, dtype=torch.float64)
+        return ADJ.loss_of("BGK", f0, a0, b0, steps, omega, 1.0 / tau_, k)
+
+    gen = torch.Generator(device="cpu").manual_seed(99)
+    worst64 = worst32 = 0.0
+    for trial in range(32):
+        v = ADJ.gaussian_random_field((n, n), 32.0, 1.0, seed=int(torch.randint(0, 2**31, (1,), generator=gen)),
+                                      device=gpu)
+        h = 1e-4
+        fd = (J(du + h * v) - J(du - h * v)) / (2 * h)
+        ad64 = (g64 * v).sum().item()
+        ad32 = (g32.double() * v).sum().item()
+        e64, e32 = abs(ad64 - fd) / abs(fd), abs(ad32 - fd) / abs(fd)
+        worst64, worst32 = max(worst64, e64), max(worst32, e32)
+        print(f"direction {trial}: fd {fd:.10e} adjoint64 {ad64:.10e} ({e64:.1e}) adjoint32 ({e32:.1e})", flush=True)
+    ht = 1e-5
+    fdt = (J(du, tau_s + ht) - J(du, tau_s - ht)) / (2 * ht)
+    print(f"tau_s: fd {fdt:.10e} adjoint64 {gt64:.10e} adjoint32 {gt32:.10e}", flush=True)
+    print(f"worst relative error: fp64 {worst64:.2e}, fp32 {worst32:.2e}", flush=True)
+    assert worst64 <= 1e-4 and abs(gt64 - fdt) <= 1e-4 * abs(fdt)
+    assert worst32 <= 1e-3
