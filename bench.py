@@ -87,15 +87,34 @@ def _time(torch, fn, reps=1):
 def adjoint_ratio(torch, n=512, steps=4096, interval=64):
     """Differentiated 2D reactive run (D2Q9 BGK + two D2Q5, A+B->C), FP32:
     time of the forward loop vs time of the full gradient (forward with
-    checkpoints, segment recompute, adjoint sweep), per step."""
+    checkpoints, segment recompute, adjoint sweep), per step.  The adjoint
+    time is the gradient time minus one forward run (recompute included)."""
     from lbcube import adjoint as ADJ
 
     du = ADJ.gaussian_random_field((n, n), 32.0, 0.01, seed=0x5EED1234)
     f0, a0, b0 = ADJ.initial_state(du, dtype=torch.float32)
     omega, omega_c, k = 1 / 0.6, 1 / 0.8, 0.05
-    ADJ.reactive_run("BGK", f0, a0, b0, 8, omega, omega_c, k)
-    t_fwd = _time(torch, lambda: ADJ.reactive_run("BGK", f0, a0, b0, steps, omega, omega_c, k))
+    # forward: 64 steps captured in a CUDA graph (no per-step host overhead),
+    # replayed steps / 64 times
     import lbcube
+    st = [f0.clone(), torch.empty_like(f0), a0.clone(), torch.empty_like(a0), b0.clone(), torch.empty_like(b0)]
+
+    def fwd_chunk():
+        for t in range(64):
+            c, x = (0, 1) if t % 2 == 0 else (1, 0)
+            lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", st[c], st[x], st[2 + c], st[2 + x], st[4 + c],
+                                       st[4 + x], omega, omega_c, k)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fwd_chunk()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        fwd_chunk()
+    graph.replay()
+    t_fwd = _time(torch, lambda: [graph.replay() for _ in range(steps // 64)])
     block = lbcube.default_block("D2Q9", n, n)
     ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block), dtype=torch.uint8,
                      device="cuda")
@@ -104,6 +123,7 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
     t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("BGK", f0, a0, b0, steps, omega, omega_c, k,
                                                                      interval, workspace=ws)))
     loss = out["r"][0]
+    del graph, st
     return {"workload": f"D2Q9 BGK + 2xD2Q5 A+B->C, {n}^2, {steps} steps, FP32, checkpoint every {interval}",
             "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
             "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,

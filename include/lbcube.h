@@ -106,8 +106,9 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
  * checkpointed every `interval` steps and each segment is recomputed before
  * it is walked backward.  storage: LBC_F32 or LBC_F64.  The caller passes a
  * device workspace of at least lbc_reactive_gradient_workspace() bytes;
- * nothing is allocated.  Synchronizes `stream`.  No counterpart in the
- * reference. */
+ * nothing is allocated.  block_x * block_y must be a multiple of 64 (whole
+ * wavefronts), else hipErrorInvalidConfiguration.  Synchronizes `stream`.
+ * No counterpart in the reference. */
 int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
                                     uint32_t block_x, uint32_t block_y, uint64_t* bytes);
 int lbc_reactive_gradient(int collision, int storage, const void* f0, const void* a0, const void* b0, uint64_t nx,
@@ -129,7 +130,8 @@ int lbc_ensemble_create(int collision, int storage, int count, const uint32_t* n
                         const double* omega, const double* omega_c, const double* k_react, lbc_ensemble** out);
 int lbc_ensemble_layout(const lbc_ensemble* ensemble, uint64_t* fluid_elems, uint64_t* species_elems,
                         uint64_t* fluid_offsets, uint64_t* species_offsets);
-/* tile_cap in {64, 128, 256, 512} (work-items along x per tile), threads in {256, 512}. */
+/* tile_cap in {64, 128, 256, 512} (work-items along x per tile), threads in
+ * {256, 512}.  Waits for the device first (the table is rewritten in place). */
 int lbc_ensemble_set_tiling(lbc_ensemble* ensemble, uint32_t tile_cap, uint32_t threads);
 int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* next_fluid, const void* current_a,
                       void* next_a, const void* current_b, void* next_b, void* stream);
@@ -138,8 +140,8 @@ int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* n
  * and replayed. */
 int lbc_ensemble_run(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
                      int steps, int use_graph, void* stream);
-/* Times the 8 tiling candidates over 2*trial_steps steps each (advancing the
- * buffers), keeps the fastest; chosen[2] = {tile_cap, threads}; ms[8] (optional)
+/* Times the 8 tiling candidates over 2*trial_steps steps each, after 2
+ * warm-up steps (advancing the buffers by an even count), keeps the fastest; chosen[2] = {tile_cap, threads}; ms[8] (optional)
  * = ms per step of each candidate (-1 when not applicable). */
 int lbc_ensemble_autotune(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
                           int trial_steps, uint32_t* chosen, float* ms, void* stream);

@@ -189,10 +189,10 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
 
 // Lambda[i][y] = nu[i][y + c_i] (+ shift): the adjoint of stored populations.
 template <class R, class L>
-__global__ __launch_bounds__(256) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, int nx,
-                                                           int ny, R shift) {
+__global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, int nx,
+                                                            int ny, R shift) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= nx || y >= ny) return;
     const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny));
     const Axis ax(x, nx), ay(y, ny);
@@ -348,10 +348,9 @@ hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape
     }
     // adjoints of the stored initial populations
     State<R> fin(nu1, cells);
-    const dim3 rg(static_cast<unsigned>((s.nx + 255) / 256), static_cast<unsigned>(s.ny));
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), rg, dim3(256), 0, st, fin.f, gf, s.nx, s.ny, R(0));
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), rg, dim3(256), 0, st, fin.a, ga, s.nx, s.ny, inv_n);
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), rg, dim3(256), 0, st, fin.b, gb, s.nx, s.ny, R(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.nx, s.ny, R(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.nx, s.ny, inv_n);
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.nx, s.ny, R(0));
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, part, L.blocks, red + 2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double h[3];
@@ -373,6 +372,7 @@ extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_
     LaunchShape s;
     const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
     if (e != hipSuccess) return e;
+    if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;
     *bytes = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid).total;
     return hipSuccess;
 }
@@ -390,6 +390,7 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, const void* f0,
     LaunchShape s;
     const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
     if (e != hipSuccess) return e;
+    if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
     if (collision != kBGK && collision != kTRT) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;

@@ -91,6 +91,9 @@ static uint32_t pow2_at_least(uint32_t v) {
 // Tile policy: tile width = min(cap, threads, next power of two >= nx), at
 // least one wave.
 static hipError_t build_table(lbc_ensemble* e, uint32_t cap, uint32_t threads) {
+    // The table is rewritten in place: no launch that reads it may be in flight.
+    hipError_t err = hipDeviceSynchronize();
+    if (err != hipSuccess) return err;
     uint64_t blocks = 0;
     for (auto& in : e->host) {
         uint32_t tw = std::min(std::min(cap, threads), std::max<uint32_t>(64, pow2_at_least(in.nx)));
@@ -104,8 +107,7 @@ static hipError_t build_table(lbc_ensemble* e, uint32_t cap, uint32_t threads) {
     e->blocks = blocks;
     e->cap = cap;
     e->threads = threads;
-    hipError_t err = hipMemcpy(e->table, e->host.data(), e->host.size() * sizeof(EnsembleInstance),
-                               hipMemcpyHostToDevice);
+    err = hipMemcpy(e->table, e->host.data(), e->host.size() * sizeof(EnsembleInstance), hipMemcpyHostToDevice);
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
@@ -272,7 +274,10 @@ extern "C" int lbc_ensemble_run(lbc_ensemble* e, void* f0, void* f1, void* a0, v
         }
         hipEvent_t before, after;
         if ((err = hipEventCreateWithFlags(&before, hipEventDisableTiming)) != hipSuccess) return err;
-        if ((err = hipEventCreateWithFlags(&after, hipEventDisableTiming)) != hipSuccess) return err;
+        if ((err = hipEventCreateWithFlags(&after, hipEventDisableTiming)) != hipSuccess) {
+            (void)hipEventDestroy(before);
+            return err;
+        }
         err = hipEventRecord(before, st);
         if (err == hipSuccess) err = hipStreamWaitEvent(e->capture, before, 0);
         while (err == hipSuccess && steps - done >= chunk) {

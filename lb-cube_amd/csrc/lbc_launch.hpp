@@ -34,8 +34,10 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
     if (static_cast<uint64_t>(bx) * by * bz > 1024) return hipErrorInvalidConfiguration;
     // Byte offsets inside one population plane are 32-bit (the global-load
     // saddr form), so a plane holds at most 2^29 cells (4 GiB of doubles).
-    if (nx > 0x7fffffffull || ny > 0x7fffffffull || nz > 0x7fffffffull) return hipErrorInvalidValue;
-    if (nx * ny * nz > (1ull << 29)) return hipErrorInvalidValue;
+    // (each factor and partial product checked first, so nothing overflows)
+    constexpr uint64_t kMaxCells = 1ull << 29;
+    if (nx > kMaxCells || ny > kMaxCells || nz > kMaxCells) return hipErrorInvalidValue;
+    if (nx * ny > kMaxCells || nx * ny * nz > kMaxCells) return hipErrorInvalidValue;
     const uint64_t gx = (nx + bx - 1) / bx, gy = (ny + by - 1) / by, gz = (nz + bz - 1) / bz;
     if (gx > 0x7fffffffull || gy > 65535 || gz > 65535) return hipErrorInvalidConfiguration;
     s.grid = dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy), static_cast<unsigned>(gz));

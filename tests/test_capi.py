@@ -48,6 +48,15 @@ def test_validation_order_fluid():
     assert fluid(FAKE_A, FAKE_B, block=(64, 32, 1)) == HIP_INVALID_CONFIGURATION  # > 1024 work-items
     assert fluid(FAKE_A, FAKE_A) == HIP_INVALID_VALUE  # ping-pong buffers must not alias
     assert fluid(FAKE_A, FAKE_B, nx=1 << 20, ny=1 << 10, nz=1) == HIP_INVALID_VALUE  # plane > 2^29 cells
+    # extents whose 64-bit product wraps to 0 (2^31 * 2^17 * 2^16) are still rejected
+    assert fluid(FAKE_A, FAKE_B, nx=1 << 31, ny=1 << 17, nz=1 << 16, block=(2, 4, 2)) == HIP_INVALID_VALUE
+
+
+def test_gradient_needs_whole_waves():
+    out = ctypes.c_uint64()
+    lib = lbcube.lib()
+    assert lib.lbc_reactive_gradient_workspace(0, 64, 64, 8, 4, 256, 1, ctypes.byref(out)) == 0 and out.value > 0
+    assert lib.lbc_reactive_gradient_workspace(0, 64, 64, 8, 4, 100, 1, ctypes.byref(out)) == HIP_INVALID_CONFIGURATION
 
 
 def test_two_d_lattice_ignores_z():
