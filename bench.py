@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "lb-cube_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E spec peak
 BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16"): 76, ("D3Q19", "f64"): 304,
-                  ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108}
+                  ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108, ("D3Q27", "bf16"): 108, ("D3Q27", "f64"): 432}
 
 
 def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False):
@@ -71,6 +71,44 @@ def copy_ceiling(lbcube, torch, nbytes, reps=20):
     del src, dst
     torch.cuda.empty_cache()
     return nbytes / (ms * 1e-3) / 1e9
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_D3Q19_TRT_f32_256.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counters cannot be read inside the timed
+    run); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def lattice_sweep(lbcube, torch, steps=10, warmup=3):
+    """Taylor-Green Re = 1600 on D3Q19 and D3Q27 at 256^3 and 384^3, TRT and
+    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT)."""
+    from lbcube import init as lbinit
+
+    out = {}
+    for n in (256, 384):
+        u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
+        omega = 1.0 / lbinit.tgv_re1600_tau(n)
+        for lattice in ("D3Q19", "D3Q27"):
+            for coll, storage, fast in (("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False)):
+                key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}"
+                r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast)
+                out[key] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                print(f"# {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s", file=sys.stderr,
+                      flush=True)
+        del u
+        torch.cuda.empty_cache()
+    return out
 
 
 def _time(torch, fn, reps=1):
@@ -132,12 +170,13 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
 
 def ensemble_rate(torch, steps=1000):
     """512 independent 2D reactive instances (D2Q9 TRT + 2xD2Q5), edges
-    log-uniform in 64..1024, FP16 storage, autotuned tiling, hipGraph replay."""
+    log-uniform in 64..1024, FP16 storage with FP32 deviation-form arithmetic,
+    autotuned tiling, hipGraph replay."""
     from lbcube import init as lbinit
     from lbcube.ensemble import Ensemble, reactive_mixing_ensemble
 
     n, omega, omega_c, k = reactive_mixing_ensemble()
-    ens = Ensemble("TRT", "f16", [(int(x), int(x)) for x in n], omega, omega_c, k)
+    ens = Ensemble("TRT", "f16", [(int(x), int(x)) for x in n], omega, omega_c, k, fast=True)
     for i, ni in enumerate(n):
         rho, u = lbinit.taylor_green_2d(int(ni), int(ni), 0.04)
         ca, cb = lbinit.tanh_half_planes(int(ni), delta=max(2.0, ni / 32))
@@ -151,7 +190,8 @@ def ensemble_rate(torch, steps=1000):
     cells = ens.cells
     ens.close()
     gbs = 76 * cells * steps / (ms * 1e-3) / 1e9
-    return {"workload": "512 instances D2Q9 TRT + 2xD2Q5 A+B->C, N log-uniform 64..1024, FP16 storage, "
+    return {"workload": "512 instances D2Q9 TRT + 2xD2Q5 A+B->C, N log-uniform 64..1024, FP16 storage "
+                        "(FP32 deviation-form arithmetic), "
                         f"{steps} steps, one launch per step, hipGraph replay",
             "cells": cells, "mlups": cells * steps / (ms * 1e-3) / 1e6, "achieved_gbs": gbs,
             "bytes_per_cell": 76, "tiling": {"tile_cap": chosen[0], "threads": chosen[1]},
@@ -269,6 +309,7 @@ def main():
               flush=True)
         extras["ensemble"] = ensemble_rate(torch)
         print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        extras["lattices"] = lattice_sweep(lbcube, torch)
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
     cpu = None
     if not args.no_cpu_baseline:
@@ -290,7 +331,9 @@ def main():
                                "seeded divergence-free perturbation rms 0.01*u0",
                    "cells": n ** 3, "lattice": "D3Q19", "collision": "TRT", "storage": "f32"},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4),
+                     "traffic": measured_traffic() if (n == 256 and block is None) else None,
+                     "algorithmic_bytes": 152 * n ** 3,
                      "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "bytes_per_cell": 152,
                      "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
         "cpu_baseline": cpu,

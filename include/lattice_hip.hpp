@@ -16,8 +16,9 @@
 //       lattice_cuda.cuh:276-291, src/lattice_cuda.cu:610-666
 //
 // Differences: launchers return hipError_t; the default block is
-// default_cuda_block_size() = {256, 1, 1} (whole 256-cell rows along x for
-// wave64 lanes) instead of {16, 8, 1}; an optional trailing hipStream_t selects
+// default_cuda_block_size() = {256, 1, 1} (four wave64s) instead of {16, 8, 1};
+// the kernels cover cells linearly, so any block gives the same results and
+// only its size matters for speed; an optional trailing hipStream_t selects
 // the stream (the reference always uses the default stream).
 #pragma once
 

@@ -98,6 +98,19 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
                             double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
                             uint32_t block_z, void* stream);
 
+/* lbc_reactive_fused_step with flags (0 = lbc_reactive_fused_step exactly).
+ * LBC_FLAG_FAST (BGK and TRT; ignored by MRT and RLBM): the fluid part in the
+ * deviation form of lbc_fluid_step_ex, the velocity taken from the stored
+ * deviations, the scalar collision with 1/cs2 multiplied and FMAs, and the
+ * reaction increment with a hardware exponential and reciprocal.  Results
+ * differ from the reference order by rounding only (bounded by the tests).
+ * Unknown flag bits -> hipErrorInvalidValue.  No counterpart in the reference. */
+int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_lattice, int storage, uint32_t flags,
+                               const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
+                               const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                               double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
+                               uint32_t block_z, void* stream);
+
 /* Gradient of the 2D reactive loop (D2Q9 BGK or TRT + two D2Q5 species,
  * A + B -> C; `steps` calls of lbc_reactive_fused_step from f0, a0, b0) by the
  * discrete adjoint.  loss = <C>(T) = <A>(0) - <A>(T) (domain means).  Outputs:
@@ -133,6 +146,10 @@ int lbc_ensemble_layout(const lbc_ensemble* ensemble, uint64_t* fluid_elems, uin
 /* tile_cap in {64, 128, 256, 512} (work-items along x per tile), threads in
  * {256, 512}.  Waits for the device first (the table is rewritten in place). */
 int lbc_ensemble_set_tiling(lbc_ensemble* ensemble, uint32_t tile_cap, uint32_t threads);
+/* 0 or LBC_FLAG_FAST: every instance then advances as
+ * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...) on it alone.  Waits for
+ * the device first. */
+int lbc_ensemble_set_flags(lbc_ensemble* ensemble, uint32_t flags);
 int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* next_fluid, const void* current_a,
                       void* next_a, const void* current_b, void* next_b, void* stream);
 /* `steps` steps between buffer sets 0 and 1 (result in set steps % 2); with
@@ -165,9 +182,10 @@ int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, c
  * counterpart in the reference. */
 int lbc_stream_copy(const void* src, void* dst, uint64_t bytes, void* stream);
 
-/* Block shape the library picks for a domain (the C++ launcher mirror uses it
- * as its default argument; the reference's default is {16, 8, 1},
- * include/lattice_cuda.cuh:52-54). */
+/* Block shape the library picks for a domain: {256, 1, 1}.  Every launcher
+ * covers the cells linearly (x fastest), so only bx * by * bz (the work-items
+ * per block) affects the launch, never the results.  The reference's default
+ * is {16, 8, 1} (include/lattice_cuda.cuh:52-54). */
 void lbc_default_block(int lattice, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t* block3);
 
 /* Human-readable name of a returned status code. */

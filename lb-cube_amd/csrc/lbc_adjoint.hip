@@ -81,15 +81,14 @@ template <class R, int CT, int MAXT>
 __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
     const R* __restrict__ f, const R* __restrict__ a, const R* __restrict__ b, const R* __restrict__ nuf1,
     const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
-    R* __restrict__ nub0, int nx, int ny, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const bool live = x < nx && y < ny;
+    R* __restrict__ nub0, Cells c, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     double acc = 0.0;
-    if (live) {
-        const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny));
-        const Axis ax(x, nx), ay(y, ny);
-        const uint32_t cell = static_cast<uint32_t>(y) * nx + x;
+    if (cell < c.count) {
+        const CellCoord q = cell_coord(cell, c);
+        const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
+        const size_t plane = c.count;
+        const Axis ax(q.x, nx), ay(q.y, ny);
         R fl[9], fp[9], lf[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
@@ -176,30 +175,31 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
     __shared__ double wsum[MAXT / 64];
-    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    const int t = threadIdx.x;
     if ((t & 63) == 0) wsum[t >> 6] = acc;
     __syncthreads();
     if (t == 0) {
         double s = 0.0;
-        const int nw = (blockDim.x * blockDim.y + 63) / 64;
+        const int nw = (blockDim.x + 63) / 64;
         for (int w = 0; w < nw; ++w) s += wsum[w];
-        djdw[blockIdx.y * gridDim.x + blockIdx.x] += s;
+        djdw[blockIdx.x] += s;
     }
 }
 
 // Lambda[i][y] = nu[i][y + c_i] (+ shift): the adjoint of stored populations.
 template <class R, class L>
-__global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, int nx,
-                                                            int ny, R shift) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= nx || y >= ny) return;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny));
-    const Axis ax(x, nx), ay(y, ny);
+__global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, Cells c,
+                                                            R shift) {
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= c.count) return;
+    const CellCoord q = cell_coord(cell, c);
+    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
+    const size_t plane = c.count;
+    const Axis ax(q.x, nx), ay(q.y, ny);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const uint32_t rev = static_cast<uint32_t>(ay.pull(-L::cy[i])) * nx + ax.pull(-L::cx[i]);
-        lam[i * plane + static_cast<uint32_t>(y) * nx + x] = plane_ptr(nu, i, plane)[rev] + shift;
+        lam[i * plane + cell] = plane_ptr(nu, i, plane)[rev] + shift;
     }
 }
 
@@ -337,10 +337,10 @@ hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape
             State<R> fw(seg + t * se, cells), in(nu1, cells), out(nu0, cells);
             if (s.threads() <= 256)
                 hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a, fw.b,
-                                   in.f, in.a, in.b, out.f, out.a, out.b, s.nx, s.ny, omega, wm, omega_c, k, part);
+                                   in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k, part);
             else
                 hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 1024>), s.grid, s.block, 0, st, fw.f, fw.a,
-                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.nx, s.ny, omega, wm, omega_c, k,
+                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k,
                                    part);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             std::swap(nu0, nu1);
@@ -348,9 +348,9 @@ hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape
     }
     // adjoints of the stored initial populations
     State<R> fin(nu1, cells);
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.nx, s.ny, R(0));
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.nx, s.ny, inv_n);
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.nx, s.ny, R(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.cells, R(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.cells, inv_n);
+    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.cells, R(0));
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, part, L.blocks, red + 2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double h[3];

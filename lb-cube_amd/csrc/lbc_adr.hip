@@ -4,8 +4,9 @@
 // (src/lattice_cuda.cu:349-457), plus a single-pass kernel that fuses the fluid
 // step with the reaction step.
 //
-// Same mapping as the fluid kernel: one work-item per cell, x along the lanes
-// of a wave64, unit-stride loads and stores of every population plane.
+// Same mapping as the fluid kernel: one work-item per cell, cells numbered
+// linearly along the lanes of a wave64, unit-stride loads and stores of every
+// population plane.
 #include <hip/hip_runtime.h>
 
 #include "../../include/lbcube.h"
@@ -24,17 +25,16 @@ template <class F, class S, int ST, bool REACT, int MAXT>
 __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b,
-    const typename Storage<ST>::S* __restrict__ cur_f, int nx, int ny, int nz, typename Storage<ST>::R omega_c,
+    const typename Storage<ST>::S* __restrict__ cur_f, Cells c, typename Storage<ST>::R omega_c,
     typename Storage<ST>::R param) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
-    if (x >= nx || y >= ny || z >= nz) return;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
-                                             static_cast<uint32_t>(nz));
-    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= c.count) return;
+    const CellCoord q = cell_coord(cell, c);
+    const int x = q.x, y = q.y, z = q.z;
+    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
+    const size_t plane = c.count;
     R fl[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i)
@@ -65,19 +65,24 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     }
 }
 
-template <class F, int CT, class S, int ST, int MAXT>
+template <class F, int CT, class S, int ST, bool FAST, int MAXT>
 __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_f, typename Storage<ST>::S* __restrict__ nxt_f,
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
-    const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, int nx, int ny,
-    int nz, typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
+    const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, Cells c,
+    typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
     typename Storage<ST>::R k_react) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
-    if (x >= nx || y >= ny || z >= nz) return;
-    reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, x, y, z, nx, ny, nz, omega, omega_minus,
-                                omega_c, k_react);
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.count) return;
+    const CellCoord q = cell_coord(p, c);
+    if constexpr (FAST)
+        reactive_cell_fast<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, q.x, q.y, q.z,
+                                         static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz), omega,
+                                         omega_minus, omega_c, k_react);
+    else
+        reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, q.x, q.y, q.z, static_cast<int>(c.nx),
+                                    static_cast<int>(c.ny), static_cast<int>(c.nz), omega, omega_minus, omega_c,
+                                    k_react);
 }
 
 template <class F, class S, int ST, bool REACT>
@@ -88,8 +93,8 @@ hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, con
     using R = typename St::R;
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(ca), static_cast<T*>(na),
-                           static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.nx, s.ny,
-                           s.nz, static_cast<R>(omega_c), static_cast<R>(param));
+                           static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells,
+                           static_cast<R>(omega_c), static_cast<R>(param));
     };
     if (s.threads() <= 256) args(scalar_step_kernel<F, S, ST, REACT, 256>);
     else args(scalar_step_kernel<F, S, ST, REACT, 1024>);
@@ -120,7 +125,7 @@ hipError_t scalar_dispatch(int fl, int sl, int storage, const void* ca, void* na
     return hipErrorInvalidValue;
 }
 
-template <class F, int CT, class S, int ST>
+template <class F, int CT, class S, int ST, bool FAST>
 hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
                         const LaunchShape& s, double omega, double omega_c, double k, hipStream_t stream) {
     using St = Storage<ST>;
@@ -131,35 +136,40 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
                            static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
-                           static_cast<T*>(nb), s.nx, s.ny, s.nz, w, wm, static_cast<R>(omega_c),
-                           static_cast<R>(k));
+                           static_cast<T*>(nb), s.cells, w, wm, static_cast<R>(omega_c), static_cast<R>(k));
     };
-    if (s.threads() <= 256) args(reactive_fused_kernel<F, CT, S, ST, 256>);
-    else args(reactive_fused_kernel<F, CT, S, ST, 1024>);
+    if (s.threads() <= 256) args(reactive_fused_kernel<F, CT, S, ST, FAST, 256>);
+    else args(reactive_fused_kernel<F, CT, S, ST, FAST, 1024>);
     return hipGetLastError();
 }
 
-template <class F, class S, int CT>
+template <class F, class S, int CT, bool FAST>
 hipError_t fused_by_storage(int storage, const void* cf, void* nf, const void* ca, void* na, const void* cb,
                             void* nb, const LaunchShape& s, double omega, double omega_c, double k, hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_fused<F, CT, S, kF32>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kF64: return launch_fused<F, CT, S, kF64>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kF16: return launch_fused<F, CT, S, kF16>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kBF16: return launch_fused<F, CT, S, kBF16>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF32: return launch_fused<F, CT, S, kF32, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF64: return launch_fused<F, CT, S, kF64, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF16: return launch_fused<F, CT, S, kF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kBF16: return launch_fused<F, CT, S, kBF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class F, class S>
-hipError_t fused_by_collision(int collision, int storage, const void* cf, void* nf, const void* ca, void* na,
-                              const void* cb, void* nb, const LaunchShape& s, double omega, double omega_c, double k,
-                              hipStream_t st) {
+hipError_t fused_by_collision(int collision, int storage, bool fast, const void* cf, void* nf, const void* ca,
+                              void* na, const void* cb, void* nb, const LaunchShape& s, double omega, double omega_c,
+                              double k, hipStream_t st) {
     switch (collision) {
-        case kBGK: return fused_by_storage<F, S, kBGK>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kTRT: return fused_by_storage<F, S, kTRT>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kMRT: return fused_by_storage<F, S, kMRT>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kRLBM: return fused_by_storage<F, S, kRLBM>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kBGK:
+            return fast ? fused_by_storage<F, S, kBGK, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
+                        : fused_by_storage<F, S, kBGK, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kTRT:
+            return fast ? fused_by_storage<F, S, kTRT, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
+                        : fused_by_storage<F, S, kTRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kMRT:
+            return fused_by_storage<F, S, kMRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kRLBM:
+            return fused_by_storage<F, S, kRLBM, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -197,11 +207,12 @@ extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int s
                                  current_fluid, s, omega_c, k_react, as_stream(stream));
 }
 
-extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
-                                       const void* current_fluid, void* next_fluid, const void* current_a,
-                                       void* next_a, const void* current_b, void* next_b, uint64_t nx, uint64_t ny,
-                                       uint64_t nz, double omega, double omega_c, double k_react, uint32_t block_x,
-                                       uint32_t block_y, uint32_t block_z, void* stream) {
+extern "C" int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_lattice, int storage,
+                                          uint32_t flags, const void* current_fluid, void* next_fluid,
+                                          const void* current_a, void* next_a, const void* current_b, void* next_b,
+                                          uint64_t nx, uint64_t ny, uint64_t nz, double omega, double omega_c,
+                                          double k_react, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                          void* stream) {
     using namespace lbc;
     if (current_fluid == nullptr || next_fluid == nullptr || current_a == nullptr || next_a == nullptr ||
         current_b == nullptr || next_b == nullptr)
@@ -211,15 +222,27 @@ extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int sca
     if (e != hipSuccess) return e;
     if (current_fluid == next_fluid || current_a == next_a || current_b == next_b || next_a == next_b)
         return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    const bool fast = (flags & LBC_FLAG_FAST) != 0;
     const hipStream_t st = as_stream(stream);
     if (fluid_lattice == kD2Q9 && scalar_lattice == kD2Q5)
-        return fused_by_collision<D2Q9, D2Q5>(collision, storage, current_fluid, next_fluid, current_a, next_a,
+        return fused_by_collision<D2Q9, D2Q5>(collision, storage, fast, current_fluid, next_fluid, current_a, next_a,
                                               current_b, next_b, s, omega, omega_c, k_react, st);
     if (fluid_lattice == kD3Q19 && scalar_lattice == kD3Q7)
-        return fused_by_collision<D3Q19, D3Q7>(collision, storage, current_fluid, next_fluid, current_a, next_a,
-                                               current_b, next_b, s, omega, omega_c, k_react, st);
+        return fused_by_collision<D3Q19, D3Q7>(collision, storage, fast, current_fluid, next_fluid, current_a,
+                                               next_a, current_b, next_b, s, omega, omega_c, k_react, st);
     if (fluid_lattice == kD3Q27 && scalar_lattice == kD3Q7)
-        return fused_by_collision<D3Q27, D3Q7>(collision, storage, current_fluid, next_fluid, current_a, next_a,
-                                               current_b, next_b, s, omega, omega_c, k_react, st);
+        return fused_by_collision<D3Q27, D3Q7>(collision, storage, fast, current_fluid, next_fluid, current_a,
+                                               next_a, current_b, next_b, s, omega, omega_c, k_react, st);
     return hipErrorInvalidValue;
+}
+
+extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
+                                       const void* current_fluid, void* next_fluid, const void* current_a,
+                                       void* next_a, const void* current_b, void* next_b, uint64_t nx, uint64_t ny,
+                                       uint64_t nz, double omega, double omega_c, double k_react, uint32_t block_x,
+                                       uint32_t block_y, uint32_t block_z, void* stream) {
+    return lbc_reactive_fused_step_ex(fluid_lattice, collision, scalar_lattice, storage, 0u, current_fluid,
+                                      next_fluid, current_a, next_a, current_b, next_b, nx, ny, nz, omega, omega_c,
+                                      k_react, block_x, block_y, block_z, stream);
 }

@@ -35,7 +35,7 @@ struct EnsembleInstance {
     double omega_d, omega_minus_d, omega_c_d, k_d;
 };
 
-template <int CT, int ST, int THREADS>
+template <int CT, int ST, bool FAST, int THREADS>
 __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storage<ST>::S* __restrict__ cf,
                                                            typename Storage<ST>::S* __restrict__ nf,
                                                            const typename Storage<ST>::S* __restrict__ ca,
@@ -61,9 +61,14 @@ __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storag
     const R wm = std::is_same<R, double>::value ? R(in.omega_minus_d) : R(in.omega_minus);
     const R wc = std::is_same<R, double>::value ? R(in.omega_c_d) : R(in.omega_c);
     const R k = std::is_same<R, double>::value ? R(in.k_d) : R(in.k);
-    reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
-                                      nb + in.s_off, x, y, 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1,
-                                      omega, wm, wc, k);
+    if constexpr (FAST)
+        reactive_cell_fast<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off,
+                                               cb + in.s_off, nb + in.s_off, x, y, 0, static_cast<int>(in.nx),
+                                               static_cast<int>(in.ny), 1, omega, wm, wc, k);
+    else
+        reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
+                                          nb + in.s_off, x, y, 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1,
+                                          omega, wm, wc, k);
 }
 
 }  // namespace lbc
@@ -74,6 +79,7 @@ struct lbc_ensemble {
     lbc::EnsembleInstance* table = nullptr;
     uint64_t f_elems = 0, s_elems = 0, blocks = 0;
     uint32_t cap = 256, threads = 256;
+    uint32_t flags = 0;
     hipStream_t capture = nullptr;
     hipGraphExec_t graph = nullptr;
     const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -115,7 +121,7 @@ static hipError_t build_table(lbc_ensemble* e, uint32_t cap, uint32_t threads) {
     return err;
 }
 
-template <int CT, int ST>
+template <int CT, int ST, bool FAST>
 static void launch_ensemble(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na, const void* cb,
                             void* nb, hipStream_t st) {
     using T = typename Storage<ST>::S;
@@ -124,18 +130,18 @@ static void launch_ensemble(const lbc_ensemble* e, const void* cf, void* nf, con
                            static_cast<const T*>(cf), static_cast<T*>(nf), static_cast<const T*>(ca),
                            static_cast<T*>(na), static_cast<const T*>(cb), static_cast<T*>(nb), e->table, e->count);
     };
-    if (e->threads == 256) go(ensemble_kernel<CT, ST, 256>);
-    else go(ensemble_kernel<CT, ST, 512>);
+    if (e->threads == 256) go(ensemble_kernel<CT, ST, FAST, 256>);
+    else go(ensemble_kernel<CT, ST, FAST, 512>);
 }
 
-template <int CT>
+template <int CT, bool FAST>
 static hipError_t step_ct(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na, const void* cb,
                           void* nb, hipStream_t st) {
     switch (e->storage) {
-        case kF32: launch_ensemble<CT, kF32>(e, cf, nf, ca, na, cb, nb, st); break;
-        case kF64: launch_ensemble<CT, kF64>(e, cf, nf, ca, na, cb, nb, st); break;
-        case kF16: launch_ensemble<CT, kF16>(e, cf, nf, ca, na, cb, nb, st); break;
-        case kBF16: launch_ensemble<CT, kBF16>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kF32: launch_ensemble<CT, kF32, FAST>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kF64: launch_ensemble<CT, kF64, FAST>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kF16: launch_ensemble<CT, kF16, FAST>(e, cf, nf, ca, na, cb, nb, st); break;
+        case kBF16: launch_ensemble<CT, kBF16, FAST>(e, cf, nf, ca, na, cb, nb, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -143,8 +149,10 @@ static hipError_t step_ct(const lbc_ensemble* e, const void* cf, void* nf, const
 
 static hipError_t ensemble_step(const lbc_ensemble* e, const void* cf, void* nf, const void* ca, void* na,
                                 const void* cb, void* nb, hipStream_t st) {
-    return e->collision == kBGK ? step_ct<kBGK>(e, cf, nf, ca, na, cb, nb, st)
-                                : step_ct<kTRT>(e, cf, nf, ca, na, cb, nb, st);
+    const bool fast = (e->flags & LBC_FLAG_FAST) != 0;
+    if (e->collision == kBGK)
+        return fast ? step_ct<kBGK, true>(e, cf, nf, ca, na, cb, nb, st) : step_ct<kBGK, false>(e, cf, nf, ca, na, cb, nb, st);
+    return fast ? step_ct<kTRT, true>(e, cf, nf, ca, na, cb, nb, st) : step_ct<kTRT, false>(e, cf, nf, ca, na, cb, nb, st);
 }
 
 }  // namespace lbc
@@ -221,6 +229,19 @@ extern "C" int lbc_ensemble_set_tiling(lbc_ensemble* e, uint32_t tile_cap, uint3
                                                tile_cap != 512))
         return hipErrorInvalidConfiguration;
     return lbc::build_table(e, tile_cap, threads);
+}
+
+extern "C" int lbc_ensemble_set_flags(lbc_ensemble* e, uint32_t flags) {
+    if (e == nullptr) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    const hipError_t err = hipDeviceSynchronize();  // a captured graph may still be replaying
+    if (err != hipSuccess) return err;
+    if (e->graph) {
+        (void)hipGraphExecDestroy(e->graph);
+        e->graph = nullptr;
+    }
+    e->flags = flags;
+    return hipSuccess;
 }
 
 extern "C" int lbc_ensemble_step(lbc_ensemble* e, const void* cur_f, void* next_f, const void* cur_a, void* next_a,

@@ -112,4 +112,38 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
     }
 }
 
+// ---- scalar ADR and A + B -> C in the same style (the fast fused reactive step)
+
+__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
+
+// g_i += omega_c (w_i C (1 + c.u / cs2) - g_i) + w_i S with 1/cs2 (3 for D2Q5,
+// 4 for D3Q7, exact) multiplied instead of divided, and FMAs.
+template <class S, class R>
+__device__ __forceinline__ void collide_scalar_fast(R* g, R conc, R ux, R uy, R uz, R omega_c, R source) {
+    constexpr double inv_cs2 = S::cs2 == 0.25 ? 4.0 : 3.0;  // D3Q7 : D2Q5
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        const R wc = R(S::w[i]) * conc;
+        const R cu = dir_dot<S, R>(i, ux, uy, uz);
+        const R eq = fmadd(wc, R(inv_cs2) * cu, wc);
+        g[i] = fmadd(omega_c, eq - g[i], fmadd(R(S::w[i]), source, g[i]));
+    }
+}
+
+// The exact one-step increment of dA/dt = dB/dt = -k A B (same branches as
+// reaction_source) with a hardware exponential and reciprocal.
+template <class R> __device__ __forceinline__ R reaction_source_fast(R a, R b, R k) {
+    const R diff = b - a;
+    const R tol = R(64.0) * Eps<R>::v * (R(1.0) + fabs(a) + fabs(b));
+    R after;
+    if (fabs(diff) <= tol) {
+        after = a * fast_rcp(fmadd(k, a, R(1.0)));
+    } else {
+        const R att = fast_exp(-k * diff);
+        after = (a * diff * att) * fast_rcp(fmadd(-a, att, b));
+    }
+    return after - a;
+}
+
 }  // namespace lbc

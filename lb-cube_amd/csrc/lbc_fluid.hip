@@ -2,9 +2,9 @@
 // replacement of LB-Cube's kernel_step (src/lattice_cuda.cu:149-213) behind
 // lbc_fluid_step (launch_step_gpu, src/lattice_cuda.cu:476-520).
 //
-// One work-item owns one destination cell, work-items along x are
-// consecutive lanes of a wave64, so each of the Q population planes is read
-// and written with unit stride.  A population is pulled from x - c_i
+// One work-item owns one destination cell; cells are numbered linearly (x
+// fastest), so the 64 lanes of a wave cover 64 consecutive cells and each of
+// the Q population planes is read and written with unit stride.  A population is pulled from x - c_i
 // (periodic), collided in registers and stored once: Q loads + Q stores per
 // cell, the whole algorithmic traffic.  Periodic wrap uses selects on 32-bit
 // coordinates; the plane offset of population i is a wave-uniform 64-bit base.
@@ -22,19 +22,17 @@ namespace lbc {
 // LBC_FLAG_FAST; every other combination evaluates the reference's order.
 template <class L, int CT, int ST, bool FAST, int MAXT>
 __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
-                                                          typename Storage<ST>::S* __restrict__ nxt, int nx, int ny,
-                                                          int nz, typename Storage<ST>::R omega,
+                                                          typename Storage<ST>::S* __restrict__ nxt, Cells c,
+                                                          typename Storage<ST>::R omega,
                                                           typename Storage<ST>::R omega_minus) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
-    if (x >= nx || y >= ny || z >= nz) return;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
-                                             static_cast<uint32_t>(nz));
-    const Axis ax(x, nx), ay(y, ny), az(z, nz);
-    const uint32_t dst = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    const uint32_t dst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dst >= c.count) return;
+    const CellCoord q = cell_coord(dst, c);
+    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
+    const size_t plane = c.count;
+    const Axis ax(q.x, nx), ay(q.y, ny), az(q.z, nz);
     R f[L::Q];
     if constexpr (FAST) {
 #pragma unroll
@@ -72,12 +70,12 @@ hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
     if (s.threads() <= 256)
         hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 256>), s.grid, s.block, 0, stream,
-                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
-                           s.nz, w, wm);
+                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.cells, w,
+                           wm);
     else
         hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 1024>), s.grid, s.block, 0, stream,
-                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.nx, s.ny,
-                           s.nz, w, wm);
+                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.cells, w,
+                           wm);
     return hipGetLastError();
 }
 

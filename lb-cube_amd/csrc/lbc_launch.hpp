@@ -11,15 +11,57 @@ namespace lbc {
 
 inline bool is_2d(int lattice) { return lattice == kD2Q9 || lattice == kD2Q5; }
 
+// Division by a launch constant d (1 <= d <= 2^29) of dividends n < 2^29:
+// q = (n * m) >> s with l = ceil(log2 d), s = 29 + l, m = ceil(2^s / d) <= 2^30
+// (the round-up reciprocal method of Granlund & Montgomery, PLDI 1994).  Exact:
+// n * m / 2^s = n / d + n e / 2^s with 0 <= e < 1, n e / 2^s < 2^-l, which is
+// < 1/d unless d = 2^l, where e = 0.
+struct FastDiv {
+    uint32_t m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    const uint32_t s = 29 + l;
+    return FastDiv{static_cast<uint32_t>(((1ull << s) + d - 1) / d), s};
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(n) * f.m) >> f.s);
+}
+
+// Cells of a domain numbered linearly, x fastest ((z * ny + y) * nx + x): the
+// kernels map work-item p of the launch to cell p, so every wave covers 64
+// consecutive cells whatever nx and the block shape are (no idle lanes at row
+// ends beyond the last wave of the domain).
+struct Cells {
+    uint32_t nx, ny, nz, count;
+    FastDiv by_nx, by_ny;
+};
+
+struct CellCoord {
+    int x, y, z;
+};
+
+__device__ __forceinline__ CellCoord cell_coord(uint32_t p, const Cells& c) {
+    const uint32_t row = fast_div(p, c.by_nx);
+    const uint32_t z = fast_div(row, c.by_ny);
+    return CellCoord{static_cast<int>(p - row * c.nx), static_cast<int>(row - z * c.ny), static_cast<int>(z)};
+}
+
 struct LaunchShape {
-    dim3 grid, block;
+    dim3 grid, block;  // 1D: ceil(cells / threads) blocks of `threads` work-items
     int nx, ny, nz;
-    unsigned threads() const { return block.x * block.y * block.z; }
+    Cells cells;
+    unsigned threads() const { return block.x; }
 };
 
 // Kernels are compiled twice, for blocks of up to 256 work-items (the
 // register budget the library tunes for) and up to 1024 (the largest block
-// HIP accepts, which the reference's launcher also accepts).
+// HIP accepts, which the reference's launcher also accepts).  The block shape
+// the caller passes sets the work-items per block (bx * by * bz); the cells
+// are covered linearly.
 
 // Validation order of src/lattice_cuda.cu:488-503: pointers, block, 2D
 // normalisation, extents.  Then this library's own limits.
@@ -33,18 +75,21 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
     if (nx == 0 || ny == 0 || nz == 0) return hipErrorInvalidValue;
     if (static_cast<uint64_t>(bx) * by * bz > 1024) return hipErrorInvalidConfiguration;
     // Byte offsets inside one population plane are 32-bit (the global-load
-    // saddr form), so a plane holds at most 2^29 cells (4 GiB of doubles).
-    // (each factor and partial product checked first, so nothing overflows)
+    // saddr form), so a plane holds at most 2^29 cells (4 GiB of doubles);
+    // each factor and partial product is checked first, so nothing overflows.
     constexpr uint64_t kMaxCells = 1ull << 29;
     if (nx > kMaxCells || ny > kMaxCells || nz > kMaxCells) return hipErrorInvalidValue;
     if (nx * ny > kMaxCells || nx * ny * nz > kMaxCells) return hipErrorInvalidValue;
-    const uint64_t gx = (nx + bx - 1) / bx, gy = (ny + by - 1) / by, gz = (nz + bz - 1) / bz;
-    if (gx > 0x7fffffffull || gy > 65535 || gz > 65535) return hipErrorInvalidConfiguration;
-    s.grid = dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy), static_cast<unsigned>(gz));
-    s.block = dim3(bx, by, bz);
+    const uint32_t threads = bx * by * bz;
+    const uint64_t count = nx * ny * nz;
+    s.block = dim3(threads, 1, 1);
+    s.grid = dim3(static_cast<unsigned>((count + threads - 1) / threads), 1, 1);
     s.nx = static_cast<int>(nx);
     s.ny = static_cast<int>(ny);
     s.nz = static_cast<int>(nz);
+    s.cells = Cells{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz),
+                    static_cast<uint32_t>(count), make_fastdiv(static_cast<uint32_t>(nx)),
+                    make_fastdiv(static_cast<uint32_t>(ny))};
     return hipSuccess;
 }
 

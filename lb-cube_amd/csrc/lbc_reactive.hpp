@@ -2,6 +2,7 @@
 // single-domain and the ensemble kernels).
 #pragma once
 
+#include "lbc_fast.hpp"
 #include "lbc_physics.hpp"
 #include "lbc_storage.hpp"
 
@@ -57,6 +58,62 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     const R source = reaction_source<R>(ca, cb, k_react);
     collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, source);
     collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));
+        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(0));
+    }
+}
+
+// The same step in the deviation form (LBC_FLAG_FAST; BGK and TRT): fluid
+// populations carried as f_i - w_i with FMAs and a hardware reciprocal
+// (lbc_fast.hpp), the velocity taken from the deviations as stored, the scalar
+// collision with 1/cs2 multiplied, the reaction with hardware exp and rcp.
+template <class F, int CT, class S, int ST>
+__device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S* __restrict__ cur_f,
+                                                   typename Storage<ST>::S* __restrict__ nxt_f,
+                                                   const typename Storage<ST>::S* __restrict__ cur_a,
+                                                   typename Storage<ST>::S* __restrict__ nxt_a,
+                                                   const typename Storage<ST>::S* __restrict__ cur_b,
+                                                   typename Storage<ST>::S* __restrict__ nxt_b, int x, int y, int z,
+                                                   int nx, int ny, int nz, typename Storage<ST>::R omega,
+                                                   typename Storage<ST>::R omega_minus,
+                                                   typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react) {
+    static_assert(CT == kBGK || CT == kTRT, "the deviation form covers BGK and TRT");
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
+                                             static_cast<uint32_t>(nz));
+    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
+    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    R d[F::Q];
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(F::cz[i])) * ny + ay.pull(F::cy[i])) * nx +
+                             ax.pull(F::cx[i]);
+        d[i] = St::load_dev(at(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
+    }
+    if constexpr (CT == kBGK) collide_bgk_dev<F, R>(d, omega);
+    else collide_trt_dev<F, R>(d, omega, omega_minus);
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        St::store_dev(at(plane_ptr(nxt_f, i, plane), cell), d[i], R(F::w[i]));
+        d[i] = St::roundtrip_dev(d[i], R(F::w[i]));
+    }
+    const DevMacro<F, R> m = dev_macro<F, R>(d);
+    R ga[S::Q], gb[S::Q];
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
+                             ax.pull(S::cx[i]);
+        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
+        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
+    }
+    const R ca = concentration<S, R>(ga);
+    const R cb = concentration<S, R>(gb);
+    const R source = reaction_source_fast<R>(ca, cb, k_react);
+    collide_scalar_fast<S, R>(ga, ca, m.ux, m.uy, m.uz, omega_c, source);
+    collide_scalar_fast<S, R>(gb, cb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));

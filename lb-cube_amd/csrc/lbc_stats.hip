@@ -269,15 +269,14 @@ extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* s
 }
 
 extern "C" void lbc_default_block(int lattice, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t* block3) {
+    (void)lattice;
+    (void)nx;
     (void)ny;
     (void)nz;
-    (void)lattice;
-    // Whole rows along x for 256 work-items per block: x-shifted loads of a
-    // wave then stay inside lines the same block reads anyway.
-    uint32_t bx = 256;
-    if (nx < 256) bx = nx <= 64 ? 64 : (nx <= 128 ? 128 : 256);
-    block3[0] = bx;
-    block3[1] = 256 / bx;
+    // 256 work-items (4 waves) per block; cells are covered linearly, so the
+    // shape of the block does not matter, only its size.
+    block3[0] = 256;
+    block3[1] = 1;
     block3[2] = 1;
 }
 

@@ -30,6 +30,8 @@ template <> struct Storage<kF32> {
     __device__ static R load_dev(const S* p, R w) { return *p - w; }
     __device__ static void store_dev(S* p, R d, R w) { put(p, static_cast<S>(w + d)); }
     __device__ static R roundtrip(R v, R) { return v; }
+    // the deviation a later load_dev returns after store_dev(d, w)
+    __device__ static R roundtrip_dev(R d, R w) { return (w + d) - w; }
 };
 
 template <> struct Storage<kF64> {
@@ -40,6 +42,7 @@ template <> struct Storage<kF64> {
     __device__ static R load_dev(const S* p, R w) { return *p - w; }
     __device__ static void store_dev(S* p, R d, R w) { put(p, static_cast<S>(w + d)); }
     __device__ static R roundtrip(R v, R) { return v; }
+    __device__ static R roundtrip_dev(R d, R w) { return (w + d) - w; }
 };
 
 template <> struct Storage<kF16> {
@@ -54,6 +57,7 @@ template <> struct Storage<kF16> {
     // fluid deviations are stored as they are
     __device__ static R load_dev(const S* p, R) { return static_cast<float>(__builtin_bit_cast(_Float16, *p)); }
     __device__ static void store_dev(S* p, R d, R) { put(p, __builtin_bit_cast(uint16_t, static_cast<_Float16>(d))); }
+    __device__ static R roundtrip_dev(R d, R) { return static_cast<float>(static_cast<_Float16>(d)); }
 };
 
 template <> struct Storage<kBF16> {
@@ -74,6 +78,9 @@ template <> struct Storage<kBF16> {
     }
     __device__ static R load_dev(const S* p, R) { return __builtin_bit_cast(float, static_cast<uint32_t>(*p) << 16); }
     __device__ static void store_dev(S* p, R d, R) { put(p, encode(d)); }
+    __device__ static R roundtrip_dev(R d, R) {
+        return __builtin_bit_cast(float, static_cast<uint32_t>(encode(d)) << 16);
+    }
 };
 
 // Base of population plane i: a wave-uniform pointer, so the per-lane part of

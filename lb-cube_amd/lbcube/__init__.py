@@ -123,19 +123,20 @@ def reaction_step(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur
 
 
 def reactive_fused_step(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, omega,
-                        omega_c, k_react, block=None, stream=None):
-    """Fluid step + A + B -> C step in one pass (same results as the two calls)."""
+                        omega_c, k_react, block=None, stream=None, fast=False):
+    """Fluid step + A + B -> C step in one pass (same results as the two calls);
+    fast=True selects the deviation-form arithmetic (LBC_FLAG_FAST)."""
     _check_pop(fluid_lattice, cur_f, "cur_f")
     _check_pop(fluid_lattice, nxt_f, "nxt_f")
     for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
         _check_pop(scalar_lattice, t, n)
     nx, ny, nz = _extents(scalar_lattice, cur_a)
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
-    check(lib().lbc_reactive_fused_step(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
-                                        STORAGE[cur_f.dtype], cur_f.data_ptr(), nxt_f.data_ptr(), cur_a.data_ptr(),
-                                        nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(), nx, ny, nz,
-                                        float(omega), float(omega_c), float(k_react), bx, by, bz, _stream(stream)),
-          "lbc_reactive_fused_step")
+    check(lib().lbc_reactive_fused_step_ex(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
+                                           STORAGE[cur_f.dtype], 1 if fast else 0, cur_f.data_ptr(), nxt_f.data_ptr(),
+                                           cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(), nx,
+                                           ny, nz, float(omega), float(omega_c), float(k_react), bx, by, bz,
+                                           _stream(stream)), "lbc_reactive_fused_step_ex")
 
 
 def fluid_diagnostics(lattice, f, stream=None):

@@ -30,6 +30,8 @@ SIGNATURES = {
                               c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_fused_step": ([c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64,
                                  c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_reactive_fused_step_ex": ([c_int, c_int, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64,
+                                    c_u64, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, _dp, c_vp], c_int),
     "lbc_reactive_gradient_workspace": ([c_int, c_u64, c_u64, c_int, c_int, c_u32, c_u32,
@@ -40,6 +42,7 @@ SIGNATURES = {
     "lbc_ensemble_layout": ([c_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), c_vp, c_vp],
                             c_int),
     "lbc_ensemble_set_tiling": ([c_vp, c_u32, c_u32], c_int),
+    "lbc_ensemble_set_flags": ([c_vp, c_u32], c_int),
     "lbc_ensemble_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbc_ensemble_run": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),
     "lbc_ensemble_autotune": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, _u32p, ctypes.POINTER(ctypes.c_float),

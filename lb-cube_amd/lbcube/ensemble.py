@@ -19,7 +19,7 @@ _STORE = {"f32": 0, "f64": 1, "f16": 2, "bf16": 3}
 
 
 class Ensemble:
-    def __init__(self, collision, storage, sizes, omega, omega_c, k_react, device="cuda"):
+    def __init__(self, collision, storage, sizes, omega, omega_c, k_react, device="cuda", fast=False):
         self.collision, self.storage = collision, storage
         self.sizes = [(int(nx), int(ny)) for nx, ny in sizes]
         n = len(self.sizes)
@@ -42,6 +42,12 @@ class Ensemble:
         self.B = [torch.zeros(se.value, dtype=dt, device=device) for _ in range(2)]
         self.level = 0
         self.cells = sum(x * y for x, y in self.sizes)
+        self.set_fast(fast)
+
+    def set_fast(self, fast):
+        """Deviation-form arithmetic (LBC_FLAG_FAST) for every instance."""
+        check(lib().lbc_ensemble_set_flags(self.handle, 1 if fast else 0), "lbc_ensemble_set_flags")
+        self.fast = bool(fast)
 
     def fluid(self, i, level=None):
         nx, ny = self.sizes[i]

@@ -90,6 +90,7 @@ def test_validation_scalar_and_reaction():
                                      0.1, 16, 8, 1, None) == HIP_INVALID_VALUE
 
 
-def test_default_block_covers_rows():
+def test_default_block():
     assert lbcube.default_block("D3Q19", 256, 256, 256) == (256, 1, 1)
-    assert lbcube.default_block("D2Q9", 64, 64, 1) == (64, 4, 1)
+    assert lbcube.default_block("D2Q9", 64, 64, 1) == (256, 1, 1)
+    assert lbcube.default_block("D3Q27", 384, 384, 384) == (256, 1, 1)

@@ -38,27 +38,28 @@ def fill(ens, storage, seed):
         sb.copy_(tb)
 
 
-def standalone(ens, i, coll, omega, omega_c, k, steps):
+def standalone(ens, i, coll, omega, omega_c, k, steps, fast=False):
     f = ens.fluid(i).clone()
     a, b = (x.clone() for x in ens.species(i))
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
     for _ in range(steps):
-        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k)
+        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
     return f, a, b
 
 
+@pytest.mark.parametrize("fast", [False, True])
 @pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
 @pytest.mark.parametrize("coll", ["BGK", "TRT"])
-def test_instances_match_standalone_runs(gpu, coll, storage):
-    print(f"case ensemble {coll} {storage}", flush=True)
+def test_instances_match_standalone_runs(gpu, coll, storage, fast):
+    print(f"case ensemble {coll} {storage} fast={fast}", flush=True)
     n = len(SIZES)
     omega = [1 / (0.55 + 0.05 * i) for i in range(n)]
     omega_c = [1 / (0.7 + 0.03 * i) for i in range(n)]
     k = [0.01 * (i + 1) for i in range(n)]
-    ens = Ensemble(coll, storage, SIZES, omega, omega_c, k)
+    ens = Ensemble(coll, storage, SIZES, omega, omega_c, k, fast=fast)
     fill(ens, storage, seed=21)
-    want = [standalone(ens, i, coll, omega[i], omega_c[i], k[i], 6) for i in range(n)]
+    want = [standalone(ens, i, coll, omega[i], omega_c[i], k[i], 6, fast=fast) for i in range(n)]
     start = [(ens.fluid(i).clone(),) + tuple(x.clone() for x in ens.species(i)) for i in range(n)]
     for cap, threads in [(64, 256), (128, 256), (256, 256), (128, 512), (512, 512)]:
         for graph in (False, True):
@@ -80,7 +81,7 @@ def test_instances_match_standalone_runs(gpu, coll, storage):
 
 def test_autotune_and_graph_replay(gpu):
     sizes = [(int(s), int(s)) for s in (64, 96, 200, 333, 512)]
-    ens = Ensemble("TRT", "f16", sizes, [1.8] * 5, [1.8] * 5, [0.01] * 5)
+    ens = Ensemble("TRT", "f16", sizes, [1.8] * 5, [1.8] * 5, [0.01] * 5, fast=True)
     fill(ens, "f16", seed=3)
     chosen, timings = ens.autotune(trial_steps=2)
     assert chosen[0] in (64, 128, 256, 512) and chosen[1] in (256, 512)

@@ -70,6 +70,18 @@ def test_block_shape_does_not_change_results(gpu, block):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("lattice,shape", [("D3Q19", (3, 5, 263)), ("D2Q9", (70001, 3)), ("D3Q27", (2, 3, 1000)),
+                                           ("D2Q9", (2, 4099))])
+def test_linear_cell_mapping_odd_extents(gpu, lattice, shape):
+    """Cells are numbered linearly across row ends: extents that are not
+    multiples of the block, rows shorter than a wave, ny above 65535."""
+    print(f"case linear {lattice} {shape}", flush=True)
+    f0 = cases.random_perturbed(lattice, shape, "f32", seed=17)
+    want = O.fluid_run(lattice, "TRT", "f32", f0, 1.6, 2)
+    for block in (None, (16, 8, 1), (1024, 1, 1)):
+        assert np.array_equal(gpu_run(lattice, "TRT", "f32", f0, 1.6, 2, gpu, block=block), want), block
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 @pytest.mark.parametrize("lattice", FLUIDS)
 def test_streaming_permutation_kat(gpu, lattice, storage):
