@@ -123,7 +123,8 @@ def _time(torch, fn, reps=1):
 
 
 def adjoint_ratio(torch, n=512, steps=4096, interval=64):
-    """Differentiated 2D reactive run (D2Q9 BGK + two D2Q5, A+B->C), FP32:
+    """Differentiated 2D reactive run (D2Q9 BGK + two D2Q5, A+B->C), FP32 with
+    the fast (deviation-form, cancellation-free reaction) forward:
     time of the forward loop vs time of the full gradient (forward with
     checkpoints, segment recompute, adjoint sweep), per step.  The adjoint
     time is the gradient time minus one forward run (recompute included)."""
@@ -141,7 +142,7 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
         for t in range(64):
             c, x = (0, 1) if t % 2 == 0 else (1, 0)
             lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", st[c], st[x], st[2 + c], st[2 + x], st[4 + c],
-                                       st[4 + x], omega, omega_c, k)
+                                       st[4 + x], omega, omega_c, k, fast=True)
 
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -156,13 +157,14 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
     block = lbcube.default_block("D2Q9", n, n)
     ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block), dtype=torch.uint8,
                      device="cuda")
-    ADJ.reactive_gradient("BGK", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws)
+    ADJ.reactive_gradient("BGK", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws, fast=True)
     out = {}
     t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("BGK", f0, a0, b0, steps, omega, omega_c, k,
-                                                                     interval, workspace=ws)))
+                                                                     interval, workspace=ws, fast=True)))
     loss = out["r"][0]
     del graph, st
-    return {"workload": f"D2Q9 BGK + 2xD2Q5 A+B->C, {n}^2, {steps} steps, FP32, checkpoint every {interval}",
+    return {"workload": f"D2Q9 BGK + 2xD2Q5 A+B->C, {n}^2, {steps} steps, FP32 (fast forward), checkpoint every "
+                        f"{interval}",
             "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
             "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
             "loss": loss}

@@ -102,8 +102,11 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
  * LBC_FLAG_FAST (BGK and TRT; ignored by MRT and RLBM): the fluid part in the
  * deviation form of lbc_fluid_step_ex, the velocity taken from the stored
  * deviations, the scalar collision with 1/cs2 multiplied and FMAs, and the
- * reaction increment with a hardware exponential and reciprocal.  Results
- * differ from the reference order by rounding only (bounded by the tests).
+ * reaction increment of the same exact one-step map written without the
+ * cancellation of the reference's formula at A ~ B
+ * (S = -A N / (1 + N), N = k A expm1(z) / z + expm1(z), z = k (B - A)).
+ * Results differ from the reference order by rounding only, and near A = B
+ * the fast form is the more accurate one (bounded by the tests).
  * Unknown flag bits -> hipErrorInvalidValue.  No counterpart in the reference. */
 int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_lattice, int storage, uint32_t flags,
                                const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
@@ -117,14 +120,18 @@ int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_latt
  * *loss, the adjoints dloss/d(f0, a0, b0) of the initial populations (same
  * layout and type as the inputs) and dloss/d(omega_c).  States are
  * checkpointed every `interval` steps and each segment is recomputed before
- * it is walked backward.  storage: LBC_F32 or LBC_F64.  The caller passes a
+ * it is walked backward.  storage: LBC_F32 or LBC_F64.  flags: 0 (forward =
+ * lbc_reactive_fused_step) or LBC_FLAG_FAST (forward =
+ * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...)); the adjoint sweep is
+ * the same for both.  The caller passes a
  * device workspace of at least lbc_reactive_gradient_workspace() bytes;
  * nothing is allocated.  block_x * block_y must be a multiple of 64 (whole
  * wavefronts), else hipErrorInvalidConfiguration.  Synchronizes `stream`.
  * No counterpart in the reference. */
 int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
                                     uint32_t block_x, uint32_t block_y, uint64_t* bytes);
-int lbc_reactive_gradient(int collision, int storage, const void* f0, const void* a0, const void* b0, uint64_t nx,
+int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void* f0, const void* a0, const void* b0,
+                          uint64_t nx,
                           uint64_t ny, int steps, int interval, double omega, double omega_c, double k_react,
                           double* loss, void* grad_f0, void* grad_a0, void* grad_b0, double* dloss_domega_c,
                           void* workspace, uint64_t workspace_bytes, uint32_t block_x, uint32_t block_y,

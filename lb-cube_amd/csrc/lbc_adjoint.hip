@@ -23,6 +23,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "../../include/lbcube.h"
 #include "lbc_launch.hpp"
@@ -273,7 +274,8 @@ template <class R> struct State {
 };
 
 template <class R, int CT>
-hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape& s, int steps, int interval,
+hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, const LaunchShape& s, int steps,
+                        int interval,
                         double omega_d, double omega_c_d, double k_d, double* J_out, R* gf, R* ga, R* gb,
                         double* djdwc_out, char* ws, hipStream_t st) {
     const int storage = std::is_same<R, double>::value ? kF64 : kF32;
@@ -289,9 +291,9 @@ hipError_t run_gradient(const R* f0, const R* a0, const R* b0, const LaunchShape
     const R wm = CT == kTRT ? trt_omega_minus<R>(omega) : R(0);
     hipError_t e;
     auto step = [&](State<R> in, State<R> out) {
-        return static_cast<hipError_t>(lbc_reactive_fused_step(kD2Q9, CT, kD2Q5, storage, in.f, out.f, in.a, out.a,
-                                                               in.b, out.b, s.nx, s.ny, 1, omega_d, omega_c_d, k_d,
-                                                               s.block.x, s.block.y, 1, st));
+        return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, flags, in.f, out.f, in.a,
+                                                                  out.a, in.b, out.b, s.nx, s.ny, 1, omega_d,
+                                                                  omega_c_d, k_d, s.block.x, s.block.y, 1, st));
     };
     // forward with checkpoints at the segment starts
     if ((e = hipMemcpyAsync(ckpt, f0, 9 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
@@ -377,8 +379,8 @@ extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_
     return hipSuccess;
 }
 
-extern "C" int lbc_reactive_gradient(int collision, int storage, const void* f0, const void* a0, const void* b0,
-                                     uint64_t nx, uint64_t ny, int steps, int interval, double omega, double omega_c,
+extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void* f0, const void* a0,
+                                     const void* b0, uint64_t nx, uint64_t ny, int steps, int interval, double omega, double omega_c,
                                      double k_react, double* loss, void* grad_f0, void* grad_a0, void* grad_b0,
                                      double* dloss_domega_c, void* workspace, uint64_t workspace_bytes,
                                      uint32_t block_x, uint32_t block_y, void* stream) {
@@ -393,25 +395,21 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, const void* f0,
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
     if (collision != kBGK && collision != kTRT) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
     const GradLayout L = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid);
     if (workspace_bytes < L.total) return hipErrorInvalidValue;
     char* ws = static_cast<char*>(workspace);
     const hipStream_t st = as_stream(stream);
-    if (storage == kF64) {
-        auto P = [](const void* p) { return static_cast<const double*>(p); };
-        auto Q = [](void* p) { return static_cast<double*>(p); };
-        return collision == kBGK
-                   ? run_gradient<double, kBGK>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react,
-                                                loss, Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st)
-                   : run_gradient<double, kTRT>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react,
-                                                loss, Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st);
-    }
-    auto P = [](const void* p) { return static_cast<const float*>(p); };
-    auto Q = [](void* p) { return static_cast<float*>(p); };
-    return collision == kBGK
-               ? run_gradient<float, kBGK>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react, loss,
-                                           Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st)
-               : run_gradient<float, kTRT>(P(f0), P(a0), P(b0), s, steps, interval, omega, omega_c, k_react, loss,
-                                           Q(grad_f0), Q(grad_a0), Q(grad_b0), dloss_domega_c, ws, st);
+    auto run = [&](auto tag, auto ct) {
+        using R = decltype(tag);
+        return run_gradient<R, decltype(ct)::value>(
+            flags, static_cast<const R*>(f0), static_cast<const R*>(a0), static_cast<const R*>(b0), s, steps, interval,
+            omega, omega_c, k_react, loss, static_cast<R*>(grad_f0), static_cast<R*>(grad_a0),
+            static_cast<R*>(grad_b0), dloss_domega_c, ws, st);
+    };
+    using BGK = std::integral_constant<int, kBGK>;
+    using TRT = std::integral_constant<int, kTRT>;
+    if (storage == kF64) return collision == kBGK ? run(0.0, BGK{}) : run(0.0, TRT{});
+    return collision == kBGK ? run(0.0f, BGK{}) : run(0.0f, TRT{});
 }

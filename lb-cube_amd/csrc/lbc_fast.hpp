@@ -112,10 +112,7 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
     }
 }
 
-// ---- scalar ADR and A + B -> C in the same style (the fast fused reactive step)
-
-__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
-__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
+// ---- scalar ADR and A + B -> C for the fast fused reactive step
 
 // g_i += omega_c (w_i C (1 + c.u / cs2) - g_i) + w_i S with 1/cs2 (3 for D2Q5,
 // 4 for D3Q7, exact) multiplied instead of divided, and FMAs.
@@ -131,19 +128,34 @@ __device__ __forceinline__ void collide_scalar_fast(R* g, R conc, R ux, R uy, R 
     }
 }
 
-// The exact one-step increment of dA/dt = dB/dt = -k A B (same branches as
-// reaction_source) with a hardware exponential and reciprocal.
+template <class R> struct SeriesLimit;
+template <> struct SeriesLimit<float> { static constexpr float v = 0.0625f; };
+template <> struct SeriesLimit<double> { static constexpr double v = 1e-3; };
+
+// The exact one-step increment of dA/dt = dB/dt = -k A B, the quantity of
+// reaction_source, in a form without cancellation.  With z = k (B - A),
+// phi(z) = expm1(z) / z and N = k A phi(z) + expm1(z):
+//   A' = A B e^{-z} / (B - A e^{-z}) = A / (1 + N),   S = A' - A = -A N / (1 + N).
+// This is one expression for both of the reference's branches (z = 0 gives
+// A / (1 + k A)), and it loses no digits at A ~ B, where the reference's
+// B - A e^{-z} cancels (about eps A / |B - A| relative error).  phi is a
+// degree-5 Taylor polynomial below SeriesLimit (truncation < z^6 / 5040:
+// 1.2e-11 for float, 2e-22 for double), else expm1(z) / z.
 template <class R> __device__ __forceinline__ R reaction_source_fast(R a, R b, R k) {
-    const R diff = b - a;
-    const R tol = R(64.0) * Eps<R>::v * (R(1.0) + fabs(a) + fabs(b));
-    R after;
-    if (fabs(diff) <= tol) {
-        after = a * fast_rcp(fmadd(k, a, R(1.0)));
+    const R z = k * (b - a);
+    R phi, em1;
+    if (fabs(z) < SeriesLimit<R>::v) {
+        phi = fmadd(z, fmadd(z, fmadd(z, fmadd(z, fmadd(z, R(1.0 / 720.0), R(1.0 / 120.0)), R(1.0 / 24.0)),
+                                      R(1.0 / 6.0)),
+                             R(0.5)),
+                    R(1.0));
+        em1 = z * phi;
     } else {
-        const R att = fast_exp(-k * diff);
-        after = (a * diff * att) * fast_rcp(fmadd(-a, att, b));
+        em1 = expm1(z);
+        phi = em1 / z;
     }
-    return after - a;
+    const R num = fmadd(k * a, phi, em1);
+    return -a * num * fast_rcp(R(1.0) + num);
 }
 
 }  // namespace lbc

@@ -30,8 +30,9 @@ def workspace_bytes(dtype, nx, ny, steps, interval, block):
 
 
 def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, interval=64, block=None,
-                      workspace=None, stream=None):
-    """(loss, dloss/df0, dloss/da0, dloss/db0, dloss/domega_c) by the discrete adjoint."""
+                      workspace=None, stream=None, fast=False):
+    """(loss, dloss/df0, dloss/da0, dloss/db0, dloss/domega_c) by the discrete adjoint;
+    fast=True differentiates the forward of reactive_fused_step(..., fast=True)."""
     _check_pop("D2Q9", f0, "f0")
     _check_pop("D2Q5", a0, "a0")
     _check_pop("D2Q5", b0, "b0")
@@ -44,7 +45,8 @@ def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, int
         workspace = torch.empty(nbytes, dtype=torch.uint8, device=f0.device)
     gf, ga, gb = torch.empty_like(f0), torch.empty_like(a0), torch.empty_like(b0)
     loss, dw = ctypes.c_double(), ctypes.c_double()
-    check(lib().lbc_reactive_gradient(_COLL[collision], _STORE[f0.dtype], f0.data_ptr(), a0.data_ptr(), b0.data_ptr(),
+    check(lib().lbc_reactive_gradient(_COLL[collision], _STORE[f0.dtype], 1 if fast else 0, f0.data_ptr(),
+                                      a0.data_ptr(), b0.data_ptr(),
                                       nx, ny, int(steps), int(interval), float(omega), float(omega_c),
                                       float(k_react), ctypes.byref(loss), gf.data_ptr(), ga.data_ptr(),
                                       gb.data_ptr(), ctypes.byref(dw), workspace.data_ptr(), workspace.numel(),
@@ -52,19 +54,20 @@ def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, int
     return loss.value, gf, ga, gb, dw.value
 
 
-def reactive_run(collision, f0, a0, b0, steps, omega, omega_c, k_react, block=None):
+def reactive_run(collision, f0, a0, b0, steps, omega, omega_c, k_react, block=None, fast=False):
     """Forward only: returns the final (f, a, b) (inputs untouched)."""
     f, a, b = f0.clone(), a0.clone(), b0.clone()
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
     for _ in range(steps):
-        reactive_fused_step("D2Q9", collision, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k_react, block=block)
+        reactive_fused_step("D2Q9", collision, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k_react, block=block,
+                            fast=fast)
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
     return f, a, b
 
 
-def loss_of(collision, f0, a0, b0, steps, omega, omega_c, k_react):
+def loss_of(collision, f0, a0, b0, steps, omega, omega_c, k_react, fast=False):
     """<A>(0) - <A>(T) from a forward run, accumulated in float64."""
-    _, a, _ = reactive_run(collision, f0, a0, b0, steps, omega, omega_c, k_react)
+    _, a, _ = reactive_run(collision, f0, a0, b0, steps, omega, omega_c, k_react, fast=fast)
     n = a0.shape[1] * a0.shape[2]
     return (a0.double().sum() - a.double().sum()).item() / n
 
@@ -73,10 +76,11 @@ class ReactiveLoss(torch.autograd.Function):
     """loss = <C>(T) of the reactive loop; differentiable in (f0, a0, b0, tau_s)."""
 
     @staticmethod
-    def forward(ctx, f0, a0, b0, tau_s, collision, steps, omega, k_react, interval):
+    def forward(ctx, f0, a0, b0, tau_s, collision, steps, omega, k_react, interval, fast):
         omega_c = 1.0 / float(tau_s)
         loss, gf, ga, gb, dw = reactive_gradient(collision, f0.detach().contiguous(), a0.detach().contiguous(),
-                                                 b0.detach().contiguous(), steps, omega, omega_c, k_react, interval)
+                                                 b0.detach().contiguous(), steps, omega, omega_c, k_react, interval,
+                                                 fast=fast)
         dtau = -dw * omega_c * omega_c
         ctx.save_for_backward(gf, ga, gb, torch.tensor(dtau, dtype=torch.float64, device=f0.device))
         return torch.tensor(loss, dtype=f0.dtype, device=f0.device)
@@ -84,12 +88,14 @@ class ReactiveLoss(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         gf, ga, gb, dtau = ctx.saved_tensors
-        return grad * gf, grad * ga, grad * gb, (grad.double() * dtau).to(grad.dtype), None, None, None, None, None
+        return (grad * gf, grad * ga, grad * gb, (grad.double() * dtau).to(grad.dtype), None, None, None, None, None,
+                None)
 
 
-def reactive_loss(f0, a0, b0, tau_s, collision="BGK", steps=4096, omega=1.0 / 0.6, k_react=0.05, interval=64):
+def reactive_loss(f0, a0, b0, tau_s, collision="BGK", steps=4096, omega=1.0 / 0.6, k_react=0.05, interval=64,
+                  fast=False):
     tau = tau_s if torch.is_tensor(tau_s) else torch.tensor(float(tau_s), dtype=torch.float64, device=f0.device)
-    return ReactiveLoss.apply(f0, a0, b0, tau, collision, steps, omega, k_react, interval)
+    return ReactiveLoss.apply(f0, a0, b0, tau, collision, steps, omega, k_react, interval, fast)
 
 
 def equilibrium_torch(rho, u):

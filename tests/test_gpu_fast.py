@@ -122,20 +122,36 @@ def test_reduced_storage_error_against_fp64(gpu, lattice, n):
 
 # Fast fused reactive step against the oracle's reference order after 4 steps:
 # fluid populations absolutely (FAST_TOL), species relative to max(|g|, 1e-3).
+# The species differ in A - B by 0.1 or more, or not at all: there the
+# reference's reaction formula is well conditioned (near A = B it is not, and
+# the fast form is the accurate one: test_reaction_increment_near_equal).
 REACT_FAST_TOL = {"f64": 1e-13, "f32": 5e-6, "f16": 2e-3, "bf16": 1.6e-2}
+
+
+def separated_species(sl, shape, storage, seed):
+    g = cases.rng(seed)
+    d = O.D_OF[sl]
+    ca = g.uniform(0.0, 0.35, size=shape)
+    ca = np.where(g.uniform(size=shape) < 0.5, ca, 1.0 - ca)
+    cb = 1.0 - ca + g.uniform(-0.05, 0.05, size=shape)
+    cb[..., 0] = ca[..., 0]  # exactly equal: the z = 0 case
+    u = g.uniform(-0.02, 0.02, size=(d,) + tuple(shape))
+    dt = storage if storage in ("f32", "f64") else "f64"
+    fa, fb = O.scalar_feq_field(sl, dt, ca, u), O.scalar_feq_field(sl, dt, cb, u)
+    if storage in ("f16", "bf16"):
+        fa, fb = cases.encode(sl, storage, fa), cases.encode(sl, storage, fb)
+    return fa, fb
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
 @pytest.mark.parametrize("coll", ["BGK", "TRT"])
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
 def test_reactive_fast_against_oracle(gpu, pair, coll, storage):
-    from test_gpu_parity import _species
-
     fl, sl = pair
     print(f"case reactive fast {fl} {sl} {coll} {storage}", flush=True)
     shape = SHAPE[fl]
     f0 = cases.random_perturbed(fl, shape, storage, seed=2)
-    a0, b0 = _species(sl, shape, storage, seed=8)
+    a0, b0 = separated_species(sl, shape, storage, seed=8)
     omega, omega_c, k, steps = 1.3, 1.05, 0.02, 4
     wf, wa, wb = O.reactive_run(fl, coll, sl, storage, f0, a0, b0, omega, omega_c, k, steps)
     f, a, b = (to_dev(x, storage, gpu) for x in (f0, a0, b0))
@@ -150,6 +166,55 @@ def test_reactive_fast_against_oracle(gpu, pair, coll, storage):
         g, w = O.decode(sl, storage, to_host(got, storage)), O.decode(sl, storage, want)
         err = np.max(np.abs(g - w) / np.maximum(np.abs(w), 1e-3))
         assert err <= REACT_FAST_TOL[storage], err
+
+
+def exact_increment(a, b, k):
+    """A' - A of the reference's formula (lattice_physics.hpp reaction source)
+    evaluated in 60-digit decimal arithmetic."""
+    import decimal
+
+    ctx = decimal.Context(prec=60)
+    A, B, K = (decimal.Decimal(float(x)) for x in (a, b, k))
+    diff = B - A
+    if diff == 0:
+        after = ctx.divide(A, 1 + K * A)
+    else:
+        att = ctx.exp(-K * diff)
+        after = ctx.divide(A * diff * att, B - A * att)
+    return float(after - A)
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_reaction_increment_near_equal(gpu, storage):
+    """At rest (u = 0, f = w) with A = 1/2 and B = A + delta, delta from 1e-7
+    to 1e-1 along x, one fast step leaves C' = C + S in every cell: the
+    increment S against the 60-digit value.  The reference order is printed
+    beside it (its B - A e^{-z} cancels as delta -> 0)."""
+    nx, k = 64, 0.05
+    delta = np.concatenate([[0.0], np.logspace(-7, -1, nx - 1)])
+    ca = np.full((1, nx), 0.5)
+    cb = ca + delta
+    dt = torch.float64 if storage == "f64" else torch.float32
+    w = torch.tensor(lbcube.WEIGHTS["D2Q9"], dtype=torch.float64).view(9, 1, 1)
+    f = (w * torch.ones((1, 1, nx), dtype=torch.float64)).to(dt).to(gpu)
+    ws = torch.tensor(lbcube.WEIGHTS["D2Q5"], dtype=torch.float64).view(5, 1, 1)
+    a = (ws * torch.from_numpy(ca)).to(dt).to(gpu)
+    b = (ws * torch.from_numpy(cb)).to(dt).to(gpu)
+    c0 = a.double().sum(0).cpu().numpy()[0]
+    truth = np.array([exact_increment(x, y, k) for x, y in zip(a.double().sum(0).cpu().numpy()[0],
+                                                                 b.double().sum(0).cpu().numpy()[0])])
+    res = {}
+    for fast in (True, False):
+        out = [torch.empty_like(x) for x in (f, a, b)]
+        lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", f, out[0], a, out[1], b, out[2], 1.0, 1.0, k, fast=fast)
+        torch.cuda.synchronize()
+        s = out[1].double().sum(0).cpu().numpy()[0] - c0
+        res[fast] = np.abs(s - truth) / np.abs(truth)
+    # extracting S from C' - C costs eps(C) / |S| ~ 1e-5 in FP32
+    tol = 1e-12 if storage == "f64" else 2e-5
+    print(f"{storage}: worst relative error of S: fast {res[True].max():.2e}, reference order "
+          f"{res[False].max():.2e}", flush=True)
+    assert res[True].max() <= tol
 
 
 def test_reactive_fast_rejects_unknown_flags(gpu):
@@ -187,7 +252,9 @@ def test_reactive_reduced_storage_error_against_fp64(gpu):
     segregation intensity of each storage/arithmetic against FP64."""
     n, steps = 512, 2000
     ref = reactive_mixing(n, "f64", False, steps)
-    for storage, fast, tol in (("f32", False, 1e-5), ("f32", True, 1e-5), ("f16", True, 2e-3), ("bf16", True, None)):
+    # FP32 in the reference order inherits the reaction formula's cancellation
+    # at A ~ B (a few 1e-4 in mean A here); the fast form does not.
+    for storage, fast, tol in (("f32", False, 1e-3), ("f32", True, 1e-5), ("f16", True, 2e-3), ("bf16", True, None)):
         got = reactive_mixing(n, storage, fast, steps)
         e_mean = float(np.max(np.abs(got[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])))
         e_seg = float(np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])))
