@@ -89,12 +89,12 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         const CellCoord q = cell_coord(cell, c);
         const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
         const size_t plane = c.count;
-        const Axis ax(q.x, nx), ay(q.y, ny);
+        const Pull pull(cell, q.x, q.y, 0, nx, ny, 1);
         R fl[9], fp[9], lf[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            const uint32_t src = static_cast<uint32_t>(ay.pull(D2Q9::cy[i])) * nx + ax.pull(D2Q9::cx[i]);
-            const uint32_t rev = static_cast<uint32_t>(ay.pull(-D2Q9::cy[i])) * nx + ax.pull(-D2Q9::cx[i]);
+            const uint32_t src = pull.src(D2Q9::cx[i], D2Q9::cy[i], 0);
+            const uint32_t rev = pull.src(-D2Q9::cx[i], -D2Q9::cy[i], 0);
             fl[i] = plane_ptr(f, i, plane)[src];
             lf[i] = plane_ptr(nuf1, i, plane)[rev];
             fp[i] = fl[i];
@@ -104,8 +104,8 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         R ga[5], gb[5], la[5], lb[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-            const uint32_t src = static_cast<uint32_t>(ay.pull(D2Q5::cy[j])) * nx + ax.pull(D2Q5::cx[j]);
-            const uint32_t rev = static_cast<uint32_t>(ay.pull(-D2Q5::cy[j])) * nx + ax.pull(-D2Q5::cx[j]);
+            const uint32_t src = pull.src(D2Q5::cx[j], D2Q5::cy[j], 0);
+            const uint32_t rev = pull.src(-D2Q5::cx[j], -D2Q5::cy[j], 0);
             ga[j] = plane_ptr(a, j, plane)[src];
             gb[j] = plane_ptr(b, j, plane)[src];
             la[j] = plane_ptr(nua1, j, plane)[rev];
@@ -196,10 +196,10 @@ __global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict_
     const CellCoord q = cell_coord(cell, c);
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
     const size_t plane = c.count;
-    const Axis ax(q.x, nx), ay(q.y, ny);
+    const Pull pull(cell, q.x, q.y, 0, nx, ny, 1);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        const uint32_t rev = static_cast<uint32_t>(ay.pull(-L::cy[i])) * nx + ax.pull(-L::cx[i]);
+        const uint32_t rev = pull.src(-L::cx[i], -L::cy[i], 0);
         lam[i * plane + cell] = plane_ptr(nu, i, plane)[rev] + shift;
     }
 }

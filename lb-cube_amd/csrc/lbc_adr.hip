@@ -40,12 +40,11 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     for (int i = 0; i < F::Q; ++i)
         fl[i] = St::load(at(plane_ptr(cur_f, i, plane), cell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     const Macro<R> m = macro_state<F, R>(fl);
-    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    const Pull pull(cell, x, y, z, nx, ny, nz);
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
-                             ax.pull(S::cx[i]);
+        const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
         ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
         if constexpr (REACT) gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
     }

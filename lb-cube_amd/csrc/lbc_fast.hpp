@@ -115,17 +115,26 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
 // ---- scalar ADR and A + B -> C for the fast fused reactive step
 
 // g_i += omega_c (w_i C (1 + c.u / cs2) - g_i) + w_i S with 1/cs2 (3 for D2Q5,
-// 4 for D3Q7, exact) multiplied instead of divided, and FMAs.
+// 4 for D3Q7, exact) multiplied instead of divided, and FMAs.  The moving
+// populations' equilibria and source shares are formed first and the rest
+// population takes the remainder (C - sum, S - sum): the D2Q5 weights 1/3 and
+// 1/6 rounded to binary sum to 1 + 3e-8 in FP32, which in the plain form adds
+// a systematic 3e-8 omega_c C of mass per step wherever g is off equilibrium.
 template <class S, class R>
 __device__ __forceinline__ void collide_scalar_fast(R* g, R conc, R ux, R uy, R uz, R omega_c, R source) {
     constexpr double inv_cs2 = S::cs2 == 0.25 ? 4.0 : 3.0;  // D3Q7 : D2Q5
+    R eq_moving = R(0), src_moving = R(0);
 #pragma unroll
-    for (int i = 0; i < S::Q; ++i) {
+    for (int i = 1; i < S::Q; ++i) {
         const R wc = R(S::w[i]) * conc;
         const R cu = dir_dot<S, R>(i, ux, uy, uz);
         const R eq = fmadd(wc, R(inv_cs2) * cu, wc);
-        g[i] = fmadd(omega_c, eq - g[i], fmadd(R(S::w[i]), source, g[i]));
+        const R src = R(S::w[i]) * source;
+        eq_moving += eq;
+        src_moving += src;
+        g[i] = fmadd(omega_c, eq - g[i], g[i] + src);
     }
+    g[0] = fmadd(omega_c, (conc - eq_moving) - g[0], g[0] + (source - src_moving));
 }
 
 template <class R> struct SeriesLimit;

@@ -32,13 +32,12 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     const CellCoord q = cell_coord(dst, c);
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
     const size_t plane = c.count;
-    const Axis ax(q.x, nx), ay(q.y, ny), az(q.z, nz);
+    const Pull pull(dst, q.x, q.y, q.z, nx, ny, nz);
     R f[L::Q];
     if constexpr (FAST) {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
-            const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
-                                 ax.pull(L::cx[i]);
+            const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             f[i] = St::load_dev(at(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
         if constexpr (CT == kBGK) collide_bgk_dev<L, R>(f, omega);
@@ -48,8 +47,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     } else {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
-            const uint32_t src = (static_cast<uint32_t>(az.pull(L::cz[i])) * ny + ay.pull(L::cy[i])) * nx +
-                                 ax.pull(L::cx[i]);
+            const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
             f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
         }

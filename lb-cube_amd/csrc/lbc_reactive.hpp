@@ -29,12 +29,11 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
                                              static_cast<uint32_t>(nz));
     const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
-    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    const Pull pull(cell, x, y, z, nx, ny, nz);
     R f[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(F::cz[i])) * ny + ay.pull(F::cy[i])) * nx +
-                             ax.pull(F::cx[i]);
+        const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
         f[i] = St::load(at(plane_ptr(cur_f, i, plane), src), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     }
     collide_cell<F, CT, R>(f, omega, omega_minus);
@@ -48,8 +47,7 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
-                             ax.pull(S::cx[i]);
+        const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
         ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
         gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
     }
@@ -85,12 +83,11 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
                                              static_cast<uint32_t>(nz));
     const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
-    const Axis ax(x, nx), ay(y, ny), az(z, nz);
+    const Pull pull(cell, x, y, z, nx, ny, nz);
     R d[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(F::cz[i])) * ny + ay.pull(F::cy[i])) * nx +
-                             ax.pull(F::cx[i]);
+        const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
         d[i] = St::load_dev(at(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
     }
     if constexpr (CT == kBGK) collide_bgk_dev<F, R>(d, omega);
@@ -104,8 +101,7 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = (static_cast<uint32_t>(az.pull(S::cz[i])) * ny + ay.pull(S::cy[i])) * nx +
-                             ax.pull(S::cx[i]);
+        const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
         ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
         gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
     }

@@ -100,13 +100,37 @@ template <class T> __device__ __forceinline__ T* at(T* plane_base, uint32_t cell
 
 template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
 
-// Periodic neighbour coordinates along one axis for pull streaming: the
-// source of a population with velocity component c at coordinate x is x - c
-// (include/lattice_core.hpp:70-78), so c = +1 reads x - 1 and c = -1 reads x + 1.
-struct Axis {
-    int lo, mid, hi;  // x - 1, x, x + 1 with periodic wrap
-    __device__ Axis(int x, int n) : lo(x == 0 ? n - 1 : x - 1), mid(x), hi(x + 1 == n ? 0 : x + 1) {}
-    __device__ int pull(int c) const { return c > 0 ? lo : (c < 0 ? hi : mid); }
+// Pull streaming with periodic wrap, as offsets from the destination cell: a
+// population with velocity c is read from x - c (include/lattice_core.hpp:70-78),
+// so c_x = +1 reads x - 1 (offset -1, or nx - 1 at x = 0) and c_x = -1 reads
+// x + 1 (offset +1, or 1 - nx at the last column); y and z likewise in units
+// of a row and of an xy-plane.  Six selects per cell, then each population's
+// source is the destination plus up to three of them (no multiplies).  The
+// sums are taken modulo 2^32, where every source lands back in [0, cells).
+struct Pull {
+    uint32_t dst;
+    uint32_t xm, xp, ym, yp, zm, zp;
+    __device__ __forceinline__ Pull(uint32_t cell, int x, int y, int z, int nx, int ny, int nz) : dst(cell) {
+        const uint32_t row = static_cast<uint32_t>(nx);
+        const uint32_t slab = row * static_cast<uint32_t>(ny);
+        xm = x == 0 ? row - 1u : ~0u;
+        xp = x + 1 == nx ? 1u - row : 1u;
+        ym = y == 0 ? slab - row : 0u - row;
+        yp = y + 1 == ny ? row - slab : row;
+        zm = z == 0 ? slab * static_cast<uint32_t>(nz - 1) : 0u - slab;
+        zp = z + 1 == nz ? 0u - slab * static_cast<uint32_t>(nz - 1) : slab;
+    }
+    // source cell of a population with velocity (cx, cy, cz)
+    __device__ __forceinline__ uint32_t src(int cx, int cy, int cz) const {
+        uint32_t s = dst;
+        if (cx > 0) s += xm;
+        if (cx < 0) s += xp;
+        if (cy > 0) s += ym;
+        if (cy < 0) s += yp;
+        if (cz > 0) s += zm;
+        if (cz < 0) s += zp;
+        return s;
+    }
 };
 
 }  // namespace lbc

@@ -117,7 +117,10 @@ struct D3Q7 {
 
 enum Lat { L_D2Q9 = 0, L_D3Q19 = 1, L_D3Q27 = 2, L_D2Q5 = 3, L_D3Q7 = 4 };
 enum Coll { C_BGK = 0, C_TRT = 1, C_MRT = 2, C_RLBM = 3 };
-enum Dt { T_F32 = 0, T_F64 = 1, T_F16 = 2, T_BF16 = 3 };
+// T_F80: x87 extended precision (long double, 64-bit significand), test-only:
+// the same algorithm with rounding errors ~2^11 times smaller than FP64, the
+// yardstick for comparing two FP64 (or FP32) evaluation orders.
+enum Dt { T_F32 = 0, T_F64 = 1, T_F16 = 2, T_BF16 = 3, T_F80 = 4 };
 
 // ---------------------------------------------------------------------------
 // Storage codecs.  F32/F64 store populations as-is (the reference's
@@ -188,6 +191,11 @@ template <> struct Codec<T_F32> {
 };
 template <> struct Codec<T_F64> {
     using S = double; using R = double;
+    static R load(S v, R) { return v; }
+    static S store(R f, R) { return f; }
+};
+template <> struct Codec<T_F80> {
+    using S = long double; using R = long double;
     static R load(S v, R) { return v; }
     static S store(R f, R) { return f; }
 };
@@ -1150,8 +1158,11 @@ using namespace orc;
         case T_F64: { constexpr int DT = T_F64; __VA_ARGS__; break; }   \
         case T_F16: { constexpr int DT = T_F16; __VA_ARGS__; break; }   \
         case T_BF16: { constexpr int DT = T_BF16; __VA_ARGS__; break; } \
+        case T_F80: { constexpr int DT = T_F80; __VA_ARGS__; break; }   \
         default: return -1;                           \
     }
+
+static size_t elem_size(int dt) { return dt == T_F80 ? sizeof(long double) : dt == T_F64 ? 8 : (dt == T_F32 ? 4 : 2); }
 
 #define ORC_CT_SWITCH(CTV, ...)                      \
     switch (CTV) {                                    \
@@ -1191,7 +1202,7 @@ int orc_fluid_run(int lat, int ct, int dt, void* buf, void* tmp, long long nx, l
                   double omega, int steps, int threads) {
     const Grid g = make_grid(lat, nx, ny, nz);
     int q = lat == L_D2Q9 ? 9 : (lat == L_D3Q19 ? 19 : 27);
-    const size_t es = dt == T_F64 ? 8 : (dt == T_F32 ? 4 : 2);
+    const size_t es = elem_size(dt);
     void* a = buf;
     void* b = tmp;
     for (int s = 0; s < steps; ++s) {
@@ -1241,7 +1252,7 @@ int orc_reactive_run(int flat, int ct, int slat, int dt, void* f, void* ft, void
     const Grid g = make_grid(flat, nx, ny, nz);
     const int qf = flat == L_D2Q9 ? 9 : (flat == L_D3Q19 ? 19 : 27);
     const int qs = slat == L_D2Q5 ? 5 : 7;
-    const size_t es = dt == T_F64 ? 8 : (dt == T_F32 ? 4 : 2);
+    const size_t es = elem_size(dt);
     void *f0 = f, *f1 = ft, *a0 = a, *a1 = at, *b0 = b, *b1 = bt;
     for (int s = 0; s < steps; ++s) {
         int rc = orc_fluid_step(flat, ct, dt, f0, f1, nx, ny, nz, omega, threads);

@@ -22,8 +22,10 @@ LIB_PATH = os.path.join(HERE, "liblbm_oracle.so")
 
 LATTICES = {"D2Q9": 0, "D3Q19": 1, "D3Q27": 2, "D2Q5": 3, "D3Q7": 4}
 COLLISIONS = {"BGK": 0, "TRT": 1, "MRT": 2, "RLBM": 3}
-DTYPES = {"f32": 0, "f64": 1, "f16": 2, "bf16": 3}
-NP_STORAGE = {"f32": np.float32, "f64": np.float64, "f16": np.uint16, "bf16": np.uint16}
+# "f80": long double (x87 extended precision), the yardstick for comparing
+# FP64 and FP32 evaluation orders; steps and runs accept it.
+DTYPES = {"f32": 0, "f64": 1, "f16": 2, "bf16": 3, "f80": 4}
+NP_STORAGE = {"f32": np.float32, "f64": np.float64, "f16": np.uint16, "bf16": np.uint16, "f80": np.longdouble}
 Q_OF = {"D2Q9": 9, "D3Q19": 19, "D3Q27": 27, "D2Q5": 5, "D3Q7": 7}
 D_OF = {"D2Q9": 2, "D3Q19": 3, "D3Q27": 3, "D2Q5": 2, "D3Q7": 3}
 
@@ -231,6 +233,8 @@ def half_roundtrip(x, bf16=False):
 
 def decode(lattice, dtype, pops):
     """Populations as float64 from a storage array (adds back w_i for f16/bf16 fluid storage)."""
+    if dtype == "f80":
+        return pops
     if dtype in ("f32", "f64"):
         return pops.astype(np.float64)
     if dtype == "f16":
