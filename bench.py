@@ -173,7 +173,7 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
 def ensemble_rate(torch, steps=1000):
     """512 independent 2D reactive instances (D2Q9 TRT + 2xD2Q5), edges
     log-uniform in 64..1024, FP16 storage with FP32 deviation-form arithmetic,
-    autotuned tiling, hipGraph replay."""
+    autotuned block size, hipGraph replay."""
     from lbcube import init as lbinit
     from lbcube.ensemble import Ensemble, reactive_mixing_ensemble
 
@@ -196,8 +196,8 @@ def ensemble_rate(torch, steps=1000):
                         "(FP32 deviation-form arithmetic), "
                         f"{steps} steps, one launch per step, hipGraph replay",
             "cells": cells, "mlups": cells * steps / (ms * 1e-3) / 1e6, "achieved_gbs": gbs,
-            "bytes_per_cell": 76, "tiling": {"tile_cap": chosen[0], "threads": chosen[1]},
-            "autotune_ms_per_step": {f"{c}x{t}": round(v, 4) for (c, t), v in timings.items()}}
+            "bytes_per_cell": 76, "threads_per_block": chosen,
+            "autotune_ms_per_step": {str(t): round(v, 4) for t, v in timings.items()}}
 
 
 def reactive_2d_rate(lbcube, torch, n=2048, steps=100):

@@ -150,9 +150,10 @@ int lbc_ensemble_create(int collision, int storage, int count, const uint32_t* n
                         const double* omega, const double* omega_c, const double* k_react, lbc_ensemble** out);
 int lbc_ensemble_layout(const lbc_ensemble* ensemble, uint64_t* fluid_elems, uint64_t* species_elems,
                         uint64_t* fluid_offsets, uint64_t* species_offsets);
-/* tile_cap in {64, 128, 256, 512} (work-items along x per tile), threads in
- * {256, 512}.  Waits for the device first (the table is rewritten in place). */
-int lbc_ensemble_set_tiling(lbc_ensemble* ensemble, uint32_t tile_cap, uint32_t threads);
+/* Work-items per block, threads in {128, 256, 512, 1024} (each instance's
+ * cells are covered linearly by whole blocks of its own).  Waits for the device
+ * first (the tables are rewritten in place). */
+int lbc_ensemble_set_block(lbc_ensemble* ensemble, uint32_t threads);
 /* 0 or LBC_FLAG_FAST: every instance then advances as
  * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...) on it alone.  Waits for
  * the device first. */
@@ -164,9 +165,10 @@ int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* n
  * and replayed. */
 int lbc_ensemble_run(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
                      int steps, int use_graph, void* stream);
-/* Times the 8 tiling candidates over 2*trial_steps steps each, after 2
- * warm-up steps (advancing the buffers by an even count), keeps the fastest; chosen[2] = {tile_cap, threads}; ms[8] (optional)
- * = ms per step of each candidate (-1 when not applicable). */
+/* Times the 4 block sizes over 2*trial_steps steps each, after 2 warm-up
+ * steps (advancing the buffers by an even count), keeps the fastest;
+ * *chosen = its work-items per block; ms[4] (optional) = ms per step of each
+ * of 128, 256, 512, 1024. */
 int lbc_ensemble_autotune(lbc_ensemble* ensemble, void* fluid0, void* fluid1, void* a0, void* a1, void* b0, void* b1,
                           int trial_steps, uint32_t* chosen, float* ms, void* stream);
 int lbc_ensemble_destroy(lbc_ensemble* ensemble);

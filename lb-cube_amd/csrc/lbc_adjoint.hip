@@ -62,20 +62,28 @@ template <class R> __device__ __forceinline__ void reaction_grad(R a, R b, R k, 
 
 // alpha = sum lam_q w_q P_q, beta = sum lam_q w_q grad_u P_q: the pieces of the
 // transposed equilibrium Jacobian (d feq_q / d f_i = w_q [P_q + G_q . (c_i - u)]).
+// sum_q w_q P_q = 1 exactly, but not with the weights rounded to binary (their
+// FP32 sum is 1 + 7e-9 for D2Q9), which would make a constant adjoint grow by
+// that much per step: the rest direction takes w_0 P_0 = 1 - sum_{q>0} w_q P_q.
 template <class R>
 __device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R& al, R& bx, R& by) {
     const R uu = ux * ux + uy * uy;
     al = bx = by = R(0);
+    R wp_moving = R(0);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
+    for (int q = 1; q < 9; ++q) {
         const R w = R(D2Q9::w[q]);
         const R cx = R(D2Q9::cx[q]), cy = R(D2Q9::cy[q]);
         const R cu = cx * ux + cy * uy;
-        const R P = R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu;
-        al += lam[q] * w * P;
+        const R wP = w * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu);
+        wp_moving += wP;
+        al += lam[q] * wP;
         bx += lam[q] * w * (R(3) * cx + R(9) * cu * cx - R(3) * ux);
         by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * uy);
     }
+    al += lam[0] * (R(1) - wp_moving);
+    bx += lam[0] * R(D2Q9::w[0]) * (-R(3) * ux);
+    by += lam[0] * R(D2Q9::w[0]) * (-R(3) * uy);
 }
 
 template <class R, int CT, int MAXT>
@@ -115,21 +123,34 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         R dsa, dsb;
         reaction_grad<R>(ca, cb, k, dsa, dsb);
         const R inv_cs2 = R(3);
-        R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0;
+        // Rest direction by remainder (w_0 e_0 = 1 - sum_{j>0} w_j e_j, and
+        // likewise w_0), so a constant adjoint is carried without the 3e-8
+        // per-step growth of the D2Q5 weights rounded to FP32.
+        R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0, we_moving = 0, w_moving = 0;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
+        for (int j = 1; j < 5; ++j) {
             const R w = R(D2Q5::w[j]);
             const R cx = R(D2Q5::cx[j]), cy = R(D2Q5::cy[j]);
-            const R e = R(1) + (cx * mp.ux + cy * mp.uy) * inv_cs2;
-            ea += la[j] * w * e;
-            eb += lb[j] * w * e;
+            const R we = w * (R(1) + (cx * mp.ux + cy * mp.uy) * inv_cs2);
+            we_moving += we;
+            w_moving += w;
+            ea += la[j] * we;
+            eb += lb[j] * we;
             wa += la[j] * w;
             wb += lb[j] * w;
             pax += la[j] * w * cx;
             pay += la[j] * w * cy;
             pbx += lb[j] * w * cx;
             pby += lb[j] * w * cy;
-            acc += static_cast<double>(la[j] * (w * ca * e - ga[j]) + lb[j] * (w * cb * e - gb[j]));
+            acc += static_cast<double>(la[j] * (ca * we - ga[j]) + lb[j] * (cb * we - gb[j]));
+        }
+        {
+            const R we0 = R(1) - we_moving, w0 = R(1) - w_moving;
+            ea += la[0] * we0;
+            eb += lb[0] * we0;
+            wa += la[0] * w0;
+            wb += lb[0] * w0;
+            acc += static_cast<double>(la[0] * (ca * we0 - ga[0]) + lb[0] * (cb * we0 - gb[0]));
         }
         const R sw = wa + wb;
 #pragma unroll

@@ -5,14 +5,14 @@
 //
 // Layout: three packed buffers per time level (fluid, species A, species B);
 // instance i owns a [Q][ny_i][nx_i] block of each at element offset off_i.
-// Work decomposition: instance i is cut into tiles of `tile_w` cells along x by
-// `threads / tile_w` rows; a block-start prefix table maps blockIdx.x to its
-// instance (binary search over a wave-uniform table), so small and large
-// instances share one grid with no idle padding beyond each tile's edge.
-// Every cell runs the same code as lbc_reactive_fused_step, so an instance
-// advances bit-identically to a standalone run.  The tile width per size class
-// is chosen by timing (lbc_ensemble_autotune), and long runs replay a
-// captured hipGraph of consecutive steps.
+// Work decomposition: the cells of instance i are numbered linearly (x
+// fastest) and cut into ceil(cells_i / threads) blocks of `threads`
+// work-items; a per-block table gives each block its instance (one uniform
+// load), so instances of every size share one grid and only the last block of
+// each instance has idle lanes.  Every cell runs the same code as
+// lbc_reactive_fused_step(_ex), so an instance advances bit-identically to a
+// standalone run.  The block size is chosen by timing (lbc_ensemble_autotune),
+// and long runs replay a captured hipGraph of consecutive steps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,12 +28,13 @@ namespace lbc {
 
 struct EnsembleInstance {
     uint64_t f_off, s_off;  // element offsets of this instance in the fluid / species buffers
-    uint32_t nx, ny;
-    uint32_t block_start, tiles_x;
-    uint32_t tile_w, rows;
+    uint32_t nx, ny, cells, block_start;
+    FastDiv by_nx;
     float omega, omega_minus, omega_c, k;
     double omega_d, omega_minus_d, omega_c_d, k_d;
 };
+
+constexpr uint32_t kEnsembleThreads[4] = {128, 256, 512, 1024};
 
 template <int CT, int ST, bool FAST, int THREADS>
 __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storage<ST>::S* __restrict__ cf,
@@ -42,33 +43,26 @@ __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storag
                                                            typename Storage<ST>::S* __restrict__ na,
                                                            const typename Storage<ST>::S* __restrict__ cb,
                                                            typename Storage<ST>::S* __restrict__ nb,
-                                                           const EnsembleInstance* __restrict__ table, int count) {
+                                                           const EnsembleInstance* __restrict__ table,
+                                                           const uint32_t* __restrict__ block_instance) {
     using R = typename Storage<ST>::R;
-    const uint32_t blk = blockIdx.x;
-    int lo = 0, hi = count - 1;  // last instance with block_start <= blk
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (table[mid].block_start <= blk) lo = mid;
-        else hi = mid - 1;
-    }
-    const EnsembleInstance& in = table[lo];
-    const uint32_t lb = blk - in.block_start;
-    const uint32_t ty = lb / in.tiles_x, tx = lb - ty * in.tiles_x;
-    const int x = static_cast<int>(tx * in.tile_w + threadIdx.x % in.tile_w);
-    const int y = static_cast<int>(ty * in.rows + threadIdx.x / in.tile_w);
-    if (x >= static_cast<int>(in.nx) || y >= static_cast<int>(in.ny)) return;
+    const EnsembleInstance& in = table[block_instance[blockIdx.x]];
+    const uint32_t p = (blockIdx.x - in.block_start) * THREADS + threadIdx.x;
+    if (p >= in.cells) return;
+    const uint32_t y = fast_div(p, in.by_nx);
+    const int x = static_cast<int>(p - y * in.nx);
     const R omega = std::is_same<R, double>::value ? R(in.omega_d) : R(in.omega);
     const R wm = std::is_same<R, double>::value ? R(in.omega_minus_d) : R(in.omega_minus);
     const R wc = std::is_same<R, double>::value ? R(in.omega_c_d) : R(in.omega_c);
     const R k = std::is_same<R, double>::value ? R(in.k_d) : R(in.k);
     if constexpr (FAST)
         reactive_cell_fast<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off,
-                                               cb + in.s_off, nb + in.s_off, x, y, 0, static_cast<int>(in.nx),
-                                               static_cast<int>(in.ny), 1, omega, wm, wc, k);
+                                               cb + in.s_off, nb + in.s_off, x, static_cast<int>(y), 0,
+                                               static_cast<int>(in.nx), static_cast<int>(in.ny), 1, omega, wm, wc, k);
     else
         reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
-                                          nb + in.s_off, x, y, 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1,
-                                          omega, wm, wc, k);
+                                          nb + in.s_off, x, static_cast<int>(y), 0, static_cast<int>(in.nx),
+                                          static_cast<int>(in.ny), 1, omega, wm, wc, k);
 }
 
 }  // namespace lbc
@@ -77,8 +71,9 @@ struct lbc_ensemble {
     int collision, storage, count;
     std::vector<lbc::EnsembleInstance> host;
     lbc::EnsembleInstance* table = nullptr;
+    uint32_t* block_instance = nullptr;  // sized for the smallest block (the most blocks)
     uint64_t f_elems = 0, s_elems = 0, blocks = 0;
-    uint32_t cap = 256, threads = 256;
+    uint32_t threads = 256;
     uint32_t flags = 0;
     hipStream_t capture = nullptr;
     hipGraphExec_t graph = nullptr;
@@ -88,32 +83,30 @@ struct lbc_ensemble {
 
 namespace lbc {
 
-static uint32_t pow2_at_least(uint32_t v) {
-    uint32_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
+static uint64_t blocks_for(const std::vector<EnsembleInstance>& host, uint32_t threads) {
+    uint64_t blocks = 0;
+    for (const auto& in : host) blocks += (in.cells + threads - 1) / threads;
+    return blocks;
 }
 
-// Tile policy: tile width = min(cap, threads, next power of two >= nx), at
-// least one wave.
-static hipError_t build_table(lbc_ensemble* e, uint32_t cap, uint32_t threads) {
-    // The table is rewritten in place: no launch that reads it may be in flight.
+// Lays out the blocks for `threads` work-items per block and uploads the
+// instance and block tables.
+static hipError_t build_table(lbc_ensemble* e, uint32_t threads) {
+    // The tables are rewritten in place: no launch that reads them may be in flight.
     hipError_t err = hipDeviceSynchronize();
     if (err != hipSuccess) return err;
-    uint64_t blocks = 0;
-    for (auto& in : e->host) {
-        uint32_t tw = std::min(std::min(cap, threads), std::max<uint32_t>(64, pow2_at_least(in.nx)));
-        in.tile_w = tw;
-        in.rows = threads / tw;
-        in.tiles_x = (in.nx + tw - 1) / tw;
-        in.block_start = static_cast<uint32_t>(blocks);
-        blocks += static_cast<uint64_t>(in.tiles_x) * ((in.ny + in.rows - 1) / in.rows);
+    std::vector<uint32_t> owner;
+    owner.reserve(blocks_for(e->host, threads));
+    for (uint32_t i = 0; i < e->host.size(); ++i) {
+        EnsembleInstance& in = e->host[i];
+        in.block_start = static_cast<uint32_t>(owner.size());
+        owner.insert(owner.end(), (in.cells + threads - 1) / threads, i);
     }
-    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-    e->blocks = blocks;
-    e->cap = cap;
+    e->blocks = owner.size();
     e->threads = threads;
     err = hipMemcpy(e->table, e->host.data(), e->host.size() * sizeof(EnsembleInstance), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(e->block_instance, owner.data(), owner.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e->graph) {
         (void)hipGraphExecDestroy(e->graph);
         e->graph = nullptr;
@@ -128,10 +121,15 @@ static void launch_ensemble(const lbc_ensemble* e, const void* cf, void* nf, con
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(e->blocks)), dim3(e->threads), 0, st,
                            static_cast<const T*>(cf), static_cast<T*>(nf), static_cast<const T*>(ca),
-                           static_cast<T*>(na), static_cast<const T*>(cb), static_cast<T*>(nb), e->table, e->count);
+                           static_cast<T*>(na), static_cast<const T*>(cb), static_cast<T*>(nb), e->table,
+                           e->block_instance);
     };
-    if (e->threads == 256) go(ensemble_kernel<CT, ST, FAST, 256>);
-    else go(ensemble_kernel<CT, ST, FAST, 512>);
+    switch (e->threads) {
+        case 128: go(ensemble_kernel<CT, ST, FAST, 128>); break;
+        case 256: go(ensemble_kernel<CT, ST, FAST, 256>); break;
+        case 512: go(ensemble_kernel<CT, ST, FAST, 512>); break;
+        default: go(ensemble_kernel<CT, ST, FAST, 1024>); break;
+    }
 }
 
 template <int CT, bool FAST>
@@ -151,8 +149,10 @@ static hipError_t ensemble_step(const lbc_ensemble* e, const void* cf, void* nf,
                                 const void* cb, void* nb, hipStream_t st) {
     const bool fast = (e->flags & LBC_FLAG_FAST) != 0;
     if (e->collision == kBGK)
-        return fast ? step_ct<kBGK, true>(e, cf, nf, ca, na, cb, nb, st) : step_ct<kBGK, false>(e, cf, nf, ca, na, cb, nb, st);
-    return fast ? step_ct<kTRT, true>(e, cf, nf, ca, na, cb, nb, st) : step_ct<kTRT, false>(e, cf, nf, ca, na, cb, nb, st);
+        return fast ? step_ct<kBGK, true>(e, cf, nf, ca, na, cb, nb, st)
+                    : step_ct<kBGK, false>(e, cf, nf, ca, na, cb, nb, st);
+    return fast ? step_ct<kTRT, true>(e, cf, nf, ca, na, cb, nb, st)
+                : step_ct<kTRT, false>(e, cf, nf, ca, na, cb, nb, st);
 }
 
 }  // namespace lbc
@@ -175,18 +175,20 @@ extern "C" int lbc_ensemble_create(int collision, int storage, int count, const 
     e->host.resize(count);
     uint64_t fo = 0, so = 0;
     for (int i = 0; i < count; ++i) {
-        if (nx[i] == 0 || ny[i] == 0 || static_cast<uint64_t>(nx[i]) * ny[i] > 0x7fffffffull) {
+        // an instance is one plane of at most 2^29 cells (32-bit offsets, exact division)
+        if (nx[i] == 0 || ny[i] == 0 || static_cast<uint64_t>(nx[i]) * ny[i] > (1ull << 29)) {
             delete e;
             return hipErrorInvalidValue;
         }
         EnsembleInstance& in = e->host[i];
         in.nx = nx[i];
         in.ny = ny[i];
+        in.cells = nx[i] * ny[i];
+        in.by_nx = make_fastdiv(nx[i]);
         in.f_off = fo;
         in.s_off = so;
-        const uint64_t cells = static_cast<uint64_t>(nx[i]) * ny[i];
-        fo += 9 * cells;
-        so += 5 * cells;
+        fo += 9ull * in.cells;
+        so += 5ull * in.cells;
         in.omega_d = omega[i];
         in.omega_c_d = omega_c[i];
         in.k_d = k_react[i];
@@ -198,11 +200,15 @@ extern "C" int lbc_ensemble_create(int collision, int storage, int count, const 
     }
     e->f_elems = fo;
     e->s_elems = so;
-    hipError_t err = hipMalloc(reinterpret_cast<void**>(&e->table), count * sizeof(EnsembleInstance));
+    const uint64_t max_blocks = blocks_for(e->host, kEnsembleThreads[0]);
+    hipError_t err = max_blocks > 0x7fffffffull ? hipErrorInvalidValue : hipSuccess;
+    if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&e->table), count * sizeof(EnsembleInstance));
+    if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&e->block_instance), max_blocks * sizeof(uint32_t));
     if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->capture, hipStreamNonBlocking);
-    if (err == hipSuccess) err = build_table(e, 256, 256);
+    if (err == hipSuccess) err = build_table(e, 256);
     if (err != hipSuccess) {
         if (e->table) (void)hipFree(e->table);
+        if (e->block_instance) (void)hipFree(e->block_instance);
         if (e->capture) (void)hipStreamDestroy(e->capture);
         delete e;
         return err;
@@ -223,12 +229,12 @@ extern "C" int lbc_ensemble_layout(const lbc_ensemble* e, uint64_t* fluid_elems,
     return hipSuccess;
 }
 
-extern "C" int lbc_ensemble_set_tiling(lbc_ensemble* e, uint32_t tile_cap, uint32_t threads) {
+extern "C" int lbc_ensemble_set_block(lbc_ensemble* e, uint32_t threads) {
     if (e == nullptr) return hipErrorInvalidValue;
-    if ((threads != 256 && threads != 512) || (tile_cap != 64 && tile_cap != 128 && tile_cap != 256 &&
-                                               tile_cap != 512))
+    if (std::find(std::begin(lbc::kEnsembleThreads), std::end(lbc::kEnsembleThreads), threads) ==
+        std::end(lbc::kEnsembleThreads))
         return hipErrorInvalidConfiguration;
-    return lbc::build_table(e, tile_cap, threads);
+    return lbc::build_table(e, threads);
 }
 
 extern "C" int lbc_ensemble_set_flags(lbc_ensemble* e, uint32_t flags) {
@@ -284,7 +290,8 @@ extern "C" int lbc_ensemble_run(lbc_ensemble* e, void* f0, void* f1, void* a0, v
             for (int s = 0; s < chunk && err == hipSuccess; ++s)
                 err = ensemble_step(e, F[s & 1], F[(s + 1) & 1], A[s & 1], A[(s + 1) & 1], B[s & 1], B[(s + 1) & 1],
                                     e->capture);
-            hipError_t end = hipStreamEndCapture(e->capture, &g);
+            const hipError_t end = hipStreamEndCapture(e->capture, &g);
+            if (end == hipSuccess && err != hipSuccess) (void)hipGraphDestroy(g);
             if (err != hipSuccess) return err;
             if (end != hipSuccess) return end;
             err = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
@@ -317,58 +324,48 @@ extern "C" int lbc_ensemble_run(lbc_ensemble* e, void* f0, void* f1, void* a0, v
     return err;
 }
 
-// Times every tiling candidate for `trial_steps` steps on the given buffers
-// (their contents are advanced) and keeps the fastest.  Candidates: tile
-// width caps {64, 128, 256, 512} x block sizes {256, 512}.  Writes the chosen
-// (cap, threads) and, if ms is non-null, the 8 measured times in ms per step.
+// Times every block size in {128, 256, 512, 1024} for 2 * trial_steps steps
+// (after 2 warm-up steps) on the given buffers (their contents advance by an
+// even number of steps) and keeps the fastest.  Writes the chosen block size
+// and, if ms is non-null, the 4 measured times in ms per step.
 extern "C" int lbc_ensemble_autotune(lbc_ensemble* e, void* f0, void* f1, void* a0, void* a1, void* b0, void* b1,
                                      int trial_steps, uint32_t* chosen, float* ms, void* stream) {
     using namespace lbc;
     if (e == nullptr || trial_steps <= 0) return hipErrorInvalidValue;
     const hipStream_t st = as_stream(stream);
-    const uint32_t caps[4] = {64, 128, 256, 512};
-    const uint32_t thr[2] = {256, 512};
     hipEvent_t t0, t1;
     hipError_t err;
     if ((err = hipEventCreate(&t0)) != hipSuccess) return err;
-    if ((err = hipEventCreate(&t1)) != hipSuccess) return err;
+    if ((err = hipEventCreate(&t1)) != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return err;
+    }
     float best = 1e30f;
-    uint32_t bc = 256, bt = 256;
-    int idx = 0;
-    for (uint32_t t : thr) {
-        for (uint32_t c : caps) {
-            float m = -1.0f;
-            if (c <= t) {
-                if ((err = build_table(e, c, t)) != hipSuccess) break;
-                err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2, 0, stream));
-                if (err == hipSuccess) err = hipEventRecord(t0, st);
-                if (err == hipSuccess)
-                    err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2 * trial_steps, 0,
-                                                                   stream));
-                if (err == hipSuccess) err = hipEventRecord(t1, st);
-                if (err == hipSuccess) err = hipEventSynchronize(t1);
-                if (err == hipSuccess) err = hipEventElapsedTime(&m, t0, t1);
-                if (err != hipSuccess) break;
-                m /= 2.0f * trial_steps;
-                if (m < best) {
-                    best = m;
-                    bc = c;
-                    bt = t;
-                }
-            }
-            if (ms) ms[idx] = m;
-            ++idx;
-        }
+    uint32_t bt = 256;
+    for (int idx = 0; idx < 4 && err == hipSuccess; ++idx) {
+        const uint32_t t = kEnsembleThreads[idx];
+        float m = -1.0f;
+        err = build_table(e, t);
+        if (err == hipSuccess) err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2, 0, stream));
+        if (err == hipSuccess) err = hipEventRecord(t0, st);
+        if (err == hipSuccess)
+            err = static_cast<hipError_t>(lbc_ensemble_run(e, f0, f1, a0, a1, b0, b1, 2 * trial_steps, 0, stream));
+        if (err == hipSuccess) err = hipEventRecord(t1, st);
+        if (err == hipSuccess) err = hipEventSynchronize(t1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&m, t0, t1);
         if (err != hipSuccess) break;
+        m /= 2.0f * trial_steps;
+        if (m < best) {
+            best = m;
+            bt = t;
+        }
+        if (ms) ms[idx] = m;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (err != hipSuccess) return err;
-    if (chosen) {
-        chosen[0] = bc;
-        chosen[1] = bt;
-    }
-    return build_table(e, bc, bt);
+    if (chosen) *chosen = bt;
+    return build_table(e, bt);
 }
 
 extern "C" int lbc_ensemble_destroy(lbc_ensemble* e) {
@@ -376,6 +373,7 @@ extern "C" int lbc_ensemble_destroy(lbc_ensemble* e) {
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
     if (e->capture) (void)hipStreamDestroy(e->capture);
     if (e->table) (void)hipFree(e->table);
+    if (e->block_instance) (void)hipFree(e->block_instance);
     delete e;
     return hipSuccess;
 }

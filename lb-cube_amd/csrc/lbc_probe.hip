@@ -1,7 +1,8 @@
-// Streaming copy used as the measured HBM ceiling next to the spec peak: the
-// bench reports the fluid kernels' achieved bandwidth against both, and the
-// HBM byte counters (FETCH_SIZE / WRITE_SIZE) are scaled by what they read for
-// this copy of known size.
+// Streaming copy (non-temporal loads and stores, 16 bytes per lane: the fastest
+// plain copy measured on MI355X) used as the measured HBM ceiling next to the
+// spec peak: the bench reports the fluid kernels' achieved bandwidth against
+// both, and the HBM byte counters (FETCH_SIZE / WRITE_SIZE) are scaled by what
+// they read for this copy of known size.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -15,7 +16,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void stream_copy_kernel(const v4f* __restrict__ src, v4f* __restrict__ dst,
                                                           uint64_t n4) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n4) __builtin_nontemporal_store(src[i], dst + i);
+    if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 }  // namespace lbc

@@ -41,7 +41,7 @@ SIGNATURES = {
     "lbc_ensemble_create": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)], c_int),
     "lbc_ensemble_layout": ([c_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), c_vp, c_vp],
                             c_int),
-    "lbc_ensemble_set_tiling": ([c_vp, c_u32, c_u32], c_int),
+    "lbc_ensemble_set_block": ([c_vp, c_u32], c_int),
     "lbc_ensemble_set_flags": ([c_vp, c_u32], c_int),
     "lbc_ensemble_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbc_ensemble_run": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),

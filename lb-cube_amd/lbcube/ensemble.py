@@ -60,8 +60,9 @@ class Ensemble:
         sl = slice(self.s_off[i], self.s_off[i] + 5 * nx * ny)
         return self.A[lv][sl].view(5, ny, nx), self.B[lv][sl].view(5, ny, nx)
 
-    def set_tiling(self, tile_cap, threads):
-        check(lib().lbc_ensemble_set_tiling(self.handle, tile_cap, threads), "lbc_ensemble_set_tiling")
+    def set_block(self, threads):
+        """Work-items per block: 128, 256, 512 or 1024."""
+        check(lib().lbc_ensemble_set_block(self.handle, threads), "lbc_ensemble_set_block")
 
     def _ptrs(self):
         c, n = self.level, 1 - self.level
@@ -76,13 +77,12 @@ class Ensemble:
 
     def autotune(self, trial_steps=3, stream=None):
         f0, f1, a0, a1, b0, b1 = self._ptrs()
-        chosen = (ctypes.c_uint32 * 2)()
-        ms = (ctypes.c_float * 8)()
-        check(lib().lbc_ensemble_autotune(self.handle, f0, f1, a0, a1, b0, b1, int(trial_steps), chosen, ms,
-                                          _stream(stream)), "lbc_ensemble_autotune")
-        # the candidates advance the state by 2 + 2 * trial_steps steps each: an even count
-        caps = [(c, t) for t in (256, 512) for c in (64, 128, 256, 512)]
-        return tuple(chosen), {caps[i]: ms[i] for i in range(8) if ms[i] >= 0}
+        chosen = ctypes.c_uint32()
+        ms = (ctypes.c_float * 4)()
+        check(lib().lbc_ensemble_autotune(self.handle, f0, f1, a0, a1, b0, b1, int(trial_steps),
+                                          ctypes.byref(chosen), ms, _stream(stream)), "lbc_ensemble_autotune")
+        # every candidate advances the state by 2 + 2 * trial_steps steps: an even count
+        return chosen.value, {t: ms[i] for i, t in enumerate((128, 256, 512, 1024))}
 
     def close(self):
         if self.handle:

@@ -228,42 +228,68 @@ def test_reactive_fast_rejects_unknown_flags(gpu):
     assert lbcube.lib().lbc_reactive_fused_step_ex(0, 0, 3, 0, 2, *args) == 1
 
 
-def reactive_mixing(n, storage, fast, steps, coll="TRT", sample=100):
+def reactive_mixing(n, storage, fast, steps, coll="TRT", sample=250, k_scale=1.0, perturbation=0.0):
     """One reactive-mixing instance: 2D Taylor-Green (u0 = 0.04), species in
-    tanh-smoothed half planes, Re = 1000, Sc = 1, Da = 10."""
+    tanh-smoothed half planes, Re = 1000, Sc = 1, Da = 10 (k_scale = 0: no
+    reaction).  `perturbation`: relative random change of the initial A.
+    Returns per sample (total A, total B, total fluid mass), float64 sums."""
     u0 = 0.04
     tau = 3.0 * u0 * n / 1000.0 + 0.5
-    k = 10.0 * u0 / n
+    k = 10.0 * u0 / n * k_scale
     rho, u = lbinit.taylor_green_2d(n, n, u0)
     ca, cb = lbinit.tanh_half_planes(n, delta=max(2.0, n / 32))
+    if perturbation:
+        g = torch.Generator(device="cpu").manual_seed(5)
+        ca = ca * (1 + perturbation * torch.randn(ca.shape, generator=g, dtype=torch.float64).to(ca.device))
     f = lbinit.equilibrium("D2Q9", rho, u, storage)
     a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage)
     b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage)
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
-    stats = []
+    out = [(lbcube.decode("D2Q5", a).sum().item(), lbcube.decode("D2Q5", b).sum().item(),
+            lbcube.decode("D2Q9", f).sum().item())]
     for s in range(1, steps + 1):
         lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, 1 / tau, 1 / tau, k, fast=fast)
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
         if s % sample == 0:
-            st = lbcube.reactive_stats("D2Q5", a, b, k)
-            stats.append((st["mean_A"], st["segregation_intensity"]))
-    return np.array(stats)
+            out.append((lbcube.decode("D2Q5", a).sum().item(), lbcube.decode("D2Q5", b).sum().item(),
+                        lbcube.decode("D2Q9", f).sum().item()))
+    return np.array(out)
+
+
+def test_reactive_mass_conservation_fp32(gpu):
+    """No reaction (k = 0), 512^2, 2000 steps: species and fluid mass in FP32.
+    The fast form conserves both to rounding; the reference order drifts (the
+    D2Q5 and D2Q9 weights rounded to binary do not sum to 1)."""
+    n, steps = 512, 2000
+    for fast in (True, False):
+        r = reactive_mixing(n, "f32", fast, steps, k_scale=0.0)
+        da = np.max(np.abs(r[:, 0] - r[0, 0])) / r[0, 0]
+        dm = np.max(np.abs(r[:, 2] - r[0, 2])) / r[0, 2]
+        print(f"fp32 fast={fast}: species mass drift {da:.2e}, fluid mass drift {dm:.2e}", flush=True)
+        if fast:
+            assert da <= 1e-6 and dm <= 1e-6
 
 
 def test_reactive_reduced_storage_error_against_fp64(gpu):
-    """512^2 reactive mixing, 2000 steps: mean A (the loss's complement) and the
-    segregation intensity of each storage/arithmetic against FP64."""
-    n, steps = 512, 2000
-    ref = reactive_mixing(n, "f64", False, steps)
-    # FP32 in the reference order drifts in species mass: the D2Q5 weights
-    # rounded to binary sum to 1 + 3e-8, a bias of ~3e-8 omega_c per step
-    # wherever g is off equilibrium (a few 1e-4 in mean A here).  The fast form
-    # takes the rest population's equilibrium as the remainder and does not.
-    for storage, fast, tol in (("f32", False, 1e-3), ("f32", True, 1e-5), ("f16", True, 2e-3), ("bf16", True, None)):
-        got = reactive_mixing(n, storage, fast, steps)
-        e_mean = float(np.max(np.abs(got[:, 0] - ref[:, 0]) / np.abs(ref[:, 0])))
-        e_seg = float(np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])))
-        print(f"reactive {n}^2 storage {storage} fast={fast}: mean A rel err {e_mean:.3e}, "
-              f"segregation rel err {e_seg:.3e}", flush=True)
-        if tol is not None:
-            assert e_mean <= tol and e_seg <= 10 * tol, (storage, fast, e_mean, e_seg)
+    """512^2 reactive mixing, 2000 steps: total A (the loss's complement) of
+    each storage/arithmetic against FP64.  From step ~1000 this flow is
+    sensitive: an FP64 run whose initial A is perturbed by 1e-7 (relative)
+    departs by ~3e-4, so beyond that the yardstick is that spread; before it,
+    the errors are the formats' own."""
+    n, steps, sample = 512, 2000, 250
+    ref = reactive_mixing(n, "f64", False, steps, sample=sample)
+    spread = np.abs(reactive_mixing(n, "f64", False, steps, sample=sample, perturbation=1e-7)[:, 0] - ref[:, 0])
+    spread /= ref[:, 0]
+    early = slice(0, 750 // sample + 1)
+    print(f"FP64 perturbed by 1e-7: relative spread of total A per sample {np.array2string(spread, precision=2)}",
+          flush=True)
+    for storage, fast, early_tol in (("f32", True, 1e-5), ("f32", False, None), ("f16", True, 2e-3),
+                                     ("bf16", True, None)):
+        got = reactive_mixing(n, storage, fast, steps, sample=sample)
+        err = np.abs(got[:, 0] - ref[:, 0]) / ref[:, 0]
+        print(f"reactive {n}^2 storage {storage} fast={fast}: relative error of total A per sample "
+              f"{np.array2string(err, precision=2)}", flush=True)
+        if early_tol is not None:
+            assert np.max(err[early]) <= early_tol, (storage, fast, err)
+        if (storage, fast) == ("f32", True):
+            assert np.max(err) <= 2 * np.max(spread) + 1e-5, (err, spread)
