@@ -139,17 +139,20 @@ int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void
 
 /* Ensembles of independent 2D reactive instances (D2Q9 BGK/TRT + two D2Q5
  * species, A + B -> C), each with its own extents, omega, omega_c and k, in
- * three packed buffers per time level (fluid, A, B; instance i at the element
- * offsets reported by lbc_ensemble_layout, each a [Q][ny_i][nx_i] block) and
- * advanced by one launch per step.  Each instance advances bit-identically to
- * lbc_reactive_fused_step on it alone.  create allocates the instance table
- * and a capture stream; step/run/autotune allocate nothing.  No counterpart
- * in the reference. */
+ * three packed buffers per time level (fluid, A, B) and advanced by one launch
+ * per step.  Instance i starts at the element offsets lbc_ensemble_layout
+ * reports; its Q population planes follow each other plane_strides[i]
+ * elements apart (nx_i * ny_i rounded up to a multiple of 64, so that planes
+ * start on whole cache lines), each holding the [ny_i][nx_i] cells; buffers
+ * must be 256-byte aligned.  Each instance advances bit-identically to
+ * lbc_reactive_fused_step(_ex) on it alone.  create allocates the instance
+ * and block tables and a capture stream; step/run/autotune allocate nothing.
+ * No counterpart in the reference. */
 typedef struct lbc_ensemble lbc_ensemble;
 int lbc_ensemble_create(int collision, int storage, int count, const uint32_t* nx, const uint32_t* ny,
                         const double* omega, const double* omega_c, const double* k_react, lbc_ensemble** out);
 int lbc_ensemble_layout(const lbc_ensemble* ensemble, uint64_t* fluid_elems, uint64_t* species_elems,
-                        uint64_t* fluid_offsets, uint64_t* species_offsets);
+                        uint64_t* fluid_offsets, uint64_t* species_offsets, uint64_t* plane_strides);
 /* Work-items per block, threads in {128, 256, 512, 1024} (each instance's
  * cells are covered linearly by whole blocks of its own).  Waits for the device
  * first (the tables are rewritten in place). */

@@ -74,14 +74,13 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= c.count) return;
     const CellCoord q = cell_coord(p, c);
+    const Pull pull(p, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
     if constexpr (FAST)
-        reactive_cell_fast<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, q.x, q.y, q.z,
-                                         static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz), omega,
-                                         omega_minus, omega_c, k_react);
+        reactive_cell_fast<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
+                                         omega_c, k_react);
     else
-        reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, q.x, q.y, q.z, static_cast<int>(c.nx),
-                                    static_cast<int>(c.ny), static_cast<int>(c.nz), omega, omega_minus, omega_c,
-                                    k_react);
+        reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
+                                    omega_c, k_react);
 }
 
 template <class F, class S, int ST, bool REACT>

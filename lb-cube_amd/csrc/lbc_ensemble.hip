@@ -4,7 +4,8 @@
 // No counterpart in the reference (its drivers run one domain per process).
 //
 // Layout: three packed buffers per time level (fluid, species A, species B);
-// instance i owns a [Q][ny_i][nx_i] block of each at element offset off_i.
+// instance i owns Q planes of `plane_i` elements (nx_i * ny_i cells rounded up
+// to 64) of each, starting at element offset off_i.
 // Work decomposition: the cells of instance i are numbered linearly (x
 // fastest) and cut into ceil(cells_i / threads) blocks of `threads`
 // work-items; a per-block table gives each block its instance (one uniform
@@ -29,12 +30,20 @@ namespace lbc {
 struct EnsembleInstance {
     uint64_t f_off, s_off;  // element offsets of this instance in the fluid / species buffers
     uint32_t nx, ny, cells, block_start;
+    uint32_t plane;  // elements between population planes: cells rounded up to kPlaneAlign
     FastDiv by_nx;
     float omega, omega_minus, omega_c, k;
     double omega_d, omega_minus_d, omega_c_d, k_d;
 };
 
 constexpr uint32_t kEnsembleThreads[4] = {128, 256, 512, 1024};
+
+// Population planes start on multiples of 64 cells (128 B at FP16, 512 B at
+// FP64), so every wave's 64 unshifted cells fill whole cache lines: with
+// planes packed back to back an instance of arbitrary size leaves every store
+// of a wave straddling two lines, and the ensemble ran at two thirds of the
+// bandwidth of equal, aligned instances.
+constexpr uint32_t kPlaneAlign = 64;
 
 template <int CT, int ST, bool FAST, int THREADS>
 __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storage<ST>::S* __restrict__ cf,
@@ -55,14 +64,13 @@ __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storag
     const R wm = std::is_same<R, double>::value ? R(in.omega_minus_d) : R(in.omega_minus);
     const R wc = std::is_same<R, double>::value ? R(in.omega_c_d) : R(in.omega_c);
     const R k = std::is_same<R, double>::value ? R(in.k_d) : R(in.k);
+    const Pull pull(p, x, static_cast<int>(y), 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1);
     if constexpr (FAST)
         reactive_cell_fast<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off,
-                                               cb + in.s_off, nb + in.s_off, x, static_cast<int>(y), 0,
-                                               static_cast<int>(in.nx), static_cast<int>(in.ny), 1, omega, wm, wc, k);
+                                               cb + in.s_off, nb + in.s_off, pull, in.plane, omega, wm, wc, k);
     else
         reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
-                                          nb + in.s_off, x, static_cast<int>(y), 0, static_cast<int>(in.nx),
-                                          static_cast<int>(in.ny), 1, omega, wm, wc, k);
+                                          nb + in.s_off, pull, in.plane, omega, wm, wc, k);
 }
 
 }  // namespace lbc
@@ -184,11 +192,12 @@ extern "C" int lbc_ensemble_create(int collision, int storage, int count, const 
         in.nx = nx[i];
         in.ny = ny[i];
         in.cells = nx[i] * ny[i];
+        in.plane = (in.cells + kPlaneAlign - 1) / kPlaneAlign * kPlaneAlign;
         in.by_nx = make_fastdiv(nx[i]);
         in.f_off = fo;
         in.s_off = so;
-        fo += 9ull * in.cells;
-        so += 5ull * in.cells;
+        fo += 9ull * in.plane;
+        so += 5ull * in.plane;
         in.omega_d = omega[i];
         in.omega_c_d = omega_c[i];
         in.k_d = k_react[i];
@@ -218,13 +227,14 @@ extern "C" int lbc_ensemble_create(int collision, int storage, int count, const 
 }
 
 extern "C" int lbc_ensemble_layout(const lbc_ensemble* e, uint64_t* fluid_elems, uint64_t* species_elems,
-                                   uint64_t* fluid_offsets, uint64_t* species_offsets) {
+                                   uint64_t* fluid_offsets, uint64_t* species_offsets, uint64_t* plane_strides) {
     if (e == nullptr) return hipErrorInvalidValue;
     if (fluid_elems) *fluid_elems = e->f_elems;
     if (species_elems) *species_elems = e->s_elems;
     for (int i = 0; i < e->count; ++i) {
         if (fluid_offsets) fluid_offsets[i] = e->host[i].f_off;
         if (species_offsets) species_offsets[i] = e->host[i].s_off;
+        if (plane_strides) plane_strides[i] = e->host[i].plane;
     }
     return hipSuccess;
 }

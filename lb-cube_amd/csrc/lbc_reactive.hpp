@@ -8,8 +8,10 @@
 
 namespace lbc {
 
-// Fluid step + A + B -> C step of one cell.  The reaction step of the
-// reference reads the fluid populations that the fluid step has just written
+// Fluid step + A + B -> C step of one cell: `pull` locates the cell and its
+// periodic sources, `plane` is the element stride between population planes.
+// The reaction step of the reference reads the fluid populations that the
+// fluid step has just written
 // at the same cell (src/simulate_reaction.cpp:527-541); here they are still in
 // registers.  With reduced-precision storage they are passed through the
 // storage rounding first, so the velocity is the one the unfused pair sees.
@@ -20,16 +22,13 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
                                               const typename Storage<ST>::S* __restrict__ cur_a,
                                               typename Storage<ST>::S* __restrict__ nxt_a,
                                               const typename Storage<ST>::S* __restrict__ cur_b,
-                                              typename Storage<ST>::S* __restrict__ nxt_b, int x, int y, int z,
-                                              int nx, int ny, int nz, typename Storage<ST>::R omega,
+                                              typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
+                                              size_t plane, typename Storage<ST>::R omega,
                                               typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
                                               typename Storage<ST>::R k_react) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
-                                             static_cast<uint32_t>(nz));
-    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
-    const Pull pull(cell, x, y, z, nx, ny, nz);
+    const uint32_t cell = pull.dst;
     R f[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
@@ -66,24 +65,22 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
 // The same step in the deviation form (LBC_FLAG_FAST; BGK and TRT): fluid
 // populations carried as f_i - w_i with FMAs and a hardware reciprocal
 // (lbc_fast.hpp), the velocity taken from the deviations as stored, the scalar
-// collision with 1/cs2 multiplied, the reaction with hardware exp and rcp.
+// collision with 1/cs2 multiplied and the rest equilibrium by remainder, the
+// reaction increment without cancellation (lbc_fast.hpp).
 template <class F, int CT, class S, int ST>
 __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S* __restrict__ cur_f,
                                                    typename Storage<ST>::S* __restrict__ nxt_f,
                                                    const typename Storage<ST>::S* __restrict__ cur_a,
                                                    typename Storage<ST>::S* __restrict__ nxt_a,
                                                    const typename Storage<ST>::S* __restrict__ cur_b,
-                                                   typename Storage<ST>::S* __restrict__ nxt_b, int x, int y, int z,
-                                                   int nx, int ny, int nz, typename Storage<ST>::R omega,
+                                                   typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
+                                                   size_t plane, typename Storage<ST>::R omega,
                                                    typename Storage<ST>::R omega_minus,
                                                    typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react) {
     static_assert(CT == kBGK || CT == kTRT, "the deviation form covers BGK and TRT");
     using St = Storage<ST>;
     using R = typename St::R;
-    const size_t plane = static_cast<size_t>(static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny) *
-                                             static_cast<uint32_t>(nz));
-    const uint32_t cell = (static_cast<uint32_t>(z) * ny + y) * nx + x;
-    const Pull pull(cell, x, y, z, nx, ny, nz);
+    const uint32_t cell = pull.dst;
     R d[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {

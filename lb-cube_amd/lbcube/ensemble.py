@@ -1,7 +1,9 @@
 """Batched ensembles of 2D reactive-mixing instances (lbc_ensemble_* of the C ABI).
 
 An Ensemble owns two time levels of three packed buffers (fluid, A, B) in
-torch CUDA memory; instance i is a [Q][ny_i][nx_i] block in each.  One launch
+torch CUDA memory; instance i holds Q population planes in each, padded to a
+multiple of 64 cells apart (`fluid(i)` and `species(i)` are views of shape
+(Q, ny_i, nx_i)).  One launch
 advances every instance; `run(..., graph=True)` replays a captured hipGraph.
 """
 from __future__ import annotations
@@ -33,9 +35,10 @@ class Ensemble:
                                         ctypes.byref(handle)), "lbc_ensemble_create")
         self.handle = handle
         fe, se = ctypes.c_uint64(), ctypes.c_uint64()
-        fo, so = (ctypes.c_uint64 * n)(), (ctypes.c_uint64 * n)()
-        check(lib().lbc_ensemble_layout(handle, ctypes.byref(fe), ctypes.byref(se), fo, so), "lbc_ensemble_layout")
-        self.f_off, self.s_off = list(fo), list(so)
+        fo, so, ps = (ctypes.c_uint64 * n)(), (ctypes.c_uint64 * n)(), (ctypes.c_uint64 * n)()
+        check(lib().lbc_ensemble_layout(handle, ctypes.byref(fe), ctypes.byref(se), fo, so, ps),
+              "lbc_ensemble_layout")
+        self.f_off, self.s_off, self.plane = list(fo), list(so), list(ps)
         dt = STORAGE_NAMES[storage]
         self.F = [torch.zeros(fe.value, dtype=dt, device=device) for _ in range(2)]
         self.A = [torch.zeros(se.value, dtype=dt, device=device) for _ in range(2)]
@@ -49,16 +52,19 @@ class Ensemble:
         check(lib().lbc_ensemble_set_flags(self.handle, 1 if fast else 0), "lbc_ensemble_set_flags")
         self.fast = bool(fast)
 
-    def fluid(self, i, level=None):
+    def _planes(self, buf, off, q, i):
         nx, ny = self.sizes[i]
+        ps = self.plane[i]
+        return buf[off:off + q * ps].view(q, ps)[:, :nx * ny].view(q, ny, nx)
+
+    def fluid(self, i, level=None):
+        """Instance i's fluid populations, a (9, ny, nx) view (planes padded apart)."""
         lv = self.level if level is None else level
-        return self.F[lv][self.f_off[i]:self.f_off[i] + 9 * nx * ny].view(9, ny, nx)
+        return self._planes(self.F[lv], self.f_off[i], 9, i)
 
     def species(self, i, level=None):
-        nx, ny = self.sizes[i]
         lv = self.level if level is None else level
-        sl = slice(self.s_off[i], self.s_off[i] + 5 * nx * ny)
-        return self.A[lv][sl].view(5, ny, nx), self.B[lv][sl].view(5, ny, nx)
+        return (self._planes(self.A[lv], self.s_off[i], 5, i), self._planes(self.B[lv], self.s_off[i], 5, i))
 
     def set_block(self, threads):
         """Work-items per block: 128, 256, 512 or 1024."""

@@ -74,7 +74,8 @@ def test_instances_match_standalone_runs(gpu, coll, storage, fast):
             for i in range(n):
                 sa, sb = ens.species(i)
                 for got, exp in ((ens.fluid(i), want[i][0]), (sa, want[i][1]), (sb, want[i][2])):
-                    assert torch.equal(got.view(torch.uint8), exp.contiguous().view(torch.uint8)), (threads, graph, i)
+                    assert torch.equal(got.contiguous().view(torch.uint8), exp.contiguous().view(torch.uint8)), (
+                        threads, graph, i)
     ens.close()
 
 
