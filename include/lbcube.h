@@ -114,7 +114,7 @@ int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_latt
                                double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
                                uint32_t block_z, void* stream);
 
-/* Gradient of the 2D reactive loop (D2Q9 BGK or TRT + two D2Q5 species,
+/* Gradient of the 2D reactive loop (D2Q9 BGK, TRT or MRT + two D2Q5 species,
  * A + B -> C; `steps` calls of lbc_reactive_fused_step from f0, a0, b0) by the
  * discrete adjoint.  loss = <C>(T) = <A>(0) - <A>(T) (domain means).  Outputs:
  * *loss, the adjoints dloss/d(f0, a0, b0) of the initial populations (same

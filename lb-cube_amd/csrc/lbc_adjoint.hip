@@ -1,4 +1,4 @@
-// Discrete adjoint of the 2D reactive loop: D2Q9 BGK or TRT fluid plus two
+// Discrete adjoint of the 2D reactive loop: D2Q9 BGK, TRT or MRT fluid plus two
 // D2Q5 species reacting A + B -> C (the loop of src/simulate_reaction.cpp:516-546,
 // run by lbc_reactive_fused_step).  No counterpart in the reference.
 //
@@ -15,8 +15,9 @@
 //   3. reaction step transpose: scalar BGK + source Jacobians, the reaction
 //      Jacobian of the exact one-step map, and the velocity coupling into the
 //      fluid adjoint through du'/df' = (c - u') / rho';
-//   4. fluid collision transpose at the pre-collision state (BGK, or TRT with
-//      its even/odd split), written to nu_f of step n.
+//   4. fluid collision transpose at the pre-collision state (BGK, TRT with
+//      its even/odd split, or MRT through the moment basis), written to nu_f
+//      of step n.
 // Long runs use checkpoints every `interval` steps and recompute each
 // segment forward before walking it backward (lbc_reactive_gradient).
 #include <hip/hip_runtime.h>
@@ -165,7 +166,34 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         for (int i = 0; i < 9; ++i)
             lf[i] += (lux * (R(D2Q9::cx[i]) - mp.ux) + luy * (R(D2Q9::cy[i]) - mp.uy)) * irho;
         const Macro<R> m = macro_state<D2Q9, R>(fl);
-        if constexpr (CT == kBGK) {
+        if constexpr (CT == kMRT) {
+            // f' = M^-1 [(I - S) M f + S m_eq(rho, j)], M^-1 = M^T D^-1
+            // (collide_mrt_d2q9): J^T lam = M^T (I - S) mu + (d m_eq/d f)^T S mu
+            // with mu = D^-1 M lam and d/df_i = d/drho + c_ix d/djx + c_iy d/djy.
+            const R s[9] = {R(0), R(1.63), R(1.14), R(0), R(1.92), R(0), R(1.92), omega, omega};
+            R mu[9], nm[9], ne[9];
+            mrt_forward_d2q9<R>(lf, mu);
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                mu[r] /= R(kD2Q9Basis.norm[r]);
+                nm[r] = (R(1) - s[r]) * mu[r];
+                ne[r] = s[r] * mu[r];
+            }
+            const R ux = m.ux, uy = m.uy, uu = ux * ux + uy * uy;
+            const R grho = ne[1] * (R(-2) - R(3) * uu) + ne[2] * (R(1) + R(3) * uu) - ne[7] * (ux * ux - uy * uy) -
+                           ne[8] * ux * uy;
+            const R gjx = R(6) * ux * (ne[1] - ne[2]) - ne[4] + R(2) * ux * ne[7] + uy * ne[8];
+            const R gjy = R(6) * uy * (ne[1] - ne[2]) - ne[6] - R(2) * uy * ne[7] + ux * ne[8];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                R t = R(0);
+#pragma unroll
+                for (int r = 0; r < 9; ++r)
+                    if (kD2Q9Basis.m[r][i] != 0) t += R(kD2Q9Basis.m[r][i]) * nm[r];
+                put(plane_ptr(nuf0, i, plane) + cell,
+                    t + grho + R(D2Q9::cx[i]) * gjx + R(D2Q9::cy[i]) * gjy);
+            }
+        } else if constexpr (CT == kBGK) {
             R al, bx, by;
             feq_transpose<R>(lf, m.ux, m.uy, al, bx, by);
 #pragma unroll
@@ -415,7 +443,7 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags,
     if (e != hipSuccess) return e;
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
-    if (collision != kBGK && collision != kTRT) return hipErrorInvalidValue;
+    if (collision != kBGK && collision != kTRT && collision != kMRT) return hipErrorInvalidValue;
     if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
     const GradLayout L = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid);
@@ -431,6 +459,8 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags,
     };
     using BGK = std::integral_constant<int, kBGK>;
     using TRT = std::integral_constant<int, kTRT>;
-    if (storage == kF64) return collision == kBGK ? run(0.0, BGK{}) : run(0.0, TRT{});
-    return collision == kBGK ? run(0.0f, BGK{}) : run(0.0f, TRT{});
+    using MRT = std::integral_constant<int, kMRT>;
+    if (storage == kF64)
+        return collision == kBGK ? run(0.0, BGK{}) : collision == kTRT ? run(0.0, TRT{}) : run(0.0, MRT{});
+    return collision == kBGK ? run(0.0f, BGK{}) : collision == kTRT ? run(0.0f, TRT{}) : run(0.0f, MRT{});
 }

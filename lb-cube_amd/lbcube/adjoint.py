@@ -1,6 +1,6 @@
 """Gradients of the 2D reactive loop through the HIP discrete adjoint.
 
-The loop is D2Q9 (BGK or TRT) + two D2Q5 species with A + B -> C, stepped by
+The loop is D2Q9 (BGK, TRT or MRT) + two D2Q5 species with A + B -> C, stepped by
 lbc_reactive_fused_step; the loss is the domain-mean product <C>(T) =
 <A>(0) - <A>(T).  `reactive_gradient` calls lbc_reactive_gradient (forward
 with checkpoints every `interval` steps, segment recompute, adjoint sweep);
@@ -19,7 +19,7 @@ import torch
 from . import VELOCITIES, WEIGHTS, _check_pop, _stream, check, default_block, lib, reactive_fused_step
 
 _STORE = {torch.float32: 0, torch.float64: 1}
-_COLL = {"BGK": 0, "TRT": 1}
+_COLL = {"BGK": 0, "TRT": 1, "MRT": 2}
 
 
 def workspace_bytes(dtype, nx, ny, steps, interval, block):

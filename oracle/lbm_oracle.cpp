@@ -956,7 +956,7 @@ template <class S, int DT> void scalar_feq_field(const double* conc, const doubl
 
 // ===========================================================================
 // Discrete adjoint of the 2D reactive loop (project-written; no counterpart in
-// the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK or TRT),
+// the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK, TRT or MRT),
 // then (a_{n+1}, b_{n+1}) = A+B->C step of (a_n, b_n) advected by u(f_{n+1})
 // at the same cell (the order of src/simulate_reaction.cpp:548-554).
 // Adjoint buffers are kept "at the collision cell": nu[i][x] holds the
@@ -1066,7 +1066,40 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                     by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * m.u[1]);
                 }
             };
-            if constexpr (CT == C_BGK) {
+            if constexpr (CT == C_MRT) {
+                // f' = M^-1 [(I - S) M f + S m_eq(rho, j)] with M^-1 = M^T D^-1
+                // (collide_mrt_d2q9, lattice_mrt.hpp:548-603), so
+                // J^T lam = M^T (I - S) mu + (d m_eq / d f)^T S mu, mu = D^-1 M lam,
+                // and d/df_i = d/drho + c_ix d/djx + c_iy d/djy.
+                static const int rows[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},    {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+                                               {4, -2, -2, -2, -2, 1, 1, 1, 1}, {0, 1, 0, -1, 0, 1, -1, -1, 1},
+                                               {0, -2, 0, 2, 0, 1, -1, -1, 1},  {0, 0, 1, 0, -1, 1, 1, -1, -1},
+                                               {0, 0, -2, 0, 2, 1, 1, -1, -1},  {0, 1, -1, 1, -1, 0, 0, 0, 0},
+                                               {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+                const R s[9] = {R(0), R{1.63}, R{1.14}, R(0), R{1.92}, R(0), R{1.92}, omega, omega};
+                R mu[9], nm[9], ne[9];
+                for (int r = 0; r < 9; ++r) {
+                    R acc_r = 0, norm = 0;
+                    for (int i = 0; i < 9; ++i) {
+                        acc_r += R(rows[r][i]) * lf[i];
+                        norm += R(rows[r][i] * rows[r][i]);
+                    }
+                    mu[r] = acc_r / norm;
+                    nm[r] = (R(1) - s[r]) * mu[r];
+                    ne[r] = s[r] * mu[r];
+                }
+                const R ux = m.u[0], uy = m.u[1];
+                const R grho = ne[1] * (R(-2) - R(3) * uu) + ne[2] * (R(1) + R(3) * uu) - ne[7] * (ux * ux - uy * uy) -
+                               ne[8] * ux * uy;
+                const R gjx = R(6) * ux * (ne[1] - ne[2]) - ne[4] + R(2) * ux * ne[7] + uy * ne[8];
+                const R gjy = R(6) * uy * (ne[1] - ne[2]) - ne[6] - R(2) * uy * ne[7] + ux * ne[8];
+                for (int i = 0; i < 9; ++i) {
+                    R t = 0;
+                    for (int r = 0; r < 9; ++r) t += R(rows[r][i]) * nm[r];
+                    const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
+                    nuf0[i * N + cell] = t + grho + cx * gjx + cy * gjy;
+                }
+            } else if constexpr (CT == C_BGK) {
                 R al, bx, by;
                 moments(lf, al, bx, by);
                 for (int i = 0; i < 9; ++i) {
@@ -1360,7 +1393,7 @@ int orc_f16_roundtrip(const float* in, uint16_t* h, float* back, long long n, in
     return 0;
 }
 
-// Reactive loop gradient (D2Q9 BGK/TRT + two D2Q5 species); dt 0 = float, 1 = double.
+// Reactive loop gradient (D2Q9 BGK/TRT/MRT + two D2Q5 species); dt 0 = float, 1 = double.
 int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const void* b0, long long nx, long long ny,
                           double omega, double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb,
                           double* djdwc) {
@@ -1373,6 +1406,9 @@ int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const 
         else if (ct == C_TRT)
             reactive_gradient<double, C_TRT>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
                                              omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
+        else if (ct == C_MRT)
+            reactive_gradient<double, C_MRT>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
+                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
         else
             return -1;
         *J = j;
@@ -1384,6 +1420,9 @@ int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const 
                                             (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
         else if (ct == C_TRT)
             reactive_gradient<float, C_TRT>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
+                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
+        else if (ct == C_MRT)
+            reactive_gradient<float, C_MRT>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
                                             (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
         else
             return -1;

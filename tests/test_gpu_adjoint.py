@@ -26,7 +26,7 @@ def dev(x, gpu, dtype=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(x)).to(gpu).to(dtype)
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
 @pytest.mark.parametrize("interval", [1, 5, 64])
 def test_adjoint_matches_oracle_fp64(gpu, coll, interval):
     print(f"case adjoint {coll} interval={interval}", flush=True)

@@ -1,4 +1,4 @@
-"""The oracle's FP64 discrete adjoint of the 2D reactive loop (D2Q9 BGK/TRT +
+"""The oracle's FP64 discrete adjoint of the 2D reactive loop (D2Q9 BGK/TRT/MRT +
 two D2Q5 species, A + B -> C), pinned by central finite differences of the
 oracle's own forward loop.  The reference has no adjoint; this CPU adjoint is
 the checker for the HIP adjoint kernel."""
@@ -42,7 +42,7 @@ def test_reaction_map_derivatives_against_fd():
     assert np.max(np.abs(da - sa)) < 1e-8 and np.max(np.abs(db - sb)) < 1e-8
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
 def test_adjoint_matches_central_differences(coll):
     f, a, b = small_state()
     omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 12
