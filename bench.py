@@ -155,19 +155,30 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
     graph.replay()
     t_fwd = _time(torch, lambda: [graph.replay() for _ in range(steps // 64)])
     block = lbcube.default_block("D2Q9", n, n)
-    ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block), dtype=torch.uint8,
-                     device="cuda")
-    ADJ.reactive_gradient("BGK", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws, fast=True)
-    out = {}
-    t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("BGK", f0, a0, b0, steps, omega, omega_c, k,
-                                                                     interval, workspace=ws, fast=True)))
-    loss = out["r"][0]
     del graph, st
+
+    def gradient_time(iv):
+        ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, iv, block), dtype=torch.uint8,
+                         device="cuda")
+        ADJ.reactive_gradient("BGK", f0, a0, b0, 64, omega, omega_c, k, iv, workspace=ws, fast=True)
+        out = {}
+        t = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("BGK", f0, a0, b0, steps, omega, omega_c, k, iv,
+                                                                    workspace=ws, fast=True)))
+        gb = ws.numel() / 1e9
+        del ws
+        torch.cuda.empty_cache()
+        return t, out["r"][0], gb
+
+    t_grad, loss, ws_gb = gradient_time(interval)
+    # every state kept (interval 1: no recompute), 4096 x 19.9 MB in HBM
+    t_all, _, ws_all_gb = gradient_time(1)
     return {"workload": f"D2Q9 BGK + 2xD2Q5 A+B->C, {n}^2, {steps} steps, FP32 (fast forward), checkpoint every "
                         f"{interval}",
             "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
             "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
-            "loss": loss}
+            "workspace_gb": ws_gb, "loss": loss,
+            "every_state_kept": {"workspace_gb": ws_all_gb, "gradient_ms_per_step": t_all / steps,
+                                 "adjoint_over_forward": (t_all - t_fwd) / t_fwd}}
 
 
 def ensemble_rate(torch, steps=1000):
@@ -201,37 +212,58 @@ def ensemble_rate(torch, steps=1000):
 
 
 def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
-    """2048^2 D2Q9 MRT double shear layer + two D2Q5 species, fused reactive
-    step, FP32 and FP64, with reactive statistics every 50 steps."""
+    """2048^2 D2Q9 MRT double shear layer + two D2Q5 species (Da = 10), FP32
+    and FP64: the fused single-pass step (152 / 304 B per cell), the reference's
+    two launches per step (fluid step, then reaction step: 188 / 376 B per
+    cell), and the whole loop with the reactive statistics every 50 steps."""
     from lbcube import init as lbinit
 
     out = {}
     u = lbinit.double_shear_layer(n, u0=0.04, delta=80.0)
     ca, cb = lbinit.tanh_half_planes(n, delta=80.0)
     k = 10.0 * 0.04 / 80.0
-    for storage, bpc in (("f32", 152), ("f64", 304)):
+    for storage, es in (("f32", 4), ("f64", 8)):
         f = lbinit.equilibrium("D2Q9", None, u, storage)
         a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage)
         b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage)
         ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
         state = [f, ft, a, at, b, bt]
 
-        def step(state=state):
+        def fused(state=state):
             f, ft, a, at, b, bt = state
             lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k)
             state[:] = [ft, f, at, a, bt, b]
 
+        def unfused(state=state):
+            f, ft, a, at, b, bt = state
+            lbcube.fluid_step("D2Q9", "MRT", f, ft, 1 / 0.6)
+            lbcube.reaction_step("D2Q9", "D2Q5", a, at, b, bt, ft, 1 / 0.6, k)
+            state[:] = [ft, f, at, a, bt, b]
+
+        def loop(state=state):
+            for s in range(1, steps + 1):
+                fused(state)
+                if s % 50 == 0:
+                    lbcube.reactive_stats("D2Q5", state[2], state[4], k)
+
         for _ in range(4):
-            step()
-        ms = _time(torch, step, reps=steps)
+            fused()
+        ms = _time(torch, fused, reps=steps)
+        ms_unfused = _time(torch, unfused, reps=steps)
+        ms_loop = _time(torch, loop) / steps
         stats = lbcube.reactive_stats("D2Q5", state[2], state[4], k)
         ms_stats = _time(torch, lambda: lbcube.reactive_stats("D2Q5", state[2], state[4], k), reps=5)
-        out[storage] = {"mlups": n * n / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * n * n / (ms * 1e-3) / 1e9,
-                        "bytes_per_cell": bpc, "stats_ms": ms_stats, "segregation_intensity":
-                        stats["segregation_intensity"]}
+        cells = n * n
+        out[storage] = {"mlups": cells / (ms * 1e-3) / 1e6, "achieved_gbs": 38 * es * cells / (ms * 1e-3) / 1e9,
+                        "bytes_per_cell": 38 * es,
+                        "unfused_mlups": cells / (ms_unfused * 1e-3) / 1e6,
+                        "unfused_achieved_gbs": 47 * es * cells / (ms_unfused * 1e-3) / 1e9,
+                        "unfused_bytes_per_cell": 47 * es,
+                        "loop_with_stats_every_50_mlups": cells / (ms_loop * 1e-3) / 1e6,
+                        "stats_ms": ms_stats, "segregation_intensity": stats["segregation_intensity"]}
         del f, ft, a, at, b, bt, state
         torch.cuda.empty_cache()
-    out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C fused, {n}^2 double shear layer, Da=10"
+    out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C, {n}^2 double shear layer, Da=10 (k = Da u0 / delta = 0.005)"
     return out
 
 

@@ -365,8 +365,11 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
         }
     }
     // J = <A>(0) - <A>(T)
-    hipLaunchKernelGGL((sum_kernel<R>), dim3(1), dim3(256), 0, st, a0, 5 * cells, red);
-    hipLaunchKernelGGL((sum_kernel<R>), dim3(1), dim3(256), 0, st, last + 9 * cells, 5 * cells, red + 1);
+    // (fixed-order two-level sums: 512 per-block partials in red[512..1023], then one block)
+    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, a0, 5 * cells, red + 512);
+    hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red);
+    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, last + 9 * cells, 5 * cells, red + 512);
+    hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red + 1);
     // adjoint initial condition: dJ/da_T = -1/N everywhere
     const R inv_n = R(1) / static_cast<R>(cells);
     const unsigned fb = static_cast<unsigned>(std::min<size_t>(4096, (se + 255) / 256));
