@@ -380,15 +380,15 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     if ((e = hipGetLastError()) != hipSuccess) return e;
     for (size_t sgp = L.nseg; sgp-- > 0;) {
         const int len = static_cast<int>(std::min<size_t>(interval, steps - sgp * interval));
-        // recompute the segment's states 0..len-1 into seg slots (slot 0 = checkpoint)
-        if ((e = hipMemcpyAsync(seg, ckpt + sgp * se, se * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess)
-            return e;
+        // recompute the segment's states 1..len-1 into seg slots 1..len-1; state 0
+        // is the checkpoint itself (no copy: with interval 1 nothing is recomputed)
+        R* const base = ckpt + sgp * se;
+        auto state_at = [&](int t) { return t == 0 ? base : seg + static_cast<size_t>(t) * se; };
         for (int t = 0; t + 1 < len; ++t) {
-            if ((e = step(State<R>(seg + t * se, cells), State<R>(seg + (t + 1) * se, cells))) != hipSuccess)
-                return e;
+            if ((e = step(State<R>(state_at(t), cells), State<R>(state_at(t + 1), cells))) != hipSuccess) return e;
         }
         for (int t = len - 1; t >= 0; --t) {
-            State<R> fw(seg + t * se, cells), in(nu1, cells), out(nu0, cells);
+            State<R> fw(state_at(t), cells), in(nu1, cells), out(nu0, cells);
             if (s.threads() <= 256)
                 hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a, fw.b,
                                    in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k, part);
