@@ -178,14 +178,17 @@ int lbc_ensemble_destroy(lbc_ensemble* ensemble);
 
 /* Fluid integrals of log_diagnostics (include/lattice_io.hpp:182-234):
  * out[0] = sum rho, out[1] = sum rho |u|^2 / 2, out[2] = max |u|.
- * Accumulated in double on the device; synchronizes `stream`. */
+ * Accumulated in double on the device in a fixed order (deterministic);
+ * synchronizes `stream`.  Allocates a small per-device workspace on its first
+ * call on a device; statistics calls are serialized across host threads. */
 int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
                           double* out3, void* stream);
 
 /* Reactive statistics of compute_reactive_stats (include/lattice_io.hpp:256-338):
  * out = {mean_A, mean_B, var_A, var_B, covariance, segregation_intensity,
  *        true_reaction_rate, mixed_reaction_rate}.
- * Accumulated in double on the device; synchronizes `stream`. */
+ * Accumulated in double on the device in a fixed order; synchronizes `stream`;
+ * shares the workspace and serialization of lbc_fluid_diagnostics. */
 int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b, uint64_t nx,
                        uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream);
 

@@ -152,9 +152,14 @@ __global__ __launch_bounds__(kStatThreads) void finish_kernel(const double* part
     }
 }
 
-// Per-device workspace: kStatBlocks partial slots plus the result slots.
+// Per-device workspace: kStatBlocks partial slots plus the result slots,
+// allocated on the first call on a device.  Every statistics call holds
+// `stats_mutex` from launch to read-back (each call synchronizes its stream
+// before returning), so calls from several host threads or streams never
+// share the workspace concurrently.
+std::mutex stats_mutex;
+
 double* workspace(hipError_t& err) {
-    static std::mutex mu;
     static double* ws[64] = {nullptr};
     int dev = 0;
     err = hipGetDevice(&dev);
@@ -163,7 +168,6 @@ double* workspace(hipError_t& err) {
         err = hipErrorInvalidDevice;
         return nullptr;
     }
-    std::lock_guard<std::mutex> lock(mu);
     if (ws[dev] == nullptr) {
         err = hipMalloc(reinterpret_cast<void**>(&ws[dev]), (kStatBlocks + 1) * kStatSlots * sizeof(double));
         if (err != hipSuccess) {
@@ -182,6 +186,7 @@ hipError_t read_back(const double* dev_result, double* host, int n, hipStream_t 
 
 template <class L, int ST>
 hipError_t run_fluid_diag(const void* f, uint32_t cells, double* out, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(stats_mutex);
     hipError_t e;
     double* ws = workspace(e);
     if (!ws) return e;
@@ -196,6 +201,7 @@ hipError_t run_fluid_diag(const void* f, uint32_t cells, double* out, hipStream_
 
 template <class S, int ST>
 hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double* out, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(stats_mutex);
     hipError_t e;
     double* ws = workspace(e);
     if (!ws) return e;
