@@ -112,6 +112,109 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
     }
 }
 
+// D3Q27 MRT on deviations.  The reference's D3Q27 equilibrium moments are the
+// moments of feq (compute_equilibrium_moments_d3q27, lattice_mrt.hpp:1242-1268),
+// so its collision f' = M^-1 [m - S (m - M feq)] equals f - M^-1 S M (f - feq):
+// one separable forward transform of f_neq = f - feq (three passes of the 1D
+// basis {1, c, 3c^2 - 2}, integer coefficients), the rates and the inverse
+// norms 1/(n_a n_b n_c) folded into one factor per moment, and one separable
+// inverse transform (integer coefficients again).  Rates as the reference's
+// (:161-204, :1284-1305): bulk s_b 1.19 on m4 = X + Y + Z, shear s_nu on
+// m5 = 2X - Y - Z, m6 = Y - Z and the three off-diagonal second moments,
+// s_e 1.1, s_eps 1.2, s_q 1.2 on the rest; conserved moments do not relax.
+// About a quarter of the reference order's VALU work (its inverse divides).
+__host__ __device__ constexpr int g27(int a, int b, int c) { return (a * 3 + b) * 3 + c; }
+
+template <class R> __device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R s_nu) {
+    using L = D3Q27;
+    const DevMacro<L, R> m = dev_macro<L, R>(d);
+    const R r3 = R(3) * m.rho;
+    R p[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+        const R cu = dev_cu<L, R>(i, m);
+        const R w = R(L::w[i]);
+        const R deq = fmadd(w, fmadd(m.rho, fmadd(R(4.5) * cu, cu, m.m15uu), m.drho), (w * r3) * cu);
+        p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)] = d[i] - deq;
+    }
+    // forward: along x, then y, then z; per line (q-1, q0, q+1) -> (sum, c, 3c^2 - 2)
+    auto fwd = [](R& lo, R& mid, R& hi) {
+        const R s = lo + hi;
+        const R m0 = s + mid, m1 = hi - lo, m2 = fmadd(R(-2), mid, s);
+        lo = m0;
+        mid = m1;
+        hi = m2;
+    };
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(j, 0, k)], p[g27(j, 1, k)], p[g27(j, 2, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(j, k, 0)], p[g27(j, k, 1)], p[g27(j, k, 2)]);
+    // p[g27(a, b, c)] now holds the raw moment (a, b, c) of f_neq.  Relaxed and
+    // normalized amounts (n_0 = 3, n_1 = 2, n_2 = 6):
+    constexpr double n[3] = {3.0, 2.0, 6.0};
+    const R s_b = R(1.19), s_e = R(1.1), s_eps = R(1.2), s_q = R(1.2);
+    {
+        const R X = p[g27(2, 0, 0)], Y = p[g27(0, 2, 0)], Z = p[g27(0, 0, 2)];
+        const R dm4 = s_b * (X + Y + Z), dm5 = s_nu * (R(2) * X - Y - Z), dm6 = s_nu * (Y - Z);
+        constexpr double k = 1.0 / 54.0;  // 1 / (n_2 n_0 n_0)
+        p[g27(2, 0, 0)] = (dm4 + dm5) * R(k / 3.0);
+        p[g27(0, 2, 0)] = fmadd(R(3), dm6, R(2) * dm4 - dm5) * R(k / 6.0);
+        p[g27(0, 0, 2)] = fmadd(R(-3), dm6, R(2) * dm4 - dm5) * R(k / 6.0);
+    }
+    p[g27(0, 0, 0)] = p[g27(1, 0, 0)] = p[g27(0, 1, 0)] = p[g27(0, 0, 1)] = R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int order = a + b + c;
+                const int nonzero = (a > 0) + (b > 0) + (c > 0);
+                if (order <= 1) continue;                     // conserved
+                if (order == 2 && nonzero == 1) continue;     // the diagonal second moments, above
+                // the off-diagonal second moments are shear; (1,1,1) and the
+                // other third-order moments are s_e or s_q as in the reference's
+                // row order: rows 10-15 (a 2 and a 1) s_e, row 16 (1,1,1) s_eps,
+                // rows 17-26 (orders >= 4, and (2,1,1)-type) s_q.
+                R rate;
+                if (order == 2) rate = s_nu;
+                else if (order == 3 && nonzero == 2) rate = s_e;
+                else if (order == 3) rate = s_eps;
+                else rate = s_q;
+                p[g27(a, b, c)] *= rate * R(1.0 / (n[a] * n[b] * n[c]));
+            }
+    // inverse: along z, then y, then x; per line (m0, m1, m2) -> (m0 - m1 + m2, m0 - 2 m2, m0 + m1 + m2)
+    auto inv = [](R& lo, R& mid, R& hi) {
+        const R m0 = lo, m1 = mid, m2 = hi;
+        const R t = m0 + m2;
+        lo = t - m1;
+        mid = fmadd(R(-2), m2, m0);
+        hi = t + m1;
+    };
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(j, k, 0)], p[g27(j, k, 1)], p[g27(j, k, 2)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(j, 0, k)], p[g27(j, 1, k)], p[g27(j, 2, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) d[i] -= p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)];
+}
+
 // ---- scalar ADR and A + B -> C for the fast fused reactive step
 
 // g_i += omega_c (w_i C (1 + c.u / cs2) - g_i) + w_i S with 1/cs2 (3 for D2Q5,

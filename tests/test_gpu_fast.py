@@ -46,6 +46,35 @@ def test_fast_against_oracle(gpu, lattice, coll, storage):
     assert np.max(np.abs(got - want)) <= FAST_TOL[storage]
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+def test_fast_d3q27_mrt_against_oracle(gpu, storage):
+    """D3Q27 MRT as f - M^-1 S M (f - feq) with separable transforms, against
+    the reference-order oracle (tolerances as for BGK/TRT)."""
+    print(f"case fast D3Q27 MRT {storage}", flush=True)
+    f0 = cases.random_perturbed("D3Q27", SHAPE["D3Q27"], storage, seed=13)
+    for omega in (1.0 / 0.57, 1.0 / 0.9):
+        a = to_dev(f0, storage, gpu)
+        b = torch.empty_like(a)
+        for _ in range(6):
+            lbcube.fluid_step("D3Q27", "MRT", a, b, omega, fast=True)
+            a, b = b, a
+        torch.cuda.synchronize()
+        got = O.decode("D3Q27", storage, to_host(a, storage))
+        want = O.decode("D3Q27", storage, O.fluid_run("D3Q27", "MRT", storage, f0, omega, 6))
+        assert np.max(np.abs(got - want)) <= FAST_TOL[storage], (omega, np.max(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19"])
+def test_fast_flag_ignored_by_other_mrt(gpu, lattice):
+    """LBC_FLAG_FAST leaves D2Q9 and D3Q19 MRT in the reference order (bit-exact)."""
+    f0 = cases.random_perturbed(lattice, SHAPE[lattice], "f32", seed=14)
+    a = to_dev(f0, "f32", gpu)
+    b = torch.empty_like(a)
+    lbcube.fluid_step(lattice, "MRT", a, b, 1.7, fast=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(b, "f32"), O.fluid_run(lattice, "MRT", "f32", f0, 1.7, 1))
+
+
 def test_fast_f64_conserves_mass(gpu):
     f0 = cases.random_perturbed("D3Q19", (16, 16, 16), "f64", seed=3)
     a = torch.from_numpy(f0).to(gpu)
@@ -55,6 +84,13 @@ def test_fast_f64_conserves_mass(gpu):
         lbcube.fluid_step("D3Q19", "TRT", a, b, 1.9, fast=True)
         a, b = b, a
     assert abs(a.sum().item() - m0) <= 1e-12 * m0
+    f27 = torch.from_numpy(cases.random_perturbed("D3Q27", (12, 12, 12), "f64", seed=4)).to(gpu)
+    g27 = torch.empty_like(f27)
+    m27 = f27.sum().item()
+    for _ in range(100):
+        lbcube.fluid_step("D3Q27", "MRT", f27, g27, 1.9, fast=True)
+        f27, g27 = g27, f27
+    assert abs(f27.sum().item() - m27) <= 1e-12 * m27
 
 
 def run_tgv(n, steps, coll, storage, fast, u0=0.05, sample=50, lattice="D3Q19", tau=None):
