@@ -92,7 +92,8 @@ def measured_traffic():
 
 def lattice_sweep(lbcube, torch, steps=10, warmup=3):
     """Taylor-Green Re = 1600 on D3Q19 and D3Q27 at 256^3 and 384^3, TRT and
-    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT)."""
+    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT; D3Q27
+    MRT in both forms)."""
     from lbcube import init as lbinit
 
     out = {}
@@ -100,7 +101,10 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
         omega = 1.0 / lbinit.tgv_re1600_tau(n)
         for lattice in ("D3Q19", "D3Q27"):
-            for coll, storage, fast in (("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False)):
+            combos = [("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False)]
+            if lattice == "D3Q27":  # the deviation-form MRT exists for D3Q27
+                combos += [("MRT", "f32", True), ("MRT", "f16", True)]
+            for coll, storage, fast in combos:
                 key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}"
                 r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast)
                 out[key] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
