@@ -197,18 +197,21 @@ def test_mrt_reduced_storage_error_against_fp64(gpu):
 
 
 def test_d3q27_mrt_reduced_storage_error_against_fp64(gpu):
-    """D3Q27 MRT (its equilibrium moments are those of feq, and it is stable at
-    Re = 1600), Taylor-Green 256^3, 2000 steps: the reference order in FP32,
-    the deviation form in FP32 and FP16, against the FP64 reference order."""
+    """D3Q27 MRT, Taylor-Green at tau = 0.6, 256^3, 2000 steps: the reference
+    order in FP32, the deviation form in FP32 and FP16, against the FP64
+    reference order.  (At the Re = 1600 setting this MRT is marginally stable:
+    FP64 at 256^3 holds for 2000 steps, but FP32 runs of either form and FP64
+    at 128^3 jump to 10-20x the kinetic energy between steps 800 and 1400, so
+    no precision can be read off there.)"""
     lattice, n, steps = "D3Q27", 256, 2000
     print(f"case precision MRT {lattice} {n}^3", flush=True)
-    ref, ke_ref = run_tgv(n, steps, "MRT", "f64", fast=False, lattice=lattice)
+    ref, ke_ref = run_tgv(n, steps, "MRT", "f64", fast=False, lattice=lattice, tau=0.6)
     u_ref = velocity(lattice, ref)
     del ref
     norm = torch.sqrt((u_ref * u_ref).sum()).item()
     report = {}
     for storage, fast in (("f32", False), ("f32", True), ("f16", True)):
-        f, ke = run_tgv(n, steps, "MRT", storage, fast, lattice=lattice)
+        f, ke = run_tgv(n, steps, "MRT", storage, fast, lattice=lattice, tau=0.6)
         u = velocity(lattice, f)
         del f
         l2 = torch.sqrt(((u - u_ref) ** 2).sum()).item() / norm
