@@ -92,8 +92,8 @@ def measured_traffic():
 
 def lattice_sweep(lbcube, torch, steps=10, warmup=3):
     """Taylor-Green Re = 1600 on D3Q19 and D3Q27 at 256^3 and 384^3, TRT and
-    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT; D3Q27
-    MRT in both forms)."""
+    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT; MRT in
+    both forms)."""
     from lbcube import init as lbinit
 
     out = {}
@@ -101,9 +101,8 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
         omega = 1.0 / lbinit.tgv_re1600_tau(n)
         for lattice in ("D3Q19", "D3Q27"):
-            combos = [("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False)]
-            if lattice == "D3Q27":  # the deviation-form MRT exists for D3Q27
-                combos += [("MRT", "f32", True), ("MRT", "f16", True)]
+            combos = [("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False), ("MRT", "f32", True),
+                      ("MRT", "f16", True)]
             for coll, storage, fast in combos:
                 key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}"
                 r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast)
@@ -304,7 +303,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", default="trt_f32,trt_f16fast,trt_f32fast,trt_f16,bgk_f32,bgk_f16fast,mrt_f32,"
-                                          "mrt_f16,trt_bf16fast")
+                                          "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
     args = ap.parse_args()

@@ -57,12 +57,14 @@ int lbc_fluid_step(int lattice, int collision, int storage, const void* current,
 
 /* Flags of lbc_fluid_step_ex. */
 enum lbc_flags {
-    /* BGK and TRT on every lattice, MRT on D3Q27 (ignored by the other MRTs
-     * and by RLBM): deviation-form arithmetic (f_i - w_i carried through the
-     * collision) with fused multiply-adds and a hardware reciprocal of the
-     * density; D3Q27 MRT as f - M^-1 S M (f - feq) with separable transforms.
-     * Half (MRT: a quarter) of the VALU work of the reference-order form;
-     * results differ from it by rounding only (bounded by the tests). */
+    /* BGK and TRT on every lattice, MRT on D3Q19 and D3Q27 (ignored by the
+     * D2Q9 MRT and by RLBM): deviation-form arithmetic (f_i - w_i carried
+     * through the collision) with fused multiply-adds and a hardware
+     * reciprocal of the density; D3Q19 MRT transforms only the five shear
+     * moments and sets the others from the equilibrium; D3Q27 MRT as
+     * f - M^-1 S M (f - feq) with separable transforms.  Half (D3Q27 MRT: a
+     * quarter) of the VALU work of the reference-order form; results differ
+     * from it by rounding only (bounded by the tests). */
     LBC_FLAG_FAST = 1u
 };
 

@@ -24,7 +24,7 @@ __device__ __forceinline__ float fmadd(float a, float b, float c) { return __bui
 __device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 template <class L, class R> struct DevMacro {
-    R drho, rho, ux, uy, uz, m15uu;
+    R drho, rho, jx, jy, jz, ux, uy, uz, m15uu;
 };
 
 template <class L, class R> __device__ __forceinline__ DevMacro<L, R> dev_macro(const R* d) {
@@ -46,6 +46,9 @@ template <class L, class R> __device__ __forceinline__ DevMacro<L, R> dev_macro(
     DevMacro<L, R> m;
     m.drho = s;
     m.rho = R(1) + s;
+    m.jx = jx;
+    m.jy = jy;
+    m.jz = jz;
     const R inv = fast_rcp(m.rho);
     m.ux = jx * inv;
     m.uy = jy * inv;
@@ -213,6 +216,74 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R
         for (int k = 0; k < 3; ++k) inv(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
 #pragma unroll
     for (int i = 0; i < 27; ++i) d[i] -= p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)];
+}
+
+// D3Q19 MRT on deviations, with the reference's rates and equilibrium
+// moments (collide_mrt_d3q19, lattice_mrt.hpp:908-1009: rho and j conserved,
+// the five shear moments 9-13 relaxed with omega toward their equilibria --
+// including the reference's factor 3 on moment 9 -- every other moment set to
+// its equilibrium, ghosts to zero).  Since M w = m_eq(1, 0) and the rest state
+// has no shear, f' - w = M^-1 Delta with Delta_r = m'_r - m_r(w): the conserved
+// deviations, eq_r(rho, u) - eq_r(1, 0) for the rate-1 moments, and
+// (1 - omega) m_r(d) + omega eq_r for the shear moments.  Only the five shear
+// moments of d are transformed; the inverse uses the 14 non-ghost rows, with
+// coefficients M[r][i] / |M_r|^2 folded at compile time for the non-conserved
+// rows.  About half the VALU work of the reference order.
+template <class R> __device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R omega) {
+    using L = D3Q19;
+    constexpr const Basis<19>& B = kD3Q19Basis;
+    const DevMacro<L, R> m = dev_macro<L, R>(d);
+    const R rho = m.rho, ux = m.ux, uy = m.uy, uz = m.uz;
+    const R ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    const R usq = ux2 + uy2 + uz2;
+    R delta[14];
+    // the conserved moments pass through unchanged (not rho * u, whose
+    // rounding would bias the momentum step after step)
+    delta[0] = m.drho;
+    delta[3] = m.jx;
+    delta[5] = m.jy;
+    delta[7] = m.jz;
+    delta[1] = fmadd(R(19) * rho, usq, R(-11) * m.drho);
+    delta[2] = fmadd(R(-5.5) * rho, usq, R(3) * m.drho);
+    delta[4] = R(-2.0 / 3.0) * delta[3];
+    delta[6] = R(-2.0 / 3.0) * delta[5];
+    delta[8] = R(-2.0 / 3.0) * delta[7];
+    const R eq[5] = {R(3) * rho * (R(2) * ux2 - uy2 - uz2), rho * (uy2 - uz2), rho * ux * uy, rho * uy * uz,
+                     rho * ux * uz};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        R sm = R(0);
+#pragma unroll
+        for (int i = 0; i < 19; ++i)
+            if (B.m[9 + k][i] != 0) sm = fmadd(R(B.m[9 + k][i]), d[i], sm);
+        delta[9 + k] = fmadd(omega, eq[k] - sm, sm);
+    }
+    // Conserved moments: 1/|M_r|^2 (1/19, 1/10) rounded to R is off by up to
+    // 1.5e-8 relative in FP32, which would scale mass and momentum by that
+    // factor every step; split it into hi + lo so the scaled moment is rounded
+    // once and the drift stays at rounding noise.  The other moments are reset
+    // or relaxed each step, so their folded coefficients cannot accumulate.
+    constexpr int kConserved[4] = {0, 3, 5, 7};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int r = kConserved[k];
+        const double inv = 1.0 / B.norm[r];
+        const R hi = R(inv);
+        const R lo = R(inv - static_cast<double>(hi));
+        delta[r] = fmadd(delta[r], hi, delta[r] * lo);
+    }
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        R v = R(0);
+#pragma unroll
+        for (int r = 0; r < 14; ++r) {
+            if (B.m[r][i] == 0) continue;
+            const bool conserved = r == 0 || r == 3 || r == 5 || r == 7;
+            const double coef = conserved ? B.m[r][i] : static_cast<double>(B.m[r][i]) / B.norm[r];
+            v = fmadd(R(coef), delta[r], v);
+        }
+        d[i] = v;
+    }
 }
 
 // ---- scalar ADR and A + B -> C for the fast fused reactive step

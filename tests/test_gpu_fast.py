@@ -47,26 +47,29 @@ def test_fast_against_oracle(gpu, lattice, coll, storage):
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
-def test_fast_d3q27_mrt_against_oracle(gpu, storage):
-    """D3Q27 MRT as f - M^-1 S M (f - feq) with separable transforms, against
-    the reference-order oracle (tolerances as for BGK/TRT)."""
-    print(f"case fast D3Q27 MRT {storage}", flush=True)
-    f0 = cases.random_perturbed("D3Q27", SHAPE["D3Q27"], storage, seed=13)
+@pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
+def test_fast_3d_mrt_against_oracle(gpu, lattice, storage):
+    """3D MRT on deviations (D3Q19: the five shear moments transformed, the
+    rest set from the equilibrium; D3Q27: f - M^-1 S M (f - feq) with separable
+    transforms), against the reference-order oracle (tolerances as for
+    BGK/TRT)."""
+    print(f"case fast {lattice} MRT {storage}", flush=True)
+    f0 = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=13)
     for omega in (1.0 / 0.57, 1.0 / 0.9):
         a = to_dev(f0, storage, gpu)
         b = torch.empty_like(a)
         for _ in range(6):
-            lbcube.fluid_step("D3Q27", "MRT", a, b, omega, fast=True)
+            lbcube.fluid_step(lattice, "MRT", a, b, omega, fast=True)
             a, b = b, a
         torch.cuda.synchronize()
-        got = O.decode("D3Q27", storage, to_host(a, storage))
-        want = O.decode("D3Q27", storage, O.fluid_run("D3Q27", "MRT", storage, f0, omega, 6))
+        got = O.decode(lattice, storage, to_host(a, storage))
+        want = O.decode(lattice, storage, O.fluid_run(lattice, "MRT", storage, f0, omega, 6))
         assert np.max(np.abs(got - want)) <= FAST_TOL[storage], (omega, np.max(np.abs(got - want)))
 
 
-@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19"])
+@pytest.mark.parametrize("lattice", ["D2Q9"])
 def test_fast_flag_ignored_by_other_mrt(gpu, lattice):
-    """LBC_FLAG_FAST leaves D2Q9 and D3Q19 MRT in the reference order (bit-exact)."""
+    """LBC_FLAG_FAST leaves D2Q9 MRT in the reference order (bit-exact)."""
     f0 = cases.random_perturbed(lattice, SHAPE[lattice], "f32", seed=14)
     a = to_dev(f0, "f32", gpu)
     b = torch.empty_like(a)
@@ -84,13 +87,14 @@ def test_fast_f64_conserves_mass(gpu):
         lbcube.fluid_step("D3Q19", "TRT", a, b, 1.9, fast=True)
         a, b = b, a
     assert abs(a.sum().item() - m0) <= 1e-12 * m0
-    f27 = torch.from_numpy(cases.random_perturbed("D3Q27", (12, 12, 12), "f64", seed=4)).to(gpu)
-    g27 = torch.empty_like(f27)
-    m27 = f27.sum().item()
-    for _ in range(100):
-        lbcube.fluid_step("D3Q27", "MRT", f27, g27, 1.9, fast=True)
-        f27, g27 = g27, f27
-    assert abs(f27.sum().item() - m27) <= 1e-12 * m27
+    for lattice, seed in (("D3Q19", 5), ("D3Q27", 4)):
+        f = torch.from_numpy(cases.random_perturbed(lattice, (12, 12, 12), "f64", seed=seed)).to(gpu)
+        g = torch.empty_like(f)
+        m = f.sum().item()
+        for _ in range(100):
+            lbcube.fluid_step(lattice, "MRT", f, g, 1.9, fast=True)
+            f, g = g, f
+        assert abs(f.sum().item() - m) <= 1e-12 * m, lattice
 
 
 def run_tgv(n, steps, coll, storage, fast, u0=0.05, sample=50, lattice="D3Q19", tau=None):
@@ -171,9 +175,10 @@ def test_mrt_gains_energy_at_re1600(gpu):
 
 
 def test_mrt_reduced_storage_error_against_fp64(gpu):
-    """D3Q19 MRT (reference order in every format), Taylor-Green at tau = 0.6
-    (the Re = 1600 setting diverges for this MRT, see above), 256^3, 2000
-    steps: FP32 and FP16/BF16 deviation storage against the FP64 run."""
+    """D3Q19 MRT, Taylor-Green at tau = 0.6 (the Re = 1600 setting diverges
+    for this MRT, see above), 256^3, 2000 steps: the reference order in FP32,
+    FP16 and BF16 and the deviation form in FP32 and FP16, against the FP64
+    reference order."""
     lattice, n, steps = "D3Q19", 256, 2000
     print(f"case precision MRT {lattice} {n}^3", flush=True)
     ref, ke_ref = run_tgv(n, steps, "MRT", "f64", fast=False, lattice=lattice, tau=0.6)
@@ -181,8 +186,8 @@ def test_mrt_reduced_storage_error_against_fp64(gpu):
     del ref
     norm = torch.sqrt((u_ref * u_ref).sum()).item()
     report = {}
-    for storage in ("f32", "f16", "bf16"):
-        f, ke = run_tgv(n, steps, "MRT", storage, False, lattice=lattice, tau=0.6)
+    for storage, fast in (("f32", False), ("f16", False), ("bf16", False), ("f32", True), ("f16", True)):
+        f, ke = run_tgv(n, steps, "MRT", storage, fast, lattice=lattice, tau=0.6)
         u = velocity(lattice, f)
         del f
         l2 = torch.sqrt(((u - u_ref) ** 2).sum()).item() / norm
@@ -190,10 +195,15 @@ def test_mrt_reduced_storage_error_against_fp64(gpu):
         torch.cuda.empty_cache()
         ke_err = float(np.max(np.abs(ke[:, 1] - ke_ref[:, 1]) / ke_ref[:, 1]))
         mass_drift = abs(ke[-1, 0] - n ** 3) / n ** 3
-        report[storage] = (ke_err, l2)
-        print(f"MRT {lattice} {n}^3 storage {storage}: max KE rel err {ke_err:.3e}, velocity L2 {l2:.3e}, "
-              f"mass drift {mass_drift:.3e}", flush=True)
-    assert report["f32"][0] <= 5e-4 and report["f16"][0] <= 2e-3  # see the D3Q27 study below
+        report[(storage, fast)] = (ke_err, l2)
+        print(f"MRT {lattice} {n}^3 storage {storage} fast={fast}: max KE rel err {ke_err:.3e}, velocity L2 "
+              f"{l2:.3e}, mass drift {mass_drift:.3e}", flush=True)
+    # see the D3Q27 study below for the growth of the relative error.  The
+    # FP32 deviation form measures 1.6e-6; with 1/|M_r|^2 of the conserved
+    # moments rounded to FP32 (1/10 is 1.5e-8 high) it read 5.6e-5, the
+    # momentum growing by that factor every step
+    assert report[("f32", False)][0] <= 5e-4 and report[("f16", False)][0] <= 2e-3
+    assert report[("f32", True)][0] <= 1e-5 and report[("f16", True)][0] <= 2e-3
 
 
 def test_d3q27_mrt_reduced_storage_error_against_fp64(gpu):
