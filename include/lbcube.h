@@ -195,6 +195,30 @@ int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uin
 int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b, uint64_t nx,
                        uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream);
 
+/* Stability guard of compute_max_velocity (src/turbulent_tgv_3d.cpp:149-173):
+ * out[0] = max |u| over the cells, +infinity if any cell's speed is not
+ * finite.  Synchronizes `stream`; shares the workspace and serialization of
+ * lbc_fluid_diagnostics. */
+int lbc_max_speed(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
+                  double* out1, void* stream);
+
+/* Macroscopic fields per cell, the values the VTK writers print
+ * (write_vtk, include/lattice_io.hpp:386-451; write_reaction_vtk,
+ * src/simulate_reaction.cpp:267-333): rho[cell] and u[d * cells + cell]
+ * (d < D) from compute_macro_state (include/lattice_physics.hpp:227-249) at
+ * the storage's arithmetic precision, stored as double in device memory, in
+ * the cells' [Z][Y][X] order.  Copying these back instead of the populations
+ * moves (1 + D) doubles per cell instead of Q values.  Asynchronous on
+ * `stream`; allocates nothing. */
+int lbc_macro_fields(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
+                     double* rho, double* u, void* stream);
+
+/* Concentration per cell of a scalar population buffer (compute_concentration,
+ * include/lattice_physics.hpp:618-627), as double in device memory.
+ * Asynchronous on `stream`; allocates nothing. */
+int lbc_scalar_fields(int scalar_lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
+                      double* concentration, void* stream);
+
 /* Streaming device copy of `bytes` (a multiple of 16) from src to dst: the
  * measured HBM ceiling the benchmark reports next to the spec peak.  No
  * counterpart in the reference. */

@@ -1,6 +1,9 @@
 // Domain statistics on the device: the fluid integrals of log_diagnostics
-// (include/lattice_io.hpp:182-234) and the reactive statistics of
-// compute_reactive_stats (include/lattice_io.hpp:256-338).
+// (include/lattice_io.hpp:182-234), the reactive statistics of
+// compute_reactive_stats (include/lattice_io.hpp:256-338) and the stability
+// guard's maximum speed (compute_max_velocity, src/turbulent_tgv_3d.cpp:149-173);
+// and the per-cell macroscopic fields the VTK writers print
+// (include/lattice_io.hpp:386-451, src/simulate_reaction.cpp:267-333).
 //
 // The reference sums serially on the host in Real after copying the whole
 // population buffers back.  Here every work-item reduces a grid-stride slice
@@ -101,6 +104,29 @@ __global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const type
     block_reduce<5>(v, op, partial + blockIdx.x * kStatSlots);
 }
 
+// compute_max_velocity: the largest |u| over the cells, +inf as soon as one
+// cell's speed is not finite (the reference returns infinity there so the
+// guard stops the run).  Max is exact, so any reduction order gives the same
+// value.
+template <class L, int ST>
+__global__ __launch_bounds__(kStatThreads) void max_speed_kernel(const typename Storage<ST>::S* __restrict__ f,
+                                                                 uint32_t cells, double* partial) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    double v[1] = {0.0};
+    constexpr int op[1] = {1};
+    const size_t plane = cells;
+    for (uint32_t c = blockIdx.x * kStatThreads + threadIdx.x; c < cells; c += kStatBlocks * kStatThreads) {
+        R loc[L::Q];
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) loc[i] = St::load(plane_ptr(f, i, plane) + c, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+        const Macro<R> m = macro_state<L, R>(loc);
+        const double speed = static_cast<double>(sqrt(vel_sq<L, R>(m)));
+        v[0] = isfinite(speed) ? fmax(v[0], speed) : INFINITY;
+    }
+    block_reduce<1>(v, op, partial + blockIdx.x * kStatSlots);
+}
+
 // Sums the per-block partials in a fixed order and derives the reported
 // quantities: N = 3 fluid diagnostics, N = 5 reactive statistics
 // (formulas of include/lattice_io.hpp:309-337 evaluated in double; eps is the
@@ -109,13 +135,14 @@ __global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const type
 template <int N>
 __global__ __launch_bounds__(kStatThreads) void finish_kernel(const double* partial, double* result, uint32_t cells,
                                                               double k_react, double eps) {
-    constexpr int mode = N == 3 ? 0 : 1;
+    // N = 1: the maximum speed; N = 3: fluid diagnostics; N = 5: reactive
+    constexpr int mode = N == 5 ? 1 : 0;
     double v[N];
     int op[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
         v[k] = 0.0;
-        op[k] = (mode == 0 && k == 2) ? 1 : 0;
+        op[k] = (N == 1 || (N == 3 && k == 2)) ? 1 : 0;
     }
     for (int blk = threadIdx.x; blk < kStatBlocks; blk += kStatThreads) {
 #pragma unroll
@@ -132,9 +159,8 @@ __global__ __launch_bounds__(kStatThreads) void finish_kernel(const double* part
     __syncthreads();
     if (threadIdx.x != 0) return;
     if constexpr (mode == 0) {
-        result[0] = tot[0];
-        result[1] = tot[1];
-        result[2] = tot[2];
+#pragma unroll
+        for (int k = 0; k < N; ++k) result[k] = tot[k];
         return;
     } else {
     const double inv = 1.0 / static_cast<double>(cells);
@@ -199,6 +225,75 @@ hipError_t run_fluid_diag(const void* f, uint32_t cells, double* out, hipStream_
     return read_back(ws + kStatBlocks * kStatSlots, out, 3, st);
 }
 
+template <class L, int ST>
+hipError_t run_max_speed(const void* f, uint32_t cells, double* out, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(stats_mutex);
+    hipError_t e;
+    double* ws = workspace(e);
+    if (!ws) return e;
+    hipLaunchKernelGGL((max_speed_kernel<L, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
+                       static_cast<const typename Storage<ST>::S*>(f), cells, ws);
+    hipLaunchKernelGGL((finish_kernel<1>), dim3(1), dim3(kStatThreads), 0, st, ws, ws + kStatBlocks * kStatSlots,
+                       cells, 0.0, 0.0);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return read_back(ws + kStatBlocks * kStatSlots, out, 1, st);
+}
+
+// Per-cell fields, one work-item per cell: density and velocity
+// (compute_macro_state at the storage's arithmetic precision, as the
+// reference's macro_at evaluates it in Real), or a scalar's concentration
+// (compute_concentration), written as double in the cell order of the
+// populations.  They replace the copy of every population to the host that
+// the reference's writers start from: (1 + D) or 1 doubles per cell instead
+// of Q values.
+template <class L, int ST>
+__global__ __launch_bounds__(kStatThreads) void macro_fields_kernel(const typename Storage<ST>::S* __restrict__ f,
+                                                                    uint32_t cells, double* __restrict__ rho,
+                                                                    double* __restrict__ u) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const uint32_t c = blockIdx.x * kStatThreads + threadIdx.x;
+    if (c >= cells) return;
+    const size_t plane = cells;
+    R loc[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) loc[i] = St::load(plane_ptr(f, i, plane) + c, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+    const Macro<R> m = macro_state<L, R>(loc);
+    rho[c] = static_cast<double>(m.rho);
+    u[c] = static_cast<double>(m.ux);
+    u[plane + c] = static_cast<double>(m.uy);
+    if constexpr (L::D == 3) u[2 * plane + c] = static_cast<double>(m.uz);
+}
+
+template <class S, int ST>
+__global__ __launch_bounds__(kStatThreads) void scalar_fields_kernel(const typename Storage<ST>::S* __restrict__ g,
+                                                                     uint32_t cells, double* __restrict__ conc) {
+    using St = Storage<ST>;
+    using R = typename St::R;
+    const uint32_t c = blockIdx.x * kStatThreads + threadIdx.x;
+    if (c >= cells) return;
+    const size_t plane = cells;
+    R loc[S::Q];
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) loc[i] = St::load(plane_ptr(g, i, plane) + c, R(0));
+    conc[c] = static_cast<double>(concentration<S, R>(loc));
+}
+
+template <class L, int ST>
+hipError_t run_macro_fields(const void* f, uint32_t cells, double* rho, double* u, hipStream_t st) {
+    hipLaunchKernelGGL((macro_fields_kernel<L, ST>), dim3((cells + kStatThreads - 1) / kStatThreads),
+                       dim3(kStatThreads), 0, st, static_cast<const typename Storage<ST>::S*>(f), cells, rho, u);
+    return hipGetLastError();
+}
+
+template <class S, int ST>
+hipError_t run_scalar_fields(const void* g, uint32_t cells, double* conc, hipStream_t st) {
+    hipLaunchKernelGGL((scalar_fields_kernel<S, ST>), dim3((cells + kStatThreads - 1) / kStatThreads),
+                       dim3(kStatThreads), 0, st, static_cast<const typename Storage<ST>::S*>(g), cells, conc);
+    return hipGetLastError();
+}
+
 template <class S, int ST>
 hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double* out, hipStream_t st) {
     std::lock_guard<std::mutex> lock(stats_mutex);
@@ -225,6 +320,30 @@ hipError_t fluid_diag_storage(int storage, const void* f, uint32_t cells, double
         case kBF16: return run_fluid_diag<L, kBF16>(f, cells, out, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+// Storage dispatch: RUNNER<LAT, storage code>(args...).
+#define LBC_STORAGE_DISPATCH(RUNNER, LAT, storage, ...)                  \
+    switch (storage) {                                                   \
+        case kF32: return RUNNER<LAT, kF32>(__VA_ARGS__);                \
+        case kF64: return RUNNER<LAT, kF64>(__VA_ARGS__);                \
+        case kF16: return RUNNER<LAT, kF16>(__VA_ARGS__);                \
+        case kBF16: return RUNNER<LAT, kBF16>(__VA_ARGS__);              \
+        default: return hipErrorInvalidValue;                            \
+    }
+
+template <class L> hipError_t max_speed_storage(int storage, const void* f, uint32_t cells, double* out, hipStream_t st) {
+    LBC_STORAGE_DISPATCH(run_max_speed, L, storage, f, cells, out, st)
+}
+
+template <class L>
+hipError_t macro_fields_storage(int storage, const void* f, uint32_t cells, double* rho, double* u, hipStream_t st) {
+    LBC_STORAGE_DISPATCH(run_macro_fields, L, storage, f, cells, rho, u, st)
+}
+
+template <class S>
+hipError_t scalar_fields_storage(int storage, const void* g, uint32_t cells, double* conc, hipStream_t st) {
+    LBC_STORAGE_DISPATCH(run_scalar_fields, S, storage, g, cells, conc, st)
 }
 
 template <class S>
@@ -270,6 +389,56 @@ extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* s
     switch (scalar_lattice) {
         case kD2Q5: return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, out8, st);
         case kD3Q7: return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, out8, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" int lbc_max_speed(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,
+                             uint64_t nz, double* out1, void* stream) {
+    using namespace lbc;
+    if (populations == nullptr || out1 == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (lattice) {
+        case kD2Q9: return max_speed_storage<D2Q9>(storage, populations, cells, out1, st);
+        case kD3Q19: return max_speed_storage<D3Q19>(storage, populations, cells, out1, st);
+        case kD3Q27: return max_speed_storage<D3Q27>(storage, populations, cells, out1, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" int lbc_macro_fields(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,
+                                uint64_t nz, double* rho, double* u, void* stream) {
+    using namespace lbc;
+    if (populations == nullptr || rho == nullptr || u == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (lattice) {
+        case kD2Q9: return macro_fields_storage<D2Q9>(storage, populations, cells, rho, u, st);
+        case kD3Q19: return macro_fields_storage<D3Q19>(storage, populations, cells, rho, u, st);
+        case kD3Q27: return macro_fields_storage<D3Q27>(storage, populations, cells, rho, u, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+extern "C" int lbc_scalar_fields(int scalar_lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,
+                                 uint64_t nz, double* concentration, void* stream) {
+    using namespace lbc;
+    if (populations == nullptr || concentration == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, 1, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (scalar_lattice) {
+        case kD2Q5: return scalar_fields_storage<D2Q5>(storage, populations, cells, concentration, st);
+        case kD3Q7: return scalar_fields_storage<D3Q7>(storage, populations, cells, concentration, st);
         default: return hipErrorInvalidValue;
     }
 }

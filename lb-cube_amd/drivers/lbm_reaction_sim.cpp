@@ -2,8 +2,8 @@
 // driver (src/simulate_reaction.cpp) on the MI355X kernel library, through
 // include/lattice_hip.hpp: a D2Q9 fluid (BGK) and two D2Q5 species reacting
 // A + B -> C, one launch_step_gpu and one launch_reaction_AB_gpu per step
-// (src/simulate_reaction.cpp:516-546), reactive statistics every -out_freq
-// steps.
+// (src/simulate_reaction.cpp:516-546), a VTK snapshot and a statistics row at
+// step 0 and every -out_freq steps.
 //
 // Options (src/simulate_reaction.cpp:109-168, same names, defaults and checks):
 //   -nx N -ny N (256)  -steps N (5000)  -out_freq N (250)  -u0 U (0.04)
@@ -12,9 +12,13 @@
 //   -fused                 one lbc_reactive_fused_step per step (same results)
 //   -load PREFIX / -dump PREFIX   raw float64 populations PREFIX.{f,a,b}
 //   -csv FILE              statistics file (default reaction_diagnostics.csv)
+//   -vtk_dir DIR           folder of the reaction_<step>.vtk snapshots (default .)
+//   -no_vtk                no snapshots
 // Rows: step,mean_A,mean_B,var_A,var_B,covariance,segregation_intensity,
 // true_reaction_rate,mixed_reaction_rate with %.17g (include/lattice_io.hpp:352-368),
-// computed on the device with FP64 accumulation.
+// computed on the device with FP64 accumulation.  Snapshots: C_A, C_B,
+// reaction_rate, velocity (write_reaction_vtk, src/simulate_reaction.cpp:267-333),
+// reconstructed on the device.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -28,6 +32,7 @@
 #include <vector>
 
 #include "lattice_hip.hpp"
+#include "vtk.hpp"
 
 namespace {
 
@@ -42,6 +47,8 @@ struct Config {
     double tau = 0.6, tau_c = 0.8, u0 = 0.04, k_react = 0.05;
     bool use_gpu = true, fused = false;
     std::string load, dump, csv = "reaction_diagnostics.csv";
+    std::string vtk_dir = ".";
+    bool vtk = true;
 };
 
 void check(hipError_t e, const char* what) {
@@ -75,6 +82,8 @@ Config parse(int argc, char** argv) {
         else if (a == "-load") c.load = value(i);
         else if (a == "-dump") c.dump = value(i);
         else if (a == "-csv") c.csv = value(i);
+        else if (a == "-vtk_dir") c.vtk_dir = value(i);
+        else if (a == "-no_vtk") c.vtk = false;
         else throw std::invalid_argument("unknown argument: " + a);
     }
     if (c.steps < 0) throw std::invalid_argument("-steps must be non-negative");
@@ -185,6 +194,18 @@ int main(int argc, char** argv) {
                           out[2], out[3], out[4], out[5], out[6], out[7]);
             csv << row;
         };
+        lbc_vtk::Staging staging(c.nx * c.ny, 2, true);
+        auto snapshot = [&](int s) {
+            if (!c.vtk) return;
+            char name[40];
+            std::snprintf(name, sizeof(name), "/reaction_%06d.vtk", s);
+            const auto fields = lbc_vtk::fluid_fields(staging, FluidLattice::abi, f.cur, c.nx, c.ny, 1);
+            const auto ca = lbc_vtk::concentration(staging, staging.ca, ScalarLattice::abi, a.cur, c.nx, c.ny, 1);
+            const auto cb = lbc_vtk::concentration(staging, staging.cb, ScalarLattice::abi, b.cur, c.nx, c.ny, 1);
+            lbc_vtk::write_reactive(c.vtk_dir + name, "LB-Cube reactive mixing step " + std::to_string(s), false,
+                                    fields, ca, cb, c.k_react, c.nx, c.ny);
+        };
+        snapshot(0);
         log(0);
         const auto start = std::chrono::steady_clock::now();
         for (int s = 1; s <= c.steps; ++s) {
@@ -205,6 +226,7 @@ int main(int argc, char** argv) {
             std::swap(a.cur, a.next);
             std::swap(b.cur, b.next);
             if (s % c.output_frequency == 0) {
+                snapshot(s);
                 log(s);
                 std::cout << "Step " << s << " / " << c.steps << " complete\n" << std::flush;
             }

@@ -2,7 +2,8 @@
 // MI355X kernel library, written against include/lattice_hip.hpp exactly as
 // the reference's driver is written against lattice_cuda.cuh: host populations
 // in a LatticeMemory, device buffers from hipMalloc, one launch_step_gpu per
-// step and a pointer swap, diagnostics rows every -out_freq steps.
+// step and a pointer swap, a VTK snapshot and a diagnostics row at step 0 and
+// every -out_freq steps (src/main.cpp:572-607).
 //
 // Options (src/main.cpp:149-195, same names and checks):
 //   -nx N -ny N -nz N      grid (default 64 x 64 x 64)
@@ -16,8 +17,12 @@
 //   -collision bgk|trt|mrt|rlbm   (default bgk, the reference's default)
 //   -load FILE / -dump FILE  raw float64 populations [19][Z][Y][X] in/out
 //   -csv FILE              diagnostics file (default diagnostics.csv)
+//   -vtk_dir DIR           folder of the lbm_<step>.vtk snapshots (default .)
+//   -no_vtk                no snapshots
 // Diagnostics rows: step,total_mass,total_kinetic_energy,max_velocity_magnitude
 // with %.17g (include/lattice_io.hpp:182-234), summed on the device in FP64.
+// Snapshots: density, velocity_magnitude, velocity (write_vtk,
+// include/lattice_io.hpp:386-451), reconstructed on the device.
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
@@ -31,6 +36,7 @@
 #include <vector>
 
 #include "lattice_hip.hpp"
+#include "vtk.hpp"
 
 namespace {
 
@@ -47,6 +53,8 @@ struct Config {
     double u0 = 0.01;
     std::string collision = "bgk";
     std::string load, dump, csv = "diagnostics.csv";
+    std::string vtk_dir = ".";
+    bool vtk = true;
 };
 
 void check(hipError_t e, const char* what) {
@@ -80,6 +88,8 @@ Config parse(int argc, char** argv) {
         else if (a == "-load") c.load = value(i);
         else if (a == "-dump") c.dump = value(i);
         else if (a == "-csv") c.csv = value(i);
+        else if (a == "-vtk_dir") c.vtk_dir = value(i);
+        else if (a == "-no_vtk") c.vtk = false;
         else throw std::invalid_argument("unknown argument: " + a);
     }
     if (c.steps < 0) throw std::invalid_argument("-steps must be non-negative");
@@ -169,12 +179,21 @@ int main(int argc, char** argv) {
             std::snprintf(row, sizeof(row), "%d,%.17g,%.17g,%.17g\n", s, out[0], out[1], out[2]);
             csv << row;
         };
+        lbc_vtk::Staging staging(count / Lattice::Q, Lattice::D, false);
+        auto snapshot = [&](int s) {
+            if (!c.vtk) return;
+            lbc_vtk::write_fluid(c.vtk_dir, static_cast<std::size_t>(s),
+                                 lbc_vtk::fluid_fields(staging, Lattice::abi, current, c.nx, c.ny, c.nz), c.nx, c.ny,
+                                 c.nz, Lattice::D);
+        };
+        snapshot(0);
         log(0);
         const auto start = std::chrono::steady_clock::now();
         for (int s = 1; s <= c.steps; ++s) {
             check(launch(current, next, c, omega), "launch_step_gpu");
             std::swap(current, next);
             if (s % c.output_frequency == 0) {
+                snapshot(s);
                 log(s);
                 std::cout << "Step " << s << " / " << c.steps << " complete\n" << std::flush;
             }

@@ -150,6 +150,38 @@ def fluid_diagnostics(lattice, f, stream=None):
     return tuple(out)
 
 
+def max_speed(lattice, f, stream=None):
+    """Stability guard (compute_max_velocity): max |u|, inf if any cell's speed
+    is not finite; synchronizes."""
+    _check_pop(lattice, f, "f")
+    nx, ny, nz = _extents(lattice, f)
+    out = (ctypes.c_double * 1)()
+    check(lib().lbc_max_speed(LATTICES[lattice], STORAGE[f.dtype], f.data_ptr(), nx, ny, nz, out, _stream(stream)),
+          "lbc_max_speed")
+    return out[0]
+
+
+def macro_fields(lattice, f, stream=None):
+    """(rho, u): float64 device tensors of shape f.shape[1:] and (D,) + f.shape[1:]."""
+    _check_pop(lattice, f, "f")
+    nx, ny, nz = _extents(lattice, f)
+    rho = torch.empty(f.shape[1:], dtype=torch.float64, device=f.device)
+    u = torch.empty((len(VELOCITIES[lattice][0]),) + tuple(f.shape[1:]), dtype=torch.float64, device=f.device)
+    check(lib().lbc_macro_fields(LATTICES[lattice], STORAGE[f.dtype], f.data_ptr(), nx, ny, nz, rho.data_ptr(),
+                                 u.data_ptr(), _stream(stream)), "lbc_macro_fields")
+    return rho, u
+
+
+def scalar_fields(scalar_lattice, g, stream=None):
+    """Concentration per cell: a float64 device tensor of shape g.shape[1:]."""
+    _check_pop(scalar_lattice, g, "g")
+    nx, ny, nz = _extents(scalar_lattice, g)
+    conc = torch.empty(g.shape[1:], dtype=torch.float64, device=g.device)
+    check(lib().lbc_scalar_fields(LATTICES[scalar_lattice], STORAGE[g.dtype], g.data_ptr(), nx, ny, nz,
+                                  conc.data_ptr(), _stream(stream)), "lbc_scalar_fields")
+    return conc
+
+
 STAT_NAMES = ("mean_A", "mean_B", "var_A", "var_B", "covariance", "segregation_intensity", "true_reaction_rate",
               "mixed_reaction_rate")
 

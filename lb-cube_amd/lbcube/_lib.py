@@ -34,6 +34,9 @@ SIGNATURES = {
                                     c_u64, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, _dp, c_vp], c_int),
+    "lbc_max_speed": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
+    "lbc_macro_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_vp, c_vp, c_vp], c_int),
+    "lbc_scalar_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_vp, c_vp], c_int),
     "lbc_reactive_gradient_workspace": ([c_int, c_u64, c_u64, c_int, c_int, c_u32, c_u32,
                                          ctypes.POINTER(ctypes.c_uint64)], c_int),
     "lbc_reactive_gradient": ([c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_u64, c_u64, c_int, c_int, c_double, c_double,

@@ -923,6 +923,61 @@ template <class L, int DT> void fluid_diagnostics(const void* fv, Grid g, double
     out[2] = static_cast<double>(umax);
 }
 
+// Per-cell fields printed by the VTK writers (macro_at of write_vtk,
+// include/lattice_io.hpp:386-451; concentration_at of write_reaction_vtk,
+// src/simulate_reaction.cpp:253-333): compute_macro_state / compute_concentration
+// in the storage's arithmetic type, widened to double.
+template <class L, int DT> void macro_fields(const void* fv, Grid g, double* rho, double* u) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* f = static_cast<const St*>(fv);
+    const i64 N = g.cells();
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    for (i64 cell = 0; cell < N; ++cell) {
+        R loc[L::Q];
+        for (int i = 0; i < L::Q; ++i) loc[i] = C::load(f[i * N + cell], off ? static_cast<R>(L::w[i]) : R{0});
+        const Macro<L, R> m = macro_state<L, R>(loc);
+        rho[cell] = static_cast<double>(m.rho);
+        for (int d = 0; d < L::D; ++d) u[d * N + cell] = static_cast<double>(m.u[d]);
+    }
+}
+
+template <class S, int DT> void scalar_fields(const void* gv, Grid g, double* conc) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* p = static_cast<const St*>(gv);
+    const i64 N = g.cells();
+    for (i64 cell = 0; cell < N; ++cell) {
+        R loc[S::Q];
+        for (int i = 0; i < S::Q; ++i) loc[i] = C::load(p[i * N + cell], R{0});
+        conc[cell] = static_cast<double>(concentration<S, R>(loc));
+    }
+}
+
+// compute_max_velocity, src/turbulent_tgv_3d.cpp:149-173: the largest |u|
+// (sqrt of the squared norm in the arithmetic type), infinity as soon as one
+// cell's speed is not finite.
+template <class L, int DT> double max_speed(const void* fv, Grid g) {
+    using C = Codec<DT>;
+    using St = typename C::S;
+    using R = typename C::R;
+    const St* f = static_cast<const St*>(fv);
+    const i64 N = g.cells();
+    const bool off = (DT == T_F16 || DT == T_BF16);
+    R best{};
+    for (i64 cell = 0; cell < N; ++cell) {
+        R loc[L::Q];
+        for (int i = 0; i < L::Q; ++i) loc[i] = C::load(f[i * N + cell], off ? static_cast<R>(L::w[i]) : R{0});
+        const Macro<L, R> m = macro_state<L, R>(loc);
+        const R speed = std::sqrt(vel_dot<L, R>(m.u));
+        if (!std::isfinite(speed)) return std::numeric_limits<double>::infinity();
+        best = std::max(best, speed);
+    }
+    return static_cast<double>(best);
+}
+
 // Equilibrium fields from macroscopic fields (initialisers of
 // src/main.cpp:368-411 and src/simulate_reaction.cpp:197-232 go through
 // compute_equilibrium / compute_scalar_equilibrium per cell).
@@ -1366,6 +1421,31 @@ int orc_fluid_diagnostics(int lat, int dt, const void* f, long long nx, long lon
                           double* out) {
     const Grid g = make_grid(lat, nx, ny, nz);
     ORC_FLUID_SWITCH(lat, ORC_DT_SWITCH(dt, (fluid_diagnostics<L, DT>(f, g, out, wide != 0))));
+    return 0;
+}
+
+int orc_macro_fields(int lat, int dt, const void* f, long long nx, long long ny, long long nz, double* rho,
+                     double* u) {
+    const Grid g = make_grid(lat, nx, ny, nz);
+    ORC_FLUID_SWITCH(lat, ORC_DT_SWITCH(dt, (macro_fields<L, DT>(f, g, rho, u))));
+    return 0;
+}
+
+int orc_scalar_fields(int slat, int dt, const void* p, long long nx, long long ny, long long nz, double* conc) {
+    const Grid g = make_grid(slat, nx, ny, nz);
+    if (slat == L_D2Q5) {
+        ORC_DT_SWITCH(dt, (scalar_fields<D2Q5, DT>(p, g, conc)));
+    } else if (slat == L_D3Q7) {
+        ORC_DT_SWITCH(dt, (scalar_fields<D3Q7, DT>(p, g, conc)));
+    } else {
+        return -1;
+    }
+    return 0;
+}
+
+int orc_max_speed(int lat, int dt, const void* f, long long nx, long long ny, long long nz, double* out) {
+    const Grid g = make_grid(lat, nx, ny, nz);
+    ORC_FLUID_SWITCH(lat, ORC_DT_SWITCH(dt, (*out = max_speed<L, DT>(f, g))));
     return 0;
 }
 

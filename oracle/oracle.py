@@ -58,6 +58,9 @@ def lib() -> ctypes.CDLL:
             "orc_reaction_source": [i, vp, vp, d, vp, ll],
             "orc_reactive_stats": [i, i, vp, vp, ll, ll, ll, d, i, dp],
             "orc_fluid_diagnostics": [i, i, vp, ll, ll, ll, i, dp],
+            "orc_macro_fields": [i, i, vp, ll, ll, ll, vp, vp],
+            "orc_scalar_fields": [i, i, vp, ll, ll, ll, vp],
+            "orc_max_speed": [i, i, vp, ll, ll, ll, dp],
             "orc_feq_field": [i, i, vp, vp, vp, ll],
             "orc_scalar_feq_field": [i, i, vp, vp, vp, ll],
             "orc_f16_roundtrip": [vp, vp, vp, ll, i],
@@ -203,6 +206,33 @@ def fluid_diagnostics(lattice, dtype, f, wide=False):
     _check(lib().orc_fluid_diagnostics(LATTICES[lattice], DTYPES[dtype], _p(f), nx, ny, nz, int(wide), out),
            "fluid_diagnostics")
     return np.array(out[:])
+
+
+def macro_fields(lattice, dtype, f):
+    """Per-cell (rho, u) of the VTK writers (lattice_io.hpp:386-451), as float64."""
+    nx, ny, nz = _dims(lattice, f.shape)
+    rho = np.empty(f.shape[1:], dtype=np.float64)
+    u = np.empty((D_OF[lattice],) + tuple(f.shape[1:]), dtype=np.float64)
+    _check(lib().orc_macro_fields(LATTICES[lattice], DTYPES[dtype], _p(f), nx, ny, nz, _p(rho), _p(u)),
+           "macro_fields")
+    return rho, u
+
+
+def scalar_fields(scalar_lattice, dtype, g):
+    """Per-cell concentration (compute_concentration), as float64."""
+    nx, ny, nz = _dims(scalar_lattice, g.shape)
+    conc = np.empty(g.shape[1:], dtype=np.float64)
+    _check(lib().orc_scalar_fields(LATTICES[scalar_lattice], DTYPES[dtype], _p(g), nx, ny, nz, _p(conc)),
+           "scalar_fields")
+    return conc
+
+
+def max_speed(lattice, dtype, f):
+    """compute_max_velocity (src/turbulent_tgv_3d.cpp:149-173): max |u|, inf if any speed is not finite."""
+    out = (ctypes.c_double * 1)()
+    nx, ny, nz = _dims(lattice, f.shape)
+    _check(lib().orc_max_speed(LATTICES[lattice], DTYPES[dtype], _p(f), nx, ny, nz, out), "max_speed")
+    return out[0]
 
 
 def feq_field(lattice, dtype, rho, u):

@@ -94,3 +94,16 @@ def test_default_block():
     assert lbcube.default_block("D3Q19", 256, 256, 256) == (256, 1, 1)
     assert lbcube.default_block("D2Q9", 64, 64, 1) == (256, 1, 1)
     assert lbcube.default_block("D3Q27", 384, 384, 384) == (256, 1, 1)
+
+
+def test_validation_fields_and_guard():
+    L = lbcube.lib()
+    out = (ctypes.c_double * 1)()
+    assert L.lbc_max_speed(1, 0, None, 8, 8, 8, out, None) == HIP_INVALID_DEVICE_POINTER
+    assert L.lbc_max_speed(1, 0, FAKE_A, 8, 0, 8, out, None) == HIP_INVALID_VALUE
+    assert L.lbc_max_speed(4, 0, FAKE_A, 8, 8, 8, out, None) == HIP_INVALID_VALUE  # scalar lattice
+    assert L.lbc_macro_fields(1, 0, FAKE_A, 8, 8, 8, None, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
+    assert L.lbc_macro_fields(1, 0, FAKE_A, 0, 8, 8, FAKE_B, 0x3000, None) == HIP_INVALID_VALUE
+    assert L.lbc_macro_fields(3, 0, FAKE_A, 8, 8, 8, FAKE_B, 0x3000, None) == HIP_INVALID_VALUE
+    assert L.lbc_scalar_fields(3, 0, None, 8, 8, 1, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
+    assert L.lbc_scalar_fields(0, 0, FAKE_A, 8, 8, 1, FAKE_B, None) == HIP_INVALID_VALUE  # fluid lattice

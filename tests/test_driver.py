@@ -1,8 +1,9 @@
-"""The C++ drivers lb-cube_amd/lbm_sim and lbm_reaction_sim (the GPU paths of
-src/main.cpp and src/simulate_reaction.cpp, written against
-include/lattice_hip.hpp): argument checks on the CPU; on the GPU, a run from an
-oracle-made state must end as the oracle's run does (bit-identical fluid; the
-species within exp() ulps) and write the oracle's diagnostics rows."""
+"""The C++ drivers lb-cube_amd/lbm_sim, lbm_reaction_sim and
+lbm_compare_operators (the GPU paths of src/main.cpp, src/simulate_reaction.cpp
+and src/compare_operators.cpp, written against include/lattice_hip.hpp):
+argument checks on the CPU; on the GPU, a run from an oracle-made state must
+end as the oracle's run does (bit-identical fluid; the species within exp()
+ulps) and write the oracle's diagnostics rows and fields."""
 from __future__ import annotations
 
 import os
@@ -21,6 +22,25 @@ REACT_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_reaction_sim")
 
 def run(*args, cwd=None, exe=EXE):
     return subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=120, cwd=cwd)
+
+
+def parse_vtk(path):
+    """Header lines and {field: values} of a legacy ASCII STRUCTURED_POINTS file
+    (scalars as (n,), vectors as (n, 3))."""
+    lines = path.read_text().split("\n")
+    n = int(next(ln for ln in lines if ln.startswith("POINT_DATA")).split()[1])
+    fields, i = {}, 0
+    while i < len(lines):
+        parts = lines[i].split()
+        if parts[:1] == ["SCALARS"]:
+            fields[parts[1]] = np.array([float(t) for t in lines[i + 2:i + 2 + n]])
+            i += 2 + n
+        elif parts[:1] == ["VECTORS"]:
+            fields[parts[1]] = np.array([[float(t) for t in ln.split()] for ln in lines[i + 1:i + 1 + n]])
+            i += 1 + n
+        else:
+            i += 1
+    return lines[:8], fields
 
 
 @pytest.mark.parametrize("args,message", [(["-steps", "-1"], "-steps must be non-negative"),
@@ -58,6 +78,14 @@ def test_run_matches_oracle(gpu, tmp_path, collision):
         assert int(step) == k * freq
         ref = O.fluid_diagnostics("D3Q19", "f64", state, wide=True)
         assert np.allclose([float(mass), float(ke), float(umax)], ref, rtol=1e-13, atol=0)
+        # the snapshot of the same step: fields reconstructed on the device,
+        # printed with 17 digits, so they read back bit-identical
+        head, fields = parse_vtk(tmp_path / f"lbm_{k * freq:06d}.vtk")
+        assert head[1] == f"LBM output step {k * freq}" and head[4] == "DIMENSIONS 12 10 8"
+        rho, u = O.macro_fields("D3Q19", "f64", state)
+        assert np.array_equal(fields["density"], rho.ravel())
+        assert np.array_equal(fields["velocity"], u.reshape(3, -1).T)
+        assert np.array_equal(fields["velocity_magnitude"], np.sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]).ravel())
         state = O.fluid_run("D3Q19", collision.upper(), "f64", state, 1.0 / tau, freq)
     assert "MLUPS" in r.stdout
 
@@ -71,6 +99,19 @@ def test_reaction_argument_errors(args, message):
     assert os.path.exists(REACT_EXE), "build first: make -C lb-cube_amd"
     r = run(*args, exe=REACT_EXE)
     assert r.returncode == 1 and message in r.stderr
+
+
+def check_reactive_fields(fields, state, k):
+    """VTK fields of a reactive snapshot against the oracle's state: the fluid
+    velocity bit-identical; the species within exp() ulps of the oracle."""
+    _, u = O.macro_fields("D2Q9", "f64", state[0])
+    assert np.array_equal(fields["velocity"][:, :2], u.reshape(2, -1).T)
+    assert np.all(fields["velocity"][:, 2] == 0)
+    ca = O.scalar_fields("D2Q5", "f64", state[1]).ravel()
+    cb = O.scalar_fields("D2Q5", "f64", state[2]).ravel()
+    assert np.allclose(fields["C_A"], ca, rtol=1e-12, atol=1e-15)
+    assert np.allclose(fields["C_B"], cb, rtol=1e-12, atol=1e-15)
+    assert np.allclose(fields["reaction_rate"], k * ca * cb, rtol=1e-12, atol=1e-15)
 
 
 @pytest.mark.gpu
@@ -101,4 +142,131 @@ def test_reaction_run_matches_oracle(gpu, tmp_path, fused):
         assert int(vals[0]) == j * freq
         ref = O.reactive_stats("D2Q5", "f64", state[1], state[2], k, wide=True)
         assert np.allclose([float(v) for v in vals[1:]], ref, rtol=1e-10, atol=1e-14)
+        head, fields = parse_vtk(tmp_path / f"reaction_{j * freq:06d}.vtk")
+        assert head[1] == f"LB-Cube reactive mixing step {j * freq}" and head[4] == f"DIMENSIONS {n} {n} 1"
+        check_reactive_fields(fields, state, k)
         state = O.reactive_run("D2Q9", "BGK", "D2Q5", "f64", *state, 1 / tau, 1 / tau_c, k, freq)
+
+
+CMP_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_compare_operators")
+
+
+@pytest.mark.parametrize("args,message", [(["-operators", "bgk,xyz"], "-operators takes bgk, trt, mrt, rlbm: xyz"),
+                                          (["-stats_freq", "0"], "-stats_freq must be greater than zero"),
+                                          (["-vtk_freq", "-1"], "-vtk_freq must be non-negative"),
+                                          (["-tau", "0.5"], "relaxation times must be greater than 0.5"),
+                                          (["-steps", "-2"], "-steps must be non-negative"),
+                                          (["-bogus"], "unknown argument: -bogus")])
+def test_compare_operators_argument_errors(args, message):
+    assert os.path.exists(CMP_EXE), "build first: make -C lb-cube_amd"
+    r = run(*args, exe=CMP_EXE)
+    assert r.returncode == 1 and message in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", [False, True])
+def test_compare_operators_matches_oracle(gpu, tmp_path, fused):
+    """Every operator's statistics rows and VTK snapshots against the oracle's
+    reactive run of the same state."""
+    print(f"case compare_operators fused={fused}", flush=True)
+    nx, ny, steps, sfreq, vfreq = 24, 20, 12, 4, 6
+    f0 = cases.random_perturbed("D2Q9", (ny, nx), seed=41)  # nx != ny: x fastest everywhere
+    a0, b0 = cases.split_species(nx, ny, cases.macro("D2Q9", f0)[1])
+    for x, suffix in ((f0, "f"), (a0, "a"), (b0, "b")):
+        np.ascontiguousarray(x, dtype=np.float64).tofile(tmp_path / f"init.{suffix}")
+    tau, tau_c, k = 0.6, 0.8, 0.05
+    r = run("-nx", nx, "-ny", ny, "-steps", steps, "-stats_freq", sfreq, "-vtk_freq", vfreq, "-tau", tau,
+            "-tau_c", tau_c, "-k_react", k, "-load", tmp_path / "init", "-out", tmp_path,
+            *(["-fused"] if fused else []), cwd=tmp_path, exe=CMP_EXE)
+    assert r.returncode == 0, r.stderr
+    for coll in ("BGK", "TRT", "MRT", "RLBM"):
+        assert f"{coll} complete in" in r.stdout
+        rows = (tmp_path / f"output_{coll}" / f"stats_{coll}.csv").read_text().strip().split("\n")
+        assert rows[0].startswith("step,mean_A,mean_B,var_A")
+        assert len(rows) == 1 + steps // sfreq + 1
+        state = (f0, a0, b0)
+        for j, row in enumerate(rows[1:]):
+            s = j * sfreq
+            vals = row.split(",")
+            assert int(vals[0]) == s
+            ref = O.reactive_stats("D2Q5", "f64", state[1], state[2], k, wide=True)
+            assert np.allclose([float(v) for v in vals[1:]], ref, rtol=1e-10, atol=1e-14), (coll, s)
+            if s % vfreq == 0:
+                head, fields = parse_vtk(tmp_path / f"output_{coll}" / f"comparison_{coll}_{s:06d}.vtk")
+                assert head[1] == f"LB-Cube {coll} comparison step {s}" and head[4] == f"DIMENSIONS {nx} {ny} 1"
+                check_reactive_fields(fields, state, k)
+            state = O.reactive_run("D2Q9", coll, "D2Q5", "f64", *state, 1 / tau, 1 / tau_c, k, sfreq)
+
+
+TGV_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_turbulent_tgv")
+
+
+@pytest.mark.parametrize("args,message", [(["-configs", "D2Q9_BGK"], "-configs lattice must be D3Q19 or D3Q27"),
+                                          (["-configs", "D3Q19_XYZ"], "-configs operator must be BGK, TRT, MRT"),
+                                          (["-configs", "D3Q19"], "-configs entries are LATTICE_OPERATOR"),
+                                          (["-storage", "f16"], "-storage must be f64 or f32"),
+                                          (["-check_freq", "0"], "-check_freq must be greater than zero"),
+                                          (["-tau", "0.5"], "-tau must be greater than 0.5"),
+                                          (["-threshold", "0"], "-threshold must be positive"),
+                                          (["-bogus"], "unknown argument: -bogus")])
+def test_turbulent_tgv_argument_errors(args, message):
+    assert os.path.exists(TGV_EXE), "build first: make -C lb-cube_amd"
+    r = run(*args, exe=TGV_EXE)
+    assert r.returncode == 1 and message in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", ["D3Q19_MRT", "D3Q27_RLBM"])
+def test_turbulent_tgv_matches_oracle(gpu, tmp_path, config):
+    """Stability rows (the device max-speed guard) and snapshots against the
+    oracle's run of the same state, FP64 bit for bit."""
+    print(f"case turbulent_tgv {config}", flush=True)
+    lattice, coll = config.split("_")
+    nx, ny, nz, steps, check, vfreq, tau = 12, 10, 8, 40, 10, 20, 0.52
+    f0 = cases.random_perturbed(lattice, (nz, ny, nx), seed=43, eps=2e-2)
+    f0.tofile(tmp_path / "init.bin")
+    r = run("-nx", nx, "-ny", ny, "-nz", nz, "-steps", steps, "-check_freq", check, "-vtk_freq", vfreq, "-tau", tau,
+            "-configs", config, "-load", tmp_path / "init.bin", "-out", tmp_path, cwd=tmp_path, exe=TGV_EXE)
+    assert r.returncode == 0, r.stderr
+    assert "terminated early" not in r.stdout and f"[{config}] elapsed=" in r.stdout
+    out = tmp_path / f"output_tgv3d_{config}"
+    rows = (out / f"stability_{config}.csv").read_text().strip().split("\n")
+    assert rows[0] == "step,max_velocity" and len(rows) == 2 + steps // check
+    state = f0
+    for j, row in enumerate(rows[1:]):
+        s, v = row.split(",")
+        assert int(s) == j * check
+        assert float(v) == O.max_speed(lattice, "f64", state)
+        if j * check % vfreq == 0:
+            head, fields = parse_vtk(out / f"tgv3d_{config}_{j * check:06d}.vtk")
+            assert head[1] == f"LB-Cube 3D turbulent TGV {config} step {j * check}"
+            rho, u = O.macro_fields(lattice, "f64", state)
+            assert np.array_equal(fields["density"], rho.ravel())
+            assert np.array_equal(fields["velocity"], u.reshape(3, -1).T)
+        state = O.fluid_run(lattice, coll, "f64", state, 1.0 / tau, check)
+
+
+@pytest.mark.gpu
+def test_turbulent_tgv_guard_stops_the_run(gpu, tmp_path):
+    """The guard ends a configuration at the first check whose max speed
+    exceeds the threshold (forced here by a threshold below u0) or is not
+    finite (a NaN population in the loaded state), and the next one runs."""
+    print("case turbulent_tgv guard", flush=True)
+    r = run("-nx", 16, "-ny", 16, "-nz", 16, "-steps", 100, "-check_freq", 10, "-vtk_freq", 0, "-threshold", 0.01,
+            "-configs", "D3Q19_BGK,D3Q27_TRT", "-out", tmp_path, cwd=tmp_path, exe=TGV_EXE)
+    assert r.returncode == 0, r.stderr
+    for config in ("D3Q19_BGK", "D3Q27_TRT"):
+        assert f"[{config}] Solver Crashed due to instability! step=10" in r.stdout
+        rows = (tmp_path / f"output_tgv3d_{config}" / f"stability_{config}.csv").read_text().strip().split("\n")
+        assert [row.split(",")[0] for row in rows[1:]] == ["0", "10"]
+        assert float(rows[1].split(",")[1]) > 0.01
+    assert r.stdout.count("(terminated early)") == 2
+    f = cases.random_perturbed("D3Q19", (8, 8, 8), seed=44)
+    f[4, 3, 2, 1] = np.nan
+    f.tofile(tmp_path / "nan.bin")
+    r = run("-nx", 8, "-ny", 8, "-nz", 8, "-steps", 20, "-check_freq", 5, "-vtk_freq", 0, "-configs", "D3Q19_MRT",
+            "-load", tmp_path / "nan.bin", "-out", tmp_path / "nan", cwd=tmp_path, exe=TGV_EXE)
+    assert r.returncode == 0, r.stderr
+    rows = (tmp_path / "nan" / "output_tgv3d_D3Q19_MRT" / "stability_D3Q19_MRT.csv").read_text().strip().split("\n")
+    assert rows[1] == "0,inf" and rows[2] == "5,inf"
+    assert "Solver Crashed due to instability! step=5, Umax=inf" in r.stdout

@@ -246,3 +246,52 @@ def test_stats_closed_forms(gpu):
     assert abs(s["var_A"] - 0.25) < 1e-15 and abs(s["covariance"] + 0.25) < 1e-15
     assert abs(s["segregation_intensity"] + 1.0) < 1e-14
     assert abs(s["true_reaction_rate"]) < 1e-15 and abs(s["mixed_reaction_rate"] - 0.125) < 1e-15
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_macro_fields(gpu, lattice, storage):
+    """Per-cell density and velocity (the VTK writers' fields): bit-identical
+    to the oracle's compute_macro_state in the storage's arithmetic."""
+    print(f"case macro_fields {lattice} {storage}", flush=True)
+    f = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=19)
+    want_rho, want_u = O.macro_fields(lattice, storage, f)
+    rho, u = lbcube.macro_fields(lattice, to_dev(f, storage, gpu))
+    assert np.array_equal(rho.cpu().numpy(), want_rho)
+    assert np.array_equal(u.cpu().numpy(), want_u)
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("sl", ["D2Q5", "D3Q7"])
+def test_scalar_fields(gpu, sl, storage):
+    print(f"case scalar_fields {sl} {storage}", flush=True)
+    shape = SHAPE["D2Q9"] if sl == "D2Q5" else SHAPE["D3Q19"]
+    a, _ = _species(sl, shape, storage, seed=20)
+    got = lbcube.scalar_fields(sl, to_dev(a, storage, gpu))
+    assert np.array_equal(got.cpu().numpy(), O.scalar_fields(sl, storage, a))
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_max_speed(gpu, lattice, storage):
+    """Stability guard: the oracle's max |u| exactly (max of correctly
+    rounded square roots), and +inf once a population is NaN or infinite."""
+    print(f"case max_speed {lattice} {storage}", flush=True)
+    f = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=24)
+    assert lbcube.max_speed(lattice, to_dev(f, storage, gpu)) == O.max_speed(lattice, storage, f)
+    if storage in ("f64", "f32"):
+        for bad in (np.nan, np.inf):
+            g = f.copy()
+            g[3].flat[g[3].size // 2] = bad
+            assert lbcube.max_speed(lattice, to_dev(g, storage, gpu)) == np.inf
+
+
+def test_max_speed_large_domain(gpu):
+    """Many blocks: one NaN cell far into the domain still reports +inf."""
+    f = torch.full((19, 64, 64, 64), 1.0 / 36.0, dtype=torch.float32, device=gpu)
+    for i, w in enumerate(O.WEIGHTS["D3Q19"]):
+        f[i] = w
+    f[1, 50, 33, 17] += 0.01
+    assert abs(lbcube.max_speed("D3Q19", f) - 0.01 / 1.01) < 1e-7
+    f[5, 63, 63, 63] = float("nan")
+    assert lbcube.max_speed("D3Q19", f) == np.inf

@@ -263,3 +263,47 @@ def test_reduced_storage_step_is_float_step_with_rounded_io():
     want = cases.encode("D3Q19", "f16", step32.astype(np.float64))
     got = O.fluid_step("D3Q19", "BGK", "f16", h, 1.5)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_macro_fields_recover_equilibrium_moments(lattice):
+    """Per-cell fields of feq(rho, u) give back rho and u (lattice isotropy, in
+    exact arithmetic; here to rounding), in [Z][Y][X] cell order, and the
+    stability guard's max speed is the largest |u|."""
+    shape = (6, 7) if lattice == "D2Q9" else (3, 4, 5)
+    g = cases.rng(21)
+    d = O.D_OF[lattice]
+    rho = 1.0 + g.uniform(-0.05, 0.05, size=shape)
+    u = g.uniform(-0.03, 0.03, size=(d,) + shape)
+    f = O.feq_field(lattice, "f64", rho, u)
+    r, v = O.macro_fields(lattice, "f64", f)
+    assert r.shape == shape and v.shape == (d,) + shape
+    assert np.allclose(r, rho, rtol=0, atol=1e-15) and np.allclose(v, u, rtol=0, atol=1e-15)
+    speed = np.sqrt((u * u).sum(0))
+    assert abs(O.max_speed(lattice, "f64", f) - speed.max()) < 1e-15
+    assert O.max_speed(lattice, "f64", f) == O.fluid_diagnostics(lattice, "f64", f)[2]
+    # extended precision: only the final rounding to double remains
+    rl, vl = O.macro_fields(lattice, "f80", O.feq_field(lattice, "f80", rho, u))
+    assert np.allclose(rl, rho, rtol=0, atol=2.3e-16) and np.allclose(vl, u, rtol=0, atol=1e-17)
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_max_speed_reports_non_finite_as_infinity(bad):
+    """compute_max_velocity returns infinity for any non-finite speed (NaN
+    included, which a plain max would skip)."""
+    f = cases.random_perturbed("D3Q19", (3, 4, 5), seed=22)
+    assert np.isfinite(O.max_speed("D3Q19", "f64", f))
+    f[7, 2, 1, 3] = bad
+    assert O.max_speed("D3Q19", "f64", f) == np.inf
+    if np.isnan(bad):  # log_diagnostics' max keeps skipping the NaN cell
+        assert np.isfinite(O.fluid_diagnostics("D3Q19", "f64", f)[2])
+
+
+@pytest.mark.parametrize("sl", ["D2Q5", "D3Q7"])
+def test_scalar_fields_recover_concentration(sl):
+    shape = (6, 7) if sl == "D2Q5" else (3, 4, 5)
+    g = cases.rng(23)
+    c = g.uniform(0.0, 1.0, size=shape)
+    u = g.uniform(-0.03, 0.03, size=(O.D_OF[sl],) + shape)
+    p = O.scalar_feq_field(sl, "f64", c, u)
+    assert np.allclose(O.scalar_fields(sl, "f64", p), c, rtol=0, atol=1e-15)
