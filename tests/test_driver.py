@@ -270,3 +270,69 @@ def test_turbulent_tgv_guard_stops_the_run(gpu, tmp_path):
     rows = (tmp_path / "nan" / "output_tgv3d_D3Q19_MRT" / "stability_D3Q19_MRT.csv").read_text().strip().split("\n")
     assert rows[1] == "0,inf" and rows[2] == "5,inf"
     assert "Solver Crashed due to instability! step=5, Umax=inf" in r.stdout
+
+
+SHEAR_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_shear_layer")
+
+
+@pytest.mark.parametrize("args,message", [(["-operators", "bgk,foo"], "-operators takes bgk, trt, mrt, rlbm: foo"),
+                                          (["-check_freq", "0"], "-check_freq must be greater than zero"),
+                                          (["-tau_c", "0.5"], "relaxation times must be greater than 0.5"),
+                                          (["-threshold", "-1"], "-threshold must be positive"),
+                                          (["-bogus"], "unknown argument: -bogus")])
+def test_shear_layer_argument_errors(args, message):
+    assert os.path.exists(SHEAR_EXE), "build first: make -C lb-cube_amd"
+    r = run(*args, exe=SHEAR_EXE)
+    assert r.returncode == 1 and message in r.stderr
+
+
+@pytest.mark.gpu
+def test_shear_layer_matches_oracle(gpu, tmp_path):
+    """Every operator's stability rows and snapshots against the oracle's
+    fluid step followed by two passive scalar steps on the new fluid state
+    (all bit-identical in FP64: no exp() on this path)."""
+    print("case shear_layer", flush=True)
+    nx, ny, steps, check, vfreq, tau, tau_c = 16, 12, 20, 5, 10, 0.55, 0.8
+    f0 = cases.random_perturbed("D2Q9", (ny, nx), seed=45, eps=1e-2)
+    a0, b0 = cases.split_species(nx, ny, cases.macro("D2Q9", f0)[1])
+    for x, suffix in ((f0, "f"), (a0, "a"), (b0, "b")):
+        np.ascontiguousarray(x, dtype=np.float64).tofile(tmp_path / f"init.{suffix}")
+    r = run("-nx", nx, "-ny", ny, "-steps", steps, "-check_freq", check, "-vtk_freq", vfreq, "-tau", tau, "-tau_c",
+            tau_c, "-load", tmp_path / "init", "-out", tmp_path, cwd=tmp_path, exe=SHEAR_EXE)
+    assert r.returncode == 0, r.stderr
+    for coll in ("BGK", "TRT", "MRT", "RLBM"):
+        name = f"D2Q9_{coll}"
+        assert f"[{name}] elapsed=" in r.stdout and "terminated early" not in r.stdout
+        out = tmp_path / f"output_shear2d_{name}"
+        rows = (out / f"stability_{name}.csv").read_text().strip().split("\n")
+        assert rows[0] == "step,max_velocity" and len(rows) == 2 + steps // check
+        f, a, b = f0, a0, b0
+        for s in range(steps + 1):
+            if s % check == 0:
+                step, v = rows[1 + s // check].split(",")
+                assert int(step) == s
+                want = O.max_speed("D2Q9", "f64", f)
+                assert float(v) == (float(f"{want:.6g}") if s == 0 else want)  # row 0: 6 digits
+            if s % vfreq == 0:
+                head, fields = parse_vtk(out / f"shear_{name}_{s:06d}.vtk")
+                assert head[1] == f"LB-Cube turbulent shear layer {name} step {s}"
+                _, u = O.macro_fields("D2Q9", "f64", f)
+                assert np.array_equal(fields["velocity"][:, :2], u.reshape(2, -1).T)
+                assert np.array_equal(fields["velocity_magnitude"], np.sqrt(u[0] * u[0] + u[1] * u[1]).ravel())
+                assert np.array_equal(fields["C_A"], O.scalar_fields("D2Q5", "f64", a).ravel())
+                assert np.array_equal(fields["C_B"], O.scalar_fields("D2Q5", "f64", b).ravel())
+            f = O.fluid_step("D2Q9", coll, "f64", f, 1 / tau)
+            a = O.scalar_step("D2Q9", "D2Q5", "f64", a, f, 1 / tau_c, 0.0)
+            b = O.scalar_step("D2Q9", "D2Q5", "f64", b, f, 1 / tau_c, 0.0)
+
+
+@pytest.mark.gpu
+def test_shear_layer_guard_writes_last_snapshot(gpu, tmp_path):
+    print("case shear_layer guard", flush=True)
+    r = run("-nx", 32, "-ny", 32, "-steps", 40, "-check_freq", 10, "-vtk_freq", 1000, "-threshold", 0.01,
+            "-operators", "mrt", "-out", tmp_path, cwd=tmp_path, exe=SHEAR_EXE)
+    assert r.returncode == 0, r.stderr
+    assert "[D2Q9_MRT] Solver Crashed due to instability! step=10" in r.stdout
+    assert "(terminated early)" in r.stdout
+    out = tmp_path / "output_shear2d_D2Q9_MRT"
+    assert sorted(p.name for p in out.glob("*.vtk")) == ["shear_D2Q9_MRT_000000.vtk", "shear_D2Q9_MRT_000010.vtk"]
