@@ -218,7 +218,9 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     """2048^2 D2Q9 MRT double shear layer + two D2Q5 species (Da = 10), FP32
     and FP64: the fused single-pass step (152 / 304 B per cell), the reference's
     two launches per step (fluid step, then reaction step: 188 / 376 B per
-    cell), and the whole loop with the reactive statistics every 50 steps."""
+    cell), and the whole loop with the reactive statistics every 50 steps; the
+    fused step in the deviation form (FP32, on the same buffers), and both
+    forms with FP16 storage (76 B)."""
     from lbcube import init as lbinit
 
     out = {}
@@ -232,9 +234,9 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
         ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
         state = [f, ft, a, at, b, bt]
 
-        def fused(state=state):
+        def fused(state=state, fast=False):
             f, ft, a, at, b, bt = state
-            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k)
+            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast)
             state[:] = [ft, f, at, a, bt, b]
 
         def unfused(state=state):
@@ -264,8 +266,31 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
                         "unfused_bytes_per_cell": 47 * es,
                         "loop_with_stats_every_50_mlups": cells / (ms_loop * 1e-3) / 1e6,
                         "stats_ms": ms_stats, "segregation_intensity": stats["segregation_intensity"]}
+        if storage == "f32":  # the deviation form (LBC_FLAG_FAST) on the same buffers
+            ms_fast = _time(torch, lambda: fused(fast=True), reps=steps)
+            out["f32_fast"] = {"mlups": cells / (ms_fast * 1e-3) / 1e6,
+                               "achieved_gbs": 38 * es * cells / (ms_fast * 1e-3) / 1e9, "bytes_per_cell": 38 * es}
         del f, ft, a, at, b, bt, state
         torch.cuda.empty_cache()
+    # FP16 storage of the deviations: the fused step in both arithmetic forms
+    f = lbinit.equilibrium("D2Q9", None, u, "f16")
+    a = lbinit.scalar_equilibrium("D2Q5", ca, u, "f16")
+    b = lbinit.scalar_equilibrium("D2Q5", cb, u, "f16")
+    state = [f, torch.empty_like(f), a, torch.empty_like(a), b, torch.empty_like(b)]
+
+    def fused16(fast):
+        f, ft, a, at, b, bt = state
+        lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast)
+        state[:] = [ft, f, at, a, bt, b]
+
+    for fast in (False, True):
+        for _ in range(4):
+            fused16(fast)
+        ms = _time(torch, lambda: fused16(fast), reps=steps)
+        out["f16_fast" if fast else "f16"] = {"mlups": n * n / (ms * 1e-3) / 1e6,
+                                             "achieved_gbs": 38 * 2 * n * n / (ms * 1e-3) / 1e9, "bytes_per_cell": 76}
+    del f, a, b, state
+    torch.cuda.empty_cache()
     out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C, {n}^2 double shear layer, Da=10 (k = Da u0 / delta = 0.005)"
     return out
 
@@ -342,8 +367,9 @@ def main():
         extras["adjoint"] = adjoint_ratio(torch)
         print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
         extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)
-        print(f"# reactive 2048^2 MRT f32: {extras['reactive_2d']['f32']['mlups']:.0f} MLUPS", file=sys.stderr,
-              flush=True)
+        r2 = extras["reactive_2d"]
+        print(f"# reactive 2048^2 MRT f32: {r2['f32']['mlups']:.0f} MLUPS, f32 fast {r2['f32_fast']['mlups']:.0f}, "
+              f"f16 fast {r2['f16_fast']['mlups']:.0f}", file=sys.stderr, flush=True)
         extras["ensemble"] = ensemble_rate(torch)
         print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         extras["lattices"] = lattice_sweep(lbcube, torch)

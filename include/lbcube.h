@@ -57,14 +57,14 @@ int lbc_fluid_step(int lattice, int collision, int storage, const void* current,
 
 /* Flags of lbc_fluid_step_ex. */
 enum lbc_flags {
-    /* BGK and TRT on every lattice, MRT on D3Q19 and D3Q27 (ignored by the
-     * D2Q9 MRT and by RLBM): deviation-form arithmetic (f_i - w_i carried
-     * through the collision) with fused multiply-adds and a hardware
-     * reciprocal of the density; D3Q19 MRT transforms only the five shear
-     * moments and sets the others from the equilibrium; D3Q27 MRT as
-     * f - M^-1 S M (f - feq) with separable transforms.  Half (D3Q27 MRT: a
-     * quarter) of the VALU work of the reference-order form; results differ
-     * from it by rounding only (bounded by the tests). */
+    /* BGK, TRT and MRT on every lattice (ignored by RLBM): deviation-form
+     * arithmetic (f_i - w_i carried through the collision) with fused
+     * multiply-adds and a hardware reciprocal of the density; D2Q9 and D3Q19
+     * MRT transform only the relaxed moments of the deviations (the rest
+     * state's moments are the rest equilibrium's); D3Q27 MRT as
+     * f - M^-1 S M (f - feq) with separable transforms.  Up to half (D3Q27
+     * MRT: a quarter) of the VALU work of the reference-order form; results
+     * differ from it by rounding only (bounded by the tests). */
     LBC_FLAG_FAST = 1u
 };
 
@@ -102,7 +102,7 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
                             uint32_t block_z, void* stream);
 
 /* lbc_reactive_fused_step with flags (0 = lbc_reactive_fused_step exactly).
- * LBC_FLAG_FAST (BGK and TRT; ignored by MRT and RLBM): the fluid part in the
+ * LBC_FLAG_FAST (BGK, TRT and MRT; ignored by RLBM): the fluid part in the
  * deviation form of lbc_fluid_step_ex, the velocity taken from the stored
  * deviations, the scalar collision with 1/cs2 multiplied and FMAs, and the
  * reaction increment of the same exact one-step map written without the

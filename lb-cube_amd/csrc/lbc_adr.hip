@@ -165,7 +165,8 @@ hipError_t fused_by_collision(int collision, int storage, bool fast, const void*
             return fast ? fused_by_storage<F, S, kTRT, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
                         : fused_by_storage<F, S, kTRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
         case kMRT:
-            return fused_by_storage<F, S, kMRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+            return fast ? fused_by_storage<F, S, kMRT, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
+                        : fused_by_storage<F, S, kMRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
         case kRLBM:
             return fused_by_storage<F, S, kRLBM, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
         default: return hipErrorInvalidValue;

@@ -286,6 +286,74 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R
     }
 }
 
+// D2Q9 MRT on deviations, with the reference's rates (s_e 1.63, s_eps 1.14,
+// s_q 1.92, s_nu on the two stress moments; collide_mrt_d2q9,
+// lattice_mrt.hpp:548-603).  As for D3Q19, m(w) = m_eq(1, 0), so
+// f' - w = M^-1 Delta with Delta_r the conserved deviations and
+// m_r(d) + s_r (Deq_r - m_r(d)) for the six relaxed moments, Deq_r =
+// eq_r(rho, u) - eq_r(1, 0).  The relaxed moments use the raw momentum j in
+// their q equilibria (the reference's -rho u is j up to rounding), and the
+// conserved rows are scaled by hi + lo splits of 1/9 and 1/6 so their
+// rounding cannot accumulate.
+template <class R> __device__ __forceinline__ void collide_mrt_d2q9_dev(R* d, R s_nu) {
+    using L = D2Q9;
+    constexpr const Basis<9>& B = kD2Q9Basis;
+    const DevMacro<L, R> m = dev_macro<L, R>(d);
+    const R rho = m.rho, ux = m.ux, uy = m.uy;
+    const R rusq = rho * fmadd(uy, uy, ux * ux);
+    R delta[9];
+    delta[0] = m.drho;
+    delta[3] = m.jx;
+    delta[5] = m.jy;
+    const R deq[9] = {R(0),   fmadd(R(3), rusq, R(-2) * m.drho), fmadd(R(-3), rusq, m.drho),
+                      R(0),   -m.jx,
+                      R(0),   -m.jy,
+                      rho * (ux * ux - uy * uy), rho * ux * uy};
+    const R rate[9] = {R(0), R(1.63), R(1.14), R(0), R(1.92), R(0), R(1.92), s_nu, s_nu};
+    constexpr int kRelaxed[6] = {1, 2, 4, 6, 7, 8};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int r = kRelaxed[k];
+        R sm = R(0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            if (B.m[r][i] != 0) sm = fmadd(R(B.m[r][i]), d[i], sm);
+        delta[r] = fmadd(rate[r], deq[r] - sm, sm);
+    }
+    constexpr int kConserved[3] = {0, 3, 5};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int r = kConserved[k];
+        const double inv = 1.0 / B.norm[r];
+        const R hi = R(inv);
+        const R lo = R(inv - static_cast<double>(hi));
+        delta[r] = fmadd(delta[r], hi, delta[r] * lo);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        R v = R(0);
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (B.m[r][i] == 0) continue;
+            const bool conserved = r == 0 || r == 3 || r == 5;
+            const double coef = conserved ? B.m[r][i] : static_cast<double>(B.m[r][i]) / B.norm[r];
+            v = fmadd(R(coef), delta[r], v);
+        }
+        d[i] = v;
+    }
+}
+
+// The deviation-form collision of LBC_FLAG_FAST for a lattice and operator
+// (BGK, TRT, MRT).
+template <class L, int CT, class R> __device__ __forceinline__ void collide_dev(R* d, R omega, R omega_minus) {
+    static_assert(CT == kBGK || CT == kTRT || CT == kMRT, "the deviation form covers BGK, TRT and MRT");
+    if constexpr (CT == kBGK) collide_bgk_dev<L, R>(d, omega);
+    else if constexpr (CT == kTRT) collide_trt_dev<L, R>(d, omega, omega_minus);
+    else if constexpr (L::Q == 9) collide_mrt_d2q9_dev<R>(d, omega);
+    else if constexpr (L::Q == 19) collide_mrt_d3q19_dev<R>(d, omega);
+    else collide_mrt_d3q27_dev<R>(d, omega);
+}
+
 // ---- scalar ADR and A + B -> C for the fast fused reactive step
 
 // g_i += omega_c (w_i C (1 + c.u / cs2) - g_i) + w_i S with 1/cs2 (3 for D2Q5,

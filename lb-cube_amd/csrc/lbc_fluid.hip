@@ -19,8 +19,8 @@
 namespace lbc {
 
 // FAST selects the deviation-form arithmetic (lbc_fast.hpp) behind
-// LBC_FLAG_FAST: BGK and TRT on every lattice, MRT on D3Q19 and D3Q27; every
-// other combination evaluates the reference's order.
+// LBC_FLAG_FAST: BGK, TRT and MRT on every lattice; RLBM always evaluates the
+// reference's order.
 template <class L, int CT, int ST, bool FAST, int MAXT>
 __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                           typename Storage<ST>::S* __restrict__ nxt, Cells c,
@@ -41,10 +41,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             f[i] = St::load_dev(at(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
-        if constexpr (CT == kBGK) collide_bgk_dev<L, R>(f, omega);
-        else if constexpr (CT == kTRT) collide_trt_dev<L, R>(f, omega, omega_minus);
-        else if constexpr (L::Q == 27) collide_mrt_d3q27_dev<R>(f, omega);
-        else collide_mrt_d3q19_dev<R>(f, omega);
+        collide_dev<L, CT, R>(f, omega, omega_minus);
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) St::store_dev(at(plane_ptr(nxt, i, plane), dst), f[i], R(L::w[i]));
     } else {
@@ -103,9 +100,8 @@ hipError_t dispatch_collision(int collision, int storage, bool fast, const void*
             return fast ? dispatch_storage<L, kTRT, true>(storage, cur, nxt, s, omega, st)
                         : dispatch_storage<L, kTRT, false>(storage, cur, nxt, s, omega, st);
         case kMRT:
-            if constexpr (L::Q != 9)
-                if (fast) return dispatch_storage<L, kMRT, true>(storage, cur, nxt, s, omega, st);
-            return dispatch_storage<L, kMRT, false>(storage, cur, nxt, s, omega, st);
+            return fast ? dispatch_storage<L, kMRT, true>(storage, cur, nxt, s, omega, st)
+                        : dispatch_storage<L, kMRT, false>(storage, cur, nxt, s, omega, st);
         case kRLBM: return dispatch_storage<L, kRLBM, false>(storage, cur, nxt, s, omega, st);
         default: return hipErrorInvalidValue;
     }

@@ -62,7 +62,7 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     }
 }
 
-// The same step in the deviation form (LBC_FLAG_FAST; BGK and TRT): fluid
+// The same step in the deviation form (LBC_FLAG_FAST; BGK, TRT, MRT): fluid
 // populations carried as f_i - w_i with FMAs and a hardware reciprocal
 // (lbc_fast.hpp), the velocity taken from the deviations as stored, the scalar
 // collision with 1/cs2 multiplied and the rest equilibrium by remainder, the
@@ -77,7 +77,6 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
                                                    size_t plane, typename Storage<ST>::R omega,
                                                    typename Storage<ST>::R omega_minus,
                                                    typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react) {
-    static_assert(CT == kBGK || CT == kTRT, "the deviation form covers BGK and TRT");
     using St = Storage<ST>;
     using R = typename St::R;
     const uint32_t cell = pull.dst;
@@ -87,8 +86,7 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
         const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
         d[i] = St::load_dev(at(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
     }
-    if constexpr (CT == kBGK) collide_bgk_dev<F, R>(d, omega);
-    else collide_trt_dev<F, R>(d, omega, omega_minus);
+    collide_dev<F, CT, R>(d, omega, omega_minus);
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
         St::store_dev(at(plane_ptr(nxt_f, i, plane), cell), d[i], R(F::w[i]));

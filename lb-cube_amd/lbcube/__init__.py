@@ -83,9 +83,8 @@ FLAG_FAST = 1
 def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False):
     """Fused pull-stream + collide: nxt <- step(cur). Replaces launch_step_gpu.
 
-    fast=True selects the deviation-form FMA arithmetic for BGK/TRT and the
-    D3Q19/D3Q27 MRT (LBC_FLAG_FAST); the default evaluates the reference's
-    operation order."""
+    fast=True selects the deviation-form FMA arithmetic for BGK, TRT and MRT
+    (LBC_FLAG_FAST); the default evaluates the reference's operation order."""
     _check_pop(lattice, cur, "cur")
     _check_pop(lattice, nxt, "nxt")
     if cur.shape != nxt.shape or cur.dtype != nxt.dtype:

@@ -47,12 +47,12 @@ def test_fast_against_oracle(gpu, lattice, coll, storage):
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
-@pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
-def test_fast_3d_mrt_against_oracle(gpu, lattice, storage):
-    """3D MRT on deviations (D3Q19: the five shear moments transformed, the
-    rest set from the equilibrium; D3Q27: f - M^-1 S M (f - feq) with separable
-    transforms), against the reference-order oracle (tolerances as for
-    BGK/TRT)."""
+@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19", "D3Q27"])
+def test_fast_mrt_against_oracle(gpu, lattice, storage):
+    """MRT on deviations (D2Q9: the six relaxed moments transformed; D3Q19:
+    the five shear moments transformed, the rest set from the equilibrium;
+    D3Q27: f - M^-1 S M (f - feq) with separable transforms), against the
+    reference-order oracle (tolerances as for BGK/TRT)."""
     print(f"case fast {lattice} MRT {storage}", flush=True)
     f0 = cases.random_perturbed(lattice, SHAPE[lattice], storage, seed=13)
     for omega in (1.0 / 0.57, 1.0 / 0.9):
@@ -67,15 +67,15 @@ def test_fast_3d_mrt_against_oracle(gpu, lattice, storage):
         assert np.max(np.abs(got - want)) <= FAST_TOL[storage], (omega, np.max(np.abs(got - want)))
 
 
-@pytest.mark.parametrize("lattice", ["D2Q9"])
-def test_fast_flag_ignored_by_other_mrt(gpu, lattice):
-    """LBC_FLAG_FAST leaves D2Q9 MRT in the reference order (bit-exact)."""
+@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19", "D3Q27"])
+def test_fast_flag_ignored_by_rlbm(gpu, lattice):
+    """LBC_FLAG_FAST leaves RLBM in the reference order (bit-exact)."""
     f0 = cases.random_perturbed(lattice, SHAPE[lattice], "f32", seed=14)
     a = to_dev(f0, "f32", gpu)
     b = torch.empty_like(a)
-    lbcube.fluid_step(lattice, "MRT", a, b, 1.7, fast=True)
+    lbcube.fluid_step(lattice, "RLBM", a, b, 1.7, fast=True)
     torch.cuda.synchronize()
-    assert np.array_equal(to_host(b, "f32"), O.fluid_run(lattice, "MRT", "f32", f0, 1.7, 1))
+    assert np.array_equal(to_host(b, "f32"), O.fluid_run(lattice, "RLBM", "f32", f0, 1.7, 1))
 
 
 def test_fast_f64_conserves_mass(gpu):
@@ -87,8 +87,9 @@ def test_fast_f64_conserves_mass(gpu):
         lbcube.fluid_step("D3Q19", "TRT", a, b, 1.9, fast=True)
         a, b = b, a
     assert abs(a.sum().item() - m0) <= 1e-12 * m0
-    for lattice, seed in (("D3Q19", 5), ("D3Q27", 4)):
-        f = torch.from_numpy(cases.random_perturbed(lattice, (12, 12, 12), "f64", seed=seed)).to(gpu)
+    for lattice, seed in (("D2Q9", 6), ("D3Q19", 5), ("D3Q27", 4)):
+        shape = (40, 40) if lattice == "D2Q9" else (12, 12, 12)
+        f = torch.from_numpy(cases.random_perturbed(lattice, shape, "f64", seed=seed)).to(gpu)
         g = torch.empty_like(f)
         m = f.sum().item()
         for _ in range(100):
@@ -249,7 +250,7 @@ REACT_FAST_TOL = {"f64": 1e-13, "f32": 5e-6, "f16": 2e-3, "bf16": 1.6e-2}
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
-@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
 def test_reactive_fast_against_oracle(gpu, pair, coll, storage):
     from test_gpu_parity import _species
@@ -410,3 +411,20 @@ def test_reactive_reduced_storage_error_against_fp64(gpu):
             assert np.max(err[early]) <= early_tol, (storage, fast, err)
         if (storage, fast) == ("f32", True):
             assert np.max(err) <= 2 * np.max(spread) + 1e-5, (err, spread)
+
+
+def test_reactive_mrt_fast_against_fp64(gpu):
+    """The C3 operator (D2Q9 MRT) in the fused reactive step: the deviation
+    form in FP32 and FP16 storage against the FP64 reference order, 512^2
+    reactive mixing, over the steps before the flow's sensitivity sets in
+    (see above)."""
+    n, steps, sample = 512, 750, 250
+    ref = reactive_mixing(n, "f64", False, steps, coll="MRT", sample=sample)
+    for storage, fast, tol in (("f32", True, 1e-5), ("f32", False, None), ("f16", True, 2e-3)):
+        got = reactive_mixing(n, storage, fast, steps, coll="MRT", sample=sample)
+        err = np.abs(got[:, 0] - ref[:, 0]) / ref[:, 0]
+        mass = np.max(np.abs(got[:, 2] - got[0, 2])) / got[0, 2]
+        print(f"reactive MRT {n}^2 storage {storage} fast={fast}: relative error of total A per sample "
+              f"{np.array2string(err, precision=2)}, fluid mass drift {mass:.2e}", flush=True)
+        if tol is not None:
+            assert np.max(err) <= tol, (storage, fast, err)
