@@ -64,12 +64,14 @@ template <class L, class R> __device__ __forceinline__ R dev_cu(int i, const Dev
 }
 
 // BGK on deviations: d_i <- d_i + omega (deq_i - d_i).
-template <class L, class R> __device__ __forceinline__ void collide_bgk_dev(R* d, R omega) {
+template <class L, class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_bgk_dev(R* d, R omega, Sink sink = {}) {
     const DevMacro<L, R> m = dev_macro<L, R>(d);
     const R r3 = R(3) * m.rho;
     {
         const R eq0 = R(L::w[0]) * fmadd(m.rho, m.m15uu, m.drho);
         d[0] = fmadd(omega, eq0 - d[0], d[0]);
+        sink(0, d[0]);
     }
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
@@ -84,18 +86,22 @@ template <class L, class R> __device__ __forceinline__ void collide_bgk_dev(R* d
         const R eqj = fmadd(w, a, -wb);
         d[i] = fmadd(omega, eqi - d[i], d[i]);
         d[j] = fmadd(omega, eqj - d[j], d[j]);
+        sink(i, d[i]);
+        sink(j, d[j]);
     }
 }
 
 // TRT on deviations, magic parameter through omega_minus (computed by the
 // launcher exactly as compute_omega_minus, lattice_physics.hpp:336-344).
-template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d, R wp, R wm) {
+template <class L, class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_trt_dev(R* d, R wp, R wm, Sink sink = {}) {
     const DevMacro<L, R> m = dev_macro<L, R>(d);
     const R r3 = R(3) * m.rho;
     {
         // the rest population is its own opposite: only the even part relaxes
         const R eq0 = R(L::w[0]) * fmadd(m.rho, m.m15uu, m.drho);
         d[0] = fmadd(wp, eq0 - d[0], d[0]);
+        sink(0, d[0]);
     }
 #pragma unroll
     for (int i = 1; i < L::Q; ++i) {
@@ -112,6 +118,8 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
         const R co = wm * po;
         d[i] = d[i] - ce - co;
         d[j] = d[j] - ce + co;
+        sink(i, d[i]);
+        sink(j, d[j]);
     }
 }
 
@@ -128,7 +136,8 @@ template <class L, class R> __device__ __forceinline__ void collide_trt_dev(R* d
 // About a quarter of the reference order's VALU work (its inverse divides).
 __host__ __device__ constexpr int g27(int a, int b, int c) { return (a * 3 + b) * 3 + c; }
 
-template <class R> __device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R s_nu) {
+template <class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R s_nu, Sink sink = {}) {
     using L = D3Q27;
     const DevMacro<L, R> m = dev_macro<L, R>(d);
     const R r3 = R(3) * m.rho;
@@ -215,7 +224,10 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R
 #pragma unroll
         for (int k = 0; k < 3; ++k) inv(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
 #pragma unroll
-    for (int i = 0; i < 27; ++i) d[i] -= p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)];
+    for (int i = 0; i < 27; ++i) {
+        d[i] -= p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)];
+        sink(i, d[i]);
+    }
 }
 
 // D3Q19 MRT on deviations, with the reference's rates and equilibrium
@@ -229,7 +241,8 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q27_dev(R* d, R
 // moments of d are transformed; the inverse uses the 14 non-ghost rows, with
 // coefficients M[r][i] / |M_r|^2 folded at compile time for the non-conserved
 // rows.  About half the VALU work of the reference order.
-template <class R> __device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R omega) {
+template <class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R omega, Sink sink = {}) {
     using L = D3Q19;
     constexpr const Basis<19>& B = kD3Q19Basis;
     const DevMacro<L, R> m = dev_macro<L, R>(d);
@@ -283,6 +296,7 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R
             v = fmadd(R(coef), delta[r], v);
         }
         d[i] = v;
+        sink(i, v);
     }
 }
 
@@ -295,7 +309,8 @@ template <class R> __device__ __forceinline__ void collide_mrt_d3q19_dev(R* d, R
 // their q equilibria (the reference's -rho u is j up to rounding), and the
 // conserved rows are scaled by hi + lo splits of 1/9 and 1/6 so their
 // rounding cannot accumulate.
-template <class R> __device__ __forceinline__ void collide_mrt_d2q9_dev(R* d, R s_nu) {
+template <class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_mrt_d2q9_dev(R* d, R s_nu, Sink sink = {}) {
     using L = D2Q9;
     constexpr const Basis<9>& B = kD2Q9Basis;
     const DevMacro<L, R> m = dev_macro<L, R>(d);
@@ -340,18 +355,20 @@ template <class R> __device__ __forceinline__ void collide_mrt_d2q9_dev(R* d, R 
             v = fmadd(R(coef), delta[r], v);
         }
         d[i] = v;
+        sink(i, v);
     }
 }
 
 // The deviation-form collision of LBC_FLAG_FAST for a lattice and operator
 // (BGK, TRT, MRT).
-template <class L, int CT, class R> __device__ __forceinline__ void collide_dev(R* d, R omega, R omega_minus) {
+template <class L, int CT, class R, class Sink = NoSink>
+__device__ __forceinline__ void collide_dev(R* d, R omega, R omega_minus, Sink sink = {}) {
     static_assert(CT == kBGK || CT == kTRT || CT == kMRT, "the deviation form covers BGK, TRT and MRT");
-    if constexpr (CT == kBGK) collide_bgk_dev<L, R>(d, omega);
-    else if constexpr (CT == kTRT) collide_trt_dev<L, R>(d, omega, omega_minus);
-    else if constexpr (L::Q == 9) collide_mrt_d2q9_dev<R>(d, omega);
-    else if constexpr (L::Q == 19) collide_mrt_d3q19_dev<R>(d, omega);
-    else collide_mrt_d3q27_dev<R>(d, omega);
+    if constexpr (CT == kBGK) collide_bgk_dev<L, R>(d, omega, sink);
+    else if constexpr (CT == kTRT) collide_trt_dev<L, R>(d, omega, omega_minus, sink);
+    else if constexpr (L::Q == 9) collide_mrt_d2q9_dev<R>(d, omega, sink);
+    else if constexpr (L::Q == 19) collide_mrt_d3q19_dev<R>(d, omega, sink);
+    else collide_mrt_d3q27_dev<R>(d, omega, sink);
 }
 
 // ---- scalar ADR and A + B -> C for the fast fused reactive step

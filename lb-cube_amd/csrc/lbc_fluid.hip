@@ -41,9 +41,8 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             f[i] = St::load_dev(at(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
-        collide_dev<L, CT, R>(f, omega, omega_minus);
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) St::store_dev(at(plane_ptr(nxt, i, plane), dst), f[i], R(L::w[i]));
+        collide_dev<L, CT, R>(f, omega, omega_minus,
+                              [&](int i, R v) { St::store_dev(at(plane_ptr(nxt, i, plane), dst), v, R(L::w[i])); });
     } else {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
@@ -51,12 +50,9 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
             f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
         }
-        collide_cell<L, CT, R>(f, omega, omega_minus);
-#pragma unroll
-        for (int i = 0; i < L::Q; ++i) {
-            const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-            St::store(at(plane_ptr(nxt, i, plane), dst), f[i], off);
-        }
+        collide_cell<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
+            St::store(at(plane_ptr(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+        });
     }
 }
 

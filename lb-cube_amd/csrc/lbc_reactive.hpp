@@ -35,12 +35,12 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
         const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
         f[i] = St::load(at(plane_ptr(cur_f, i, plane), src), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     }
-    collide_cell<F, CT, R>(f, omega, omega_minus);
+    collide_cell<F, CT, R>(f, omega, omega_minus, [&](int i, R v) {
+        St::store(at(plane_ptr(nxt_f, i, plane), cell), v, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+    });
+    if constexpr (kOffsetStorage<ST>) {
 #pragma unroll
-    for (int i = 0; i < F::Q; ++i) {
-        const R off = kOffsetStorage<ST> ? R(F::w[i]) : R(0);
-        St::store(at(plane_ptr(nxt_f, i, plane), cell), f[i], off);
-        if constexpr (kOffsetStorage<ST>) f[i] = St::roundtrip(f[i], off);
+        for (int i = 0; i < F::Q; ++i) f[i] = St::roundtrip(f[i], R(F::w[i]));
     }
     const Macro<R> m = macro_state<F, R>(f);
     R ga[S::Q], gb[S::Q];
@@ -86,12 +86,10 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
         const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
         d[i] = St::load_dev(at(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
     }
-    collide_dev<F, CT, R>(d, omega, omega_minus);
+    collide_dev<F, CT, R>(d, omega, omega_minus,
+                          [&](int i, R v) { St::store_dev(at(plane_ptr(nxt_f, i, plane), cell), v, R(F::w[i])); });
 #pragma unroll
-    for (int i = 0; i < F::Q; ++i) {
-        St::store_dev(at(plane_ptr(nxt_f, i, plane), cell), d[i], R(F::w[i]));
-        d[i] = St::roundtrip_dev(d[i], R(F::w[i]));
-    }
+    for (int i = 0; i < F::Q; ++i) d[i] = St::roundtrip_dev(d[i], R(F::w[i]));
     const DevMacro<F, R> m = dev_macro<F, R>(d);
     R ga[S::Q], gb[S::Q];
 #pragma unroll
