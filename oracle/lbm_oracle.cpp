@@ -1514,6 +1514,31 @@ int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const 
     return 0;
 }
 
+// One step of the discrete adjoint with arbitrary adjoint inputs (collision-cell
+// convention) for the dot-product test; djdw receives <nu1, dStep/d omega_c>.
+int orc_reactive_adjoint_step(int ct, int dt, const void* f, const void* a, const void* b, const void* nuf1,
+                              const void* nua1, const void* nub1, void* nuf0, void* nua0, void* nub0, long long nx,
+                              long long ny, double omega, double omega_c, double k, double* djdw) {
+    if (ct != C_BGK && ct != C_TRT && ct != C_MRT) return -1;  // the adjoint covers BGK, TRT, MRT
+    const Grid g = make_grid(L_D2Q9, nx, ny, 1);
+    long double acc = 0;
+    auto run = [&](auto tag) {
+        using R = decltype(tag);
+        ORC_CT_SWITCH(ct, (reactive_adjoint_step<R, CT>(
+                              static_cast<const R*>(f), static_cast<const R*>(a), static_cast<const R*>(b),
+                              static_cast<const R*>(nuf1), static_cast<const R*>(nua1), static_cast<const R*>(nub1),
+                              static_cast<R*>(nuf0), static_cast<R*>(nua0), static_cast<R*>(nub0), g,
+                              static_cast<R>(omega), static_cast<R>(omega_c), static_cast<R>(k), &acc)));
+        return 0;
+    };
+    int rc;
+    if (dt == T_F64) rc = run(double{});
+    else if (dt == T_F80) rc = run((long double){});
+    else return -1;
+    *djdw = static_cast<double>(acc);
+    return rc;
+}
+
 int orc_reaction_grad(const double* a, const double* b, double k, double* dsa, double* dsb, long long n) {
     for (long long j = 0; j < n; ++j) reaction_grad<double>(a[j], b[j], k, dsa[j], dsb[j]);
     return 0;

@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
             "orc_f16_roundtrip": [vp, vp, vp, ll, i],
             "orc_reactive_gradient": [i, i, vp, vp, vp, ll, ll, d, d, d, i, dp, vp, vp, vp, dp],
             "orc_reaction_grad": [vp, vp, d, vp, vp, ll],
+            "orc_reactive_adjoint_step": [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, ll, ll, d, d, d, dp],
             "orc_max_threads": [],
         }
         for name, args in sig.items():
@@ -206,6 +207,19 @@ def fluid_diagnostics(lattice, dtype, f, wide=False):
     _check(lib().orc_fluid_diagnostics(LATTICES[lattice], DTYPES[dtype], _p(f), nx, ny, nz, int(wide), out),
            "fluid_diagnostics")
     return np.array(out[:])
+
+
+def reactive_adjoint_step(collision, dtype, f, a, b, nuf1, nua1, nub1, omega, omega_c, k):
+    """One step of the oracle's discrete adjoint (D2Q9 + 2 D2Q5) from arbitrary
+    adjoint inputs at the collision cell; returns (nuf0, nua0, nub0, djdw)."""
+    arrs = [np.ascontiguousarray(x, dtype=NP_STORAGE[dtype]) for x in (f, a, b, nuf1, nua1, nub1)]
+    outs = [np.empty_like(arrs[0]), np.empty_like(arrs[1]), np.empty_like(arrs[2])]
+    nx, ny, _ = _dims("D2Q9", arrs[0].shape)
+    dw = (ctypes.c_double * 1)()
+    _check(lib().orc_reactive_adjoint_step(COLLISIONS[collision], DTYPES[dtype], *(_p(x) for x in arrs),
+                                           *(_p(x) for x in outs), nx, ny, float(omega), float(omega_c), float(k),
+                                           dw), "reactive_adjoint_step")
+    return outs[0], outs[1], outs[2], dw[0]
 
 
 def macro_fields(lattice, dtype, f):

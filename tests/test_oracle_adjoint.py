@@ -9,6 +9,7 @@ import pytest
 
 import cases
 from oracle import oracle as O
+from test_oracle_pins import VEL
 
 
 def small_state(nx=10, ny=8, seed=3):
@@ -61,3 +62,56 @@ def test_adjoint_matches_central_differences(coll):
     fd = (J_of(coll, f, a, b, omega, omega_c + hw, k, steps) - J_of(coll, f, a, b, omega, omega_c - hw, k, steps)) / (
         2 * hw)
     assert abs(fd - dw) <= 1e-6 * abs(fd), (fd, dw)
+
+
+def to_collision_cell(lam, lattice):
+    """Stored-state adjoint Lambda[i][y] -> collision-cell adjoint nu[i][x] = Lambda[i][x - c_i]."""
+    nu = np.empty_like(lam)
+    for i, (cx, cy) in enumerate(VEL[lattice]):
+        nu[i] = np.roll(lam[i], shift=(cy, cx), axis=(0, 1))
+    return nu
+
+
+def to_stored(nu, lattice):
+    """Collision-cell adjoint nu -> stored-state adjoint Lambda[i][y] = nu[i][y + c_i]."""
+    lam = np.empty_like(nu)
+    for i, (cx, cy) in enumerate(VEL[lattice]):
+        lam[i] = np.roll(nu[i], shift=(-cy, -cx), axis=(0, 1))
+    return lam
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+def test_adjoint_step_dot_product(coll):
+    """<Lambda_1, L v> = <L^T Lambda_1, v> for one step, with random v and a
+    random adjoint Lambda_1 on every population of the fluid and both species
+    (so every path of the transposed step is exercised, not only those the
+    loss reaches).  L v is the central difference of the forward step in
+    extended precision (h = 1e-6: truncation ~1e-12, rounding ~1e-13); the
+    adjoint step runs in FP64.  Also <Lambda_1, dStep/d omega_c> against the
+    adjoint's accumulated dJ/d omega_c term."""
+    f, a, b = small_state(nx=7, ny=6, seed=9)
+    omega, omega_c, k = 1 / 0.7, 1 / 0.9, 0.4
+    g = cases.rng(17)
+    v = [g.standard_normal(x.shape) * 1e-2 for x in (f, a, b)]
+    lam1 = [g.standard_normal(x.shape) for x in (f, a, b)]
+    nu1 = [to_collision_cell(lam1[0], "D2Q9"), to_collision_cell(lam1[1], "D2Q5"), to_collision_cell(lam1[2], "D2Q5")]
+    nuf0, nua0, nub0, djdw = O.reactive_adjoint_step(coll, "f64", f, a, b, *nu1, omega, omega_c, k)
+    lam0 = [to_stored(nuf0, "D2Q9"), to_stored(nua0, "D2Q5"), to_stored(nub0, "D2Q5")]
+    ext = [x.astype(np.longdouble) for x in (f, a, b)]
+    h = np.longdouble(1e-6)
+
+    def step(state, oc):
+        return O.reactive_run("D2Q9", coll, "D2Q5", "f80", *state, omega, oc, k, 1)
+
+    plus = step([x + h * d for x, d in zip(ext, v)], omega_c)
+    minus = step([x - h * d for x, d in zip(ext, v)], omega_c)
+    lv = [(p - m) / (2 * h) for p, m in zip(plus, minus)]
+    lhs = sum(float(np.sum(l.astype(np.longdouble) * x)) for l, x in zip(lam1, lv))
+    rhs = sum(float(np.sum(l * x)) for l, x in zip(lam0, v))
+    scale = sum(float(np.sum(np.abs(l) * np.abs(x))) for l, x in zip(lam0, v))
+    assert abs(lhs - rhs) <= 1e-10 * scale, (lhs, rhs, scale)
+    hw = np.longdouble(2.0 ** -20)  # omega_c +- hw exact in double
+    dp = step(ext, omega_c + float(hw))
+    dm = step(ext, omega_c - float(hw))
+    lw = sum(float(np.sum(l.astype(np.longdouble) * (p - m) / (2 * hw))) for l, p, m in zip(lam1, dp, dm))
+    assert abs(lw - djdw) <= 1e-9 * max(abs(lw), 1.0), (lw, djdw)
