@@ -170,6 +170,22 @@ std::vector<double> initial_state(const Config& c, int q) {
     return f;
 }
 
+// Ping-pong device buffers, freed on every exit path.
+struct DeviceBuffers {
+    void* cur = nullptr;
+    void* next = nullptr;
+    explicit DeviceBuffers(std::size_t bytes) {
+        check(hipMalloc(&cur, bytes), "hipMalloc populations");
+        check(hipMalloc(&next, bytes), "hipMalloc populations");
+    }
+    ~DeviceBuffers() {
+        (void)hipFree(cur);
+        (void)hipFree(next);
+    }
+    DeviceBuffers(const DeviceBuffers&) = delete;
+    DeviceBuffers& operator=(const DeviceBuffers&) = delete;
+};
+
 void run_config(const Config& c, const Run& r) {
     const std::filesystem::path dir = std::filesystem::path(c.out) / ("output_tgv3d_" + r.name);
     std::filesystem::create_directories(dir);
@@ -188,9 +204,9 @@ void run_config(const Config& c, const Run& r) {
             throw std::runtime_error("cannot read " + c.load);
     }
     const std::size_t elem = c.storage == LBC_F64 ? 8 : 4;
-    void *cur = nullptr, *next = nullptr;
-    check(hipMalloc(&cur, count * elem), "hipMalloc populations");
-    check(hipMalloc(&next, count * elem), "hipMalloc populations");
+    DeviceBuffers buffers(count * elem);
+    void*& cur = buffers.cur;
+    void*& next = buffers.next;
     if (c.storage == LBC_F64) {
         check(hipMemcpy(cur, host.data(), count * 8, hipMemcpyHostToDevice), "hipMemcpy H2D");
     } else {
@@ -268,8 +284,6 @@ void run_config(const Config& c, const Run& r) {
               << " s, MLUPS=" << static_cast<double>(cells) * completed / elapsed.count() / 1e6
               << (crashed ? " (terminated early)" : "") << "\n\n"
               << std::flush;
-    check(hipFree(cur), "hipFree");
-    check(hipFree(next), "hipFree");
 }
 
 }  // namespace
