@@ -192,14 +192,16 @@ def ensemble_rate(torch, steps=1000):
     from lbcube.ensemble import Ensemble, reactive_mixing_ensemble
 
     n, omega, omega_c, k = reactive_mixing_ensemble()
-    ens = Ensemble("TRT", "f16", [(int(x), int(x)) for x in n], omega, omega_c, k, fast=True)
+    half = [0.5] * len(n)  # species stored as g - w/2: the reference concentration of 0..1 fields
+    ens = Ensemble("TRT", "f16", [(int(x), int(x)) for x in n], omega, omega_c, k, fast=True, ref_a=half,
+                   ref_b=half)
     for i, ni in enumerate(n):
         rho, u = lbinit.taylor_green_2d(int(ni), int(ni), 0.04)
         ca, cb = lbinit.tanh_half_planes(int(ni), delta=max(2.0, ni / 32))
         ens.fluid(i).copy_(lbinit.equilibrium("D2Q9", rho, u, "f16"))
         a, b = ens.species(i)
-        a.copy_(lbinit.scalar_equilibrium("D2Q5", ca, u, "f16"))
-        b.copy_(lbinit.scalar_equilibrium("D2Q5", cb, u, "f16"))
+        a.copy_(lbinit.scalar_equilibrium("D2Q5", ca, u, "f16", 0.5))
+        b.copy_(lbinit.scalar_equilibrium("D2Q5", cb, u, "f16", 0.5))
     chosen, timings = ens.autotune(trial_steps=3)
     ens.run(2, graph=True)
     ms = _time(torch, lambda: ens.run(steps, graph=True))
@@ -272,15 +274,17 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
                                "achieved_gbs": 38 * es * cells / (ms_fast * 1e-3) / 1e9, "bytes_per_cell": 38 * es}
         del f, ft, a, at, b, bt, state
         torch.cuda.empty_cache()
-    # FP16 storage of the deviations: the fused step in both arithmetic forms
+    # FP16 storage of the deviations (species about their reference 1/2): the
+    # fused step in both arithmetic forms
     f = lbinit.equilibrium("D2Q9", None, u, "f16")
-    a = lbinit.scalar_equilibrium("D2Q5", ca, u, "f16")
-    b = lbinit.scalar_equilibrium("D2Q5", cb, u, "f16")
+    a = lbinit.scalar_equilibrium("D2Q5", ca, u, "f16", 0.5)
+    b = lbinit.scalar_equilibrium("D2Q5", cb, u, "f16", 0.5)
     state = [f, torch.empty_like(f), a, torch.empty_like(a), b, torch.empty_like(b)]
 
     def fused16(fast):
         f, ft, a, at, b, bt = state
-        lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast)
+        lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast,
+                                   ref_a=0.5, ref_b=0.5)
         state[:] = [ft, f, at, a, bt, b]
 
     for fast in (False, True):

@@ -117,8 +117,8 @@ hipError_t launch_scalar_step_gpu(const Real* current_scalar, Real* next_scalar,
     return static_cast<hipError_t>(lbc_scalar_step(FluidLattice::abi, ScalarLattice::abi, storage_code<Real>(),
                                                    current_scalar, next_scalar, current_fluid, x_extent, y_extent,
                                                    z_extent, static_cast<double>(omega_c),
-                                                   static_cast<double>(source_term), block.x, block.y, block.z,
-                                                   stream));
+                                                   static_cast<double>(source_term), 0.0, block.x, block.y,
+                                                   block.z, stream));
 }
 
 template <IsLatticeModel FluidLattice, IsLatticeModel ScalarLattice, std::floating_point Real>
@@ -145,8 +145,8 @@ hipError_t launch_reaction_AB_gpu(const Real* current_a, Real* next_a, const Rea
     return static_cast<hipError_t>(lbc_reaction_ab_step(FluidLattice::abi, ScalarLattice::abi, storage_code<Real>(),
                                                         current_a, next_a, current_b, next_b, current_fluid,
                                                         x_extent, y_extent, z_extent, static_cast<double>(omega_c),
-                                                        static_cast<double>(k_react), block.x, block.y, block.z,
-                                                        stream));
+                                                        static_cast<double>(k_react), 0.0, 0.0, block.x, block.y,
+                                                        block.z, stream));
 }
 
 }  // namespace lbm

@@ -74,21 +74,31 @@ int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, c
                       uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
                       uint32_t block_z, void* stream);
 
+/* Reference concentrations of scalar fields.  With LBC_F16 / LBC_BF16
+ * storage a scalar field's populations are stored as g_i - w_i c_ref, with
+ * c_ref the field's reference concentration (w_i alone is the fluid's rest
+ * state, not a scalar's): concentrations near c_ref keep the full 11 (8)
+ * significand bits on their deviation.  Every call that reads or writes a
+ * field must pass the same c_ref for it (0 stores g_i as it is); FP32/FP64
+ * storage ignores it.  The reference has no reduced-precision storage. */
+
 /* Passive-scalar advection-diffusion step with a uniform source.
  * Replaces launch_scalar_step_gpu<F, S, Real> (include/lattice_cuda.cuh:209-222,
- * src/lattice_cuda.cu:539-589).  Pairs: D2Q9/D2Q5, D3Q19/D3Q7, D3Q27/D3Q7. */
+ * src/lattice_cuda.cu:539-589).  Pairs: D2Q9/D2Q5, D3Q19/D3Q7, D3Q27/D3Q7.
+ * `reference`: the scalar field's reference concentration (above). */
 int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
                     void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
-                    double omega_c, double source_term, uint32_t block_x, uint32_t block_y, uint32_t block_z,
-                    void* stream);
+                    double omega_c, double source_term, double reference, uint32_t block_x, uint32_t block_y,
+                    uint32_t block_z, void* stream);
 
 /* Fused A + B -> C advection-diffusion-reaction step of two species.
  * Replaces launch_reaction_AB_gpu<F, S, Real> (include/lattice_cuda.cuh:276-291,
- * src/lattice_cuda.cu:610-666). */
+ * src/lattice_cuda.cu:610-666).  reference_a/b: the species' reference
+ * concentrations (above). */
 int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_a, void* next_a,
                          const void* current_b, void* next_b, const void* current_fluid, uint64_t nx, uint64_t ny,
-                         uint64_t nz, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
-                         uint32_t block_z, void* stream);
+                         uint64_t nz, double omega_c, double k_react, double reference_a, double reference_b,
+                         uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
 
 /* One pass that performs lbc_fluid_step followed by lbc_reaction_ab_step on
  * the freshly written fluid state (the order of src/simulate_reaction.cpp:516-546),
@@ -98,8 +108,8 @@ int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, con
 int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
                             const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
                             const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
-                            double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
-                            uint32_t block_z, void* stream);
+                            double omega, double omega_c, double k_react, double reference_a, double reference_b,
+                            uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
 
 /* lbc_reactive_fused_step with flags (0 = lbc_reactive_fused_step exactly).
  * LBC_FLAG_FAST (BGK, TRT and MRT; ignored by RLBM): the fluid part in the
@@ -114,8 +124,9 @@ int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice
 int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_lattice, int storage, uint32_t flags,
                                const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
                                const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
-                               double omega, double omega_c, double k_react, uint32_t block_x, uint32_t block_y,
-                               uint32_t block_z, void* stream);
+                               double omega, double omega_c, double k_react, double reference_a,
+                               double reference_b, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                               void* stream);
 
 /* Gradient of the 2D reactive loop (D2Q9 BGK, TRT or MRT + two D2Q5 species,
  * A + B -> C; `steps` calls of lbc_reactive_fused_step from f0, a0, b0) by the
@@ -164,6 +175,10 @@ int lbc_ensemble_set_block(lbc_ensemble* ensemble, uint32_t threads);
  * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...) on it alone.  Waits for
  * the device first. */
 int lbc_ensemble_set_flags(lbc_ensemble* ensemble, uint32_t flags);
+/* Per-instance reference concentrations of the species A and B (arrays of
+ * `count`; NULL = 0 for every instance).  Defaults to 0.  Waits for the
+ * device first (the tables are rewritten in place). */
+int lbc_ensemble_set_references(lbc_ensemble* ensemble, const double* reference_a, const double* reference_b);
 int lbc_ensemble_step(lbc_ensemble* ensemble, const void* current_fluid, void* next_fluid, const void* current_a,
                       void* next_a, const void* current_b, void* next_b, void* stream);
 /* `steps` steps between buffer sets 0 and 1 (result in set steps % 2); with
@@ -193,7 +208,8 @@ int lbc_fluid_diagnostics(int lattice, int storage, const void* populations, uin
  * Accumulated in double on the device in a fixed order; synchronizes `stream`;
  * shares the workspace and serialization of lbc_fluid_diagnostics. */
 int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b, uint64_t nx,
-                       uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream);
+                       uint64_t ny, uint64_t nz, double k_react, double reference_a, double reference_b,
+                       double* out8, void* stream);
 
 /* Stability guard of compute_max_velocity (src/turbulent_tgv_3d.cpp:149-173):
  * out[0] = max |u| over the cells, +infinity if any cell's speed is not
@@ -214,10 +230,11 @@ int lbc_macro_fields(int lattice, int storage, const void* populations, uint64_t
                      double* rho, double* u, void* stream);
 
 /* Concentration per cell of a scalar population buffer (compute_concentration,
- * include/lattice_physics.hpp:618-627), as double in device memory.
- * Asynchronous on `stream`; allocates nothing. */
+ * include/lattice_physics.hpp:618-627), as double in device memory; `reference`
+ * is the field's reference concentration.  Asynchronous on `stream`; allocates
+ * nothing. */
 int lbc_scalar_fields(int scalar_lattice, int storage, const void* populations, uint64_t nx, uint64_t ny, uint64_t nz,
-                      double* concentration, void* stream);
+                      double reference, double* concentration, void* stream);
 
 /* Streaming device copy of `bytes` (a multiple of 16) from src to dst: the
  * measured HBM ceiling the benchmark reports next to the spec peak.  No

@@ -342,7 +342,8 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     auto step = [&](State<R> in, State<R> out) {
         return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, flags, in.f, out.f, in.a,
                                                                   out.a, in.b, out.b, s.nx, s.ny, 1, omega_d,
-                                                                  omega_c_d, k_d, s.block.x, s.block.y, 1, st));
+                                                                  omega_c_d, k_d, 0.0, 0.0, s.block.x, s.block.y, 1,
+                                                                  st));
     };
     // forward with checkpoints at the segment starts
     if ((e = hipMemcpyAsync(ckpt, f0, 9 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;

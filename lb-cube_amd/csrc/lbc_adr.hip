@@ -26,7 +26,7 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b,
     const typename Storage<ST>::S* __restrict__ cur_f, Cells c, typename Storage<ST>::R omega_c,
-    typename Storage<ST>::R param) {
+    typename Storage<ST>::R param, typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,8 +45,8 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
-        if constexpr (REACT) gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
+        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        if constexpr (REACT) gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     if constexpr (REACT) {
         const R ca = concentration<S, R>(ga);
@@ -59,8 +59,8 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     }
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));
-        if constexpr (REACT) St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(0));
+        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        if constexpr (REACT) St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
@@ -70,29 +70,30 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, Cells c,
     typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
-    typename Storage<ST>::R k_react) {
+    typename Storage<ST>::R k_react, typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= c.count) return;
     const CellCoord q = cell_coord(p, c);
     const Pull pull(p, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
     if constexpr (FAST)
         reactive_cell_fast<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
-                                         omega_c, k_react);
+                                         omega_c, k_react, ref_a, ref_b);
     else
         reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
-                                    omega_c, k_react);
+                                    omega_c, k_react, ref_a, ref_b);
 }
 
 template <class F, class S, int ST, bool REACT>
 hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, const void* cf, const LaunchShape& s,
-                         double omega_c, double param, hipStream_t stream) {
+                         double omega_c, double param, double ref_a, double ref_b, hipStream_t stream) {
     using St = Storage<ST>;
     using T = typename St::S;
     using R = typename St::R;
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(ca), static_cast<T*>(na),
                            static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells,
-                           static_cast<R>(omega_c), static_cast<R>(param));
+                           static_cast<R>(omega_c), static_cast<R>(param), static_cast<R>(ref_a),
+                           static_cast<R>(ref_b));
     };
     if (s.threads() <= 256) args(scalar_step_kernel<F, S, ST, REACT, 256>);
     else args(scalar_step_kernel<F, S, ST, REACT, 1024>);
@@ -101,31 +102,34 @@ hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, con
 
 template <class F, class S, bool REACT>
 hipError_t scalar_by_storage(int storage, const void* ca, void* na, const void* cb, void* nb, const void* cf,
-                             const LaunchShape& s, double omega_c, double param, hipStream_t st) {
+                             const LaunchShape& s, double omega_c, double param, double ra, double rb,
+                             hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_scalar<F, S, kF32, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
-        case kF64: return launch_scalar<F, S, kF64, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
-        case kF16: return launch_scalar<F, S, kF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
-        case kBF16: return launch_scalar<F, S, kBF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, st);
+        case kF32: return launch_scalar<F, S, kF32, REACT>(ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
+        case kF64: return launch_scalar<F, S, kF64, REACT>(ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
+        case kF16: return launch_scalar<F, S, kF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
+        case kBF16: return launch_scalar<F, S, kBF16, REACT>(ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <bool REACT>
 hipError_t scalar_dispatch(int fl, int sl, int storage, const void* ca, void* na, const void* cb, void* nb,
-                           const void* cf, const LaunchShape& s, double omega_c, double param, hipStream_t st) {
+                           const void* cf, const LaunchShape& s, double omega_c, double param, double ra, double rb,
+                           hipStream_t st) {
     if (fl == kD2Q9 && sl == kD2Q5)
-        return scalar_by_storage<D2Q9, D2Q5, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+        return scalar_by_storage<D2Q9, D2Q5, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
     if (fl == kD3Q19 && sl == kD3Q7)
-        return scalar_by_storage<D3Q19, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+        return scalar_by_storage<D3Q19, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
     if (fl == kD3Q27 && sl == kD3Q7)
-        return scalar_by_storage<D3Q27, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, st);
+        return scalar_by_storage<D3Q27, D3Q7, REACT>(storage, ca, na, cb, nb, cf, s, omega_c, param, ra, rb, st);
     return hipErrorInvalidValue;
 }
 
 template <class F, int CT, class S, int ST, bool FAST>
 hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
-                        const LaunchShape& s, double omega, double omega_c, double k, hipStream_t stream) {
+                        const LaunchShape& s, double omega, double omega_c, double k, double ra, double rb,
+                        hipStream_t stream) {
     using St = Storage<ST>;
     using T = typename St::S;
     using R = typename St::R;
@@ -134,7 +138,8 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
                            static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
-                           static_cast<T*>(nb), s.cells, w, wm, static_cast<R>(omega_c), static_cast<R>(k));
+                           static_cast<T*>(nb), s.cells, w, wm, static_cast<R>(omega_c), static_cast<R>(k),
+                           static_cast<R>(ra), static_cast<R>(rb));
     };
     if (s.threads() <= 256) args(reactive_fused_kernel<F, CT, S, ST, FAST, 256>);
     else args(reactive_fused_kernel<F, CT, S, ST, FAST, 1024>);
@@ -143,12 +148,14 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
 
 template <class F, class S, int CT, bool FAST>
 hipError_t fused_by_storage(int storage, const void* cf, void* nf, const void* ca, void* na, const void* cb,
-                            void* nb, const LaunchShape& s, double omega, double omega_c, double k, hipStream_t st) {
+                            void* nb, const LaunchShape& s, double omega, double omega_c, double k, double ra,
+                            double rb, hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_fused<F, CT, S, kF32, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kF64: return launch_fused<F, CT, S, kF64, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kF16: return launch_fused<F, CT, S, kF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kBF16: return launch_fused<F, CT, S, kBF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kF32: return launch_fused<F, CT, S, kF32, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, ra, rb, st);
+        case kF64: return launch_fused<F, CT, S, kF64, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, ra, rb, st);
+        case kF16: return launch_fused<F, CT, S, kF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, ra, rb, st);
+        case kBF16:
+            return launch_fused<F, CT, S, kBF16, FAST>(cf, nf, ca, na, cb, nb, s, omega, omega_c, k, ra, rb, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -156,29 +163,24 @@ hipError_t fused_by_storage(int storage, const void* cf, void* nf, const void* c
 template <class F, class S>
 hipError_t fused_by_collision(int collision, int storage, bool fast, const void* cf, void* nf, const void* ca,
                               void* na, const void* cb, void* nb, const LaunchShape& s, double omega, double omega_c,
-                              double k, hipStream_t st) {
+                              double k, double ra, double rb, hipStream_t st) {
+#define LBC_FUSED(CT, FAST) fused_by_storage<F, S, CT, FAST>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, ra, rb, st)
     switch (collision) {
-        case kBGK:
-            return fast ? fused_by_storage<F, S, kBGK, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
-                        : fused_by_storage<F, S, kBGK, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kTRT:
-            return fast ? fused_by_storage<F, S, kTRT, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
-                        : fused_by_storage<F, S, kTRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kMRT:
-            return fast ? fused_by_storage<F, S, kMRT, true>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st)
-                        : fused_by_storage<F, S, kMRT, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
-        case kRLBM:
-            return fused_by_storage<F, S, kRLBM, false>(storage, cf, nf, ca, na, cb, nb, s, omega, omega_c, k, st);
+        case kBGK: return fast ? LBC_FUSED(kBGK, true) : LBC_FUSED(kBGK, false);
+        case kTRT: return fast ? LBC_FUSED(kTRT, true) : LBC_FUSED(kTRT, false);
+        case kMRT: return fast ? LBC_FUSED(kMRT, true) : LBC_FUSED(kMRT, false);
+        case kRLBM: return LBC_FUSED(kRLBM, false);
         default: return hipErrorInvalidValue;
     }
+#undef LBC_FUSED
 }
 
 }  // namespace lbc
 
 extern "C" int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
                                void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
-                               double omega_c, double source_term, uint32_t block_x, uint32_t block_y,
-                               uint32_t block_z, void* stream) {
+                               double omega_c, double source_term, double reference, uint32_t block_x,
+                               uint32_t block_y, uint32_t block_z, void* stream) {
     using namespace lbc;
     if (current_scalar == nullptr || next_scalar == nullptr || current_fluid == nullptr)
         return hipErrorInvalidDevicePointer;
@@ -187,13 +189,14 @@ extern "C" int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storag
     if (e != hipSuccess) return e;
     if (current_scalar == next_scalar || next_scalar == current_fluid) return hipErrorInvalidValue;
     return scalar_dispatch<false>(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, nullptr,
-                                  nullptr, current_fluid, s, omega_c, source_term, as_stream(stream));
+                                  nullptr, current_fluid, s, omega_c, source_term, reference, 0.0, as_stream(stream));
 }
 
 extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_a,
                                     void* next_a, const void* current_b, void* next_b, const void* current_fluid,
                                     uint64_t nx, uint64_t ny, uint64_t nz, double omega_c, double k_react,
-                                    uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
+                                    double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
+                                    uint32_t block_z, void* stream) {
     using namespace lbc;
     if (current_a == nullptr || next_a == nullptr || current_b == nullptr || next_b == nullptr ||
         current_fluid == nullptr)
@@ -203,15 +206,15 @@ extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int s
     if (e != hipSuccess) return e;
     if (current_a == next_a || current_b == next_b || next_a == next_b) return hipErrorInvalidValue;
     return scalar_dispatch<true>(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
-                                 current_fluid, s, omega_c, k_react, as_stream(stream));
+                                 current_fluid, s, omega_c, k_react, reference_a, reference_b, as_stream(stream));
 }
 
 extern "C" int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_lattice, int storage,
                                           uint32_t flags, const void* current_fluid, void* next_fluid,
                                           const void* current_a, void* next_a, const void* current_b, void* next_b,
                                           uint64_t nx, uint64_t ny, uint64_t nz, double omega, double omega_c,
-                                          double k_react, uint32_t block_x, uint32_t block_y, uint32_t block_z,
-                                          void* stream) {
+                                          double k_react, double reference_a, double reference_b, uint32_t block_x,
+                                          uint32_t block_y, uint32_t block_z, void* stream) {
     using namespace lbc;
     if (current_fluid == nullptr || next_fluid == nullptr || current_a == nullptr || next_a == nullptr ||
         current_b == nullptr || next_b == nullptr)
@@ -226,22 +229,26 @@ extern "C" int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int 
     const hipStream_t st = as_stream(stream);
     if (fluid_lattice == kD2Q9 && scalar_lattice == kD2Q5)
         return fused_by_collision<D2Q9, D2Q5>(collision, storage, fast, current_fluid, next_fluid, current_a, next_a,
-                                              current_b, next_b, s, omega, omega_c, k_react, st);
+                                              current_b, next_b, s, omega, omega_c, k_react, reference_a,
+                                              reference_b, st);
     if (fluid_lattice == kD3Q19 && scalar_lattice == kD3Q7)
         return fused_by_collision<D3Q19, D3Q7>(collision, storage, fast, current_fluid, next_fluid, current_a,
-                                               next_a, current_b, next_b, s, omega, omega_c, k_react, st);
+                                               next_a, current_b, next_b, s, omega, omega_c, k_react, reference_a,
+                                              reference_b, st);
     if (fluid_lattice == kD3Q27 && scalar_lattice == kD3Q7)
         return fused_by_collision<D3Q27, D3Q7>(collision, storage, fast, current_fluid, next_fluid, current_a,
-                                               next_a, current_b, next_b, s, omega, omega_c, k_react, st);
+                                               next_a, current_b, next_b, s, omega, omega_c, k_react, reference_a,
+                                              reference_b, st);
     return hipErrorInvalidValue;
 }
 
 extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int scalar_lattice, int storage,
                                        const void* current_fluid, void* next_fluid, const void* current_a,
                                        void* next_a, const void* current_b, void* next_b, uint64_t nx, uint64_t ny,
-                                       uint64_t nz, double omega, double omega_c, double k_react, uint32_t block_x,
-                                       uint32_t block_y, uint32_t block_z, void* stream) {
+                                       uint64_t nz, double omega, double omega_c, double k_react,
+                                       double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
+                                       uint32_t block_z, void* stream) {
     return lbc_reactive_fused_step_ex(fluid_lattice, collision, scalar_lattice, storage, 0u, current_fluid,
                                       next_fluid, current_a, next_a, current_b, next_b, nx, ny, nz, omega, omega_c,
-                                      k_react, block_x, block_y, block_z, stream);
+                                      k_react, reference_a, reference_b, block_x, block_y, block_z, stream);
 }

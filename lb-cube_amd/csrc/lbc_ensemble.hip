@@ -34,6 +34,9 @@ struct EnsembleInstance {
     FastDiv by_nx;
     float omega, omega_minus, omega_c, k;
     double omega_d, omega_minus_d, omega_c_d, k_d;
+    // reference concentrations of the species storage (F16 / BF16 hold g - w c_ref)
+    float ref_a, ref_b;
+    double ref_a_d, ref_b_d;
 };
 
 constexpr uint32_t kEnsembleThreads[4] = {128, 256, 512, 1024};
@@ -64,13 +67,16 @@ __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storag
     const R wm = std::is_same<R, double>::value ? R(in.omega_minus_d) : R(in.omega_minus);
     const R wc = std::is_same<R, double>::value ? R(in.omega_c_d) : R(in.omega_c);
     const R k = std::is_same<R, double>::value ? R(in.k_d) : R(in.k);
+    const R ra = std::is_same<R, double>::value ? R(in.ref_a_d) : R(in.ref_a);
+    const R rb = std::is_same<R, double>::value ? R(in.ref_b_d) : R(in.ref_b);
     const Pull pull(p, x, static_cast<int>(y), 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1);
     if constexpr (FAST)
         reactive_cell_fast<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off,
-                                               cb + in.s_off, nb + in.s_off, pull, in.plane, omega, wm, wc, k);
+                                               cb + in.s_off, nb + in.s_off, pull, in.plane, omega, wm, wc, k, ra,
+                                               rb);
     else
         reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
-                                          nb + in.s_off, pull, in.plane, omega, wm, wc, k);
+                                          nb + in.s_off, pull, in.plane, omega, wm, wc, k, ra, rb);
 }
 
 }  // namespace lbc
@@ -258,6 +264,19 @@ extern "C" int lbc_ensemble_set_flags(lbc_ensemble* e, uint32_t flags) {
     }
     e->flags = flags;
     return hipSuccess;
+}
+
+extern "C" int lbc_ensemble_set_references(lbc_ensemble* e, const double* reference_a, const double* reference_b) {
+    using namespace lbc;
+    if (e == nullptr) return hipErrorInvalidValue;
+    for (int i = 0; i < e->count; ++i) {
+        EnsembleInstance& in = e->host[i];
+        in.ref_a_d = reference_a ? reference_a[i] : 0.0;
+        in.ref_b_d = reference_b ? reference_b[i] : 0.0;
+        in.ref_a = static_cast<float>(in.ref_a_d);
+        in.ref_b = static_cast<float>(in.ref_b_d);
+    }
+    return build_table(e, e->threads);  // waits for the device, rewrites the tables, drops a captured graph
 }
 
 extern "C" int lbc_ensemble_step(lbc_ensemble* e, const void* cur_f, void* next_f, const void* cur_a, void* next_a,

@@ -25,7 +25,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
                                               typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
                                               size_t plane, typename Storage<ST>::R omega,
                                               typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
-                                              typename Storage<ST>::R k_react) {
+                                              typename Storage<ST>::R k_react, typename Storage<ST>::R ref_a,
+                                              typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
     const uint32_t cell = pull.dst;
@@ -47,8 +48,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
-        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
+        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     const R ca = concentration<S, R>(ga);
     const R cb = concentration<S, R>(gb);
@@ -57,8 +58,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));
-        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(0));
+        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
@@ -76,7 +77,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
                                                    typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
                                                    size_t plane, typename Storage<ST>::R omega,
                                                    typename Storage<ST>::R omega_minus,
-                                                   typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react) {
+                                                   typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react,
+                                                   typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
     const uint32_t cell = pull.dst;
@@ -95,8 +97,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(0));
-        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(0));
+        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     const R ca = concentration<S, R>(ga);
     const R cb = concentration<S, R>(gb);
@@ -105,8 +107,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     collide_scalar_fast<S, R>(gb, cb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(0));
-        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(0));
+        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 

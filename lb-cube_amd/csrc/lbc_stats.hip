@@ -80,7 +80,8 @@ __global__ __launch_bounds__(kStatThreads) void fluid_diag_kernel(const typename
 template <class S, int ST>
 __global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const typename Storage<ST>::S* __restrict__ a,
                                                                       const typename Storage<ST>::S* __restrict__ b,
-                                                                      uint32_t cells, double* partial) {
+                                                                      uint32_t cells, typename Storage<ST>::R ref_a,
+                                                                      typename Storage<ST>::R ref_b, double* partial) {
     using St = Storage<ST>;
     using R = typename St::R;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -90,8 +91,8 @@ __global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const type
         R ga[S::Q], gb[S::Q];
 #pragma unroll
         for (int i = 0; i < S::Q; ++i) {
-            ga[i] = St::load(plane_ptr(a, i, plane) + c, R(0));
-            gb[i] = St::load(plane_ptr(b, i, plane) + c, R(0));
+            ga[i] = St::load(plane_ptr(a, i, plane) + c, R(S::w[i]) * ref_a);
+            gb[i] = St::load(plane_ptr(b, i, plane) + c, R(S::w[i]) * ref_b);
         }
         const double ca = static_cast<double>(concentration<S, R>(ga));
         const double cb = static_cast<double>(concentration<S, R>(gb));
@@ -268,7 +269,8 @@ __global__ __launch_bounds__(kStatThreads) void macro_fields_kernel(const typena
 
 template <class S, int ST>
 __global__ __launch_bounds__(kStatThreads) void scalar_fields_kernel(const typename Storage<ST>::S* __restrict__ g,
-                                                                     uint32_t cells, double* __restrict__ conc) {
+                                                                     uint32_t cells, typename Storage<ST>::R ref,
+                                                                     double* __restrict__ conc) {
     using St = Storage<ST>;
     using R = typename St::R;
     const uint32_t c = blockIdx.x * kStatThreads + threadIdx.x;
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(kStatThreads) void scalar_fields_kernel(const typen
     const size_t plane = cells;
     R loc[S::Q];
 #pragma unroll
-    for (int i = 0; i < S::Q; ++i) loc[i] = St::load(plane_ptr(g, i, plane) + c, R(0));
+    for (int i = 0; i < S::Q; ++i) loc[i] = St::load(plane_ptr(g, i, plane) + c, R(S::w[i]) * ref);
     conc[c] = static_cast<double>(concentration<S, R>(loc));
 }
 
@@ -288,14 +290,16 @@ hipError_t run_macro_fields(const void* f, uint32_t cells, double* rho, double* 
 }
 
 template <class S, int ST>
-hipError_t run_scalar_fields(const void* g, uint32_t cells, double* conc, hipStream_t st) {
+hipError_t run_scalar_fields(const void* g, uint32_t cells, double ref, double* conc, hipStream_t st) {
     hipLaunchKernelGGL((scalar_fields_kernel<S, ST>), dim3((cells + kStatThreads - 1) / kStatThreads),
-                       dim3(kStatThreads), 0, st, static_cast<const typename Storage<ST>::S*>(g), cells, conc);
+                       dim3(kStatThreads), 0, st, static_cast<const typename Storage<ST>::S*>(g), cells,
+                       static_cast<typename Storage<ST>::R>(ref), conc);
     return hipGetLastError();
 }
 
 template <class S, int ST>
-hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double* out, hipStream_t st) {
+hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
+                              double* out, hipStream_t st) {
     std::lock_guard<std::mutex> lock(stats_mutex);
     hipError_t e;
     double* ws = workspace(e);
@@ -303,7 +307,7 @@ hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, doub
     using R = typename Storage<ST>::R;
     hipLaunchKernelGGL((reactive_stats_kernel<S, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
                        static_cast<const typename Storage<ST>::S*>(a), static_cast<const typename Storage<ST>::S*>(b),
-                       cells, ws);
+                       cells, static_cast<R>(ra), static_cast<R>(rb), ws);
     hipLaunchKernelGGL((finish_kernel<5>), dim3(1), dim3(kStatThreads), 0, st, ws, ws + kStatBlocks * kStatSlots,
                        cells, k, static_cast<double>(Eps<R>::v));
     e = hipGetLastError();
@@ -342,18 +346,19 @@ hipError_t macro_fields_storage(int storage, const void* f, uint32_t cells, doub
 }
 
 template <class S>
-hipError_t scalar_fields_storage(int storage, const void* g, uint32_t cells, double* conc, hipStream_t st) {
-    LBC_STORAGE_DISPATCH(run_scalar_fields, S, storage, g, cells, conc, st)
+hipError_t scalar_fields_storage(int storage, const void* g, uint32_t cells, double ref, double* conc,
+                                 hipStream_t st) {
+    LBC_STORAGE_DISPATCH(run_scalar_fields, S, storage, g, cells, ref, conc, st)
 }
 
 template <class S>
-hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, double k, double* out,
-                         hipStream_t st) {
+hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
+                         double* out, hipStream_t st) {
     switch (storage) {
-        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, out, st);
-        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, out, st);
-        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, out, st);
-        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, out, st);
+        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, ra, rb, out, st);
+        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, ra, rb, out, st);
+        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, ra, rb, out, st);
+        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, ra, rb, out, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -378,7 +383,8 @@ extern "C" int lbc_fluid_diagnostics(int lattice, int storage, const void* popul
 }
 
 extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b,
-                                  uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double* out8, void* stream) {
+                                  uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double reference_a,
+                                  double reference_b, double* out8, void* stream) {
     using namespace lbc;
     if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
@@ -387,8 +393,12 @@ extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* s
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
     switch (scalar_lattice) {
-        case kD2Q5: return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, out8, st);
-        case kD3Q7: return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, out8, st);
+        case kD2Q5:
+            return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
+                                       st);
+        case kD3Q7:
+            return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
+                                       st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -428,7 +438,7 @@ extern "C" int lbc_macro_fields(int lattice, int storage, const void* population
 }
 
 extern "C" int lbc_scalar_fields(int scalar_lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,
-                                 uint64_t nz, double* concentration, void* stream) {
+                                 uint64_t nz, double reference, double* concentration, void* stream) {
     using namespace lbc;
     if (populations == nullptr || concentration == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
@@ -437,8 +447,8 @@ extern "C" int lbc_scalar_fields(int scalar_lattice, int storage, const void* po
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
     switch (scalar_lattice) {
-        case kD2Q5: return scalar_fields_storage<D2Q5>(storage, populations, cells, concentration, st);
-        case kD3Q7: return scalar_fields_storage<D3Q7>(storage, populations, cells, concentration, st);
+        case kD2Q5: return scalar_fields_storage<D2Q5>(storage, populations, cells, reference, concentration, st);
+        case kD3Q7: return scalar_fields_storage<D3Q7>(storage, populations, cells, reference, concentration, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -457,4 +467,4 @@ extern "C" void lbc_default_block(int lattice, uint64_t nx, uint64_t ny, uint64_
 
 extern "C" const char* lbc_status_string(int status) { return hipGetErrorString(static_cast<hipError_t>(status)); }
 
-extern "C" int lbc_abi_version(void) { return 1; }
+extern "C" int lbc_abi_version(void) { return 2; }

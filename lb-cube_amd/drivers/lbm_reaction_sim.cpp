@@ -187,7 +187,7 @@ int main(int argc, char** argv) {
         auto log = [&](int s) {
             double out[8];
             check(static_cast<hipError_t>(lbc_reactive_stats(ScalarLattice::abi, LBC_F64, a.cur, b.cur, c.nx, c.ny,
-                                                             1, c.k_react, out, nullptr)),
+                                                             1, c.k_react, 0.0, 0.0, out, nullptr)),
                   "lbc_reactive_stats");
             char row[400];
             std::snprintf(row, sizeof(row), "%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g\n", s, out[0], out[1],
@@ -212,7 +212,7 @@ int main(int argc, char** argv) {
             if (c.fused) {
                 check(static_cast<hipError_t>(lbc_reactive_fused_step(
                           FluidLattice::abi, LBC_BGK, ScalarLattice::abi, LBC_F64, f.cur, f.next, a.cur, a.next, b.cur,
-                          b.next, c.nx, c.ny, 1, omega, omega_c, c.k_react, 256, 1, 1, nullptr)),
+                          b.next, c.nx, c.ny, 1, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
                       "lbc_reactive_fused_step");
                 std::swap(f.cur, f.next);
             } else {

@@ -96,7 +96,7 @@ inline FluidFields fluid_fields(Staging& s, int lattice, const void* pop, std::s
 
 inline std::vector<double> concentration(Staging& s, double* dev, int scalar_lattice, const void* pop, std::size_t nx,
                                          std::size_t ny, std::size_t nz) {
-    check(lbc_scalar_fields(scalar_lattice, LBC_F64, pop, nx, ny, nz, dev, nullptr), "lbc_scalar_fields");
+    check(lbc_scalar_fields(scalar_lattice, LBC_F64, pop, nx, ny, nz, 0.0, dev, nullptr), "lbc_scalar_fields");
     return Staging::host(dev, s.cells);
 }
 

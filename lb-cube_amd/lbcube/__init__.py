@@ -96,8 +96,10 @@ def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fas
                                   bx, by, bz, _stream(stream)), "lbc_fluid_step_ex")
 
 
-def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None):
-    """Passive-scalar ADR step. Replaces launch_scalar_step_gpu."""
+def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None,
+                ref=0.0):
+    """Passive-scalar ADR step. Replaces launch_scalar_step_gpu.  `ref`: the
+    field's reference concentration (f16/bf16 storage holds g - w ref)."""
     _check_pop(scalar_lattice, cur_s, "cur_s")
     _check_pop(scalar_lattice, nxt_s, "nxt_s")
     _check_pop(fluid_lattice, cur_f, "cur_f")
@@ -105,12 +107,13 @@ def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, sou
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
     check(lib().lbc_scalar_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_s.dtype],
                                 cur_s.data_ptr(), nxt_s.data_ptr(), cur_f.data_ptr(), nx, ny, nz, float(omega_c),
-                                float(source), bx, by, bz, _stream(stream)), "lbc_scalar_step")
+                                float(source), float(ref), bx, by, bz, _stream(stream)), "lbc_scalar_step")
 
 
 def reaction_step(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur_f, omega_c, k_react, block=None,
-                  stream=None):
-    """Fused A + B -> C ADR step. Replaces launch_reaction_AB_gpu."""
+                  stream=None, ref_a=0.0, ref_b=0.0):
+    """Fused A + B -> C ADR step. Replaces launch_reaction_AB_gpu.  ref_a/ref_b:
+    the species' reference concentrations."""
     for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
         _check_pop(scalar_lattice, t, n)
     _check_pop(fluid_lattice, cur_f, "cur_f")
@@ -118,14 +121,15 @@ def reaction_step(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
     check(lib().lbc_reaction_ab_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_a.dtype],
                                      cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(),
-                                     cur_f.data_ptr(), nx, ny, nz, float(omega_c), float(k_react), bx, by, bz,
-                                     _stream(stream)), "lbc_reaction_ab_step")
+                                     cur_f.data_ptr(), nx, ny, nz, float(omega_c), float(k_react), float(ref_a),
+                                     float(ref_b), bx, by, bz, _stream(stream)), "lbc_reaction_ab_step")
 
 
 def reactive_fused_step(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, omega,
-                        omega_c, k_react, block=None, stream=None, fast=False):
+                        omega_c, k_react, block=None, stream=None, fast=False, ref_a=0.0, ref_b=0.0):
     """Fluid step + A + B -> C step in one pass (same results as the two calls);
-    fast=True selects the deviation-form arithmetic (LBC_FLAG_FAST)."""
+    fast=True selects the deviation-form arithmetic (LBC_FLAG_FAST); ref_a/ref_b:
+    the species' reference concentrations."""
     _check_pop(fluid_lattice, cur_f, "cur_f")
     _check_pop(fluid_lattice, nxt_f, "nxt_f")
     for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
@@ -135,8 +139,8 @@ def reactive_fused_step(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, 
     check(lib().lbc_reactive_fused_step_ex(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
                                            STORAGE[cur_f.dtype], 1 if fast else 0, cur_f.data_ptr(), nxt_f.data_ptr(),
                                            cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(), nx,
-                                           ny, nz, float(omega), float(omega_c), float(k_react), bx, by, bz,
-                                           _stream(stream)), "lbc_reactive_fused_step_ex")
+                                           ny, nz, float(omega), float(omega_c), float(k_react), float(ref_a),
+                                           float(ref_b), bx, by, bz, _stream(stream)), "lbc_reactive_fused_step_ex")
 
 
 def fluid_diagnostics(lattice, f, stream=None):
@@ -171,12 +175,12 @@ def macro_fields(lattice, f, stream=None):
     return rho, u
 
 
-def scalar_fields(scalar_lattice, g, stream=None):
+def scalar_fields(scalar_lattice, g, stream=None, ref=0.0):
     """Concentration per cell: a float64 device tensor of shape g.shape[1:]."""
     _check_pop(scalar_lattice, g, "g")
     nx, ny, nz = _extents(scalar_lattice, g)
     conc = torch.empty(g.shape[1:], dtype=torch.float64, device=g.device)
-    check(lib().lbc_scalar_fields(LATTICES[scalar_lattice], STORAGE[g.dtype], g.data_ptr(), nx, ny, nz,
+    check(lib().lbc_scalar_fields(LATTICES[scalar_lattice], STORAGE[g.dtype], g.data_ptr(), nx, ny, nz, float(ref),
                                   conc.data_ptr(), _stream(stream)), "lbc_scalar_fields")
     return conc
 
@@ -185,14 +189,15 @@ STAT_NAMES = ("mean_A", "mean_B", "var_A", "var_B", "covariance", "segregation_i
               "mixed_reaction_rate")
 
 
-def reactive_stats(scalar_lattice, a, b, k_react, stream=None):
+def reactive_stats(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=0.0):
     """compute_reactive_stats on the device; returns a dict; synchronizes."""
     _check_pop(scalar_lattice, a, "a")
     _check_pop(scalar_lattice, b, "b")
     nx, ny, nz = _extents(scalar_lattice, a)
     out = (ctypes.c_double * 8)()
     check(lib().lbc_reactive_stats(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx, ny,
-                                   nz, float(k_react), out, _stream(stream)), "lbc_reactive_stats")
+                                   nz, float(k_react), float(ref_a), float(ref_b), out, _stream(stream)),
+          "lbc_reactive_stats")
     return dict(zip(STAT_NAMES, out))
 
 
@@ -200,27 +205,30 @@ def weights_tensor(lattice, dtype=torch.float32, device="cuda"):
     return torch.tensor(WEIGHTS[lattice], dtype=torch.float64).to(dtype).to(device)
 
 
-def encode(lattice, f, storage):
-    """Populations (float tensor) -> storage tensor (f16/bf16 hold f - w for fluid lattices)."""
+def _offsets(lattice, ref, device, ndim):
+    """Per-population storage offsets in float32: w_i for a fluid lattice,
+    w_i * ref for a scalar lattice (the products the kernels form)."""
+    w = weights_tensor(lattice, torch.float32, device)
+    if lattice in ("D2Q5", "D3Q7"):
+        w = w * torch.tensor(float(ref), dtype=torch.float32, device=device)
+    return w.view((-1,) + (1,) * (ndim - 1))
+
+
+def encode(lattice, f, storage, ref=0.0):
+    """Populations (float tensor) -> storage tensor: f16/bf16 hold f - w for a
+    fluid lattice and g - w * ref for a scalar lattice (ref: the field's
+    reference concentration)."""
     dt = STORAGE_NAMES[storage] if isinstance(storage, str) else storage
     if dt in (torch.float32, torch.float64):
         return f.to(dt).contiguous()
-    f32 = f.to(torch.float32)
-    if lattice in ("D2Q5", "D3Q7"):
-        return f32.to(dt).contiguous()
-    w = weights_tensor(lattice, torch.float32, f.device).view((-1,) + (1,) * (f.dim() - 1))
-    return (f32 - w).to(dt).contiguous()
+    return (f.to(torch.float32) - _offsets(lattice, ref, f.device, f.dim())).to(dt).contiguous()
 
 
-def decode(lattice, t):
+def decode(lattice, t, ref=0.0):
     """Storage tensor -> float64 populations."""
     if t.dtype in (torch.float32, torch.float64):
         return t.to(torch.float64)
-    dev = t.to(torch.float32)
-    if lattice in ("D2Q5", "D3Q7"):
-        return dev.to(torch.float64)
-    w = weights_tensor(lattice, torch.float32, t.device).view((-1,) + (1,) * (t.dim() - 1))
-    return (w + dev).to(torch.float64)
+    return (_offsets(lattice, ref, t.device, t.dim()) + t.to(torch.float32)).to(torch.float64)
 
 
 def stream_copy(src, dst, stream=None):

@@ -24,19 +24,21 @@ SIGNATURES = {
                        c_int),
     "lbc_fluid_step_ex": ([c_int, c_int, c_int, c_u32, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_u32, c_u32,
                            c_u32, c_vp], c_int),
-    "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_u32, c_u32,
-                         c_u32, c_vp], c_int),
+    "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double,
+                         c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reaction_ab_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double,
-                              c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+                              c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_fused_step": ([c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64,
-                                 c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+                                 c_double, c_double, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_fused_step_ex": ([c_int, c_int, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64,
-                                    c_u64, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+                                    c_u64, c_double, c_double, c_double, c_double, c_double, c_u32, c_u32, c_u32,
+                                    c_vp], c_int),
     "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
-    "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, _dp, c_vp], c_int),
+    "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double, _dp, c_vp],
+                           c_int),
     "lbc_max_speed": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_macro_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_vp, c_vp, c_vp], c_int),
-    "lbc_scalar_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_vp, c_vp], c_int),
+    "lbc_scalar_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_double, c_vp, c_vp], c_int),
     "lbc_reactive_gradient_workspace": ([c_int, c_u64, c_u64, c_int, c_int, c_u32, c_u32,
                                          ctypes.POINTER(ctypes.c_uint64)], c_int),
     "lbc_reactive_gradient": ([c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_u64, c_u64, c_int, c_int, c_double, c_double,
@@ -46,6 +48,7 @@ SIGNATURES = {
                              c_vp], c_int),
     "lbc_ensemble_set_block": ([c_vp, c_u32], c_int),
     "lbc_ensemble_set_flags": ([c_vp, c_u32], c_int),
+    "lbc_ensemble_set_references": ([c_vp, c_vp, c_vp], c_int),
     "lbc_ensemble_step": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_int),
     "lbc_ensemble_run": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp], c_int),
     "lbc_ensemble_autotune": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, _u32p, ctypes.POINTER(ctypes.c_float),

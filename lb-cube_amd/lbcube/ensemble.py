@@ -21,7 +21,8 @@ _STORE = {"f32": 0, "f64": 1, "f16": 2, "bf16": 3}
 
 
 class Ensemble:
-    def __init__(self, collision, storage, sizes, omega, omega_c, k_react, device="cuda", fast=False):
+    def __init__(self, collision, storage, sizes, omega, omega_c, k_react, device="cuda", fast=False, ref_a=None,
+                 ref_b=None):
         self.collision, self.storage = collision, storage
         self.sizes = [(int(nx), int(ny)) for nx, ny in sizes]
         n = len(self.sizes)
@@ -46,11 +47,21 @@ class Ensemble:
         self.level = 0
         self.cells = sum(x * y for x, y in self.sizes)
         self.set_fast(fast)
+        self.set_references(ref_a, ref_b)
 
     def set_fast(self, fast):
         """Deviation-form arithmetic (LBC_FLAG_FAST) for every instance."""
         check(lib().lbc_ensemble_set_flags(self.handle, 1 if fast else 0), "lbc_ensemble_set_flags")
         self.fast = bool(fast)
+
+    def set_references(self, ref_a=None, ref_b=None):
+        """Per-instance reference concentrations of A and B (sequences of the
+        instance count, or None for 0): f16/bf16 storage holds g - w ref."""
+        n = len(self.sizes)
+        self.ref_a = [0.0] * n if ref_a is None else [float(x) for x in ref_a]
+        self.ref_b = [0.0] * n if ref_b is None else [float(x) for x in ref_b]
+        ra, rb = (ctypes.c_double * n)(*self.ref_a), (ctypes.c_double * n)(*self.ref_b)
+        check(lib().lbc_ensemble_set_references(self.handle, ra, rb), "lbc_ensemble_set_references")
 
     def _planes(self, buf, off, q, i):
         nx, ny = self.sizes[i]

@@ -36,8 +36,9 @@ def equilibrium(lattice, rho, u, storage="f32"):
     return encode(lattice, out, storage)
 
 
-def scalar_equilibrium(lattice, conc, u, storage="f32"):
-    """g_i = w_i C (1 + c.u / cs2), cs2 = 1/3 (D2Q5) or 1/4 (D3Q7)."""
+def scalar_equilibrium(lattice, conc, u, storage="f32", ref=0.0):
+    """g_i = w_i C (1 + c.u / cs2), cs2 = 1/3 (D2Q5) or 1/4 (D3Q7); f16/bf16
+    storage holds g_i - w_i ref (ref: the field's reference concentration)."""
     device = u.device
     c = _velocities(lattice, device)
     w = torch.tensor(WEIGHTS[lattice], dtype=torch.float64, device=device)
@@ -46,7 +47,7 @@ def scalar_equilibrium(lattice, conc, u, storage="f32"):
     for i in range(Q[lattice]):
         cu = sum(c[i, d] * u[d] for d in range(DIM[lattice]))
         out[i] = w[i] * conc * (1.0 + cu / cs2)
-    return encode(lattice, out, storage)
+    return encode(lattice, out, storage, ref)
 
 
 def solenoidal_noise_3d(n, rms, seed, kmin=1, kmax=8, device="cuda"):

@@ -795,9 +795,15 @@ void fluid_step(const void* curv, void* nxtv, Grid g, double omega_d, int thread
 // (lattice_core.hpp:357-474), one cell at a time; `reaction` selects the
 // fused A+B->C form.  The fluid velocity comes from the fluid buffer at the
 // destination cell, unshifted.
+// Storage offset of scalar population i of a field with reference
+// concentration ref: F16/BF16 hold g_i - w_i ref (formed in the arithmetic
+// type, as the kernels form it); FP32/FP64 codecs ignore it.
+template <class S, class R> R scalar_offset(int i, R ref) { return static_cast<R>(S::w[i]) * ref; }
+
 template <class F, class S, int DT>
 void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void* nxt_bv, const void* cur_fv, Grid g,
-                      double omega_cd, double param, bool reaction, int threads) {
+                      double omega_cd, double param, bool reaction, int threads, double ref_a = 0.0,
+                      double ref_b = 0.0) {
     using C = Codec<DT>;
     using St = typename C::S;
     using R = typename C::R;
@@ -808,6 +814,7 @@ void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void
     const St* cf = static_cast<const St*>(cur_fv);
     const R omega_c = static_cast<R>(omega_cd);
     const R prm = static_cast<R>(param);
+    const R ra = static_cast<R>(ref_a), rb = static_cast<R>(ref_b);
     const i64 N = g.cells();
     const bool off = (DT == T_F16 || DT == T_BF16);
     (void)threads;
@@ -826,8 +833,8 @@ void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void
                 const i64 sy = pull(y, S::c[i * S::D + 1], g.ny);
                 const i64 sz = S::D == 3 ? pull(z, S::c[i * S::D + 2], g.nz) : 0;
                 const i64 src = i * N + (sz * g.ny + sy) * g.nx + sx;
-                ga[i] = C::load(ca[src], R{0});
-                if (reaction) gb[i] = C::load(cb[src], R{0});
+                ga[i] = C::load(ca[src], scalar_offset<S, R>(i, ra));
+                if (reaction) gb[i] = C::load(cb[src], scalar_offset<S, R>(i, rb));
             }
             if (reaction) {
                 const R conc_a = concentration<S, R>(ga);
@@ -839,8 +846,8 @@ void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void
                 collide_scalar<S, R>(ga, m.u, omega_c, prm);
             }
             for (int i = 0; i < S::Q; ++i) {
-                na[i * N + cell] = C::store(ga[i], R{0});
-                if (reaction) nb[i * N + cell] = C::store(gb[i], R{0});
+                na[i * N + cell] = C::store(ga[i], scalar_offset<S, R>(i, ra));
+                if (reaction) nb[i * N + cell] = C::store(gb[i], scalar_offset<S, R>(i, rb));
             }
         }
     }
@@ -849,10 +856,12 @@ void scalar_like_step(const void* cur_av, void* nxt_av, const void* cur_bv, void
 // compute_reactive_stats, lattice_io.hpp:256-338 (accumulation in Real as the
 // reference does) plus a long double accumulation used as the brute-force sum.
 template <class S, int DT>
-void reactive_stats(const void* av, const void* bv, Grid g, double k_d, double* out, bool wide) {
+void reactive_stats(const void* av, const void* bv, Grid g, double k_d, double* out, bool wide, double ref_a = 0.0,
+                    double ref_b = 0.0) {
     using C = Codec<DT>;
     using St = typename C::S;
     using R = typename C::R;
+    const R ra = static_cast<R>(ref_a), rb = static_cast<R>(ref_b);
     const St* a = static_cast<const St*>(av);
     const St* b = static_cast<const St*>(bv);
     const i64 N = g.cells();
@@ -860,8 +869,8 @@ void reactive_stats(const void* av, const void* bv, Grid g, double k_d, double* 
         R sa{}, sb{}, saa{}, sbb{}, sab{};
         for (i64 cell = 0; cell < N; ++cell) {
             R ca{}, cb{};
-            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], R{0});
-            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], R{0});
+            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], scalar_offset<S, R>(i, ra));
+            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], scalar_offset<S, R>(i, rb));
             sa += ca; sb += cb; saa += ca * ca; sbb += cb * cb; sab += ca * cb;
         }
         const R k = static_cast<R>(k_d);
@@ -877,8 +886,8 @@ void reactive_stats(const void* av, const void* bv, Grid g, double k_d, double* 
         long double sa = 0, sb = 0, saa = 0, sbb = 0, sab = 0;
         for (i64 cell = 0; cell < N; ++cell) {
             R ca{}, cb{};
-            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], R{0});
-            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], R{0});
+            for (int i = 0; i < S::Q; ++i) ca += C::load(a[i * N + cell], scalar_offset<S, R>(i, ra));
+            for (int i = 0; i < S::Q; ++i) cb += C::load(b[i * N + cell], scalar_offset<S, R>(i, rb));
             const long double la = ca, lb = cb;
             sa += la; sb += lb; saa += la * la; sbb += lb * lb; sab += la * lb;
         }
@@ -943,15 +952,16 @@ template <class L, int DT> void macro_fields(const void* fv, Grid g, double* rho
     }
 }
 
-template <class S, int DT> void scalar_fields(const void* gv, Grid g, double* conc) {
+template <class S, int DT> void scalar_fields(const void* gv, Grid g, double* conc, double ref) {
     using C = Codec<DT>;
     using St = typename C::S;
     using R = typename C::R;
+    const R r = static_cast<R>(ref);
     const St* p = static_cast<const St*>(gv);
     const i64 N = g.cells();
     for (i64 cell = 0; cell < N; ++cell) {
         R loc[S::Q];
-        for (int i = 0; i < S::Q; ++i) loc[i] = C::load(p[i * N + cell], R{0});
+        for (int i = 0; i < S::Q; ++i) loc[i] = C::load(p[i * N + cell], scalar_offset<S, R>(i, r));
         conc[cell] = static_cast<double>(concentration<S, R>(loc));
     }
 }
@@ -995,16 +1005,17 @@ template <class L, int DT> void feq_field(const double* rho, const double* u, vo
             out[i * N + cell] = C::store(feq<L, R>(i, m), off ? static_cast<R>(L::w[i]) : R{0});
     }
 }
-template <class S, int DT> void scalar_feq_field(const double* conc, const double* u, void* outv, i64 N) {
+template <class S, int DT> void scalar_feq_field(const double* conc, const double* u, void* outv, i64 N, double ref) {
     using C = Codec<DT>;
     using St = typename C::S;
     using R = typename C::R;
+    const R r = static_cast<R>(ref);
     St* out = static_cast<St*>(outv);
     for (i64 cell = 0; cell < N; ++cell) {
         R uu[3];
         for (int d = 0; d < S::D; ++d) uu[d] = static_cast<R>(u[d * N + cell]);
         for (int i = 0; i < S::Q; ++i)
-            out[i * N + cell] = C::store(scalar_feq<S, R>(i, static_cast<R>(conc[cell]), uu), R{0});
+            out[i * N + cell] = C::store(scalar_feq<S, R>(i, static_cast<R>(conc[cell]), uu), scalar_offset<S, R>(i, r));
     }
 }
 
@@ -1304,14 +1315,17 @@ int orc_fluid_run(int lat, int ct, int dt, void* buf, void* tmp, long long nx, l
 
 static int scalar_dispatch(int flat, int slat, int dt, const void* ca, void* na, const void* cb, void* nb,
                            const void* cf, long long nx, long long ny, long long nz, double omega_c, double param,
-                           bool reaction, int threads) {
+                           bool reaction, int threads, double ra, double rb) {
     const Grid g = make_grid(flat, nx, ny, nz);
     if (flat == L_D2Q9 && slat == L_D2Q5) {
-        ORC_DT_SWITCH(dt, (scalar_like_step<D2Q9, D2Q5, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+        ORC_DT_SWITCH(dt, (scalar_like_step<D2Q9, D2Q5, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads,
+                                                            ra, rb)));
     } else if (flat == L_D3Q19 && slat == L_D3Q7) {
-        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q19, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q19, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads,
+                                                             ra, rb)));
     } else if (flat == L_D3Q27 && slat == L_D3Q7) {
-        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q27, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads)));
+        ORC_DT_SWITCH(dt, (scalar_like_step<D3Q27, D3Q7, DT>(ca, na, cb, nb, cf, g, omega_c, param, reaction, threads,
+                                                             ra, rb)));
     } else {
         return -1;
     }
@@ -1319,16 +1333,16 @@ static int scalar_dispatch(int flat, int slat, int dt, const void* ca, void* na,
 }
 
 int orc_scalar_step(int flat, int slat, int dt, const void* cur_s, void* nxt_s, const void* cur_f, long long nx,
-                    long long ny, long long nz, double omega_c, double source, int threads) {
+                    long long ny, long long nz, double omega_c, double source, int threads, double ref) {
     return scalar_dispatch(flat, slat, dt, cur_s, nxt_s, nullptr, nullptr, cur_f, nx, ny, nz, omega_c, source, false,
-                           threads);
+                           threads, ref, 0.0);
 }
 
 int orc_reaction_step(int flat, int slat, int dt, const void* cur_a, void* nxt_a, const void* cur_b, void* nxt_b,
                       const void* cur_f, long long nx, long long ny, long long nz, double omega_c, double k_react,
-                      int threads) {
+                      int threads, double ref_a, double ref_b) {
     return scalar_dispatch(flat, slat, dt, cur_a, nxt_a, cur_b, nxt_b, cur_f, nx, ny, nz, omega_c, k_react, true,
-                           threads);
+                           threads, ref_a, ref_b);
 }
 
 // The reactive loop of src/simulate_reaction.cpp:548-554: a fluid step, then
@@ -1336,7 +1350,7 @@ int orc_reaction_step(int flat, int slat, int dt, const void* cur_a, void* nxt_a
 // copied back into f, a and b.
 int orc_reactive_run(int flat, int ct, int slat, int dt, void* f, void* ft, void* a, void* at, void* b, void* bt,
                      long long nx, long long ny, long long nz, double omega, double omega_c, double k_react,
-                     int steps, int threads) {
+                     int steps, int threads, double ref_a, double ref_b) {
     const Grid g = make_grid(flat, nx, ny, nz);
     const int qf = flat == L_D2Q9 ? 9 : (flat == L_D3Q19 ? 19 : 27);
     const int qs = slat == L_D2Q5 ? 5 : 7;
@@ -1346,7 +1360,7 @@ int orc_reactive_run(int flat, int ct, int slat, int dt, void* f, void* ft, void
         int rc = orc_fluid_step(flat, ct, dt, f0, f1, nx, ny, nz, omega, threads);
         if (rc) return rc;
         std::swap(f0, f1);
-        rc = orc_reaction_step(flat, slat, dt, a0, a1, b0, b1, f0, nx, ny, nz, omega_c, k_react, threads);
+        rc = orc_reaction_step(flat, slat, dt, a0, a1, b0, b1, f0, nx, ny, nz, omega_c, k_react, threads, ref_a, ref_b);
         if (rc) return rc;
         std::swap(a0, a1);
         std::swap(b0, b1);
@@ -1405,12 +1419,12 @@ int orc_reaction_source(int dt, const double* a, const double* b, double k, doub
 }
 
 int orc_reactive_stats(int slat, int dt, const void* a, const void* b, long long nx, long long ny, long long nz,
-                       double k_react, int wide, double* out) {
+                       double k_react, int wide, double* out, double ref_a, double ref_b) {
     const Grid g = make_grid(slat, nx, ny, nz);
     if (slat == L_D2Q5) {
-        ORC_DT_SWITCH(dt, (reactive_stats<D2Q5, DT>(a, b, g, k_react, out, wide != 0)));
+        ORC_DT_SWITCH(dt, (reactive_stats<D2Q5, DT>(a, b, g, k_react, out, wide != 0, ref_a, ref_b)));
     } else if (slat == L_D3Q7) {
-        ORC_DT_SWITCH(dt, (reactive_stats<D3Q7, DT>(a, b, g, k_react, out, wide != 0)));
+        ORC_DT_SWITCH(dt, (reactive_stats<D3Q7, DT>(a, b, g, k_react, out, wide != 0, ref_a, ref_b)));
     } else {
         return -1;
     }
@@ -1431,12 +1445,13 @@ int orc_macro_fields(int lat, int dt, const void* f, long long nx, long long ny,
     return 0;
 }
 
-int orc_scalar_fields(int slat, int dt, const void* p, long long nx, long long ny, long long nz, double* conc) {
+int orc_scalar_fields(int slat, int dt, const void* p, long long nx, long long ny, long long nz, double* conc,
+                      double ref) {
     const Grid g = make_grid(slat, nx, ny, nz);
     if (slat == L_D2Q5) {
-        ORC_DT_SWITCH(dt, (scalar_fields<D2Q5, DT>(p, g, conc)));
+        ORC_DT_SWITCH(dt, (scalar_fields<D2Q5, DT>(p, g, conc, ref)));
     } else if (slat == L_D3Q7) {
-        ORC_DT_SWITCH(dt, (scalar_fields<D3Q7, DT>(p, g, conc)));
+        ORC_DT_SWITCH(dt, (scalar_fields<D3Q7, DT>(p, g, conc, ref)));
     } else {
         return -1;
     }
@@ -1454,11 +1469,12 @@ int orc_feq_field(int lat, int dt, const double* rho, const double* u, void* out
     return 0;
 }
 
-int orc_scalar_feq_field(int slat, int dt, const double* conc, const double* u, void* out, long long ncells) {
+int orc_scalar_feq_field(int slat, int dt, const double* conc, const double* u, void* out, long long ncells,
+                         double ref) {
     if (slat == L_D2Q5) {
-        ORC_DT_SWITCH(dt, (scalar_feq_field<D2Q5, DT>(conc, u, out, ncells)));
+        ORC_DT_SWITCH(dt, (scalar_feq_field<D2Q5, DT>(conc, u, out, ncells, ref)));
     } else if (slat == L_D3Q7) {
-        ORC_DT_SWITCH(dt, (scalar_feq_field<D3Q7, DT>(conc, u, out, ncells)));
+        ORC_DT_SWITCH(dt, (scalar_feq_field<D3Q7, DT>(conc, u, out, ncells, ref)));
     } else {
         return -1;
     }

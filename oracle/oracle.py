@@ -48,21 +48,21 @@ def lib() -> ctypes.CDLL:
         sig = {
             "orc_fluid_step": [i, i, i, vp, vp, ll, ll, ll, d, i],
             "orc_fluid_run": [i, i, i, vp, vp, ll, ll, ll, d, i, i],
-            "orc_scalar_step": [i, i, i, vp, vp, vp, ll, ll, ll, d, d, i],
-            "orc_reaction_step": [i, i, i, vp, vp, vp, vp, vp, ll, ll, ll, d, d, i],
-            "orc_reactive_run": [i, i, i, i, vp, vp, vp, vp, vp, vp, ll, ll, ll, d, d, d, i, i],
+            "orc_scalar_step": [i, i, i, vp, vp, vp, ll, ll, ll, d, d, i, d],
+            "orc_reaction_step": [i, i, i, vp, vp, vp, vp, vp, ll, ll, ll, d, d, i, d, d],
+            "orc_reactive_run": [i, i, i, i, vp, vp, vp, vp, vp, vp, ll, ll, ll, d, d, d, i, i, d, d],
             "orc_collide_cell": [i, i, i, vp, d],
             "orc_macro_feq": [i, vp, vp, vp],
             "orc_mrt_forward": [i, vp, vp],
             "orc_mrt_inverse": [i, vp, vp],
             "orc_reaction_source": [i, vp, vp, d, vp, ll],
-            "orc_reactive_stats": [i, i, vp, vp, ll, ll, ll, d, i, dp],
+            "orc_reactive_stats": [i, i, vp, vp, ll, ll, ll, d, i, dp, d, d],
             "orc_fluid_diagnostics": [i, i, vp, ll, ll, ll, i, dp],
             "orc_macro_fields": [i, i, vp, ll, ll, ll, vp, vp],
-            "orc_scalar_fields": [i, i, vp, ll, ll, ll, vp],
+            "orc_scalar_fields": [i, i, vp, ll, ll, ll, vp, d],
             "orc_max_speed": [i, i, vp, ll, ll, ll, dp],
             "orc_feq_field": [i, i, vp, vp, vp, ll],
-            "orc_scalar_feq_field": [i, i, vp, vp, vp, ll],
+            "orc_scalar_feq_field": [i, i, vp, vp, vp, ll, d],
             "orc_f16_roundtrip": [vp, vp, vp, ll, i],
             "orc_reactive_gradient": [i, i, vp, vp, vp, ll, ll, d, d, d, i, dp, vp, vp, vp, dp],
             "orc_reaction_grad": [vp, vp, d, vp, vp, ll],
@@ -115,36 +115,39 @@ def fluid_run(lattice, collision, dtype, state, omega, steps, threads=1):
     return buf
 
 
-def scalar_step(fluid_lattice, scalar_lattice, dtype, cur_s, cur_f, omega_c, source=0.0, threads=1):
-    """step_scalar_cpu (lattice_core.hpp:238-337); returns the next scalar state."""
+def scalar_step(fluid_lattice, scalar_lattice, dtype, cur_s, cur_f, omega_c, source=0.0, threads=1, ref=0.0):
+    """step_scalar_cpu (lattice_core.hpp:238-337); returns the next scalar state.
+    ref: the field's reference concentration (f16/bf16 storage holds g - w ref)."""
     nxt = np.empty_like(cur_s)
     nx, ny, nz = _dims(fluid_lattice, cur_s.shape)
     _check(lib().orc_scalar_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], DTYPES[dtype], _p(cur_s),
-                                 _p(nxt), _p(cur_f), nx, ny, nz, float(omega_c), float(source), threads),
+                                 _p(nxt), _p(cur_f), nx, ny, nz, float(omega_c), float(source), threads,
+                                 float(ref)),
            "scalar_step")
     return nxt
 
 
-def reaction_step(fluid_lattice, scalar_lattice, dtype, cur_a, cur_b, cur_f, omega_c, k_react, threads=1):
+def reaction_step(fluid_lattice, scalar_lattice, dtype, cur_a, cur_b, cur_f, omega_c, k_react, threads=1, ref_a=0.0,
+                  ref_b=0.0):
     """step_reaction_AB (lattice_core.hpp:357-474); returns (next_a, next_b)."""
     na, nb = np.empty_like(cur_a), np.empty_like(cur_b)
     nx, ny, nz = _dims(fluid_lattice, cur_a.shape)
     _check(lib().orc_reaction_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], DTYPES[dtype], _p(cur_a),
                                    _p(na), _p(cur_b), _p(nb), _p(cur_f), nx, ny, nz, float(omega_c),
-                                   float(k_react), threads), "reaction_step")
+                                   float(k_react), threads, float(ref_a), float(ref_b)), "reaction_step")
     return na, nb
 
 
 def reactive_run(fluid_lattice, collision, scalar_lattice, dtype, f, a, b, omega, omega_c, k_react, steps,
-                 threads=1):
+                 threads=1, ref_a=0.0, ref_b=0.0):
     """Fluid step then reaction step, `steps` times (src/simulate_reaction.cpp:548-554)."""
     f, a, b = (np.array(x, copy=True) for x in (f, a, b))
     ft, at, bt = np.empty_like(f), np.empty_like(a), np.empty_like(b)
     nx, ny, nz = _dims(fluid_lattice, f.shape)
     _check(lib().orc_reactive_run(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
                                   DTYPES[dtype], _p(f), _p(ft), _p(a), _p(at), _p(b), _p(bt), nx, ny, nz,
-                                  float(omega), float(omega_c), float(k_react), int(steps), threads),
-           "reactive_run")
+                                  float(omega), float(omega_c), float(k_react), int(steps), threads, float(ref_a),
+                                  float(ref_b)), "reactive_run")
     return f, a, b
 
 
@@ -191,12 +194,13 @@ STAT_NAMES = ("mean_A", "mean_B", "var_A", "var_B", "covariance", "segregation_i
               "true_reaction_rate", "mixed_reaction_rate")
 
 
-def reactive_stats(scalar_lattice, dtype, a, b, k_react, wide=False):
+def reactive_stats(scalar_lattice, dtype, a, b, k_react, wide=False, ref_a=0.0, ref_b=0.0):
     """compute_reactive_stats (lattice_io.hpp:256-338); wide=True sums in long double."""
     out = (ctypes.c_double * 8)()
     nx, ny, nz = _dims(scalar_lattice, a.shape)
     _check(lib().orc_reactive_stats(LATTICES[scalar_lattice], DTYPES[dtype], _p(a), _p(b), nx, ny, nz,
-                                    float(k_react), int(wide), out), "reactive_stats")
+                                    float(k_react), int(wide), out, float(ref_a),
+                                    float(ref_b)), "reactive_stats")
     return np.array(out[:])
 
 
@@ -232,11 +236,12 @@ def macro_fields(lattice, dtype, f):
     return rho, u
 
 
-def scalar_fields(scalar_lattice, dtype, g):
+def scalar_fields(scalar_lattice, dtype, g, ref=0.0):
     """Per-cell concentration (compute_concentration), as float64."""
     nx, ny, nz = _dims(scalar_lattice, g.shape)
     conc = np.empty(g.shape[1:], dtype=np.float64)
-    _check(lib().orc_scalar_fields(LATTICES[scalar_lattice], DTYPES[dtype], _p(g), nx, ny, nz, _p(conc)),
+    _check(lib().orc_scalar_fields(LATTICES[scalar_lattice], DTYPES[dtype], _p(g), nx, ny, nz, _p(conc),
+                                   float(ref)),
            "scalar_fields")
     return conc
 
@@ -258,12 +263,12 @@ def feq_field(lattice, dtype, rho, u):
     return out
 
 
-def scalar_feq_field(scalar_lattice, dtype, conc, u):
+def scalar_feq_field(scalar_lattice, dtype, conc, u, ref=0.0):
     conc = np.ascontiguousarray(conc, dtype=np.float64)
     u = np.ascontiguousarray(u, dtype=np.float64)
     out = np.empty((Q_OF[scalar_lattice],) + conc.shape, dtype=NP_STORAGE[dtype])
     _check(lib().orc_scalar_feq_field(LATTICES[scalar_lattice], DTYPES[dtype], _p(conc), _p(u), _p(out),
-                                      conc.size), "scalar_feq_field")
+                                      conc.size, float(ref)), "scalar_feq_field")
     return out
 
 
@@ -275,8 +280,10 @@ def half_roundtrip(x, bf16=False):
     return h, back
 
 
-def decode(lattice, dtype, pops):
-    """Populations as float64 from a storage array (adds back w_i for f16/bf16 fluid storage)."""
+def decode(lattice, dtype, pops, ref=0.0):
+    """Populations as float64 from a storage array: adds back w_i for f16/bf16
+    fluid storage, w_i * ref for f16/bf16 scalar storage (ref: the field's
+    reference concentration; the product formed in float32 as the kernels do)."""
     if dtype == "f80":
         return pops
     if dtype in ("f32", "f64"):
@@ -285,9 +292,9 @@ def decode(lattice, dtype, pops):
         dev = pops.view(np.float16).astype(np.float32)
     else:
         dev = (pops.astype(np.uint32) << 16).view(np.float32)
-    if lattice in ("D2Q5", "D3Q7"):
-        return dev.astype(np.float64)
     w = np.asarray(WEIGHTS[lattice], dtype=np.float32).reshape((-1,) + (1,) * (pops.ndim - 1))
+    if lattice in ("D2Q5", "D3Q7"):
+        w = (w * np.float32(ref)).astype(np.float32)
     return (w + dev).astype(np.float64)
 
 

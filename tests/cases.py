@@ -26,8 +26,9 @@ def random_equilibrium(lattice, shape, dtype="f64", seed=0x5EED1234, du=0.01, dr
     return O.feq_field(lattice, dtype, rho, u)
 
 
-def random_perturbed(lattice, shape, dtype="f64", seed=7, eps=1e-3):
-    """Equilibrium plus a small random non-equilibrium part (exercises every moment)."""
+def random_perturbed(lattice, shape, dtype="f64", seed=7, eps=1e-3, ref=0.0):
+    """Equilibrium plus a small random non-equilibrium part (exercises every
+    moment); ref: a scalar field's reference concentration for f16/bf16."""
     f = random_equilibrium(lattice, shape, "f64", seed)
     g = rng(seed + 1)
     f = f + eps * g.uniform(-1, 1, size=f.shape) * np.asarray(O.WEIGHTS[lattice]).reshape((-1,) + (1,) * len(shape))
@@ -35,15 +36,18 @@ def random_perturbed(lattice, shape, dtype="f64", seed=7, eps=1e-3):
         return np.ascontiguousarray(f)
     if dtype == "f32":
         return np.ascontiguousarray(f.astype(np.float32))
-    return encode(lattice, dtype, f)
+    return encode(lattice, dtype, f, ref)
 
 
-def encode(lattice, dtype, f):
-    """float64 populations -> storage array of the oracle's codecs."""
+def encode(lattice, dtype, f, ref=0.0):
+    """float64 populations -> storage array of the oracle's codecs: f16/bf16
+    hold f - w (fluid) or g - w ref (scalar, ref its reference concentration),
+    the offset formed in float32."""
     f32 = f.astype(np.float32)
-    if lattice not in ("D2Q5", "D3Q7"):
-        w = np.asarray(O.WEIGHTS[lattice], dtype=np.float32).reshape((-1,) + (1,) * (f.ndim - 1))
-        f32 = (f32 - w).astype(np.float32)
+    w = np.asarray(O.WEIGHTS[lattice], dtype=np.float32).reshape((-1,) + (1,) * (f.ndim - 1))
+    if lattice in ("D2Q5", "D3Q7"):
+        w = (w * np.float32(ref)).astype(np.float32)
+    f32 = (f32 - w).astype(np.float32)
     h, _ = O.half_roundtrip(f32.ravel(), bf16=(dtype == "bf16"))
     return h.reshape(f.shape)
 

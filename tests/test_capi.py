@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lbcube.lib().lbc_abi_version() == 1
+    assert lbcube.lib().lbc_abi_version() == 2
 
 
 def fluid(ptr_a, ptr_b, nx=8, ny=8, nz=8, block=(16, 8, 1), lattice=1, coll=0, storage=0):
@@ -76,18 +76,18 @@ def test_unknown_codes_rejected(bad):
 
 def test_validation_scalar_and_reaction():
     L = lbcube.lib()
-    assert L.lbc_scalar_step(0, 3, 0, FAKE_A, FAKE_B, None, 8, 8, 1, 1.0, 0.0, 16, 8, 1, None) == \
+    assert L.lbc_scalar_step(0, 3, 0, FAKE_A, FAKE_B, None, 8, 8, 1, 1.0, 0.0, 0.0, 16, 8, 1, None) == \
         HIP_INVALID_DEVICE_POINTER
-    assert L.lbc_scalar_step(0, 3, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 1, 1.0, 0.0, 0, 8, 1, None) == \
+    assert L.lbc_scalar_step(0, 3, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 1, 1.0, 0.0, 0.0, 0, 8, 1, None) == \
         HIP_INVALID_CONFIGURATION
-    assert L.lbc_scalar_step(1, 4, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 0, 1.0, 0.0, 16, 8, 1, None) == HIP_INVALID_VALUE
-    assert L.lbc_scalar_step(0, 4, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 1, 1.0, 0.0, 16, 8, 1, None) == HIP_INVALID_VALUE
-    assert L.lbc_reaction_ab_step(0, 3, 0, FAKE_A, FAKE_B, 0x3000, None, 0x5000, 8, 8, 1, 1.0, 0.1, 16, 8, 1,
+    assert L.lbc_scalar_step(1, 4, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 0, 1.0, 0.0, 0.0, 16, 8, 1, None) == HIP_INVALID_VALUE
+    assert L.lbc_scalar_step(0, 4, 0, FAKE_A, FAKE_B, 0x3000, 8, 8, 1, 1.0, 0.0, 0.0, 16, 8, 1, None) == HIP_INVALID_VALUE
+    assert L.lbc_reaction_ab_step(0, 3, 0, FAKE_A, FAKE_B, 0x3000, None, 0x5000, 8, 8, 1, 1.0, 0.1, 0.0, 0.0, 16, 8, 1,
                                   None) == HIP_INVALID_DEVICE_POINTER
-    assert L.lbc_reaction_ab_step(0, 3, 0, FAKE_A, FAKE_A, 0x3000, 0x4000, 0x5000, 8, 8, 1, 1.0, 0.1, 16, 8, 1,
+    assert L.lbc_reaction_ab_step(0, 3, 0, FAKE_A, FAKE_A, 0x3000, 0x4000, 0x5000, 8, 8, 1, 1.0, 0.1, 0.0, 0.0, 16, 8, 1,
                                   None) == HIP_INVALID_VALUE
     assert L.lbc_reactive_fused_step(0, 0, 3, 0, FAKE_A, FAKE_B, 0x3000, 0x4000, 0x5000, 0x6000, 8, 0, 1, 1.0, 1.0,
-                                     0.1, 16, 8, 1, None) == HIP_INVALID_VALUE
+                                     0.1, 0.0, 0.0, 16, 8, 1, None) == HIP_INVALID_VALUE
 
 
 def test_default_block():
@@ -105,5 +105,5 @@ def test_validation_fields_and_guard():
     assert L.lbc_macro_fields(1, 0, FAKE_A, 8, 8, 8, None, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
     assert L.lbc_macro_fields(1, 0, FAKE_A, 0, 8, 8, FAKE_B, 0x3000, None) == HIP_INVALID_VALUE
     assert L.lbc_macro_fields(3, 0, FAKE_A, 8, 8, 8, FAKE_B, 0x3000, None) == HIP_INVALID_VALUE
-    assert L.lbc_scalar_fields(3, 0, None, 8, 8, 1, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
-    assert L.lbc_scalar_fields(0, 0, FAKE_A, 8, 8, 1, FAKE_B, None) == HIP_INVALID_VALUE  # fluid lattice
+    assert L.lbc_scalar_fields(3, 0, None, 8, 8, 1, 0.0, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
+    assert L.lbc_scalar_fields(0, 0, FAKE_A, 8, 8, 1, 0.0, FAKE_B, None) == HIP_INVALID_VALUE  # fluid lattice

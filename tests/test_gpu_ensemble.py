@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 SIZES = [(64, 48), (100, 37), (130, 64), (17, 9), (256, 31), (70, 70), (300, 5)]
 
 
-def fill(ens, storage, seed):
+def fill(ens, storage, seed, refs=None):
     g = cases.rng(seed)
     for i, (nx, ny) in enumerate(ens.sizes):
         u = g.uniform(-0.03, 0.03, size=(2, ny, nx))
@@ -29,21 +29,23 @@ def fill(ens, storage, seed):
         b = O.scalar_feq_field("D2Q5", dt, 1.0 - ca, u)
         tf, ta, tb = (torch.from_numpy(x).to("cuda") for x in (f, a, b))
         if storage in ("f16", "bf16"):
+            ra, rb = refs[i] if refs else (0.0, 0.0)
             tf = lbcube.encode("D2Q9", tf, storage)
-            ta = lbcube.encode("D2Q5", ta, storage)
-            tb = lbcube.encode("D2Q5", tb, storage)
+            ta = lbcube.encode("D2Q5", ta, storage, ra)
+            tb = lbcube.encode("D2Q5", tb, storage, rb)
         ens.fluid(i).copy_(tf)
         sa, sb = ens.species(i)
         sa.copy_(ta)
         sb.copy_(tb)
 
 
-def standalone(ens, i, coll, omega, omega_c, k, steps, fast=False):
+def standalone(ens, i, coll, omega, omega_c, k, steps, fast=False, refs=(0.0, 0.0)):
     f = ens.fluid(i).clone()
     a, b = (x.clone() for x in ens.species(i))
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
     for _ in range(steps):
-        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
+        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast,
+                                   ref_a=refs[0], ref_b=refs[1])
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
     return f, a, b
 
@@ -106,3 +108,29 @@ def test_config5_generator():
     assert len(n) == 512 and n.min() >= 64 and n.max() <= 1024
     tau = 1 / omega
     assert np.all(tau > 0.5) and np.all(k > 0)
+
+
+@pytest.mark.parametrize("fast", [False, True])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_instances_with_species_references(gpu, storage, fast):
+    """Per-instance reference concentrations of the species storage: each
+    instance still advances bit for bit like the fused step on it alone with
+    the same references (graph replay included)."""
+    print(f"case ensemble references {storage} fast={fast}", flush=True)
+    n = len(SIZES)
+    omega = [1 / (0.55 + 0.05 * i) for i in range(n)]
+    omega_c = [1 / (0.7 + 0.03 * i) for i in range(n)]
+    k = [0.01 * (i + 1) for i in range(n)]
+    refs = [(0.5, 0.5 - 0.05 * i) for i in range(n)]
+    ens = Ensemble("TRT", storage, SIZES, omega, omega_c, k, fast=fast, ref_a=[r[0] for r in refs],
+                   ref_b=[r[1] for r in refs])
+    fill(ens, storage, seed=23, refs=refs)
+    want = [standalone(ens, i, "TRT", omega[i], omega_c[i], k[i], 6, fast=fast, refs=refs[i]) for i in range(n)]
+    ens.run(6, graph=True)
+    torch.cuda.synchronize()
+    for i in range(n):
+        sa, sb = ens.species(i)
+        for got, exp in ((ens.fluid(i), want[i][0]), (sa, want[i][1]), (sb, want[i][2])):
+            assert torch.equal(got.contiguous().view(torch.uint8), exp.contiguous().view(torch.uint8)), i
+    ens.close()
+

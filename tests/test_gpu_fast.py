@@ -249,17 +249,19 @@ FLUID_TOL = {"f64": 1e-14, "f32": 2e-6, "f16": 4e-5, "bf16": 5e-4}
 REACT_FAST_TOL = {"f64": 1e-13, "f32": 5e-6, "f16": 2e-3, "bf16": 1.6e-2}
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage,refs", [("f64", (0.0, 0.0)), ("f32", (0.0, 0.0)), ("f16", (0.0, 0.0)),
+                                          ("bf16", (0.0, 0.0)), ("f16", (0.5, 0.5)), ("bf16", (0.5, 0.3))])
 @pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
-def test_reactive_fast_against_oracle(gpu, pair, coll, storage):
+def test_reactive_fast_against_oracle(gpu, pair, coll, storage, refs):
     from test_gpu_parity import _species
 
     fl, sl = pair
-    print(f"case reactive fast {fl} {sl} {coll} {storage}", flush=True)
+    print(f"case reactive fast {fl} {sl} {coll} {storage} refs {refs}", flush=True)
     shape = SHAPE[fl]
+    ra, rb = refs
     f0 = cases.random_perturbed(fl, shape, storage, seed=2)
-    a0, b0 = _species(sl, shape, storage, seed=8)
+    a0, b0 = _species(sl, shape, storage, seed=8, refs=refs)
     omega, omega_c, k, steps = 1.3, 1.05, 0.02, 4
     if storage in ("f64", "f32"):
         truth = O.reactive_run(fl, coll, sl, "f80", *(x.astype(np.longdouble) for x in (f0, a0, b0)), omega,
@@ -267,20 +269,21 @@ def test_reactive_fast_against_oracle(gpu, pair, coll, storage):
         ref = O.reactive_run(fl, coll, sl, storage, f0, a0, b0, omega, omega_c, k, steps)
         want = [x.astype(np.float64) for x in truth]
     else:
-        ref = O.reactive_run(fl, coll, sl, storage, f0, a0, b0, omega, omega_c, k, steps)
-        want = [O.decode(fl, storage, ref[0]), O.decode(sl, storage, ref[1]), O.decode(sl, storage, ref[2])]
+        ref = O.reactive_run(fl, coll, sl, storage, f0, a0, b0, omega, omega_c, k, steps, ref_a=ra, ref_b=rb)
+        want = [O.decode(fl, storage, ref[0]), O.decode(sl, storage, ref[1], ra), O.decode(sl, storage, ref[2], rb)]
     f, a, b = (to_dev(x, storage, gpu) for x in (f0, a0, b0))
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
     for _ in range(steps):
-        lbcube.reactive_fused_step(fl, coll, sl, f, ft, a, at, b, bt, omega, omega_c, k, fast=True)
+        lbcube.reactive_fused_step(fl, coll, sl, f, ft, a, at, b, bt, omega, omega_c, k, fast=True, ref_a=ra,
+                                   ref_b=rb)
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
     torch.cuda.synchronize()
     gf = O.decode(fl, storage, to_host(f, storage))
     assert np.max(np.abs(gf - want[0])) <= FLUID_TOL[storage]
-    for got, w, r in ((a, want[1], ref[1]), (b, want[2], ref[2])):
-        g = O.decode(sl, storage, to_host(got, storage))
+    for got, w, r, c in ((a, want[1], ref[1], ra), (b, want[2], ref[2], rb)):
+        g = O.decode(sl, storage, to_host(got, storage), c)
         err = np.max(np.abs(g - w) / np.maximum(np.abs(w), 1e-3))
-        err_ref = np.max(np.abs(O.decode(sl, storage, r) - w) / np.maximum(np.abs(w), 1e-3))
+        err_ref = np.max(np.abs(O.decode(sl, storage, r, c) - w) / np.maximum(np.abs(w), 1e-3))
         print(f"species error: fast {err:.2e}, reference order {err_ref:.2e}", flush=True)
         assert err <= REACT_FAST_TOL[storage], err
 
@@ -342,11 +345,11 @@ def test_reactive_fast_rejects_unknown_flags(gpu):
     f = torch.zeros((9, 4, 4), dtype=torch.float32, device=gpu)
     a = torch.zeros((5, 4, 4), dtype=torch.float32, device=gpu)
     args = [f.data_ptr(), torch.empty_like(f).data_ptr(), a.data_ptr(), torch.empty_like(a).data_ptr(),
-            a.clone().data_ptr(), torch.empty_like(a).data_ptr(), 4, 4, 1, 1.0, 1.0, 0.1, 256, 1, 1, None]
+            a.clone().data_ptr(), torch.empty_like(a).data_ptr(), 4, 4, 1, 1.0, 1.0, 0.1, 0.0, 0.0, 256, 1, 1, None]
     assert lbcube.lib().lbc_reactive_fused_step_ex(0, 0, 3, 0, 2, *args) == 1
 
 
-def reactive_mixing(n, storage, fast, steps, coll="TRT", sample=250, k_scale=1.0, perturbation=0.0):
+def reactive_mixing(n, storage, fast, steps, coll="TRT", sample=250, k_scale=1.0, perturbation=0.0, ref=0.0):
     """One reactive-mixing instance: 2D Taylor-Green (u0 = 0.04), species in
     tanh-smoothed half planes, Re = 1000, Sc = 1, Da = 10 (k_scale = 0: no
     reaction).  `perturbation`: relative random change of the initial A.
@@ -360,16 +363,17 @@ def reactive_mixing(n, storage, fast, steps, coll="TRT", sample=250, k_scale=1.0
         g = torch.Generator(device="cpu").manual_seed(5)
         ca = ca * (1 + perturbation * torch.randn(ca.shape, generator=g, dtype=torch.float64).to(ca.device))
     f = lbinit.equilibrium("D2Q9", rho, u, storage)
-    a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage)
-    b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage)
+    a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage, ref)
+    b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage, ref)
     ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
-    out = [(lbcube.decode("D2Q5", a).sum().item(), lbcube.decode("D2Q5", b).sum().item(),
+    out = [(lbcube.decode("D2Q5", a, ref).sum().item(), lbcube.decode("D2Q5", b, ref).sum().item(),
             lbcube.decode("D2Q9", f).sum().item())]
     for s in range(1, steps + 1):
-        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, 1 / tau, 1 / tau, k, fast=fast)
+        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, ft, a, at, b, bt, 1 / tau, 1 / tau, k, fast=fast,
+                                   ref_a=ref, ref_b=ref)
         f, ft, a, at, b, bt = ft, f, at, a, bt, b
         if s % sample == 0:
-            out.append((lbcube.decode("D2Q5", a).sum().item(), lbcube.decode("D2Q5", b).sum().item(),
+            out.append((lbcube.decode("D2Q5", a, ref).sum().item(), lbcube.decode("D2Q5", b, ref).sum().item(),
                         lbcube.decode("D2Q9", f).sum().item()))
     return np.array(out)
 
@@ -420,11 +424,17 @@ def test_reactive_mrt_fast_against_fp64(gpu):
     (see above)."""
     n, steps, sample = 512, 750, 250
     ref = reactive_mixing(n, "f64", False, steps, coll="MRT", sample=sample)
-    for storage, fast, tol in (("f32", True, 1e-5), ("f32", False, None), ("f16", True, 2e-3)):
-        got = reactive_mixing(n, storage, fast, steps, coll="MRT", sample=sample)
+    errs = {}
+    for storage, fast, c_ref, tol in (("f32", True, 0.0, 1e-5), ("f32", False, 0.0, None), ("f16", True, 0.0, 2e-3),
+                                      ("f16", True, 0.5, 2e-3), ("bf16", True, 0.0, None), ("bf16", True, 0.5, None)):
+        got = reactive_mixing(n, storage, fast, steps, coll="MRT", sample=sample, ref=c_ref)
         err = np.abs(got[:, 0] - ref[:, 0]) / ref[:, 0]
         mass = np.max(np.abs(got[:, 2] - got[0, 2])) / got[0, 2]
-        print(f"reactive MRT {n}^2 storage {storage} fast={fast}: relative error of total A per sample "
-              f"{np.array2string(err, precision=2)}, fluid mass drift {mass:.2e}", flush=True)
+        errs[(storage, c_ref)] = np.max(err)
+        print(f"reactive MRT {n}^2 storage {storage} fast={fast} species reference {c_ref}: relative error of "
+              f"total A per sample {np.array2string(err, precision=2)}, fluid mass drift {mass:.2e}", flush=True)
         if tol is not None:
             assert np.max(err) <= tol, (storage, fast, err)
+    # the species are 0 or 1 apart from the mixing layer: storing g - w/2
+    # instead of g halves the magnitude FP16 rounds
+    assert errs[("f16", 0.5)] < errs[("f16", 0.0)]

@@ -114,23 +114,25 @@ def test_config1_tgv_64cubed_1000_steps(gpu):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("ref", [0.0, 0.75])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
-def test_scalar_step_bit_exact(gpu, pair, storage):
+def test_scalar_step_bit_exact(gpu, pair, storage, ref):
+    """ref: the scalar field's reference concentration (f16/bf16 store g - w ref)."""
     fl, sl = pair
-    print(f"case scalar {fl} {sl} {storage}", flush=True)
+    print(f"case scalar {fl} {sl} {storage} ref {ref}", flush=True)
     shape = SHAPE[fl]
     f = cases.random_perturbed(fl, shape, storage, seed=5)
-    g = cases.random_perturbed(sl, shape, storage, seed=9, eps=0.05)
-    want = O.scalar_step(fl, sl, storage, g, f, 1.25, 3e-4)
+    g = cases.random_perturbed(sl, shape, storage, seed=9, eps=0.05, ref=ref)
+    want = O.scalar_step(fl, sl, storage, g, f, 1.25, 3e-4, ref=ref)
     gd, fd = to_dev(g, storage, gpu), to_dev(f, storage, gpu)
     out = torch.empty_like(gd)
-    lbcube.scalar_step(fl, sl, gd, out, fd, 1.25, 3e-4)
+    lbcube.scalar_step(fl, sl, gd, out, fd, 1.25, 3e-4, ref=ref)
     torch.cuda.synchronize()
     assert np.array_equal(to_host(out, storage), want)
 
 
-def _species(sl, shape, storage, seed):
+def _species(sl, shape, storage, seed, refs=(0.0, 0.0)):
     g = cases.rng(seed)
     d = O.D_OF[sl]
     ca = g.uniform(0.0, 1.0, size=shape)
@@ -140,7 +142,7 @@ def _species(sl, shape, storage, seed):
     fa = O.scalar_feq_field(sl, storage if storage in ("f32", "f64") else "f64", ca, u)
     fb = O.scalar_feq_field(sl, storage if storage in ("f32", "f64") else "f64", cb, u)
     if storage in ("f16", "bf16"):
-        fa, fb = cases.encode(sl, storage, fa), cases.encode(sl, storage, fb)
+        fa, fb = cases.encode(sl, storage, fa, refs[0]), cases.encode(sl, storage, fb, refs[1])
     return fa, fb
 
 
@@ -149,45 +151,48 @@ def _species(sl, shape, storage, seed):
 REACT_TOL = {"f64": 1e-13, "f32": 2e-6, "f16": 2e-3, "bf16": 1.6e-2}
 
 
+@pytest.mark.parametrize("refs", [(0.0, 0.0), (0.5, 0.3)])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
-def test_reaction_step_parity(gpu, pair, storage):
+def test_reaction_step_parity(gpu, pair, storage, refs):
     fl, sl = pair
-    print(f"case reaction {fl} {sl} {storage}", flush=True)
+    print(f"case reaction {fl} {sl} {storage} refs {refs}", flush=True)
     shape = SHAPE[fl]
     f = cases.random_perturbed(fl, shape, storage, seed=21)
-    a, b = _species(sl, shape, storage, seed=4)
+    a, b = _species(sl, shape, storage, seed=4, refs=refs)
     k = 0.05
-    wa, wb = O.reaction_step(fl, sl, storage, a, b, f, 1.1, k)
+    wa, wb = O.reaction_step(fl, sl, storage, a, b, f, 1.1, k, ref_a=refs[0], ref_b=refs[1])
     ad, bd, fd = to_dev(a, storage, gpu), to_dev(b, storage, gpu), to_dev(f, storage, gpu)
     na, nb = torch.empty_like(ad), torch.empty_like(bd)
-    lbcube.reaction_step(fl, sl, ad, na, bd, nb, fd, 1.1, k)
+    lbcube.reaction_step(fl, sl, ad, na, bd, nb, fd, 1.1, k, ref_a=refs[0], ref_b=refs[1])
     torch.cuda.synchronize()
     ga, gb = to_host(na, storage), to_host(nb, storage)
-    for got, want in ((ga, wa), (gb, wb)):
-        gd_, wd_ = O.decode(sl, storage, got), O.decode(sl, storage, want)
+    for got, want, r in ((ga, wa, refs[0]), (gb, wb, refs[1])):
+        gd_, wd_ = O.decode(sl, storage, got, r), O.decode(sl, storage, want, r)
         err = np.max(np.abs(gd_ - wd_) / np.maximum(np.abs(wd_), 1e-3))
         assert err <= REACT_TOL[storage], err
         # nearly everything is bit-identical; the exceptions are exp() ulps
         assert np.mean(got == want) > 0.5
 
 
+@pytest.mark.parametrize("refs", [(0.0, 0.0), (0.5, 0.25)])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("coll", COLLS)
-def test_fused_reactive_equals_two_launches(gpu, coll, storage):
-    print(f"case fused D2Q9 {coll} {storage}", flush=True)
+def test_fused_reactive_equals_two_launches(gpu, coll, storage, refs):
+    print(f"case fused D2Q9 {coll} {storage} refs {refs}", flush=True)
     shape = SHAPE["D2Q9"]
     f = to_dev(cases.random_perturbed("D2Q9", shape, storage, seed=2), storage, gpu)
-    a0, b0 = _species("D2Q5", shape, storage, seed=8)
+    a0, b0 = _species("D2Q5", shape, storage, seed=8, refs=refs)
     a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
     omega, omega_c, k = 1.3, 1.05, 0.02
+    ra, rb = refs
     # two launches, reference order
     f1, a1, b1 = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
     lbcube.fluid_step("D2Q9", coll, f, f1, omega)
-    lbcube.reaction_step("D2Q9", "D2Q5", a, a1, b, b1, f1, omega_c, k)
+    lbcube.reaction_step("D2Q9", "D2Q5", a, a1, b, b1, f1, omega_c, k, ref_a=ra, ref_b=rb)
     # one pass
     f2, a2, b2 = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
-    lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, f2, a, a2, b, b2, omega, omega_c, k)
+    lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", f, f2, a, a2, b, b2, omega, omega_c, k, ref_a=ra, ref_b=rb)
     torch.cuda.synchronize()
     for x, y in ((f1, f2), (a1, a2), (b1, b2)):
         assert torch.equal(x.view(torch.uint8), y.view(torch.uint8))
@@ -225,13 +230,15 @@ def test_fluid_diagnostics(gpu, lattice, storage):
     assert got[2] == want[2]  # max of correctly rounded sqrt: exact
 
 
+@pytest.mark.parametrize("refs", [(0.0, 0.0), (0.5, 0.4)])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("sl", ["D2Q5", "D3Q7"])
-def test_reactive_stats(gpu, sl, storage):
+def test_reactive_stats(gpu, sl, storage, refs):
     shape = SHAPE["D2Q9"] if sl == "D2Q5" else SHAPE["D3Q19"]
-    a, b = _species(sl, shape, storage, seed=17)
-    want = O.reactive_stats(sl, storage, a, b, 0.01, wide=True)
-    got = lbcube.reactive_stats(sl, to_dev(a, storage, gpu), to_dev(b, storage, gpu), 0.01)
+    a, b = _species(sl, shape, storage, seed=17, refs=refs)
+    want = O.reactive_stats(sl, storage, a, b, 0.01, wide=True, ref_a=refs[0], ref_b=refs[1])
+    got = lbcube.reactive_stats(sl, to_dev(a, storage, gpu), to_dev(b, storage, gpu), 0.01, ref_a=refs[0],
+                                ref_b=refs[1])
     got = np.array([got[k] for k in lbcube.STAT_NAMES])
     np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-14)
 
@@ -261,14 +268,15 @@ def test_macro_fields(gpu, lattice, storage):
     assert np.array_equal(u.cpu().numpy(), want_u)
 
 
+@pytest.mark.parametrize("ref", [0.0, 0.6])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("sl", ["D2Q5", "D3Q7"])
-def test_scalar_fields(gpu, sl, storage):
-    print(f"case scalar_fields {sl} {storage}", flush=True)
+def test_scalar_fields(gpu, sl, storage, ref):
+    print(f"case scalar_fields {sl} {storage} ref {ref}", flush=True)
     shape = SHAPE["D2Q9"] if sl == "D2Q5" else SHAPE["D3Q19"]
-    a, _ = _species(sl, shape, storage, seed=20)
-    got = lbcube.scalar_fields(sl, to_dev(a, storage, gpu))
-    assert np.array_equal(got.cpu().numpy(), O.scalar_fields(sl, storage, a))
+    a, _ = _species(sl, shape, storage, seed=20, refs=(ref, 0.0))
+    got = lbcube.scalar_fields(sl, to_dev(a, storage, gpu), ref=ref)
+    assert np.array_equal(got.cpu().numpy(), O.scalar_fields(sl, storage, a, ref))
 
 
 @pytest.mark.parametrize("storage", STORES)

@@ -307,3 +307,29 @@ def test_scalar_fields_recover_concentration(sl):
     u = g.uniform(-0.03, 0.03, size=(O.D_OF[sl],) + shape)
     p = O.scalar_feq_field(sl, "f64", c, u)
     assert np.allclose(O.scalar_fields(sl, "f64", p), c, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16"])
+def test_scalar_storage_reference_codec(dtype):
+    """F16/BF16 scalar storage holds g - w ref: a field at rest at its
+    reference concentration stores exact zeros and decodes to w ref (the
+    float32 product), and a field near ref round-trips more closely than with
+    no reference."""
+    shape = (6, 7)
+    u0 = np.zeros((2,) + shape)
+    ref = 0.5
+    rest = O.scalar_feq_field("D2Q5", dtype, np.full(shape, ref), u0, ref)
+    assert not np.any(rest)  # every stored word is +0
+    w32 = np.asarray(O.WEIGHTS["D2Q5"], dtype=np.float32).reshape(-1, 1, 1)
+    assert np.array_equal(O.decode("D2Q5", dtype, rest, ref), np.broadcast_to((w32 * np.float32(ref)).astype(
+        np.float64), rest.shape))
+    g = cases.rng(31)
+    conc = ref + g.uniform(-0.05, 0.05, size=shape)
+    u = g.uniform(-0.02, 0.02, size=(2,) + shape)
+    exact = O.scalar_feq_field("D2Q5", "f64", conc, u)
+    err = {r: np.max(np.abs(O.decode("D2Q5", dtype, O.scalar_feq_field("D2Q5", dtype, conc, u, r), r) - exact))
+           for r in (0.0, ref)}
+    assert err[ref] < err[0.0] / 4, err
+    # the test helper's encoding (offset formed in float32 too) decodes as closely
+    helper = cases.encode("D2Q5", dtype, exact, ref)
+    assert np.max(np.abs(O.decode("D2Q5", dtype, helper, ref) - exact)) <= 1.01 * err[ref] + 1e-9
