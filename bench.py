@@ -125,6 +125,26 @@ def _time(torch, fn, reps=1):
     return start.elapsed_time(end) / reps
 
 
+def _graph_ms(torch, step, steps, chunk=50):
+    """ms per call of `step` (a launch sequence advancing ping-pong buffers by
+    one step) replayed from a CUDA graph of `chunk` calls (chunk even, so the
+    buffers return to their roles): no host launch overhead in the timing."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(chunk):
+            step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(chunk):
+            step()
+    graph.replay()
+    ms = _time(torch, graph.replay, reps=max(1, steps // chunk)) / chunk
+    del graph
+    return ms
+
+
 def adjoint_ratio(torch, n=512, steps=4096, interval=64):
     """Differentiated 2D reactive run (D2Q9 BGK + two D2Q5, A+B->C), FP32 with
     the fast (deviation-form, cancellation-free reaction) forward:
@@ -222,7 +242,9 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     two launches per step (fluid step, then reaction step: 188 / 376 B per
     cell), and the whole loop with the reactive statistics every 50 steps; the
     fused step in the deviation form (FP32, on the same buffers), and both
-    forms with FP16 storage (76 B)."""
+    forms with FP16 storage (76 B).  Step rates are timed from CUDA-graph
+    replays of 50-step chunks (a 2048^2 step is ~0.12 ms, close to the host's
+    per-launch cost); the loop with statistics runs eagerly."""
     from lbcube import init as lbinit
 
     out = {}
@@ -255,9 +277,9 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
 
         for _ in range(4):
             fused()
-        ms = _time(torch, fused, reps=steps)
-        ms_unfused = _time(torch, unfused, reps=steps)
-        ms_loop = _time(torch, loop) / steps
+        ms = _graph_ms(torch, fused, steps)
+        ms_unfused = _graph_ms(torch, unfused, steps)
+        ms_loop = _time(torch, loop) / steps  # eager launches and the statistics' synchronisations
         stats = lbcube.reactive_stats("D2Q5", state[2], state[4], k)
         ms_stats = _time(torch, lambda: lbcube.reactive_stats("D2Q5", state[2], state[4], k), reps=5)
         cells = n * n
@@ -269,7 +291,7 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
                         "loop_with_stats_every_50_mlups": cells / (ms_loop * 1e-3) / 1e6,
                         "stats_ms": ms_stats, "segregation_intensity": stats["segregation_intensity"]}
         if storage == "f32":  # the deviation form (LBC_FLAG_FAST) on the same buffers
-            ms_fast = _time(torch, lambda: fused(fast=True), reps=steps)
+            ms_fast = _graph_ms(torch, lambda: fused(fast=True), steps)
             out["f32_fast"] = {"mlups": cells / (ms_fast * 1e-3) / 1e6,
                                "achieved_gbs": 38 * es * cells / (ms_fast * 1e-3) / 1e9, "bytes_per_cell": 38 * es}
         del f, ft, a, at, b, bt, state
@@ -290,7 +312,7 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     for fast in (False, True):
         for _ in range(4):
             fused16(fast)
-        ms = _time(torch, lambda: fused16(fast), reps=steps)
+        ms = _graph_ms(torch, lambda: fused16(fast), steps)
         out["f16_fast" if fast else "f16"] = {"mlups": n * n / (ms * 1e-3) / 1e6,
                                              "achieved_gbs": 38 * 2 * n * n / (ms * 1e-3) / 1e9, "bytes_per_cell": 76}
     del f, a, b, state
