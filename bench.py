@@ -357,6 +357,8 @@ def main():
                                           "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
+    ap.add_argument("--extras", default="adjoint,reactive_2d,ensemble,lattices",
+                    help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -387,18 +389,23 @@ def main():
     head = variants["trt_f32"]
     copy_gbs = copy_ceiling(lbcube, torch, 152 * n ** 3)
     extras = {}
-    if not args.no_extras:
+    chosen = set() if args.no_extras else set(args.extras.split(","))
+    if chosen:
         del u
         torch.cuda.empty_cache()
-        extras["adjoint"] = adjoint_ratio(torch)
-        print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
-        extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)
-        r2 = extras["reactive_2d"]
-        print(f"# reactive 2048^2 MRT f32: {r2['f32']['mlups']:.0f} MLUPS, f32 fast {r2['f32_fast']['mlups']:.0f}, "
-              f"f16 fast {r2['f16_fast']['mlups']:.0f}", file=sys.stderr, flush=True)
-        extras["ensemble"] = ensemble_rate(torch)
-        print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
-        extras["lattices"] = lattice_sweep(lbcube, torch)
+        if "adjoint" in chosen:
+            extras["adjoint"] = adjoint_ratio(torch)
+            print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
+        if "reactive_2d" in chosen:
+            extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)
+            r2 = extras["reactive_2d"]
+            print(f"# reactive 2048^2 MRT f32: {r2['f32']['mlups']:.0f} MLUPS, f32 fast {r2['f32_fast']['mlups']:.0f}, "
+                  f"f16 fast {r2['f16_fast']['mlups']:.0f}", file=sys.stderr, flush=True)
+        if "ensemble" in chosen:
+            extras["ensemble"] = ensemble_rate(torch)
+            print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        if "lattices" in chosen:
+            extras["lattices"] = lattice_sweep(lbcube, torch)
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
     cpu = None
     if not args.no_cpu_baseline:
@@ -432,7 +439,8 @@ def main():
     if "trt_f16fast" in variants:
         out["fp16_over_fp32"] = round(variants["trt_f16fast"]["mlups"] / head["mlups"], 3)
     if extras:
-        out["adjoint_over_forward"] = round(extras["adjoint"]["adjoint_over_forward"], 3)
+        if "adjoint" in extras:
+            out["adjoint_over_forward"] = round(extras["adjoint"]["adjoint_over_forward"], 3)
         out["configs"] = extras
     print(json.dumps(out))
 

@@ -28,7 +28,8 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
                                                           typename Storage<ST>::R omega_minus) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t dst = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t block = L::D == 2 ? xcd_block() : blockIdx.x;
+    const uint32_t dst = block * blockDim.x + threadIdx.x;
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
