@@ -104,8 +104,10 @@ inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 // one block more).  3D grids at 256^3 and above have whole rows per block and
 // no such lines, so only the 2D single-domain kernels use it (measured on the
 // 2048^2 reactive step: FETCH_SIZE back to the algorithmic bytes, +3.5% FP32,
-// +4% FP16).  The ensemble kernel keeps the dispatch order: renumbered, its
-// 512 instances ran 12% slower.
+// +4% FP16).  The ensemble kernel keeps the dispatch order (renumbered, its
+// 512 instances ran 12% slower), and so do 3D grids whose rows split blocks
+// (renumbered, D3Q19 and D3Q27 at 384^3 ran 7-8% slower: each XCD's separate
+// window of the 38 streams costs more than the edge lines it saves).
 constexpr uint32_t kXcds = 8;
 __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
