@@ -29,7 +29,7 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     typename Storage<ST>::R param, typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t block = S::D == 2 ? xcd_block() : blockIdx.x;
+    const uint32_t block = renumber_blocks<S::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
     const uint32_t cell = block * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
     const CellCoord q = cell_coord(cell, c);
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, Cells c,
     typename Storage<ST>::R omega, typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
     typename Storage<ST>::R k_react, typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
-    const uint32_t block = F::D == 2 ? xcd_block() : blockIdx.x;
+    const uint32_t block = renumber_blocks<F::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
     const uint32_t p = block * blockDim.x + threadIdx.x;
     if (p >= c.count) return;
     const CellCoord q = cell_coord(p, c);

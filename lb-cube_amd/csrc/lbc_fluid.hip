@@ -28,7 +28,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
                                                           typename Storage<ST>::R omega_minus) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t block = L::D == 2 ? xcd_block() : blockIdx.x;
+    const uint32_t block = renumber_blocks<L::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
     const uint32_t dst = block * blockDim.x + threadIdx.x;
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);

@@ -109,6 +109,14 @@ inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 // (renumbered, D3Q19 and D3Q27 at 384^3 ran 7-8% slower: each XCD's separate
 // window of the 38 streams costs more than the edge lines it saves).
 constexpr uint32_t kXcds = 8;
+
+// Where the renumbering pays (measured, 2048^2 reactive step): 2D grids with
+// elements of at most 4 bytes, whose 256-cell blocks span 8 lines per plane
+// (FP32) or 4 (FP16), so an edge line is 1/8 or 1/4 of a block's reads.  With
+// FP64 (16 lines per block) the saving is half as large and the renumbered
+// FP64 step ran 5% slower.
+template <int D, class S> __host__ __device__ constexpr bool renumber_blocks() { return D == 2 && sizeof(S) <= 4; }
+
 __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q = nb / kXcds, r = nb % kXcds;
