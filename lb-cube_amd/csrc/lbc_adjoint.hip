@@ -92,7 +92,11 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
     const R* __restrict__ f, const R* __restrict__ a, const R* __restrict__ b, const R* __restrict__ nuf1,
     const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
     R* __restrict__ nub0, Cells c, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
-    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware block order as in the forward kernels (the reverse pulls share
+    // block-edge lines the same way); each block keeps its own dJ/d omega_c
+    // slot, indexed by the dispatch id, so a slot always sums the same cells
+    const uint32_t block = renumber_blocks<2, R>() ? xcd_block() : blockIdx.x;
+    const uint32_t cell = block * blockDim.x + threadIdx.x;
     double acc = 0.0;
     if (cell < c.count) {
         const CellCoord q = cell_coord(cell, c);
