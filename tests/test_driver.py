@@ -48,10 +48,11 @@ def parse_vtk(path):
                                           (["-bogus"], "unknown argument: -bogus"),
                                           (["-init", "vortex"], "-init must be rest or taylor_green"),
                                           (["-collision", "xyz"], "-collision must be bgk, trt, mrt or rlbm")])
-def test_argument_errors(args, message):
+def test_argument_errors(tmp_path, args, message):
     assert os.path.exists(EXE), "build first: make -C lb-cube_amd"
-    r = run(*args)
+    r = run(*args, cwd=tmp_path)
     assert r.returncode == 1 and message in r.stderr
+    assert not any(tmp_path.iterdir()), "a rejected command line must not create files"
 
 
 @pytest.mark.gpu
@@ -95,10 +96,11 @@ def test_run_matches_oracle(gpu, tmp_path, collision):
                                           (["-k_react", "-1"], "-k_react must be non-negative"),
                                           (["-nx", "0"], "-nx must be greater than zero"),
                                           (["-bogus"], "unknown argument: -bogus")])
-def test_reaction_argument_errors(args, message):
+def test_reaction_argument_errors(tmp_path, args, message):
     assert os.path.exists(REACT_EXE), "build first: make -C lb-cube_amd"
-    r = run(*args, exe=REACT_EXE)
+    r = run(*args, cwd=tmp_path, exe=REACT_EXE)
     assert r.returncode == 1 and message in r.stderr
+    assert not any(tmp_path.iterdir()), "a rejected command line must not create files"
 
 
 def check_reactive_fields(fields, state, k):
@@ -157,10 +159,11 @@ CMP_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_compare_operators")
                                           (["-tau", "0.5"], "relaxation times must be greater than 0.5"),
                                           (["-steps", "-2"], "-steps must be non-negative"),
                                           (["-bogus"], "unknown argument: -bogus")])
-def test_compare_operators_argument_errors(args, message):
+def test_compare_operators_argument_errors(tmp_path, args, message):
     assert os.path.exists(CMP_EXE), "build first: make -C lb-cube_amd"
-    r = run(*args, exe=CMP_EXE)
+    r = run(*args, cwd=tmp_path, exe=CMP_EXE)
     assert r.returncode == 1 and message in r.stderr
+    assert not any(tmp_path.iterdir()), "a rejected command line must not create files"
 
 
 @pytest.mark.gpu
@@ -209,10 +212,11 @@ TGV_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_turbulent_tgv")
                                           (["-tau", "0.5"], "-tau must be greater than 0.5"),
                                           (["-threshold", "0"], "-threshold must be positive"),
                                           (["-bogus"], "unknown argument: -bogus")])
-def test_turbulent_tgv_argument_errors(args, message):
+def test_turbulent_tgv_argument_errors(tmp_path, args, message):
     assert os.path.exists(TGV_EXE), "build first: make -C lb-cube_amd"
-    r = run(*args, exe=TGV_EXE)
+    r = run(*args, cwd=tmp_path, exe=TGV_EXE)
     assert r.returncode == 1 and message in r.stderr
+    assert not any(tmp_path.iterdir()), "a rejected command line must not create files"
 
 
 @pytest.mark.gpu
@@ -280,10 +284,11 @@ SHEAR_EXE = os.path.join(ROOT, "lb-cube_amd", "lbm_shear_layer")
                                           (["-tau_c", "0.5"], "relaxation times must be greater than 0.5"),
                                           (["-threshold", "-1"], "-threshold must be positive"),
                                           (["-bogus"], "unknown argument: -bogus")])
-def test_shear_layer_argument_errors(args, message):
+def test_shear_layer_argument_errors(tmp_path, args, message):
     assert os.path.exists(SHEAR_EXE), "build first: make -C lb-cube_amd"
-    r = run(*args, exe=SHEAR_EXE)
+    r = run(*args, cwd=tmp_path, exe=SHEAR_EXE)
     assert r.returncode == 1 and message in r.stderr
+    assert not any(tmp_path.iterdir()), "a rejected command line must not create files"
 
 
 @pytest.mark.gpu
