@@ -17,8 +17,13 @@
  *     (src/main.cpp:587, src/simulate_reaction.cpp:527,542-543).
  * Checks the reference does not make, all reported as hipErrorInvalidValue:
  * an unsupported lattice/collision/storage code, current == next (the
- * ping-pong buffers must not alias), and a population plane of more than 2^29
- * cells (in-plane byte offsets are 32-bit).  Return values are hipError_t codes as int (0 = hipSuccess).
+ * ping-pong buffers must not alias), an extent of 2^31 or more, and a domain
+ * of more than 2^32 - 2^20 cells (4,293,918,720: cell numbers are 32-bit; the
+ * adjoint entry points keep at most 2^29 cells); a block size that would need
+ * more than 2^31 - 1 blocks is hipErrorInvalidConfiguration.  Population planes
+ * of more than 4 GiB are addressed with 64-bit per-lane addresses, smaller
+ * ones with 32-bit offsets; results are identical either way.  Return values
+ * are hipError_t codes as int (0 = hipSuccess).
  *
  * Streams are passed as void* (a hipStream_t) so this header needs no HIP
  * headers.  Thread safety: every entry point is stateless and re-entrant,

@@ -422,6 +422,13 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
 
 }  // namespace lbc
 
+namespace lbc {
+// The adjoint's element counts (19 planes per state, its reductions) stay
+// 32-bit: at most 2^29 cells per instance (23,170^2), beyond what its
+// checkpoints would fit in HBM for any useful number of steps.
+constexpr uint32_t kMaxAdjointCells = 1u << 29;
+}  // namespace lbc
+
 extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
                                                uint32_t block_x, uint32_t block_y, uint64_t* bytes) {
     using namespace lbc;
@@ -431,6 +438,7 @@ extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_
     LaunchShape s;
     const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
     if (e != hipSuccess) return e;
+    if (s.cells.count > kMaxAdjointCells) return hipErrorInvalidValue;
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;
     *bytes = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid).total;
     return hipSuccess;
@@ -449,6 +457,7 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags,
     LaunchShape s;
     const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
     if (e != hipSuccess) return e;
+    if (s.cells.count > kMaxAdjointCells) return hipErrorInvalidValue;
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
     if (collision != kBGK && collision != kTRT && collision != kMRT) return hipErrorInvalidValue;

@@ -6,7 +6,8 @@
 //
 // Same mapping as the fluid kernel: one work-item per cell, cells numbered
 // linearly along the lanes of a wave64, unit-stride loads and stores of every
-// population plane.
+// population plane; WIDE instantiations (64-bit per-lane addresses) for planes
+// of more than 4 GiB.
 #include <hip/hip_runtime.h>
 
 #include "../../include/lbcube.h"
@@ -21,7 +22,7 @@ namespace lbc {
 // A + B -> C step (REACT = true: two species, source from the exact one-step
 // reaction increment).  The advecting velocity is the fluid's at the
 // destination cell, unshifted (include/lattice_core.hpp:259-265, :381-387).
-template <class F, class S, int ST, bool REACT, int MAXT>
+template <class F, class S, int ST, bool REACT, int MAXT, bool WIDE>
 __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b,
@@ -39,15 +40,15 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     R fl[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i)
-        fl[i] = St::load(at(plane_ptr(cur_f, i, plane), cell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+        fl[i] = St::load(at<WIDE>(plane_ptr(cur_f, i, plane), cell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     const Macro<R> m = macro_state<F, R>(fl);
     const Pull pull(cell, x, y, z, nx, ny, nz);
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
-        if constexpr (REACT) gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
+        ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        if constexpr (REACT) gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     if constexpr (REACT) {
         const R ca = concentration<S, R>(ga);
@@ -60,12 +61,12 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     }
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        if constexpr (REACT) St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        if constexpr (REACT) St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
-template <class F, int CT, class S, int ST, bool FAST, int MAXT>
+template <class F, int CT, class S, int ST, bool FAST, int MAXT, bool WIDE>
 __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const typename Storage<ST>::S* __restrict__ cur_f, typename Storage<ST>::S* __restrict__ nxt_f,
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
@@ -78,10 +79,10 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
     const CellCoord q = cell_coord(p, c);
     const Pull pull(p, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
     if constexpr (FAST)
-        reactive_cell_fast<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
+        reactive_cell_fast<F, CT, S, ST, WIDE>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
                                          omega_c, k_react, ref_a, ref_b);
     else
-        reactive_cell<F, CT, S, ST>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
+        reactive_cell<F, CT, S, ST, WIDE>(cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, pull, c.count, omega, omega_minus,
                                     omega_c, k_react, ref_a, ref_b);
 }
 
@@ -97,8 +98,12 @@ hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, con
                            static_cast<R>(omega_c), static_cast<R>(param), static_cast<R>(ref_a),
                            static_cast<R>(ref_b));
     };
-    if (s.threads() <= 256) args(scalar_step_kernel<F, S, ST, REACT, 256>);
-    else args(scalar_step_kernel<F, S, ST, REACT, 1024>);
+    const bool wide = wide_planes<T>(s);
+    if (s.threads() <= 256)
+        wide ? args(scalar_step_kernel<F, S, ST, REACT, 256, true>) : args(scalar_step_kernel<F, S, ST, REACT, 256, false>);
+    else
+        wide ? args(scalar_step_kernel<F, S, ST, REACT, 1024, true>)
+             : args(scalar_step_kernel<F, S, ST, REACT, 1024, false>);
     return hipGetLastError();
 }
 
@@ -143,8 +148,13 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
                            static_cast<T*>(nb), s.cells, w, wm, static_cast<R>(omega_c), static_cast<R>(k),
                            static_cast<R>(ra), static_cast<R>(rb));
     };
-    if (s.threads() <= 256) args(reactive_fused_kernel<F, CT, S, ST, FAST, 256>);
-    else args(reactive_fused_kernel<F, CT, S, ST, FAST, 1024>);
+    const bool wide = wide_planes<T>(s);
+    if (s.threads() <= 256)
+        wide ? args(reactive_fused_kernel<F, CT, S, ST, FAST, 256, true>)
+             : args(reactive_fused_kernel<F, CT, S, ST, FAST, 256, false>);
+    else
+        wide ? args(reactive_fused_kernel<F, CT, S, ST, FAST, 1024, true>)
+             : args(reactive_fused_kernel<F, CT, S, ST, FAST, 1024, false>);
     return hipGetLastError();
 }
 

@@ -7,7 +7,8 @@
 // the Q population planes is read and written with unit stride.  A population is pulled from x - c_i
 // (periodic), collided in registers and stored once: Q loads + Q stores per
 // cell, the whole algorithmic traffic.  Periodic wrap uses selects on 32-bit
-// coordinates; the plane offset of population i is a wave-uniform 64-bit base.
+// coordinates; the plane offset of population i is a wave-uniform 64-bit base
+// (per-lane offsets 32-bit, or 64-bit for planes of more than 4 GiB).
 #include <hip/hip_runtime.h>
 
 #include "../../include/lbcube.h"
@@ -20,8 +21,9 @@ namespace lbc {
 
 // FAST selects the deviation-form arithmetic (lbc_fast.hpp) behind
 // LBC_FLAG_FAST: BGK, TRT and MRT on every lattice; RLBM always evaluates the
-// reference's order.
-template <class L, int CT, int ST, bool FAST, int MAXT>
+// reference's order.  WIDE: 64-bit per-lane addresses, for planes of more than
+// 4 GiB.
+template <class L, int CT, int ST, bool FAST, int MAXT, bool WIDE>
 __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                           typename Storage<ST>::S* __restrict__ nxt, Cells c,
                                                           typename Storage<ST>::R omega,
@@ -40,19 +42,19 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
             const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
-            f[i] = St::load_dev(at(plane_ptr(cur, i, plane), src), R(L::w[i]));
+            f[i] = St::load_dev(at<WIDE>(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
         collide_dev<L, CT, R>(f, omega, omega_minus,
-                              [&](int i, R v) { St::store_dev(at(plane_ptr(nxt, i, plane), dst), v, R(L::w[i])); });
+                              [&](int i, R v) { St::store_dev(at<WIDE>(plane_ptr(nxt, i, plane), dst), v, R(L::w[i])); });
     } else {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
             const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             const R off = kOffsetStorage<ST> ? R(L::w[i]) : R(0);
-            f[i] = St::load(at(plane_ptr(cur, i, plane), src), off);
+            f[i] = St::load(at<WIDE>(plane_ptr(cur, i, plane), src), off);
         }
         collide_cell<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
-            St::store(at(plane_ptr(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+            St::store(at<WIDE>(plane_ptr(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
         });
     }
 }
@@ -63,14 +65,15 @@ hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const typename St::S*>(cur),
+                           static_cast<typename St::S*>(nxt), s.cells, w, wm);
+    };
+    const bool wide = wide_planes<typename St::S>(s);
     if (s.threads() <= 256)
-        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 256>), s.grid, s.block, 0, stream,
-                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.cells, w,
-                           wm);
+        wide ? go(fluid_step_kernel<L, CT, ST, FAST, 256, true>) : go(fluid_step_kernel<L, CT, ST, FAST, 256, false>);
     else
-        hipLaunchKernelGGL((fluid_step_kernel<L, CT, ST, FAST, 1024>), s.grid, s.block, 0, stream,
-                           static_cast<const typename St::S*>(cur), static_cast<typename St::S*>(nxt), s.cells, w,
-                           wm);
+        wide ? go(fluid_step_kernel<L, CT, ST, FAST, 1024, true>) : go(fluid_step_kernel<L, CT, ST, FAST, 1024, false>);
     return hipGetLastError();
 }
 

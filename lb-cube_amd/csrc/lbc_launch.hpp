@@ -11,24 +11,28 @@ namespace lbc {
 
 inline bool is_2d(int lattice) { return lattice == kD2Q9 || lattice == kD2Q5; }
 
-// Division by a launch constant d (1 <= d <= 2^29) of dividends n < 2^29:
-// q = (n * m) >> s with l = ceil(log2 d), s = 29 + l, m = ceil(2^s / d) <= 2^30
-// (the round-up reciprocal method of Granlund & Montgomery, PLDI 1994).  Exact:
-// n * m / 2^s = n / d + n e / 2^s with 0 <= e < 1, n e / 2^s < 2^-l, which is
-// < 1/d unless d = 2^l, where e = 0.
+// Division by a launch constant d (1 <= d < 2^32) of any dividend n < 2^32,
+// with the add-back form of Granlund & Montgomery's round-up reciprocal
+// (PLDI 1994, Fig. 4.1): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1
+// (< 2^32), t = hi32(m n), q = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0).
+// The 33-bit multiplier 2^32 + m exceeds 2^(32+l) / d by at most one unit, so
+// n (2^32 + m) / 2^(32+l) exceeds n / d by less than 2^-l <= 1/d and its floor
+// is the quotient; the add-back evaluates floor((n + t) / 2^l) without a
+// 33-bit intermediate.
 struct FastDiv {
-    uint32_t m, s;
+    uint32_t m, s1, s2;
 };
 
 inline FastDiv make_fastdiv(uint32_t d) {
     uint32_t l = 0;
     while ((1ull << l) < d) ++l;
-    const uint32_t s = 29 + l;
-    return FastDiv{static_cast<uint32_t>(((1ull << s) + d - 1) / d), s};
+    const uint64_t m = ((((1ull << l) - d) << 32) / d) + 1;
+    return FastDiv{static_cast<uint32_t>(m), l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
 }
 
-__device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
-    return static_cast<uint32_t>((static_cast<uint64_t>(n) * f.m) >> f.s);
+__host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
+    const uint32_t t = static_cast<uint32_t>((static_cast<uint64_t>(n) * f.m) >> 32);  // v_mul_hi_u32
+    return (t + ((n - t) >> f.s1)) >> f.s2;
 }
 
 // Cells of a domain numbered linearly, x fastest ((z * ny + y) * nx + x): the
@@ -63,6 +67,20 @@ struct LaunchShape {
 // the caller passes sets the work-items per block (bx * by * bz); the cells
 // are covered linearly.
 
+// Largest domain: 2^32 - 2^20 cells (4,293,918,720), e.g. 1,600^3; 1,200^3
+// D3Q19 populations in FP32 ping-pong take 245 GiB of HBM.
+constexpr uint64_t kMaxCells = (1ull << 32) - (1ull << 20);
+// Coordinates are int.
+constexpr uint64_t kMaxExtent = (1ull << 31) - 1;
+// HIP's limit on blocks per launch along x.
+constexpr uint64_t kMaxBlocks = (1ull << 31) - 1;
+
+// Population planes of more than 4 GiB take 64-bit per-lane addresses (the
+// kernels' WIDE instantiation); smaller planes keep the 32-bit byte offsets.
+template <class S> bool wide_planes(const LaunchShape& s) {
+    return static_cast<uint64_t>(s.cells.count) * sizeof(S) > (1ull << 32);
+}
+
 // Validation order of src/lattice_cuda.cu:488-503: pointers, block, 2D
 // normalisation, extents.  Then this library's own limits.
 inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, uint32_t bx, uint32_t by,
@@ -74,14 +92,14 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
     }
     if (nx == 0 || ny == 0 || nz == 0) return hipErrorInvalidValue;
     if (static_cast<uint64_t>(bx) * by * bz > 1024) return hipErrorInvalidConfiguration;
-    // Byte offsets inside one population plane are 32-bit (the global-load
-    // saddr form), so a plane holds at most 2^29 cells (4 GiB of doubles);
-    // each factor and partial product is checked first, so nothing overflows.
-    constexpr uint64_t kMaxCells = 1ull << 29;
-    if (nx > kMaxCells || ny > kMaxCells || nz > kMaxCells) return hipErrorInvalidValue;
+    // Cell numbers are 32-bit, with room for a grid-stride step of the
+    // statistics kernels (kMaxCells + 2^18 < 2^32); each factor and partial
+    // product is checked first, so nothing overflows.
+    if (nx > kMaxExtent || ny > kMaxExtent || nz > kMaxExtent) return hipErrorInvalidValue;
     if (nx * ny > kMaxCells || nx * ny * nz > kMaxCells) return hipErrorInvalidValue;
     const uint32_t threads = bx * by * bz;
     const uint64_t count = nx * ny * nz;
+    if ((count + threads - 1) / threads > kMaxBlocks) return hipErrorInvalidConfiguration;
     s.block = dim3(threads, 1, 1);
     s.grid = dim3(static_cast<unsigned>((count + threads - 1) / threads), 1, 1);
     s.nx = static_cast<int>(nx);

@@ -9,14 +9,15 @@
 namespace lbc {
 
 // Fluid step + A + B -> C step of one cell: `pull` locates the cell and its
-// periodic sources, `plane` is the element stride between population planes.
+// periodic sources, `plane` is the element stride between population planes;
+// WIDE selects 64-bit per-lane addresses (planes of more than 4 GiB).
 // The reaction step of the reference reads the fluid populations that the
 // fluid step has just written
 // at the same cell (src/simulate_reaction.cpp:527-541); here they are still in
 // registers.  With reduced-precision storage they are passed through the
 // storage rounding first, so the velocity is the one the unfused pair sees.
 // Shared by the single-domain kernel below and the ensemble kernel.
-template <class F, int CT, class S, int ST>
+template <class F, int CT, class S, int ST, bool WIDE = false>
 __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __restrict__ cur_f,
                                               typename Storage<ST>::S* __restrict__ nxt_f,
                                               const typename Storage<ST>::S* __restrict__ cur_a,
@@ -34,10 +35,10 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
         const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
-        f[i] = St::load(at(plane_ptr(cur_f, i, plane), src), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+        f[i] = St::load(at<WIDE>(plane_ptr(cur_f, i, plane), src), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     }
     collide_cell<F, CT, R>(f, omega, omega_minus, [&](int i, R v) {
-        St::store(at(plane_ptr(nxt_f, i, plane), cell), v, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+        St::store(at<WIDE>(plane_ptr(nxt_f, i, plane), cell), v, kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     });
     if constexpr (kOffsetStorage<ST>) {
 #pragma unroll
@@ -48,8 +49,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
-        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
+        ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     const R ca = concentration<S, R>(ga);
     const R cb = concentration<S, R>(gb);
@@ -58,8 +59,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
@@ -68,7 +69,7 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
 // (lbc_fast.hpp), the velocity taken from the deviations as stored, the scalar
 // collision with 1/cs2 multiplied and the rest equilibrium by remainder, the
 // reaction increment without cancellation (lbc_fast.hpp).
-template <class F, int CT, class S, int ST>
+template <class F, int CT, class S, int ST, bool WIDE = false>
 __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S* __restrict__ cur_f,
                                                    typename Storage<ST>::S* __restrict__ nxt_f,
                                                    const typename Storage<ST>::S* __restrict__ cur_a,
@@ -86,10 +87,10 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
         const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
-        d[i] = St::load_dev(at(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
+        d[i] = St::load_dev(at<WIDE>(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
     }
     collide_dev<F, CT, R>(d, omega, omega_minus,
-                          [&](int i, R v) { St::store_dev(at(plane_ptr(nxt_f, i, plane), cell), v, R(F::w[i])); });
+                          [&](int i, R v) { St::store_dev(at<WIDE>(plane_ptr(nxt_f, i, plane), cell), v, R(F::w[i])); });
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) d[i] = St::roundtrip_dev(d[i], R(F::w[i]));
     const DevMacro<F, R> m = dev_macro<F, R>(d);
@@ -97,8 +98,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
-        gb[i] = St::load(at(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
+        ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
+        gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
     const R ca = concentration<S, R>(ga);
     const R cb = concentration<S, R>(gb);
@@ -107,8 +108,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     collide_scalar_fast<S, R>(gb, cb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        St::store(at(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 

@@ -370,7 +370,7 @@ extern "C" int lbc_fluid_diagnostics(int lattice, int storage, const void* popul
     using namespace lbc;
     if (populations == nullptr || out3 == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
@@ -388,7 +388,7 @@ extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* s
     using namespace lbc;
     if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, 1, 1, 1, s);
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
@@ -408,7 +408,7 @@ extern "C" int lbc_max_speed(int lattice, int storage, const void* populations, 
     using namespace lbc;
     if (populations == nullptr || out1 == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
@@ -425,7 +425,7 @@ extern "C" int lbc_macro_fields(int lattice, int storage, const void* population
     using namespace lbc;
     if (populations == nullptr || rho == nullptr || u == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 1, 1, 1, s);
+    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);
@@ -442,7 +442,7 @@ extern "C" int lbc_scalar_fields(int scalar_lattice, int storage, const void* po
     using namespace lbc;
     if (populations == nullptr || concentration == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, 1, 1, 1, s);
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
     const hipStream_t st = as_stream(stream);

@@ -93,13 +93,18 @@ template <class T> __device__ __forceinline__ T* plane_ptr(T* base, int i, size_
     return base + static_cast<size_t>(i) * plane;
 }
 
-// Element `cell` of a plane, addressed as uniform base + zero-extended 32-bit
-// byte offset: the form the global-load/store "saddr" addressing mode takes
-// directly (no per-lane 64-bit address arithmetic).  Requires the plane to be
-// smaller than 4 GiB, which the launchers check.
-template <class T> __device__ __forceinline__ T* at(T* plane_base, uint32_t cell) {
-    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-    return reinterpret_cast<T*>(reinterpret_cast<B*>(plane_base) + cell * static_cast<uint32_t>(sizeof(T)));
+// Element `cell` of a plane.  Planes of at most 4 GiB (WIDE = false): uniform
+// base + zero-extended 32-bit byte offset, the form the global-load/store
+// "saddr" addressing mode takes directly (no per-lane 64-bit address
+// arithmetic).  Larger planes (WIDE = true): a 64-bit per-lane address.  The
+// launchers pick the instantiation from the plane size.
+template <bool WIDE = false, class T> __device__ __forceinline__ T* at(T* plane_base, uint32_t cell) {
+    if constexpr (WIDE) {
+        return plane_base + static_cast<uint64_t>(cell);
+    } else {
+        using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+        return reinterpret_cast<T*>(reinterpret_cast<B*>(plane_base) + cell * static_cast<uint32_t>(sizeof(T)));
+    }
 }
 
 template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
@@ -110,7 +115,8 @@ template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
 // x + 1 (offset +1, or 1 - nx at the last column); y and z likewise in units
 // of a row and of an xy-plane.  Six selects per cell, then each population's
 // source is the destination plus up to three of them (no multiplies).  The
-// sums are taken modulo 2^32, where every source lands back in [0, cells).
+// sums are taken modulo 2^32, where every source lands back in [0, cells)
+// (cells < 2^32).
 struct Pull {
     uint32_t dst;
     uint32_t xm, xp, ym, yp, zm, zp;

@@ -47,7 +47,12 @@ def test_validation_order_fluid():
     assert fluid(FAKE_A, FAKE_B, ny=0) == HIP_INVALID_VALUE
     assert fluid(FAKE_A, FAKE_B, block=(64, 32, 1)) == HIP_INVALID_CONFIGURATION  # > 1024 work-items
     assert fluid(FAKE_A, FAKE_A) == HIP_INVALID_VALUE  # ping-pong buffers must not alias
-    assert fluid(FAKE_A, FAKE_B, nx=1 << 20, ny=1 << 10, nz=1) == HIP_INVALID_VALUE  # plane > 2^29 cells
+    # at most 2^32 - 2^20 cells, each extent below 2^31, at most 2^31 - 1 blocks
+    assert fluid(FAKE_A, FAKE_B, nx=1 << 12, ny=1 << 10, nz=1 << 10) == HIP_INVALID_VALUE  # 2^32 cells
+    assert fluid(FAKE_A, FAKE_B, nx=(1 << 12) - 1, ny=1 << 10, nz=(1 << 10) + 1) == HIP_INVALID_VALUE
+    assert fluid(FAKE_A, FAKE_B, nx=1 << 31, ny=1, nz=1) == HIP_INVALID_VALUE
+    assert fluid(FAKE_A, FAKE_B, nx=1 << 12, ny=1 << 10, nz=(1 << 10) - 1, block=(1, 1, 1)) == \
+        HIP_INVALID_CONFIGURATION  # 2^32 - 2^22 cells in 1-cell blocks
     # extents whose 64-bit product wraps to 0 (2^31 * 2^17 * 2^16) are still rejected
     assert fluid(FAKE_A, FAKE_B, nx=1 << 31, ny=1 << 17, nz=1 << 16, block=(2, 4, 2)) == HIP_INVALID_VALUE
 
@@ -57,6 +62,10 @@ def test_gradient_needs_whole_waves():
     lib = lbcube.lib()
     assert lib.lbc_reactive_gradient_workspace(0, 64, 64, 8, 4, 256, 1, ctypes.byref(out)) == 0 and out.value > 0
     assert lib.lbc_reactive_gradient_workspace(0, 64, 64, 8, 4, 100, 1, ctypes.byref(out)) == HIP_INVALID_CONFIGURATION
+    # the adjoint keeps at most 2^29 cells per instance
+    assert lib.lbc_reactive_gradient_workspace(0, 1 << 15, 1 << 14, 8, 4, 256, 1, ctypes.byref(out)) == 0
+    assert lib.lbc_reactive_gradient_workspace(0, 1 << 15, (1 << 14) + 1, 8, 4, 256, 1,
+                                               ctypes.byref(out)) == HIP_INVALID_VALUE
 
 
 def test_two_d_lattice_ignores_z():
