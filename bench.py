@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,10 +28,14 @@ BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16")
                   ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108, ("D3Q27", "bf16"): 108, ("D3Q27", "f64"): 432}
 
 
-def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False):
+def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False,
+                 state=None):
+    """MLUPS of `steps` fluid steps from the equilibrium of velocity field u
+    (or from `state`, a population tensor the call takes over)."""
     from lbcube import init as lbinit
 
-    a = lbinit.equilibrium(lattice, None, u, storage)
+    a = lbinit.equilibrium(lattice, None, u, storage) if state is None else state
+    del state
     b = torch.empty_like(a)
     for _ in range(warmup):
         lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast)
@@ -45,7 +50,7 @@ def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup,
     end.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / steps
-    cells = u[0].numel()
+    cells = a[0].numel()
     mlups = cells / (ms * 1e-3) / 1e6
     gbs = BYTES_PER_CELL[(lattice, storage)] * cells / (ms * 1e-3) / 1e9
     diag = lbcube.fluid_diagnostics(lattice, a)
@@ -110,6 +115,56 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
                 print(f"# {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s", file=sys.stderr,
                       flush=True)
         del u
+        torch.cuda.empty_cache()
+    return out
+
+
+def tgv_state_chunked(torch, lattice, n, storage, u0=0.05, chunk=32):
+    """Equilibrium populations of the Re = 1600 TGV velocity on an n^3 box
+    (no perturbation), built z-slab by z-slab so no full-size float64
+    temporary exists: the populations themselves are the only large
+    allocation."""
+    from lbcube import Q
+    from lbcube import init as lbinit
+
+    dt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}[storage]
+    a = torch.empty((Q[lattice], n, n, n), dtype=dt, device="cuda")
+    r = torch.arange(n, dtype=torch.float64, device="cuda") * (2.0 * math.pi / n)
+    for z0 in range(0, n, chunk):
+        z, y, x = torch.meshgrid(r[z0:z0 + chunk], r, r, indexing="ij")
+        u = torch.stack([u0 * torch.sin(x) * torch.cos(y) * torch.cos(z),
+                         -u0 * torch.cos(x) * torch.sin(y) * torch.cos(z), torch.zeros_like(x)])
+        a[:, z0:z0 + chunk] = lbinit.equilibrium(lattice, None, u, storage)
+        del z, y, x, u
+    return a
+
+
+def large_domain_rate(lbcube, torch, steps=10, warmup=2):
+    """D3Q19 TRT TGV (Re = 1600) on the largest cubes one MI355X holds:
+    1024^3 FP32 (2^30 cells, 4 GiB planes, 32-bit offsets), 1152^3 FP32
+    (1.53e9 cells, 232 GB of populations, 6.1 GB planes: the 64-bit-address
+    kernels) and 1440^3 FP16 deviation storage (2.99e9 cells, 227 GB)."""
+    from lbcube import init as lbinit
+
+    out = {}
+    for n, storage, fast in ((1024, "f32", False), (1152, "f32", False), (1440, "f16", True)):
+        key = f"D3Q19_{n}_trt_{storage}{'fast' if fast else ''}"
+        bpc = BYTES_PER_CELL[("D3Q19", storage)]  # both buffers: 2 x 19 values per cell
+        need = bpc * n ** 3 + (16 << 30)
+        torch.cuda.empty_cache()
+        free, _ = torch.cuda.mem_get_info()
+        if free < need:
+            out[key] = {"skipped": f"needs {need / 1e9:.0f} GB free, {free / 1e9:.0f} GB free"}
+            continue
+        state = tgv_state_chunked(torch, "D3Q19", n, storage)
+        r = time_variant(lbcube, torch, "D3Q19", "TRT", storage, None, 1.0 / lbinit.tgv_re1600_tau(n), steps, warmup,
+                         fast=fast, state=state)
+        del state
+        r["cells"] = n ** 3
+        r["population_gb"] = round(bpc * n ** 3 / 1e9, 1)
+        r["plane_bytes"] = n ** 3 * bpc // 38
+        out[key] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        print(f"# {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     return out
 
@@ -357,7 +412,7 @@ def main():
                                           "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
-    ap.add_argument("--extras", default="adjoint,reactive_2d,ensemble,lattices",
+    ap.add_argument("--extras", default="adjoint,reactive_2d,ensemble,lattices,large",
                     help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -406,6 +461,8 @@ def main():
             print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         if "lattices" in chosen:
             extras["lattices"] = lattice_sweep(lbcube, torch)
+        if "large" in chosen:
+            extras["large"] = large_domain_rate(lbcube, torch)
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
     cpu = None
     if not args.no_cpu_baseline:
