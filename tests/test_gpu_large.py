@@ -9,7 +9,7 @@ itself compared with the oracle (bit-exact for the fluid step; the reaction
 step to the tolerance of its exp(), as in the parity tests).  The statistics
 of the large domain are the small domain's, summed reps times.
 
-The tests need up to 163 GB of device memory (one MI355X has 288 GB) and skip
+The tests need up to 232 GB of device memory (one MI355X has 288 GiB) and skip
 where the device has less free.
 """
 from __future__ import annotations
@@ -84,6 +84,7 @@ def steps(lattice, coll, a, b, omega, n, fast=False):
     ("D2Q9", "MRT", "f64", (128, 4096), False),       # 2^29 + 2^19 cells, 4.004 GiB planes
     ("D3Q19", "TRT", "f32", (256, 64, 64), False),    # 2^30 + 2^20 cells, 4.004 GiB planes, 163 GB
     ("D3Q19", "TRT", "f16", (256, 64, 64), True),     # 2^30 + 2^20 cells, 2 GiB planes (32-bit offsets)
+    ("D3Q27", "MRT", "f32", (256, 64, 64), False),    # 2^30 + 2^20 cells, 4.004 GiB planes, 232 GB
 ])
 def test_fluid_large_domain_tiles(gpu, release, lattice, coll, storage, small_shape, fast):
     print(f"case large fluid {lattice} {coll} {storage} fast={fast} {small_shape} x {REPS}", flush=True)
@@ -183,3 +184,60 @@ def test_reactive_fused_large_domain_tiles(gpu, release, fast):
     st_s = lbcube.reactive_stats("D2Q5", ra, rb, k)
     for name in ("mean_A", "mean_B", "var_A", "var_B", "true_reaction_rate"):
         assert abs(st_big[name] - st_s[name]) <= 1e-11 * max(abs(st_s[name]), 1e-30), name
+
+
+def _separated_species(shape, storage, seed):
+    """A and B about 0.5 apart (no cancellation in the reference's reaction
+    formula), advected by a small random velocity."""
+    g = cases.rng(seed)
+    ca = g.uniform(0.05, 0.4, size=shape)
+    cb = ca + 0.5 + g.uniform(-0.05, 0.05, size=shape)
+    u = g.uniform(-0.02, 0.02, size=(2,) + shape)
+    return O.scalar_feq_field("D2Q5", storage, ca, u), O.scalar_feq_field("D2Q5", storage, cb, u)
+
+
+@pytest.mark.parametrize("react", [False, True])
+def test_scalar_and_reaction_steps_large_domain_tiles(gpu, release, react):
+    """The unfused ADR kernels (passive scalar; A + B -> C) on 4.004 GiB planes."""
+    what = "reaction" if react else "scalar"
+    print(f"case large {what} step D2Q9/D2Q5 f32 (256, 4096) x 1025", flush=True)
+    storage, shape = "f32", (256, 4096)
+    cells = shape[0] * shape[1] * REPS
+    need(gpu, (9 + (4 if react else 2) * 5) * cells * 4)
+    ff = cases.random_perturbed("D2Q9", shape, storage, seed=43)
+    fa, fb = _separated_species(shape, storage, 47)
+    omega_c, k, src = 1.0 / 0.8, 0.3, 1e-4
+
+    def run(f, a, b, n=2):
+        na, nb = torch.empty_like(a), (torch.empty_like(b) if react else None)
+        for _ in range(n):
+            if react:
+                lbcube.reaction_step("D2Q9", "D2Q5", a, na, b, nb, f, omega_c, k)
+                a, na, b, nb = na, a, nb, b
+            else:
+                lbcube.scalar_step("D2Q9", "D2Q5", a, na, f, omega_c, source=src)
+                a, na = na, a
+        del na, nb
+        return a, b
+
+    sf, sa, sb = (to_dev(x, storage, gpu) for x in (ff, fa, fb))
+    ra, rb = run(sf, sa.clone(), sb.clone() if react else None)
+    if react:
+        wa, wb = O.reaction_step("D2Q9", "D2Q5", storage, fa, fb, ff, omega_c, k)
+        wa, wb = O.reaction_step("D2Q9", "D2Q5", storage, wa, wb, ff, omega_c, k)
+        for got, want in ((ra, wa), (rb, wb)):  # exp() of device and host may differ by an ulp
+            np.testing.assert_allclose(got.cpu().numpy(), want, rtol=2e-6, atol=1e-7)
+    else:
+        want = O.scalar_step("D2Q9", "D2Q5", storage, fa, ff, omega_c, source=src)
+        want = O.scalar_step("D2Q9", "D2Q5", storage, want, ff, omega_c, source=src)
+        assert np.array_equal(ra.cpu().numpy(), want), "small domain against the oracle"
+    big_f, big_a = tile(sf, REPS), tile(sa, REPS)
+    big_b = tile(sb, REPS) if react else None
+    assert plane_bytes(big_a) > 1 << 32 and plane_bytes(big_f) > 1 << 32
+    del sf, sa, sb
+    oa, ob = run(big_f, big_a, big_b)
+    torch.cuda.synchronize()
+    del big_f, big_a, big_b
+    assert_tiles(oa, ra, REPS, f"{what} A")
+    if react:
+        assert_tiles(ob, rb, REPS, f"{what} B")
