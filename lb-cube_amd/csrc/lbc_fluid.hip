@@ -59,14 +59,27 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     }
 }
 
+// Kernels that run faster with residency capped at kCappedWavesPerCu (measured
+// on MI355X): 3D lattices with FP32 storage or FP16 in the deviation form,
+// except the reference-order D3Q27 MRT (VALU-bound).  Left uncapped: BF16
+// (its software rounding makes the step VALU-heavier: 71.0k against 74.4k
+// MLUPS capped), FP64 (unchanged by the cap) and RLBM (not measured).
+template <class L, int CT, int ST, bool FAST> constexpr bool capped_residency() {
+    if (L::D != 3 || ST == kF64 || ST == kBF16 || CT == kRLBM) return false;
+    if (kOffsetStorage<ST> && !FAST) return false;
+    if (L::Q == 27 && CT == kMRT && !FAST) return false;
+    return true;
+}
+
 template <class L, int CT, int ST, bool FAST>
 hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double omega, hipStream_t stream) {
     using St = Storage<ST>;
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
+    const uint32_t lds = capped_residency<L, CT, ST, FAST>() ? capped_lds(s.threads()) : 0;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const typename St::S*>(cur),
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const typename St::S*>(cur),
                            static_cast<typename St::S*>(nxt), s.cells, w, wm);
     };
     const bool wide = wide_planes<typename St::S>(s);

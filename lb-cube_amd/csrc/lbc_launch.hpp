@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "lbc_lattice.hpp"
 
@@ -112,6 +113,49 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
 }
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// Residency cap for the memory-bound 3D fluid kernels.  They need few
+// registers (45-50 VGPRs), so the hardware keeps 8 waves per SIMD (32 per CU)
+// resident; with 19-27 population streams per wave that many outstanding
+// streams run the HBM below its best rate.  Reserving LDS the kernels do not
+// use caps the blocks per CU: at 16 waves per CU the D3Q19 TRT FP32 step at
+// 256^3 ran 4.7% faster (39.1k against 37.3k MLUPS), 512^3 ~10%, D3Q27 TRT
+// FP32 and FP16 (deviation form) ~4%; 20 waves per CU gave the same, 24
+// less, 12 lost ground.  Kernels whose collision keeps the VALU busy
+// (reference-order FP16/BF16, reference-order D3Q27 MRT) lose from it, as did
+// the 2D fused reactive step, so the launchers apply it per kernel.
+constexpr uint32_t kCappedWavesPerCu = 16;
+
+// LDS per CU and per block of the current device (queried once per device).
+struct LdsLimits {
+    uint32_t per_cu = 0, per_block = 0;
+};
+
+inline LdsLimits lds_limits() {
+    static LdsLimits cache[64];
+    static std::once_flag once[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return LdsLimits{};
+    std::call_once(once[dev], [dev] {
+        int cu = 0, block = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess &&
+            hipDeviceGetAttribute(&block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            cu > 0 && block > 0)
+            cache[dev] = LdsLimits{static_cast<uint32_t>(cu), static_cast<uint32_t>(block)};
+    });
+    return cache[dev];
+}
+
+// Dynamic LDS bytes per block that hold a launch of `threads`-wide blocks to
+// kCappedWavesPerCu resident waves per CU (0: no cap, e.g. blocks of more
+// than 8 waves, or a device whose limits could not be read).
+inline uint32_t capped_lds(uint32_t threads) {
+    const uint32_t blocks = kCappedWavesPerCu / ((threads + 63) / 64);
+    if (blocks < 2) return 0;
+    const LdsLimits l = lds_limits();
+    const uint32_t lds = l.per_cu / blocks;
+    return (lds == 0 || lds > l.per_block) ? 0 : lds;
+}
 
 // The dispatcher hands workgroup b to XCD b % 8 (8 XCDs, each with its own
 // L2).  Blocks numbered linearly over the cells share the cache lines at their
