@@ -291,6 +291,54 @@ def ensemble_rate(torch, steps=1000):
             "autotune_ms_per_step": {str(t): round(v, 4) for t, v in timings.items()}}
 
 
+def adr_3d_rate(lbcube, torch, n=256, steps=40):
+    """The 3D advection-diffusion kernels on the 256^3 TGV (D3Q19 fluid, D3Q7
+    scalars, FP32): the passive-scalar step (reads the fluid, pulls one
+    scalar: (19 + 2 x 7) x 4 = 132 B/cell), the A + B -> C reaction step
+    ((19 + 4 x 7) x 4 = 188 B/cell) and the fused TRT + reaction step in both
+    arithmetic forms ((2 x 19 + 4 x 7) x 4 = 264 B/cell).  CUDA-graph replays
+    of 10-step chunks."""
+    from lbcube import init as lbinit
+
+    u = lbinit.taylor_green_3d(n, 0.05, device="cuda")
+    x = torch.arange(n, dtype=torch.float64, device="cuda") * (2.0 * math.pi / n)
+    ca = (0.5 + 0.2 * torch.sin(x)).expand(n, n, n).contiguous()
+    cb = (0.5 - 0.2 * torch.sin(x)).view(n, 1, 1).expand(n, n, n).contiguous()
+    f = lbinit.equilibrium("D3Q19", None, u, "f32")
+    a = lbinit.scalar_equilibrium("D3Q7", ca, u, "f32")
+    b = lbinit.scalar_equilibrium("D3Q7", cb, u, "f32")
+    del ca, cb, u
+    ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    state = [f, ft, a, at, b, bt]
+    omega, omega_c, k = 1.0 / 0.6, 1.0 / 0.7, 0.005
+    cells = n ** 3
+
+    def scalar():
+        lbcube.scalar_step("D3Q19", "D3Q7", state[2], state[3], state[0], omega_c, source=1e-6)
+        state[2], state[3] = state[3], state[2]
+
+    def reaction():
+        lbcube.reaction_step("D3Q19", "D3Q7", state[2], state[3], state[4], state[5], state[0], omega_c, k)
+        state[2], state[3], state[4], state[5] = state[3], state[2], state[5], state[4]
+
+    def fused(fast):
+        f, ft, a, at, b, bt = state
+        lbcube.reactive_fused_step("D3Q19", "TRT", "D3Q7", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
+        state[:] = [ft, f, at, a, bt, b]
+
+    out = {"workload": f"D3Q19 TGV {n}^3 (u0 = 0.05) + two D3Q7 species (C = 0.5 +- 0.2 sin), FP32"}
+    for name, fn, bpc in (("scalar_step", scalar, 132), ("reaction_step", reaction, 188),
+                          ("fused_trt", lambda: fused(False), 264), ("fused_trt_fast", lambda: fused(True), 264)):
+        ms = _graph_ms(torch, fn, steps, chunk=10)
+        out[name] = {"mlups": cells / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * cells / (ms * 1e-3) / 1e9,
+                     "bytes_per_cell": bpc, "ms_per_step": ms}
+        print(f"# adr 3d {name}: {out[name]['mlups']:.0f} MLUPS, {out[name]['achieved_gbs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    del state, f, ft, a, at, b, bt
+    torch.cuda.empty_cache()
+    return out
+
+
 def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     """2048^2 D2Q9 MRT double shear layer + two D2Q5 species (Da = 10), FP32
     and FP64: the fused single-pass step (152 / 304 B per cell), the reference's
@@ -412,7 +460,7 @@ def main():
                                           "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
-    ap.add_argument("--extras", default="adjoint,reactive_2d,ensemble,lattices,large",
+    ap.add_argument("--extras", default="adjoint,reactive_2d,adr_3d,ensemble,lattices,large",
                     help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -456,6 +504,8 @@ def main():
             r2 = extras["reactive_2d"]
             print(f"# reactive 2048^2 MRT f32: {r2['f32']['mlups']:.0f} MLUPS, f32 fast {r2['f32_fast']['mlups']:.0f}, "
                   f"f16 fast {r2['f16_fast']['mlups']:.0f}", file=sys.stderr, flush=True)
+        if "adr_3d" in chosen:
+            extras["adr_3d"] = adr_3d_rate(lbcube, torch)
         if "ensemble" in chosen:
             extras["ensemble"] = ensemble_rate(torch)
             print(f"# ensemble: {extras['ensemble']['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)

@@ -86,14 +86,22 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
                                     omega_c, k_react, ref_a, ref_b);
 }
 
+// Residency cap (kCappedWavesPerCu, lbc_launch.hpp) for the 3D kernels with
+// FP32 storage, measured on the 256^3 TGV with two D3Q7 species: scalar step
+// 41.2k -> 45.0k MLUPS, reaction step 30.9k -> 31.8k, fused TRT step 21.0k ->
+// 21.3k (both forms).  2D grids (the fused 2048^2 step lost 1-5%) and the
+// other storage formats (not measured) stay uncapped.
+template <int D, int ST> constexpr bool capped_residency_adr() { return D == 3 && ST == kF32; }
+
 template <class F, class S, int ST, bool REACT>
 hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, const void* cf, const LaunchShape& s,
                          double omega_c, double param, double ref_a, double ref_b, hipStream_t stream) {
     using St = Storage<ST>;
     using T = typename St::S;
     using R = typename St::R;
+    const uint32_t lds = capped_residency_adr<S::D, ST>() ? capped_lds(s.threads()) : 0;
     auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(ca), static_cast<T*>(na),
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(ca), static_cast<T*>(na),
                            static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells,
                            static_cast<R>(omega_c), static_cast<R>(param), static_cast<R>(ref_a),
                            static_cast<R>(ref_b));
@@ -142,8 +150,9 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
+    const uint32_t lds = capped_residency_adr<F::D, ST>() ? capped_lds(s.threads()) : 0;
     auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, s.grid, s.block, 0, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
                            static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
                            static_cast<T*>(nb), s.cells, w, wm, static_cast<R>(omega_c), static_cast<R>(k),
                            static_cast<R>(ra), static_cast<R>(rb));
