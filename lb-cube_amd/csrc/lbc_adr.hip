@@ -86,12 +86,16 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
                                     omega_c, k_react, ref_a, ref_b);
 }
 
-// Residency cap (kCappedWavesPerCu, lbc_launch.hpp) for the 3D kernels with
-// FP32 storage, measured on the 256^3 TGV with two D3Q7 species: scalar step
-// 41.2k -> 45.0k MLUPS, reaction step 30.9k -> 31.8k, fused TRT step 21.0k ->
-// 21.3k (both forms).  2D grids (the fused 2048^2 step lost 1-5%) and the
-// other storage formats (not measured) stay uncapped.
-template <int D, int ST> constexpr bool capped_residency_adr() { return D == 3 && ST == kF32; }
+// Residency cap (kCappedWavesPerCu, lbc_launch.hpp), measured on the 256^3
+// TGV with two D3Q7 species.  FP32: scalar step 41.2k -> 45.0k MLUPS,
+// reaction step 30.9k -> 31.8k, fused TRT step 21.0k -> 21.3k (both forms).
+// FP16: the fused step gains 3-4.5% (39.2k -> 40.4k, deviation form 39.3k ->
+// 41.1k), the scalar step loses 7%, the reaction step gains 1.4% (left
+// uncapped).  BF16 loses up to 15% and FP64 is unchanged, so both stay
+// uncapped, as do 2D grids (the fused 2048^2 step lost 1-5%).
+template <int D, int ST, bool FUSED> constexpr bool capped_residency_adr() {
+    return D == 3 && (ST == kF32 || (FUSED && ST == kF16));
+}
 
 template <class F, class S, int ST, bool REACT>
 hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, const void* cf, const LaunchShape& s,
@@ -99,7 +103,7 @@ hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, con
     using St = Storage<ST>;
     using T = typename St::S;
     using R = typename St::R;
-    const uint32_t lds = capped_residency_adr<S::D, ST>() ? capped_lds(s.threads()) : 0;
+    const uint32_t lds = capped_residency_adr<S::D, ST, false>() ? capped_lds(s.threads()) : 0;
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(ca), static_cast<T*>(na),
                            static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells,
@@ -150,7 +154,7 @@ hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, cons
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
-    const uint32_t lds = capped_residency_adr<F::D, ST>() ? capped_lds(s.threads()) : 0;
+    const uint32_t lds = capped_residency_adr<F::D, ST, true>() ? capped_lds(s.threads()) : 0;
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
                            static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
