@@ -139,12 +139,15 @@ int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_latt
  * *loss, the adjoints dloss/d(f0, a0, b0) of the initial populations (same
  * layout and type as the inputs) and dloss/d(omega_c).  States are
  * checkpointed every `interval` steps and each segment is recomputed before
- * it is walked backward.  storage: LBC_F32 or LBC_F64.  flags: 0 (forward =
+ * it is walked backward; the recompute of a segment runs on a private stream
+ * of the library (default priority, created and destroyed by the call) while
+ * `stream` walks the following segment, ordered by events.  storage: LBC_F32
+ * or LBC_F64.  flags: 0 (forward =
  * lbc_reactive_fused_step) or LBC_FLAG_FAST (forward =
  * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...)); the adjoint sweep is
  * the same for both.  The caller passes a
- * device workspace of at least lbc_reactive_gradient_workspace() bytes;
- * nothing is allocated.  block_x * block_y must be a multiple of 64 (whole
+ * device workspace of at least lbc_reactive_gradient_workspace() bytes
+ * (checkpoints plus two segment buffers); no device memory is allocated.  block_x * block_y must be a multiple of 64 (whole
  * wavefronts), else hipErrorInvalidConfiguration.  Synchronizes `stream`.
  * No counterpart in the reference. */
 int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,

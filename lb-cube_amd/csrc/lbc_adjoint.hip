@@ -10,8 +10,8 @@
 // streaming).  One adjoint step per cell:
 //   1. reverse-pull nu_f, nu_a, nu_b of step n+1 (the adjoints of the stored
 //      f_{n+1}, a_{n+1}, b_{n+1} at this cell);
-//   2. pull f_n and re-collide it (same code as the forward) to get the state
-//      and velocity u' the forward reaction step used;
+//   2. pull f_n and take its density and velocity, which the collision
+//      conserves: u' of the forward reaction step up to rounding;
 //   3. reaction step transpose: scalar BGK + source Jacobians, the reaction
 //      Jacobian of the exact one-step map, and the velocity coupling into the
 //      fluid adjoint through du'/df' = (c - u') / rho';
@@ -103,17 +103,20 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
         const size_t plane = c.count;
         const Pull pull(cell, q.x, q.y, 0, nx, ny, 1);
-        R fl[9], fp[9], lf[9];
+        R fl[9], lf[9];
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
             const uint32_t src = pull.src(D2Q9::cx[i], D2Q9::cy[i], 0);
             const uint32_t rev = pull.src(-D2Q9::cx[i], -D2Q9::cy[i], 0);
             fl[i] = plane_ptr(f, i, plane)[src];
             lf[i] = plane_ptr(nuf1, i, plane)[rev];
-            fp[i] = fl[i];
         }
-        collide_cell<D2Q9, CT, R>(fp, omega, omega_minus);
-        const Macro<R> mp = macro_state<D2Q9, R>(fp);
+        // BGK, TRT and MRT conserve rho and j, so the velocity the forward
+        // scalar step read from the post-collision state equals that of the
+        // pulled state up to rounding: the linearisation point is taken from
+        // the pulled populations, without re-colliding them.
+        const Macro<R> m = macro_state<D2Q9, R>(fl);
+        const Macro<R>& mp = m;
         R ga[5], gb[5], la[5], lb[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
@@ -169,7 +172,6 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
 #pragma unroll
         for (int i = 0; i < 9; ++i)
             lf[i] += (lux * (R(D2Q9::cx[i]) - mp.ux) + luy * (R(D2Q9::cy[i]) - mp.uy)) * irho;
-        const Macro<R> m = macro_state<D2Q9, R>(fl);
         if constexpr (CT == kMRT) {
             // f' = M^-1 [(I - S) M f + S m_eq(rho, j)], M^-1 = M^T D^-1
             // (collide_mrt_d2q9): J^T lam = M^T (I - S) mu + (d m_eq/d f)^T S mu
@@ -310,7 +312,7 @@ inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, in
     L.off_ckpt = off;
     off = align_up(off + L.nseg * L.state_elems * es);  // states at the segment starts
     L.off_seg = off;
-    off = align_up(off + (L.seg_slots + 1) * L.state_elems * es);  // one recomputed segment
+    off = align_up(off + 2 * L.seg_slots * L.state_elems * es);  // two recomputed segments
     L.off_nu = off;
     off = align_up(off + 2 * L.state_elems * es);  // adjoint ping-pong
     L.off_part = off;
@@ -324,6 +326,25 @@ inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, in
 template <class R> struct State {
     R *f, *a, *b;
     State(R* base, size_t cells) : f(base), a(base + 9 * cells), b(base + 14 * cells) {}
+};
+
+// The second stream of the reverse sweep and the events that order it
+// against the caller's stream; destroyed when the sweep ends or fails.
+struct SweepStreams {
+    hipStream_t side = nullptr;
+    hipEvent_t forward_done = nullptr, recomputed[2] = {nullptr, nullptr}, swept[2] = {nullptr, nullptr};
+    hipError_t create() {
+        hipError_t e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+        for (hipEvent_t* ev : {&forward_done, &recomputed[0], &recomputed[1], &swept[0], &swept[1]})
+            if (e == hipSuccess) e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        return e;
+    }
+    ~SweepStreams() {
+        if (side != nullptr) (void)hipStreamSynchronize(side);
+        for (hipEvent_t ev : {forward_done, recomputed[0], recomputed[1], swept[0], swept[1]})
+            if (ev != nullptr) (void)hipEventDestroy(ev);
+        if (side != nullptr) (void)hipStreamDestroy(side);
+    }
 };
 
 template <class R, int CT>
@@ -343,12 +364,13 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     const R omega = static_cast<R>(omega_d), omega_c = static_cast<R>(omega_c_d), k = static_cast<R>(k_d);
     const R wm = CT == kTRT ? trt_omega_minus<R>(omega) : R(0);
     hipError_t e;
-    auto step = [&](State<R> in, State<R> out) {
+    auto step_on = [&](hipStream_t on, State<R> in, State<R> out) {
         return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, flags, in.f, out.f, in.a,
                                                                   out.a, in.b, out.b, s.nx, s.ny, 1, omega_d,
                                                                   omega_c_d, k_d, 0.0, 0.0, s.block.x, s.block.y, 1,
-                                                                  st));
+                                                                  on));
     };
+    auto step = [&](State<R> in, State<R> out) { return step_on(st, in, out); };
     // forward with checkpoints at the segment starts
     if ((e = hipMemcpyAsync(ckpt, f0, 9 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(ckpt + 9 * cells, a0, 5 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess)
@@ -375,6 +397,19 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red);
     hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, last + 9 * cells, 5 * cells, red + 512);
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red + 1);
+    // The recomputes may overwrite the forward's scratch states from here on.
+    // Segments shorter than kOverlapSteps stay on `stream`: two cross-stream
+    // waits per segment cost more than the overlap gains (with interval 1 the
+    // step rose from 24 to 34 us at 512^2).
+    constexpr int kOverlapSteps = 8;
+    const bool overlap = interval >= kOverlapSteps && L.nseg > 1;
+    SweepStreams ss;
+    if (overlap) {
+        if ((e = ss.create()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ss.forward_done, st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(ss.side, ss.forward_done, 0)) != hipSuccess) return e;
+    }
+    const hipStream_t rs = overlap ? ss.side : st;
     // adjoint initial condition: dJ/da_T = -1/N everywhere
     const R inv_n = R(1) / static_cast<R>(cells);
     const unsigned fb = static_cast<unsigned>(std::min<size_t>(4096, (se + 255) / 256));
@@ -383,15 +418,35 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1 + 14 * cells, 5 * cells, R(0));
     hipLaunchKernelGGL((zero_double_kernel<R>), dim3(4), dim3(256), 0, st, part, L.blocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // Reverse sweep.  Segment sgp's states 1..len-1 are recomputed into
+    // segment buffer sgp % 2 (slots 1..len-1; state 0 is the checkpoint
+    // itself, and with interval 1 nothing is recomputed) on the side stream,
+    // while the caller's stream walks segment sgp + 1 backward from the other
+    // buffer: the two are independent, and at a few hundred thousand cells
+    // neither kernel fills the GPU.  Events order each walk after its
+    // recompute and each recompute after the walk that last read its buffer
+    // (without overlap everything runs in order on `stream`).
+    auto seg_len = [&](size_t sgp) { return static_cast<int>(std::min<size_t>(interval, steps - sgp * interval)); };
+    auto state_in = [&](size_t sgp, int t) {
+        return t == 0 ? ckpt + sgp * se : seg + ((sgp & 1) * L.seg_slots + static_cast<size_t>(t)) * se;
+    };
+    auto recompute = [&](size_t sgp) {
+        hipError_t err = hipSuccess;
+        for (int t = 0; t + 1 < seg_len(sgp) && err == hipSuccess; ++t)
+            err = step_on(rs, State<R>(state_in(sgp, t), cells), State<R>(state_in(sgp, t + 1), cells));
+        return err == hipSuccess && overlap ? hipEventRecord(ss.recomputed[sgp & 1], ss.side) : err;
+    };
+    if ((e = recompute(L.nseg - 1)) != hipSuccess) return e;
     for (size_t sgp = L.nseg; sgp-- > 0;) {
-        const int len = static_cast<int>(std::min<size_t>(interval, steps - sgp * interval));
-        // recompute the segment's states 1..len-1 into seg slots 1..len-1; state 0
-        // is the checkpoint itself (no copy: with interval 1 nothing is recomputed)
-        R* const base = ckpt + sgp * se;
-        auto state_at = [&](int t) { return t == 0 ? base : seg + static_cast<size_t>(t) * se; };
-        for (int t = 0; t + 1 < len; ++t) {
-            if ((e = step(State<R>(state_at(t), cells), State<R>(state_at(t + 1), cells))) != hipSuccess) return e;
+        const int len = seg_len(sgp);
+        if (sgp >= 1) {
+            if (overlap && sgp + 1 < L.nseg &&
+                (e = hipStreamWaitEvent(ss.side, ss.swept[(sgp + 1) & 1], 0)) != hipSuccess)
+                return e;
+            if (overlap && (e = recompute(sgp - 1)) != hipSuccess) return e;
         }
+        if (overlap && (e = hipStreamWaitEvent(st, ss.recomputed[sgp & 1], 0)) != hipSuccess) return e;
+        auto state_at = [&](int t) { return state_in(sgp, t); };
         for (int t = len - 1; t >= 0; --t) {
             State<R> fw(state_at(t), cells), in(nu1, cells), out(nu0, cells);
             if (s.threads() <= 256)
@@ -404,6 +459,9 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
             if ((e = hipGetLastError()) != hipSuccess) return e;
             std::swap(nu0, nu1);
         }
+        if (overlap && (e = hipEventRecord(ss.swept[sgp & 1], st)) != hipSuccess) return e;
+        // in order on one stream, the previous segment is recomputed after this walk
+        if (!overlap && sgp >= 1 && (e = recompute(sgp - 1)) != hipSuccess) return e;
     }
     // adjoints of the stored initial populations
     State<R> fin(nu1, cells);

@@ -27,8 +27,10 @@ def dev(x, gpu, dtype=torch.float64):
 
 
 @pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
-@pytest.mark.parametrize("interval", [1, 5, 64])
+@pytest.mark.parametrize("interval", [1, 5, 8, 64])
 def test_adjoint_matches_oracle_fp64(gpu, coll, interval):
+    # interval 8 over 17 steps: three segments (8, 8, 1), the recomputes
+    # overlapped with the adjoint walks on a second stream
     print(f"case adjoint {coll} interval={interval}", flush=True)
     f, a, b = TA.small_state(nx=12, ny=10, seed=4)
     omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 17
@@ -79,7 +81,9 @@ def test_checkpoint_interval_invariance(gpu):
     f, a, b = TA.small_state(nx=64, ny=48, seed=2)
     args = (dev(f, gpu), dev(a, gpu), dev(b, gpu), 50, 1 / 0.6, 1 / 0.8, 0.05)
     ref = ADJ.reactive_gradient("BGK", *args, interval=50)
-    for interval in (1, 7, 64):
+    # 1 and 7: every segment on one stream; 8 and 16: recomputes overlapped
+    # with the adjoint walks (interval >= 8 and more than one segment)
+    for interval in (1, 7, 8, 16, 64):
         got = ADJ.reactive_gradient("BGK", *args, interval=interval)
         assert got[0] == ref[0] and got[4] == ref[4]
         for x, y in zip(got[1:4], ref[1:4]):
