@@ -90,6 +90,25 @@ def test_checkpoint_interval_invariance(gpu):
             assert torch.equal(x, y)
 
 
+def test_overlapped_gradient_on_a_caller_stream(gpu):
+    """The overlapped schedule (recomputes on the library's second stream,
+    ordered by events) gives the same bits when the caller's stream is a
+    non-default stream as on the default one."""
+    print("case overlapped gradient on a caller stream, TRT interval 8", flush=True)
+    f, a, b = TA.small_state(nx=64, ny=48, seed=3)
+    args = (dev(f, gpu), dev(a, gpu), dev(b, gpu), 40, 1 / 0.6, 1 / 0.8, 0.05)
+    ref = ADJ.reactive_gradient("TRT", *args, interval=8)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        got = ADJ.reactive_gradient("TRT", *args, interval=8)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert got[0] == ref[0] and got[4] == ref[4]
+    for x, y in zip(got[1:4], ref[1:4]):
+        assert torch.equal(x, y)
+
+
 def test_autograd_gradcheck(gpu):
     f, a, b = TA.small_state(nx=6, ny=5, seed=8)
     f0, a0, b0 = (dev(x, gpu).requires_grad_(True) for x in (f, a, b))
