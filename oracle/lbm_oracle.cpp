@@ -28,11 +28,19 @@
 //                                        (tests/test_validation.cpp) restated,
 //                                        TRT(w-=w+) == BGK, D2Q9/D3Q27 MRT with
 //                                        all rates = w == BGK, conservation
+//   RLBM reconstruction ................ Hermite moments of orders 0-3 of the
+//                                        post-collision state against the
+//                                        recursive a3 = u Pi + permutations
+//                                        (Malaspinas 2015), D3Q19 pairs, and
+//                                        degree <= 3 of f_neq / w
 //   MRT transforms ..................... the reference's literal identity
 //                                        vectors (test_validation.cpp:1532-1582)
 //                                        and M M^-1 = I by brute force
 //   scalar ADR ......................... advected decaying sine closed form
-//                                        (test_validation.cpp:1590-1646)
+//                                        (test_validation.cpp:1590-1646), in 2D
+//                                        on D2Q5 and in 3D on D3Q7 (cs2 = 1/4);
+//                                        equilibrium moments C, C u, C cs2 I
+//                                        through a one-cell step
 //   reaction source .................... batch closed form C0/(1+C0 k t),
 //                                        high-precision ODE integration
 //   statistics ......................... closed forms (uniform, complementary
