@@ -433,7 +433,10 @@ def cpu_baseline(u_host, omega, n):
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
 
-    O.build()
+    # The library is built beforehand (__graft_entry__.build()); building it here
+    # would start a compiler from a process that holds the GPU.
+    if not os.path.exists(O.LIB_PATH):
+        raise RuntimeError(f"{O.LIB_PATH} is not built: run __graft_entry__.build() before the benchmark")
     rho = np.ones(u_host.shape[1:])
     f = O.feq_field("D3Q19", "f32", rho, u_host)
     threads = O.max_threads()

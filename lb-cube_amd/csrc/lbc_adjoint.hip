@@ -36,29 +36,48 @@ namespace lbc {
 __device__ __forceinline__ float expm1_dev(float z) { return expm1f(z); }
 __device__ __forceinline__ double expm1_dev(double z) { return expm1(z); }
 
-// dS/dA, dS/dB of S = A' - A for the exact one-step map of dA/dt = dB/dt =
-// -k A B.  With D = B - A and z = k D: A' = A / (1 + k A phi(z) + expm1(z)),
-// phi(z) = expm1(z) / z, which has no cancellation at A = B.
-template <class R> __device__ __forceinline__ void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
-    const R d = b - a;
-    const R z = k * d;
-    R phi, dphi, em1;
-    if (fabs(z) < R(1e-3)) {
-        phi = R(1) + z * (R(0.5) + z * (R(1) / R(6) + z * (R(1) / R(24))));
-        dphi = R(0.5) + z * (R(1) / R(3) + z * (R(0.125) + z * (R(1) / R(30))));
+// phi(z) = expm1(z) / z and phi'(z) = (z e^z - expm1(z)) / z^2 without the
+// cancellation of that quotient near z = 0 (the numerator is ~z^2/2, formed
+// from terms of size 1): below |z| = 1 (float) or 0.1 (double) both come from
+// their Taylor series through z^10 (phi'(z) = sum_n n z^(n-1) / (n+1)!; the
+// first omitted term is below one ulp at the switch), above it from expm1,
+// where z + expm1(z) (z - 1) loses at most ~5 ulps (at z = -1).
+template <class R> __device__ __forceinline__ void phi_and_slope(R z, R& phi, R& dphi, R& em1) {
+    constexpr R limit = sizeof(R) == 4 ? R(1) : R(0.1);
+    if (fabs(z) < limit) {
+        phi = R(1) + z * (R(1) / R(2) + z * (R(1) / R(6) + z * (R(1) / R(24) + z * (R(1) / R(120) + z * (
+                  R(1) / R(720) + z * (R(1) / R(5040) + z * (R(1) / R(40320) + z * (R(1) / R(362880) + z * (
+                  R(1) / R(3628800) + z * (R(1) / R(39916800)))))))))));
+        dphi = R(1) / R(2) + z * (R(1) / R(3) + z * (R(1) / R(8) + z * (R(1) / R(30) + z * (R(1) / R(144) + z * (
+                   R(1) / R(840) + z * (R(1) / R(5760) + z * (R(1) / R(45360) + z * (R(1) / R(403200) + z * (
+                   R(1) / R(3991680) + z * (R(1) / R(43545600)))))))))));
         em1 = z * phi;
     } else {
         em1 = expm1_dev(z);
         phi = em1 / z;
-        dphi = ((R(1) + em1) * (z - R(1)) + R(1)) / (z * z);
+        dphi = (z + em1 * (z - R(1))) / (z * z);
     }
+}
+
+// dS/dA, dS/dB of S = A' - A for the exact one-step map of dA/dt = dB/dt =
+// -k A B.  With D = B - A and z = k D: A' = A / (1 + s), s = k A phi(z) +
+// expm1(z), phi(z) = expm1(z) / z, which has no cancellation at A = B.
+// Differentiating, with e^z - phi = z phi' and z + k A = k B:
+//   dS/dA = (k^2 A B phi' - k B phi - s^2) / (1 + s)^2
+//   dS/dB = -k A (k A phi' + e^z) / (1 + s)^2,
+// both free of the cancellation that dA'/dA - 1 (a quotient near 1 minus 1)
+// would bring for small k.
+template <class R> __device__ __forceinline__ void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
+    const R d = b - a;
+    const R z = k * d;
+    R phi, dphi, em1;
+    phi_and_slope(z, phi, dphi, em1);
     const R e = R(1) + em1;
-    const R den = R(1) + k * a * phi + em1;
+    const R s = k * a * phi + em1;
+    const R den = R(1) + s;
     const R inv2 = R(1) / (den * den);
-    const R d_fixed = e * inv2;
-    const R d_dd = -a * k * (k * a * dphi + e) * inv2;
-    dsa = d_fixed - d_dd - R(1);
-    dsb = d_dd;
+    dsa = (k * k * a * b * dphi - k * b * phi - s * s) * inv2;
+    dsb = -a * k * (k * a * dphi + e) * inv2;
 }
 
 // alpha = sum lam_q w_q P_q, beta = sum lam_q w_q grad_u P_q: the pieces of the

@@ -67,6 +67,21 @@ def _check_pop(lattice, t, name):
         raise ValueError(f"{name}: unsupported dtype {t.dtype}")
 
 
+def _check_set(named):
+    """Every tensor of one call must cover the same cells with the same storage
+    format on the same device: the kernels take the extents from one tensor and
+    the storage code from one dtype, so a mismatch would read or write past
+    the end of a smaller allocation."""
+    (n0, t0), rest = named[0], named[1:]
+    for n, t in rest:
+        if tuple(t.shape[1:]) != tuple(t0.shape[1:]):
+            raise ValueError(f"{n} covers cells {tuple(t.shape[1:])}, {n0} covers {tuple(t0.shape[1:])}")
+        if t.dtype != t0.dtype:
+            raise ValueError(f"{n} has dtype {t.dtype}, {n0} has {t0.dtype}: one storage format per call")
+        if t.device != t0.device:
+            raise ValueError(f"{n} is on {t.device}, {n0} on {t0.device}")
+
+
 def default_block(lattice, nx, ny, nz=1):
     out = (ctypes.c_uint32 * 3)()
     lib().lbc_default_block(LATTICES[lattice], nx, ny, nz, out)
@@ -87,8 +102,7 @@ def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fas
     (LBC_FLAG_FAST); the default evaluates the reference's operation order."""
     _check_pop(lattice, cur, "cur")
     _check_pop(lattice, nxt, "nxt")
-    if cur.shape != nxt.shape or cur.dtype != nxt.dtype:
-        raise ValueError("cur and nxt must have the same shape and dtype")
+    _check_set([("cur", cur), ("nxt", nxt)])
     nx, ny, nz = _extents(lattice, cur)
     bx, by, bz = _block(lattice, block, nx, ny, nz)
     check(lib().lbc_fluid_step_ex(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
@@ -103,6 +117,7 @@ def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, sou
     _check_pop(scalar_lattice, cur_s, "cur_s")
     _check_pop(scalar_lattice, nxt_s, "nxt_s")
     _check_pop(fluid_lattice, cur_f, "cur_f")
+    _check_set([("cur_s", cur_s), ("nxt_s", nxt_s), ("cur_f", cur_f)])
     nx, ny, nz = _extents(scalar_lattice, cur_s)
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
     check(lib().lbc_scalar_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_s.dtype],
@@ -117,6 +132,7 @@ def reaction_step(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur
     for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
         _check_pop(scalar_lattice, t, n)
     _check_pop(fluid_lattice, cur_f, "cur_f")
+    _check_set([("cur_a", cur_a), ("nxt_a", nxt_a), ("cur_b", cur_b), ("nxt_b", nxt_b), ("cur_f", cur_f)])
     nx, ny, nz = _extents(scalar_lattice, cur_a)
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
     check(lib().lbc_reaction_ab_step(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_a.dtype],
@@ -134,6 +150,8 @@ def reactive_fused_step(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, 
     _check_pop(fluid_lattice, nxt_f, "nxt_f")
     for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
         _check_pop(scalar_lattice, t, n)
+    _check_set([("cur_f", cur_f), ("nxt_f", nxt_f), ("cur_a", cur_a), ("nxt_a", nxt_a), ("cur_b", cur_b),
+                ("nxt_b", nxt_b)])
     nx, ny, nz = _extents(scalar_lattice, cur_a)
     bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
     check(lib().lbc_reactive_fused_step_ex(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
@@ -193,6 +211,7 @@ def reactive_stats(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=
     """compute_reactive_stats on the device; returns a dict; synchronizes."""
     _check_pop(scalar_lattice, a, "a")
     _check_pop(scalar_lattice, b, "b")
+    _check_set([("a", a), ("b", b)])
     nx, ny, nz = _extents(scalar_lattice, a)
     out = (ctypes.c_double * 8)()
     check(lib().lbc_reactive_stats(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx, ny,

@@ -16,7 +16,7 @@ import math
 
 import torch
 
-from . import VELOCITIES, WEIGHTS, _check_pop, _stream, check, default_block, lib, reactive_fused_step
+from . import VELOCITIES, WEIGHTS, _check_pop, _check_set, _stream, check, default_block, lib, reactive_fused_step
 
 _STORE = {torch.float32: 0, torch.float64: 1}
 _COLL = {"BGK": 0, "TRT": 1, "MRT": 2}
@@ -36,7 +36,8 @@ def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, int
     _check_pop("D2Q9", f0, "f0")
     _check_pop("D2Q5", a0, "a0")
     _check_pop("D2Q5", b0, "b0")
-    if f0.dtype not in _STORE or a0.dtype != f0.dtype or b0.dtype != f0.dtype:
+    _check_set([("f0", f0), ("a0", a0), ("b0", b0)])
+    if f0.dtype not in _STORE:
         raise ValueError("f0, a0, b0 must share float32 or float64")
     ny, nx = f0.shape[1:]
     block = default_block("D2Q9", nx, ny) if block is None else block

@@ -1042,26 +1042,43 @@ template <class S, int DT> void scalar_feq_field(const double* conc, const doubl
 // Derivatives of the exact one-step A+B->C map.  With D = B - A (invariant)
 // and z = k D, A' = A / (1 + k A phi(z) + expm1(z)), phi(z) = expm1(z) / z; the
 // form has no cancellation at A = B.  Returns dS/dA and dS/dB of S = A' - A.
-template <class R> void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
-    const R d = b - a;
-    const R z = k * d;
-    R phi, dphi, em1;
-    if (std::fabs(z) < R(1e-3)) {
-        // series: phi = 1 + z/2 + z^2/6 + z^3/24, phi' = 1/2 + z/3 + z^2/8 + z^3/30
-        phi = R(1) + z * (R(0.5) + z * (R(1) / R(6) + z * (R(1) / R(24))));
-        dphi = R(0.5) + z * (R(1) / R(3) + z * (R(0.125) + z * (R(1) / R(30))));
+// phi(z) = expm1(z)/z = sum_n z^n/(n+1)! and phi'(z) = sum_n n z^(n-1)/(n+1)!.
+// The closed form phi' = (z e^z - expm1(z))/z^2 cancels near 0 (its numerator
+// is ~z^2/2), so below |z| = 1 (float) or 0.1 (double) both come from the
+// series through z^10, whose first omitted term is below one ulp there;
+// above, phi' = (z + expm1(z)(z - 1))/z^2 (at most ~5 ulps lost, at z = -1).
+template <class R> void phi_and_slope(R z, R& phi, R& dphi, R& em1) {
+    const R limit = sizeof(R) == 4 ? R(1) : R(0.1);
+    if (std::fabs(z) < limit) {
+        // phi: 1/(m+1)! for z^m, m = 0..10; phi': n/(n+1)! = 1/(n! (n+1)/n) for z^(n-1), n = 1..11
+        static constexpr double phi_den[11] = {1, 2, 6, 24, 120, 720, 5040, 40320, 362880, 3628800, 39916800};
+        static constexpr double slope_den[11] = {2, 3, 8, 30, 144, 840, 5760, 45360, 403200, 3991680, 43545600};
+        phi = R(0);
+        dphi = R(0);
+        for (int m = 10; m >= 0; --m) phi = phi * z + R(1) / static_cast<R>(phi_den[m]);
+        for (int m = 10; m >= 0; --m) dphi = dphi * z + R(1) / static_cast<R>(slope_den[m]);
         em1 = z * phi;
     } else {
         em1 = std::expm1(z);
         phi = em1 / z;
-        dphi = (std::exp(z) * (z - R(1)) + R(1)) / (z * z);
+        dphi = (z + em1 * (z - R(1))) / (z * z);
     }
+}
+
+template <class R> void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
+    const R d = b - a;
+    const R z = k * d;
+    R phi, dphi, em1;
+    phi_and_slope(z, phi, dphi, em1);
+    // A' = A / (1 + s), s = k A phi + expm1(z).  dA'/dA - 1 would subtract 1
+    // from a quotient near 1; with e^z - phi = z phi' and z + k A = k B:
+    //   dS/dA = (k^2 A B phi' - k B phi - s^2) / (1 + s)^2
+    //   dS/dB = -k A (k A phi' + e^z) / (1 + s)^2
     const R e = R(1) + em1;
-    const R den = R(1) + k * a * phi + em1;
-    const R dA_dA_fixedD = e / (den * den);                     // d A' / d A at fixed D
-    const R dA_dD = -a * k * (k * a * dphi + e) / (den * den);  // d A' / d D at fixed A
-    dsa = dA_dA_fixedD - dA_dD - R(1);
-    dsb = dA_dD;
+    const R s = k * a * phi + em1;
+    const R den = R(1) + s;
+    dsa = (k * k * a * b * dphi - k * b * phi - s * s) / (den * den);
+    dsb = -a * k * (k * a * dphi + e) / (den * den);
 }
 
 template <class R, int CT>
@@ -1563,8 +1580,35 @@ int orc_reactive_adjoint_step(int ct, int dt, const void* f, const void* a, cons
     return rc;
 }
 
-int orc_reaction_grad(const double* a, const double* b, double k, double* dsa, double* dsb, long long n) {
-    for (long long j = 0; j < n; ++j) reaction_grad<double>(a[j], b[j], k, dsa[j], dsb[j]);
+int orc_phi_slope(int dt, const double* z, double* phi, double* dphi, long long n) {
+    for (long long j = 0; j < n; ++j) {
+        if (dt == T_F64) {
+            double em1;
+            phi_and_slope<double>(z[j], phi[j], dphi[j], em1);
+        } else if (dt == T_F32) {
+            float p, dp, em1;
+            phi_and_slope<float>(static_cast<float>(z[j]), p, dp, em1);
+            phi[j] = p;
+            dphi[j] = dp;
+        } else {
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int orc_reaction_grad(int dt, const double* a, const double* b, double k, double* dsa, double* dsb, long long n) {
+    if (dt == T_F64) {
+        for (long long j = 0; j < n; ++j) reaction_grad<double>(a[j], b[j], k, dsa[j], dsb[j]);
+        return 0;
+    }
+    if (dt != T_F32) return 1;
+    for (long long j = 0; j < n; ++j) {
+        float sa, sb;
+        reaction_grad<float>(static_cast<float>(a[j]), static_cast<float>(b[j]), static_cast<float>(k), sa, sb);
+        dsa[j] = sa;
+        dsb[j] = sb;
+    }
     return 0;
 }
 

@@ -65,7 +65,8 @@ def lib() -> ctypes.CDLL:
             "orc_scalar_feq_field": [i, i, vp, vp, vp, ll, d],
             "orc_f16_roundtrip": [vp, vp, vp, ll, i],
             "orc_reactive_gradient": [i, i, vp, vp, vp, ll, ll, d, d, d, i, dp, vp, vp, vp, dp],
-            "orc_reaction_grad": [vp, vp, d, vp, vp, ll],
+            "orc_reaction_grad": [i, vp, vp, d, vp, vp, ll],
+            "orc_phi_slope": [i, vp, vp, vp, ll],
             "orc_reactive_adjoint_step": [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, ll, ll, d, d, d, dp],
             "orc_max_threads": [],
         }
@@ -322,9 +323,19 @@ def reactive_gradient(collision, dtype, f0, a0, b0, omega, omega_c, k, steps):
     return J.value, gf, ga, gb, dw.value
 
 
-def reaction_grad(a, b, k):
+def reaction_grad(a, b, k, dtype="f64"):
+    """dS/dA, dS/dB of the exact one-step A + B -> C increment, evaluated in
+    `dtype` ("f64" or "f32"; inputs rounded to it), returned as float64."""
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     da, db = np.empty_like(a), np.empty_like(b)
-    lib().orc_reaction_grad(_p(a), _p(b), float(k), _p(da), _p(db), a.size)
+    _check(lib().orc_reaction_grad(DTYPES[dtype], _p(a), _p(b), float(k), _p(da), _p(db), a.size), "reaction_grad")
     return da, db
+
+
+def phi_slope(z, dtype="f64"):
+    """phi(z) = expm1(z)/z and phi'(z) as the adjoint's reaction Jacobian forms them in `dtype`."""
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    phi, dphi = np.empty_like(z), np.empty_like(z)
+    _check(lib().orc_phi_slope(DTYPES[dtype], _p(z), _p(phi), _p(dphi), z.size), "phi_slope")
+    return phi, dphi

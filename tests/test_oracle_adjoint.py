@@ -29,6 +29,55 @@ def J_of(coll, f, a, b, omega, omega_c, k, steps):
     return a.sum() / (nx * ny) - wa.sum() / (nx * ny)
 
 
+@pytest.mark.parametrize("dtype,ulps", [("f32", 8), ("f64", 32)])
+def test_phi_slope_against_extended_precision(dtype, ulps):
+    """phi(z) = expm1(z)/z and phi'(z) of the reaction Jacobian, against 40-digit
+    mpmath, over |z| in [1e-7, 3] of both signs.  The closed form
+    phi' = (z e^z - expm1(z))/z^2 cancels near 0 (its numerator is ~z^2/2): in
+    float32 it was 7% off at z = 1e-3, the range the 512^2 adjoint's interface
+    cells fall in; the series below the switch and z + expm1(z)(z - 1) above it
+    stay within a few ulps (worst measured: 7 ulps in float32 at z = -1.6, 27 in
+    float64 just past its switch at |z| = 0.1)."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    z = np.concatenate([s * np.logspace(-7, np.log10(3), 300) for s in (1.0, -1.0)])
+    if dtype == "f32":
+        z = z.astype(np.float32).astype(np.float64)
+    eps = 2.0 ** -24 if dtype == "f32" else 2.0 ** -53
+    phi, dphi = O.phi_slope(z, dtype)
+    want_phi = np.array([float(mp.expm1(mp.mpf(x)) / mp.mpf(x)) for x in z])
+    want_dphi = np.array([float((mp.mpf(x) * mp.exp(mp.mpf(x)) - mp.expm1(mp.mpf(x))) / mp.mpf(x) ** 2) for x in z])
+    assert np.max(np.abs(phi - want_phi) / np.abs(want_phi)) <= ulps * eps
+    assert np.max(np.abs(dphi - want_dphi) / np.abs(want_dphi)) <= ulps * eps
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_reaction_map_derivatives_against_extended_precision(dtype):
+    """dS/dA and dS/dB of the exact one-step map, each relative to its own size,
+    against 40-digit mpmath derivatives, at the 512^2 run's k = 0.05, at 0.4 and
+    at 3.  dS/dA is formed without subtracting 1 from dA'/dA (which cost float32
+    ~5e-6 of a -k B sized result at k = 0.05); measured within 5 ulps at
+    k <= 0.4 and 28 at k = 3."""
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    g = cases.rng(12)
+    a = g.uniform(0.0, 1.0, 120).astype(np.float32).astype(np.float64)
+    b = np.clip(1.0 - a + g.normal(0, 0.05, a.size), 1e-3, 1.0).astype(np.float32).astype(np.float64)
+    eps = 2.0 ** -24 if dtype == "f32" else 2.0 ** -53
+    for k, ulps in ((float(np.float32(0.05)), 8), (float(np.float32(0.4)), 8), (3.0, 32)):
+        def S(A, B):
+            z = k * (B - A)
+            return A / (1 + k * A * (mp.expm1(z) / z if z != 0 else 1) + mp.expm1(z)) - A
+
+        wa = np.array([float(mp.diff(lambda x: S(x, mp.mpf(bb)), mp.mpf(aa))) for aa, bb in zip(a, b)])
+        wb = np.array([float(mp.diff(lambda y: S(mp.mpf(aa), y), mp.mpf(bb))) for aa, bb in zip(a, b)])
+        da, db = O.reaction_grad(a, b, k, dtype=dtype)
+        assert np.max(np.abs(da - wa) / np.abs(wa)) <= ulps * eps, k
+        assert np.max(np.abs(db - wb) / np.abs(wb)) <= ulps * eps, k
+
+
 def test_reaction_map_derivatives_against_fd():
     g = cases.rng(11)
     a = np.concatenate([g.uniform(0.01, 1.2, 50), [0.4, 0.7]])

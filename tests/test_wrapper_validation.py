@@ -1,0 +1,95 @@
+"""The Python wrappers refuse tensors that do not belong to one domain and one
+storage format before any kernel is enqueued.
+
+The kernels take the extents from one tensor and the storage code from one
+dtype; a fluid tensor of another shape, or a species stored in another format,
+would be read or written past the end of its allocation.  These checks run on
+the CPU: the CUDA-placement check is relaxed and the kernel library is replaced
+by a stub that fails the test if any entry point is reached.
+"""
+from __future__ import annotations
+
+import pytest
+import torch
+
+import lbcube
+from lbcube import adjoint
+
+
+class _NoKernel:
+    def __getattr__(self, name):
+        if name == "lbc_default_block":
+            return lambda lattice, nx, ny, nz, out: None
+        raise AssertionError(f"{name} was reached with mismatched tensors")
+
+
+@pytest.fixture
+def cpu_wrappers(monkeypatch):
+    def check_pop(lattice, t, name):  # the shape/dtype part of _check_pop, without the CUDA placement
+        if t.dim() != lbcube.DIM[lattice] + 1 or t.shape[0] != lbcube.Q[lattice]:
+            raise ValueError(f"{name}: bad population shape")
+
+    stub = _NoKernel()
+    monkeypatch.setattr(lbcube, "_check_pop", check_pop)
+    monkeypatch.setattr(lbcube, "lib", lambda: stub)
+    monkeypatch.setattr(adjoint, "_check_pop", check_pop)
+    monkeypatch.setattr(adjoint, "lib", lambda: stub)
+    return lbcube
+
+
+def pops(q, shape, dtype=torch.float32):
+    return torch.zeros((q,) + tuple(shape), dtype=dtype)
+
+
+def test_fluid_pair_must_match(cpu_wrappers):
+    with pytest.raises(ValueError, match="covers cells"):
+        lbcube.fluid_step("D3Q19", "BGK", pops(19, (4, 4, 4)), pops(19, (4, 4, 5)), 1.0, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="storage format"):
+        lbcube.fluid_step("D3Q19", "BGK", pops(19, (4, 4, 4)), pops(19, (4, 4, 4), torch.float16), 1.0,
+                          block=(64, 1, 1))
+
+
+def test_scalar_step_fluid_of_another_shape(cpu_wrappers):
+    s = pops(5, (8, 8))
+    with pytest.raises(ValueError, match="cur_f covers cells"):
+        lbcube.scalar_step("D2Q9", "D2Q5", s, s.clone(), pops(9, (8, 16)), 1.0, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="storage format"):
+        lbcube.scalar_step("D2Q9", "D2Q5", s, s.clone(), pops(9, (8, 8), torch.float64), 1.0, block=(64, 1, 1))
+
+
+def test_reaction_step_mismatches(cpu_wrappers):
+    a, f = pops(5, (8, 8)), pops(9, (8, 8))
+    with pytest.raises(ValueError, match="nxt_b covers cells"):
+        lbcube.reaction_step("D2Q9", "D2Q5", a, a.clone(), a.clone(), pops(5, (8, 4)), f, 1.0, 0.1,
+                             block=(64, 1, 1))
+    with pytest.raises(ValueError, match="cur_f covers cells"):
+        lbcube.reaction_step("D2Q9", "D2Q5", a, a.clone(), a.clone(), a.clone(), pops(9, (4, 8)), 1.0, 0.1,
+                             block=(64, 1, 1))
+    with pytest.raises(ValueError, match="storage format"):
+        lbcube.reaction_step("D2Q9", "D2Q5", a, a.clone(), pops(5, (8, 8), torch.float16), a.clone(), f, 1.0, 0.1,
+                             block=(64, 1, 1))
+
+
+def test_fused_step_mismatches(cpu_wrappers):
+    f, a = pops(9, (16, 16)), pops(5, (16, 16))
+    with pytest.raises(ValueError, match="storage format"):  # f16 species with an f32 fluid
+        lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", f, f.clone(), pops(5, (16, 16), torch.float16),
+                                   pops(5, (16, 16), torch.float16), a, a.clone(), 1.0, 1.0, 0.1, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="cur_a covers cells"):  # species smaller than the fluid
+        lbcube.reactive_fused_step("D2Q9", "BGK", "D2Q5", f, f.clone(), pops(5, (8, 8)), pops(5, (8, 8)), a,
+                                   a.clone(), 1.0, 1.0, 0.1, block=(64, 1, 1))
+
+
+def test_stats_pair_must_match(cpu_wrappers):
+    with pytest.raises(ValueError, match="covers cells"):
+        lbcube.reactive_stats("D2Q5", pops(5, (8, 8)), pops(5, (8, 9)), 0.1)
+
+
+def test_gradient_species_shape(cpu_wrappers):
+    f = pops(9, (64, 64))
+    with pytest.raises(ValueError, match="a0 covers cells"):
+        adjoint.reactive_gradient("BGK", f, pops(5, (32, 64)), pops(5, (64, 64)), 8, 1.0, 1.0, 0.1,
+                                  block=(256, 1))
+    with pytest.raises(ValueError, match="storage format"):
+        adjoint.reactive_gradient("BGK", f, pops(5, (64, 64)), pops(5, (64, 64), torch.float64), 8, 1.0, 1.0, 0.1,
+                                  block=(256, 1))
