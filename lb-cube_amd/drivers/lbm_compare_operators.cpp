@@ -37,7 +37,9 @@
 #include <string>
 #include <vector>
 
+#include "initial_states.hpp"
 #include "lattice_hip.hpp"
+#include "lbcube_lattices.hpp"
 #include "vtk.hpp"
 
 namespace {
@@ -124,13 +126,6 @@ Config parse(int argc, char** argv) {
     return c;
 }
 
-constexpr int FX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-constexpr int FY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-constexpr double FW[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
-constexpr int SX[5] = {0, 1, 0, -1, 0};
-constexpr int SY[5] = {0, 0, 1, 0, -1};
-constexpr double SW[5] = {1.0 / 3, 1.0 / 6, 1.0 / 6, 1.0 / 6, 1.0 / 6};
-
 // Host populations of one case: fluid [9][ny][nx], species [5][ny][nx].
 struct HostState {
     std::vector<Real> f, a, b;
@@ -140,25 +135,18 @@ struct HostState {
 // (src/compare_operators.cpp:64-118).
 HostState initial_state(const Config& c) {
     const std::size_t n = c.nx * c.ny;
-    HostState s{std::vector<Real>(9 * n), std::vector<Real>(5 * n), std::vector<Real>(5 * n)};
-    const double pi = std::acos(-1.0);
-    const double kx = 2 * pi / c.nx, ky = 2 * pi / c.ny;
+    HostState s{std::vector<Real>(FluidLattice::Q * n), std::vector<Real>(ScalarLattice::Q * n),
+                std::vector<Real>(ScalarLattice::Q * n)};
     for (std::size_t y = 0; y < c.ny; ++y)
         for (std::size_t x = 0; x < c.nx; ++x) {
-            const double px = kx * x, py = ky * y;
-            const double rho = 1.0 - (c.u0 * c.u0) / (4.0 / 3.0) * (std::cos(2 * px) + std::cos(2 * py));
-            const double ux = -c.u0 * std::cos(px) * std::sin(py), uy = c.u0 * std::sin(px) * std::cos(py);
-            const double uu = ux * ux + uy * uy;
+            const lbc_init::Macro m = lbc_init::taylor_green_2d(x, y, c.nx, c.ny, c.u0);
             const std::size_t cell = y * c.nx + x;
-            for (int i = 0; i < 9; ++i) {
-                const double cu = FX[i] * ux + FY[i] * uy;
-                s.f[i * n + cell] = FW[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-            }
-            const double ca = x < c.nx / 2 ? 1.0 : 0.0;
-            for (int i = 0; i < 5; ++i) {
-                const double e = 1.0 + 3.0 * (SX[i] * ux + SY[i] * uy);
-                s.a[i * n + cell] = SW[i] * ca * e;
-                s.b[i * n + cell] = SW[i] * (1.0 - ca) * e;
+            for (int i = 0; i < FluidLattice::Q; ++i)
+                s.f[i * n + cell] = lbc_init::equilibrium<FluidLattice>(i, m.rho, m.u);
+            const double ca = x < c.nx / 2 ? 1.0 : 0.0, cb = x < c.nx / 2 ? 0.0 : 1.0;
+            for (int i = 0; i < ScalarLattice::Q; ++i) {
+                s.a[i * n + cell] = lbc_init::scalar_equilibrium<ScalarLattice>(i, ca, m.u);
+                s.b[i * n + cell] = lbc_init::scalar_equilibrium<ScalarLattice>(i, cb, m.u);
             }
         }
     return s;
@@ -203,7 +191,7 @@ void run_case(const Config& c, const HostState& init) {
 
     auto log = [&](int s) {
         double out[8];
-        check(static_cast<hipError_t>(lbc_reactive_stats(ScalarLattice::abi, LBC_F64, a.cur, b.cur, c.nx, c.ny, 1,
+        check(static_cast<hipError_t>(lbc_reactive_stats(lbm::lattice_code<ScalarLattice>(), LBC_F64, a.cur, b.cur, c.nx, c.ny, 1,
                                                          c.k_react, 0.0, 0.0, out, nullptr)),
               "lbc_reactive_stats");
         char row[400];
@@ -215,9 +203,9 @@ void run_case(const Config& c, const HostState& init) {
     auto snapshot = [&](int s) {
         char file[96];
         std::snprintf(file, sizeof(file), "comparison_%s_%06d.vtk", name, s);
-        const auto fields = lbc_vtk::fluid_fields(staging, FluidLattice::abi, f.cur, c.nx, c.ny, 1);
-        const auto ca = lbc_vtk::concentration(staging, staging.ca, ScalarLattice::abi, a.cur, c.nx, c.ny, 1);
-        const auto cb = lbc_vtk::concentration(staging, staging.cb, ScalarLattice::abi, b.cur, c.nx, c.ny, 1);
+        const auto fields = lbc_vtk::fluid_fields(staging, lbm::lattice_code<FluidLattice>(), f.cur, c.nx, c.ny, 1);
+        const auto ca = lbc_vtk::concentration(staging, staging.ca, lbm::lattice_code<ScalarLattice>(), a.cur, c.nx, c.ny, 1);
+        const auto cb = lbc_vtk::concentration(staging, staging.cb, lbm::lattice_code<ScalarLattice>(), b.cur, c.nx, c.ny, 1);
         lbc_vtk::write_reactive((dir / file).string(),
                                 std::string("LB-Cube ") + name + " comparison step " + std::to_string(s), true, fields,
                                 ca, cb, c.k_react, c.nx, c.ny);
@@ -229,7 +217,7 @@ void run_case(const Config& c, const HostState& init) {
     for (int s = 1; s <= c.steps; ++s) {
         if (c.fused) {
             check(static_cast<hipError_t>(lbc_reactive_fused_step(
-                      FluidLattice::abi, static_cast<int>(CT), ScalarLattice::abi, LBC_F64, f.cur, f.next, a.cur,
+                      lbm::lattice_code<FluidLattice>(), static_cast<int>(CT), lbm::lattice_code<ScalarLattice>(), LBC_F64, f.cur, f.next, a.cur,
                       a.next, b.cur, b.next, c.nx, c.ny, 1, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
                   "lbc_reactive_fused_step");
             std::swap(f.cur, f.next);

@@ -31,7 +31,9 @@
 #include <string>
 #include <vector>
 
+#include "initial_states.hpp"
 #include "lattice_hip.hpp"
+#include "lbcube_lattices.hpp"
 #include "vtk.hpp"
 
 namespace {
@@ -94,34 +96,18 @@ Config parse(int argc, char** argv) {
     return c;
 }
 
-constexpr int FX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-constexpr int FY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-constexpr double FW[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
-constexpr int SX[5] = {0, 1, 0, -1, 0};
-constexpr int SY[5] = {0, 0, 1, 0, -1};
-constexpr double SW[5] = {1.0 / 3, 1.0 / 6, 1.0 / 6, 1.0 / 6, 1.0 / 6};
-
 // 2D Taylor-Green fluid and species split at x = nx / 2, at equilibrium
-// (src/simulate_reaction.cpp:173-232).
-void initialize(lbm::LatticeMemory<FluidLattice, Real>& f, lbm::LatticeMemory<ScalarLattice, Real>& a,
-                lbm::LatticeMemory<ScalarLattice, Real>& b, const Config& c) {
-    const double pi = std::acos(-1.0);
-    const double kx = 2 * pi / c.nx, ky = 2 * pi / c.ny;
+// (src/simulate_reaction.cpp:173-232), in the reference's operation order.
+void initialize(lbm::HostLattice<FluidLattice, Real>& f, lbm::HostLattice<ScalarLattice, Real>& a,
+                lbm::HostLattice<ScalarLattice, Real>& b, const Config& c) {
     for (std::size_t y = 0; y < c.ny; ++y)
         for (std::size_t x = 0; x < c.nx; ++x) {
-            const double px = kx * x, py = ky * y;
-            const double rho = 1.0 - (c.u0 * c.u0) / (4.0 / 3.0) * (std::cos(2 * px) + std::cos(2 * py));
-            const double ux = -c.u0 * std::cos(px) * std::sin(py), uy = c.u0 * std::sin(px) * std::cos(py);
-            const double uu = ux * ux + uy * uy;
-            for (int i = 0; i < 9; ++i) {
-                const double cu = FX[i] * ux + FY[i] * uy;
-                f.at(i, 0, y, x) = FW[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-            }
-            const double ca = x < c.nx / 2 ? 1.0 : 0.0;
-            for (int i = 0; i < 5; ++i) {
-                const double e = 1.0 + 3.0 * (SX[i] * ux + SY[i] * uy);
-                a.at(i, 0, y, x) = SW[i] * ca * e;
-                b.at(i, 0, y, x) = SW[i] * (1.0 - ca) * e;
+            const lbc_init::Macro m = lbc_init::taylor_green_2d(x, y, c.nx, c.ny, c.u0);
+            for (int i = 0; i < FluidLattice::Q; ++i) f(i, 0, y, x) = lbc_init::equilibrium<FluidLattice>(i, m.rho, m.u);
+            const double ca = x < c.nx / 2 ? 1.0 : 0.0, cb = x < c.nx / 2 ? 0.0 : 1.0;
+            for (int i = 0; i < ScalarLattice::Q; ++i) {
+                a(i, 0, y, x) = lbc_init::scalar_equilibrium<ScalarLattice>(i, ca, m.u);
+                b(i, 0, y, x) = lbc_init::scalar_equilibrium<ScalarLattice>(i, cb, m.u);
             }
         }
 }
@@ -141,7 +127,7 @@ struct Device {
     Device& operator=(const Device&) = delete;
 };
 
-template <class L> void read_raw(lbm::LatticeMemory<L, Real>& m, const std::string& path) {
+template <class L> void read_raw(lbm::HostLattice<L, Real>& m, const std::string& path) {
     std::ifstream in(path, std::ios::binary);
     if (!in.read(reinterpret_cast<char*>(m.current()), m.population_count() * sizeof(Real)))
         throw std::runtime_error("cannot read " + path);
@@ -161,8 +147,8 @@ int main(int argc, char** argv) {
     try {
         const Config c = parse(argc, argv);
         const Real omega = 1.0 / c.tau, omega_c = 1.0 / c.tau_c;
-        lbm::LatticeMemory<FluidLattice, Real> fluid{c.nx, c.ny};
-        lbm::LatticeMemory<ScalarLattice, Real> species_a{c.nx, c.ny}, species_b{c.nx, c.ny};
+        lbm::HostLattice<FluidLattice, Real> fluid{c.nx, c.ny};
+        lbm::HostLattice<ScalarLattice, Real> species_a{c.nx, c.ny}, species_b{c.nx, c.ny};
         if (c.load.empty()) {
             initialize(fluid, species_a, species_b, c);
         } else {
@@ -186,7 +172,7 @@ int main(int argc, char** argv) {
 
         auto log = [&](int s) {
             double out[8];
-            check(static_cast<hipError_t>(lbc_reactive_stats(ScalarLattice::abi, LBC_F64, a.cur, b.cur, c.nx, c.ny,
+            check(static_cast<hipError_t>(lbc_reactive_stats(lbm::lattice_code<ScalarLattice>(), LBC_F64, a.cur, b.cur, c.nx, c.ny,
                                                              1, c.k_react, 0.0, 0.0, out, nullptr)),
                   "lbc_reactive_stats");
             char row[400];
@@ -199,9 +185,9 @@ int main(int argc, char** argv) {
             if (!c.vtk) return;
             char name[40];
             std::snprintf(name, sizeof(name), "/reaction_%06d.vtk", s);
-            const auto fields = lbc_vtk::fluid_fields(staging, FluidLattice::abi, f.cur, c.nx, c.ny, 1);
-            const auto ca = lbc_vtk::concentration(staging, staging.ca, ScalarLattice::abi, a.cur, c.nx, c.ny, 1);
-            const auto cb = lbc_vtk::concentration(staging, staging.cb, ScalarLattice::abi, b.cur, c.nx, c.ny, 1);
+            const auto fields = lbc_vtk::fluid_fields(staging, lbm::lattice_code<FluidLattice>(), f.cur, c.nx, c.ny, 1);
+            const auto ca = lbc_vtk::concentration(staging, staging.ca, lbm::lattice_code<ScalarLattice>(), a.cur, c.nx, c.ny, 1);
+            const auto cb = lbc_vtk::concentration(staging, staging.cb, lbm::lattice_code<ScalarLattice>(), b.cur, c.nx, c.ny, 1);
             lbc_vtk::write_reactive(c.vtk_dir + name, "LB-Cube reactive mixing step " + std::to_string(s), false,
                                     fields, ca, cb, c.k_react, c.nx, c.ny);
         };
@@ -211,7 +197,7 @@ int main(int argc, char** argv) {
         for (int s = 1; s <= c.steps; ++s) {
             if (c.fused) {
                 check(static_cast<hipError_t>(lbc_reactive_fused_step(
-                          FluidLattice::abi, LBC_BGK, ScalarLattice::abi, LBC_F64, f.cur, f.next, a.cur, a.next, b.cur,
+                          lbm::lattice_code<FluidLattice>(), LBC_BGK, lbm::lattice_code<ScalarLattice>(), LBC_F64, f.cur, f.next, a.cur, a.next, b.cur,
                           b.next, c.nx, c.ny, 1, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
                       "lbc_reactive_fused_step");
                 std::swap(f.cur, f.next);

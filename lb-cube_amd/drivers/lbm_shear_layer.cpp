@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "lattice_hip.hpp"
+#include "lbcube_lattices.hpp"
 #include "vtk.hpp"
 
 namespace {
@@ -208,9 +209,9 @@ template <CollisionType CT> void run_case(const Config& c, const HostState& init
         const std::string path = (dir / file).string();
         std::ofstream out(path);
         if (!out) throw std::runtime_error("failed to open " + path);
-        const auto fl = lbc_vtk::fluid_fields(staging, FluidLattice::abi, f.cur, c.nx, c.ny, 1);
-        const auto ca = lbc_vtk::concentration(staging, staging.ca, ScalarLattice::abi, a.cur, c.nx, c.ny, 1);
-        const auto cb = lbc_vtk::concentration(staging, staging.cb, ScalarLattice::abi, b.cur, c.nx, c.ny, 1);
+        const auto fl = lbc_vtk::fluid_fields(staging, lbm::lattice_code<FluidLattice>(), f.cur, c.nx, c.ny, 1);
+        const auto ca = lbc_vtk::concentration(staging, staging.ca, lbm::lattice_code<ScalarLattice>(), a.cur, c.nx, c.ny, 1);
+        const auto cb = lbc_vtk::concentration(staging, staging.cb, lbm::lattice_code<ScalarLattice>(), b.cur, c.nx, c.ny, 1);
         lbc_vtk::header(out, "LB-Cube turbulent shear layer " + name + " step " + std::to_string(s), c.nx, c.ny, 1);
         lbc_vtk::vectors(out, "velocity", fl.u, cells, 2);
         lbc_vtk::scalars(out, "velocity_magnitude", cells,
@@ -220,7 +221,7 @@ template <CollisionType CT> void run_case(const Config& c, const HostState& init
     };
     auto max_speed = [&]() {
         double v = 0.0;
-        lbc_vtk::check(lbc_max_speed(FluidLattice::abi, LBC_F64, f.cur, c.nx, c.ny, 1, &v, nullptr), "lbc_max_speed");
+        lbc_vtk::check(lbc_max_speed(lbm::lattice_code<FluidLattice>(), LBC_F64, f.cur, c.nx, c.ny, 1, &v, nullptr), "lbc_max_speed");
         return v;
     };
 

@@ -1,9 +1,12 @@
 // lbm_sim: the GPU path of LB-Cube's lbm_sim driver (src/main.cpp) on the
 // MI355X kernel library, written against include/lattice_hip.hpp exactly as
 // the reference's driver is written against lattice_cuda.cuh: host populations
-// in a LatticeMemory, device buffers from hipMalloc, one launch_step_gpu per
-// step and a pointer swap, a VTK snapshot and a diagnostics row at step 0 and
-// every -out_freq steps (src/main.cpp:572-607).
+// in a LatticeMemory-shaped container, device buffers from hipMalloc, one
+// launch_step_gpu per step and a pointer swap, a VTK snapshot and a
+// diagnostics row at step 0 and every -out_freq steps (src/main.cpp:572-607).
+// Without -load the initial state is the reference's, bit for bit: the rest
+// state or the 3D Taylor-Green vortex with its density perturbation
+// (src/main.cpp:295-411), populations from compute_equilibrium.
 //
 // Options (src/main.cpp:149-195, same names and checks):
 //   -nx N -ny N -nz N      grid (default 64 x 64 x 64)
@@ -35,7 +38,9 @@
 #include <string>
 #include <vector>
 
+#include "initial_states.hpp"
 #include "lattice_hip.hpp"
+#include "lbcube_lattices.hpp"
 #include "vtk.hpp"
 
 namespace {
@@ -99,32 +104,13 @@ Config parse(int argc, char** argv) {
     return c;
 }
 
-// Second-order equilibrium w_i rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u).
-constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
-constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
-constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
-double weight(int i) { return i == 0 ? 1.0 / 3.0 : (i < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
-
-void initialize(lbm::LatticeMemory<Lattice, Real>& mem, const Config& c) {
-    const double pi = std::acos(-1.0);
-    const double kx = 2 * pi / c.nx, ky = 2 * pi / c.ny, kz = 2 * pi / c.nz;
+void initialize(lbm::HostLattice<Lattice, Real>& mem, const Config& c) {
     for (std::size_t z = 0; z < c.nz; ++z)
         for (std::size_t y = 0; y < c.ny; ++y)
             for (std::size_t x = 0; x < c.nx; ++x) {
-                double rho = 1.0, ux = 0.0, uy = 0.0;
-                if (c.init == "taylor_green") {
-                    // the 3D Taylor-Green state of src/main.cpp:371-411
-                    const double px = kx * x, py = ky * y, pz = kz * z;
-                    rho = 1.0 + (c.u0 * c.u0) / (16.0 / 3.0) * (std::cos(2 * px) + std::cos(2 * py)) *
-                                    (std::cos(2 * pz) + 2.0);
-                    ux = c.u0 * std::sin(px) * std::cos(py) * std::cos(pz);
-                    uy = -c.u0 * std::cos(px) * std::sin(py) * std::cos(pz);
-                }
-                const double uu = ux * ux + uy * uy;
-                for (int i = 0; i < Lattice::Q; ++i) {
-                    const double cu = CX[i] * ux + CY[i] * uy;
-                    mem.at(i, z, y, x) = weight(i) * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-                }
+                lbc_init::Macro m{1.0, {0.0, 0.0, 0.0}};
+                if (c.init == "taylor_green") m = lbc_init::taylor_green_3d(x, y, z, c.nx, c.ny, c.nz, c.u0);
+                for (int i = 0; i < Lattice::Q; ++i) mem(i, z, y, x) = lbc_init::equilibrium<Lattice>(i, m.rho, m.u);
             }
 }
 
@@ -146,7 +132,7 @@ int main(int argc, char** argv) {
         else if (c.collision == "rlbm") launch = step<lbm::CollisionType::RLBM>;
         else throw std::invalid_argument("-collision must be bgk, trt, mrt or rlbm");
 
-        lbm::LatticeMemory<Lattice, Real> memory{c.nx, c.ny, c.nz};
+        lbm::HostLattice<Lattice, Real> memory{c.nx, c.ny, c.nz};
         const std::size_t count = memory.population_count();
         if (c.load.empty()) {
             initialize(memory, c);
@@ -172,7 +158,7 @@ int main(int argc, char** argv) {
 
         auto log = [&](int s) {
             double out[3];
-            check(static_cast<hipError_t>(lbc_fluid_diagnostics(Lattice::abi, LBC_F64, current, c.nx, c.ny, c.nz, out,
+            check(static_cast<hipError_t>(lbc_fluid_diagnostics(lbm::lattice_code<Lattice>(), LBC_F64, current, c.nx, c.ny, c.nz, out,
                                                                 nullptr)),
                   "lbc_fluid_diagnostics");
             char row[160];
@@ -183,7 +169,7 @@ int main(int argc, char** argv) {
         auto snapshot = [&](int s) {
             if (!c.vtk) return;
             lbc_vtk::write_fluid(c.vtk_dir, static_cast<std::size_t>(s),
-                                 lbc_vtk::fluid_fields(staging, Lattice::abi, current, c.nx, c.ny, c.nz), c.nx, c.ny,
+                                 lbc_vtk::fluid_fields(staging, lbm::lattice_code<Lattice>(), current, c.nx, c.ny, c.nz), c.nx, c.ny,
                                  c.nz, Lattice::D);
         };
         snapshot(0);

@@ -341,3 +341,124 @@ def test_shear_layer_guard_writes_last_snapshot(gpu, tmp_path):
     assert "(terminated early)" in r.stdout
     out = tmp_path / "output_shear2d_D2Q9_MRT"
     assert sorted(p.name for p in out.glob("*.vtk")) == ["shear_D2Q9_MRT_000000.vtk", "shear_D2Q9_MRT_000010.vtk"]
+
+
+# ---------------------------------------------------------------------------
+# The drivers' own initial conditions (no -load)
+# ---------------------------------------------------------------------------
+def reference_tgv_3d(nx, ny, nz, u0):
+    """initialize_taylor_green_state_3d (src/main.cpp:371-411) restated: the
+    trigonometric values come from the C library (math.sin / math.cos, as
+    std::sin / std::cos), each product and sum in the reference's order;
+    populations from the oracle's compute_equilibrium."""
+    import math
+
+    cs2 = 1.0 / 3.0
+    k = [2.0 * math.pi / n for n in (nx, ny, nz)]
+    ph = [[kk * float(i) for i in range(n)] for kk, n in zip(k, (nx, ny, nz))]
+    sx, cx, c2x = (np.array([f(p) for p in ph[0]]) for f in (math.sin, math.cos, lambda p: math.cos(2.0 * p)))
+    sy, cy, c2y = (np.array([f(p) for p in ph[1]]) for f in (math.sin, math.cos, lambda p: math.cos(2.0 * p)))
+    sz, cz, c2z = (np.array([f(p) for p in ph[2]]) for f in (math.sin, math.cos, lambda p: math.cos(2.0 * p)))
+    pert = (u0 * u0) / (16.0 * cs2) * (c2x[None, None, :] + c2y[None, :, None]) * (c2z[:, None, None] + 2.0)
+    rho = 1.0 + pert
+    ux = u0 * sx[None, None, :] * cy[None, :, None] * cz[:, None, None]
+    uy = -u0 * cx[None, None, :] * sy[None, :, None] * cz[:, None, None]
+    u = np.stack([ux, uy, np.zeros_like(ux)])
+    return O.feq_field("D3Q19", "f64", rho, u)
+
+
+def reference_tgv_2d_split(nx, ny, u0):
+    """taylor_green_macro + initialize_fields (src/simulate_reaction.cpp:173-232)
+    restated the same way: fluid and both species at equilibrium."""
+    import math
+
+    cs2 = 1.0 / 3.0
+    px = [2.0 * math.pi / nx * float(x) for x in range(nx)]
+    py = [2.0 * math.pi / ny * float(y) for y in range(ny)]
+    cx, sx, c2x = (np.array([f(p) for p in px]) for f in (math.cos, math.sin, lambda p: math.cos(2.0 * p)))
+    cy, sy, c2y = (np.array([f(p) for p in py]) for f in (math.cos, math.sin, lambda p: math.cos(2.0 * p)))
+    rho = 1.0 - (u0 * u0) / (4.0 * cs2) * (c2x[None, :] + c2y[:, None])
+    u = np.stack([-u0 * cx[None, :] * sy[:, None], u0 * sx[None, :] * cy[:, None]])
+    ca = np.broadcast_to((np.arange(nx) < nx // 2).astype(np.float64), (ny, nx))
+    return (O.feq_field("D2Q9", "f64", rho, u), O.scalar_feq_field("D2Q5", "f64", ca, u),
+            O.scalar_feq_field("D2Q5", "f64", 1.0 - ca, u))
+
+
+@pytest.mark.gpu
+def test_lbm_sim_reference_default_run(gpu, tmp_path):
+    """lbm_sim without -load at the reference's literal default: 64^3, tau = 0.8,
+    u0 = 0.01, the density-perturbed Taylor-Green vortex, BGK, 1000 steps,
+    diagnostics every 100 (src/main.cpp:149-195, :368-411, :542).  The initial
+    populations, every diagnostics row and the final populations must be the
+    oracle's, started from the restated initial state (populations bit for
+    bit; FP64 sums to 1e-13)."""
+    print("case lbm_sim reference default, 64^3 x 1000 steps", flush=True)
+    want0 = reference_tgv_3d(64, 64, 64, 0.01)
+    r = run("-steps", 0, "-dump", tmp_path / "init.bin", "-csv", tmp_path / "d0.csv", "-no_vtk", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    got0 = np.fromfile(tmp_path / "init.bin", dtype=np.float64).reshape(want0.shape)
+    assert np.array_equal(got0, want0)
+    r = run("-dump", tmp_path / "out.bin", "-csv", tmp_path / "diag.csv", "-no_vtk", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    rows = (tmp_path / "diag.csv").read_text().strip().split("\n")
+    assert len(rows) == 1 + 11
+    state = want0
+    threads = O.max_threads()
+    for k, row in enumerate(rows[1:]):
+        step, mass, ke, umax = row.split(",")
+        assert int(step) == 100 * k
+        ref = O.fluid_diagnostics("D3Q19", "f64", state, wide=True)
+        assert np.allclose([float(mass), float(ke), float(umax)], ref, rtol=1e-13, atol=0), k
+        if k < 10:
+            state = O.fluid_run("D3Q19", "BGK", "f64", state, 1.0 / 0.8, 100, threads=threads)
+    got = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(want0.shape)
+    assert np.array_equal(got, state)
+
+
+@pytest.mark.gpu
+def test_lbm_sim_rest_initial_state(gpu, tmp_path):
+    """-init rest: every cell at w_i (initialize_rest_state, src/main.cpp:295-312),
+    a fixed point of the step."""
+    r = run("-nx", 12, "-ny", 10, "-nz", 8, "-init", "rest", "-steps", 3, "-dump", tmp_path / "out.bin", "-csv",
+            tmp_path / "d.csv", "-no_vtk", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(19, 8, 10, 12)
+    w = np.asarray(O.WEIGHTS["D3Q19"]).reshape(19, 1, 1, 1)
+    assert np.array_equal(got, np.broadcast_to(w, got.shape))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", [False, True])
+def test_reaction_sim_own_initial_state(gpu, tmp_path, fused):
+    """lbm_reaction_sim without -load: the 2D Taylor-Green fluid and the species
+    split at x = nx/2 (src/simulate_reaction.cpp:173-232) at the reference's
+    defaults (u0 = 0.04, tau = 0.6, tau_c = 0.8, k = 0.05) on a 64 x 48 grid.
+    The initial populations bit for bit; after 200 steps the fluid bit for bit
+    and the species within exp() ulps; every statistics row against the
+    oracle."""
+    print(f"case lbm_reaction_sim own initial state fused={fused}", flush=True)
+    nx, ny, steps, freq = 64, 48, 200, 50
+    want0 = reference_tgv_2d_split(nx, ny, 0.04)
+    common = ["-nx", nx, "-ny", ny, "-no_vtk", *(["-fused"] if fused else [])]
+    r = run(*common, "-steps", 0, "-dump", tmp_path / "init", "-csv", tmp_path / "s0.csv", cwd=tmp_path,
+            exe=REACT_EXE)
+    assert r.returncode == 0, r.stderr
+    for suffix, w in zip("fab", want0):
+        assert np.array_equal(np.fromfile(tmp_path / f"init.{suffix}", dtype=np.float64).reshape(w.shape), w), suffix
+    r = run(*common, "-steps", steps, "-out_freq", freq, "-dump", tmp_path / "out", "-csv", tmp_path / "stats.csv",
+            cwd=tmp_path, exe=REACT_EXE)
+    assert r.returncode == 0, r.stderr
+    rows = (tmp_path / "stats.csv").read_text().strip().split("\n")
+    assert len(rows) == 1 + steps // freq + 1
+    state = want0
+    for j, row in enumerate(rows[1:]):
+        vals = row.split(",")
+        assert int(vals[0]) == j * freq
+        ref = O.reactive_stats("D2Q5", "f64", state[1], state[2], 0.05, wide=True)
+        assert np.allclose([float(v) for v in vals[1:]], ref, rtol=1e-10, atol=1e-14), j
+        if j * freq < steps:
+            state = O.reactive_run("D2Q9", "BGK", "D2Q5", "f64", *state, 1 / 0.6, 1 / 0.8, 0.05, freq)
+    got = [np.fromfile(tmp_path / f"out.{s}", dtype=np.float64).reshape(w.shape) for s, w in zip("fab", want0)]
+    assert np.array_equal(got[0], state[0])
+    for g, w in zip(got[1:], state[1:]):
+        assert np.max(np.abs(g - w)) <= 1e-13
