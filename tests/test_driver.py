@@ -417,14 +417,21 @@ def test_lbm_sim_reference_default_run(gpu, tmp_path):
 
 @pytest.mark.gpu
 def test_lbm_sim_rest_initial_state(gpu, tmp_path):
-    """-init rest: every cell at w_i (initialize_rest_state, src/main.cpp:295-312),
-    a fixed point of the step."""
-    r = run("-nx", 12, "-ny", 10, "-nz", 8, "-init", "rest", "-steps", 3, "-dump", tmp_path / "out.bin", "-csv",
-            tmp_path / "d.csv", "-no_vtk", cwd=tmp_path)
+    """-init rest: every cell at feq(1, 0) = w_i (initialize_rest_state,
+    src/main.cpp:295-312), then three steps as the oracle takes them (the
+    density sums the rounded weights, so the state moves by ulps)."""
+    common = ["-nx", 12, "-ny", 10, "-nz", 8, "-init", "rest", "-csv", tmp_path / "d.csv", "-no_vtk"]
+    r = run(*common, "-steps", 0, "-dump", tmp_path / "init.bin", cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    got0 = np.fromfile(tmp_path / "init.bin", dtype=np.float64).reshape(19, 8, 10, 12)
+    want0 = O.feq_field("D3Q19", "f64", np.ones((8, 10, 12)), np.zeros((3, 8, 10, 12)))
+    w = np.asarray(O.WEIGHTS["D3Q19"]).reshape(19, 1, 1, 1)
+    assert np.array_equal(want0, np.broadcast_to(w, want0.shape)) and np.array_equal(got0, want0)
+    r = run(*common, "-steps", 3, "-dump", tmp_path / "out.bin", cwd=tmp_path)
     assert r.returncode == 0, r.stderr
     got = np.fromfile(tmp_path / "out.bin", dtype=np.float64).reshape(19, 8, 10, 12)
-    w = np.asarray(O.WEIGHTS["D3Q19"]).reshape(19, 1, 1, 1)
-    assert np.array_equal(got, np.broadcast_to(w, got.shape))
+    want = O.fluid_run("D3Q19", "BGK", "f64", want0, 1.0 / 0.8, 3)
+    assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
 
 
 @pytest.mark.gpu
