@@ -219,6 +219,21 @@ int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, c
                        uint64_t ny, uint64_t nz, double k_react, double reference_a, double reference_b,
                        double* out8, void* stream);
 
+/* lbc_reactive_stats without any device allocation: same arguments, kernels,
+ * validation and results (bit for bit), but the per-block partial sums live
+ * in a static device array of the library (65,600 bytes per device, the same
+ * for every domain size; the domain limits are lbc_reactive_stats') instead
+ * of a workspace allocated on the first call.  It calls no hipMalloc or
+ * hipFree and allocates nothing else on the device.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code in a tight
+ * serving or sampling loop, harnesses that forbid allocation after set-up).
+ * All calls on one device share the array; calls from several host threads
+ * of one process are serialized like lbc_reactive_stats, while calls from
+ * several processes on one device that overlap race on it. */
+int lbc_reactive_stats_noalloc(int scalar_lattice, int storage, const void* species_a, const void* species_b,
+                               uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double reference_a,
+                               double reference_b, double* out8, void* stream);
+
 /* Stability guard of compute_max_velocity (src/turbulent_tgv_3d.cpp:149-173):
  * out[0] = max |u| over the cells, +infinity if any cell's speed is not
  * finite.  Synchronizes `stream`; shares the workspace and serialization of

@@ -205,6 +205,34 @@ double* workspace(hipError_t& err) {
     return ws[dev];
 }
 
+// Scratch of lbc_reactive_stats_noalloc: the same kStatBlocks partial slots
+// plus result slots as the workspace above, as a static device array
+// (65,600 bytes on every device the library runs on, whatever the domain
+// size).  Every partial slot a finishing kernel reads is written by the
+// reduction kernel of the same call, so nothing needs clearing between calls.
+// Host code never names the array in a launch or copy: its device address is
+// taken once per device (hipGetSymbolAddress, with that device current) and
+// kept in scratch_ptr.
+static __device__ __align__(256) double scratch[(kStatBlocks + 1) * kStatSlots];
+
+double* scratch_address(hipError_t& err) {  // called with stats_mutex held
+    static double* scratch_ptr[64] = {nullptr};
+    int dev = 0;
+    err = hipGetDevice(&dev);
+    if (err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= 64) {
+        err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    if (scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess) return nullptr;
+        scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return scratch_ptr[dev];
+}
+
 hipError_t read_back(const double* dev_result, double* host, int n, hipStream_t st) {
     hipError_t e = hipMemcpyAsync(host, dev_result, n * sizeof(double), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
@@ -297,12 +325,14 @@ hipError_t run_scalar_fields(const void* g, uint32_t cells, double ref, double* 
     return hipGetLastError();
 }
 
+// no_alloc: the partial sums go to the static scratch array instead of the
+// workspace allocated on first use (lbc_reactive_stats_noalloc).
 template <class S, int ST>
 hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
-                              double* out, hipStream_t st) {
+                              double* out, hipStream_t st, bool no_alloc) {
     std::lock_guard<std::mutex> lock(stats_mutex);
     hipError_t e;
-    double* ws = workspace(e);
+    double* ws = no_alloc ? scratch_address(e) : workspace(e);
     if (!ws) return e;
     using R = typename Storage<ST>::R;
     hipLaunchKernelGGL((reactive_stats_kernel<S, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
@@ -353,12 +383,32 @@ hipError_t scalar_fields_storage(int storage, const void* g, uint32_t cells, dou
 
 template <class S>
 hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
-                         double* out, hipStream_t st) {
+                         double* out, hipStream_t st, bool no_alloc) {
     switch (storage) {
-        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, ra, rb, out, st);
-        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, ra, rb, out, st);
-        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, ra, rb, out, st);
-        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, ra, rb, out, st);
+        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, ra, rb, out, st, no_alloc);
+        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, ra, rb, out, st, no_alloc);
+        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, ra, rb, out, st, no_alloc);
+        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, ra, rb, out, st, no_alloc);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t reactive_stats_entry(int scalar_lattice, int storage, const void* species_a, const void* species_b,
+                                uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double reference_a,
+                                double reference_b, double* out8, void* stream, bool no_alloc) {
+    if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
+    if (e != hipSuccess) return e;
+    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    const hipStream_t st = as_stream(stream);
+    switch (scalar_lattice) {
+        case kD2Q5:
+            return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
+                                       st, no_alloc);
+        case kD3Q7:
+            return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
+                                       st, no_alloc);
         default: return hipErrorInvalidValue;
     }
 }
@@ -385,22 +435,16 @@ extern "C" int lbc_fluid_diagnostics(int lattice, int storage, const void* popul
 extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* species_a, const void* species_b,
                                   uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double reference_a,
                                   double reference_b, double* out8, void* stream) {
-    using namespace lbc;
-    if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
-    LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
-    if (e != hipSuccess) return e;
-    const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
-    const hipStream_t st = as_stream(stream);
-    switch (scalar_lattice) {
-        case kD2Q5:
-            return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
-                                       st);
-        case kD3Q7:
-            return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
-                                       st);
-        default: return hipErrorInvalidValue;
-    }
+    return lbc::reactive_stats_entry(scalar_lattice, storage, species_a, species_b, nx, ny, nz, k_react, reference_a,
+                                     reference_b, out8, stream, false);
+}
+
+extern "C" int lbc_reactive_stats_noalloc(int scalar_lattice, int storage, const void* species_a,
+                                          const void* species_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                                          double k_react, double reference_a, double reference_b, double* out8,
+                                          void* stream) {
+    return lbc::reactive_stats_entry(scalar_lattice, storage, species_a, species_b, nx, ny, nz, k_react, reference_a,
+                                     reference_b, out8, stream, true);
 }
 
 extern "C" int lbc_max_speed(int lattice, int storage, const void* populations, uint64_t nx, uint64_t ny,

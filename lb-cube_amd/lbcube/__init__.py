@@ -207,16 +207,19 @@ STAT_NAMES = ("mean_A", "mean_B", "var_A", "var_B", "covariance", "segregation_i
               "mixed_reaction_rate")
 
 
-def reactive_stats(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=0.0):
-    """compute_reactive_stats on the device; returns a dict; synchronizes."""
+def reactive_stats(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=0.0, no_alloc=False):
+    """compute_reactive_stats on the device; returns a dict; synchronizes.
+    no_alloc=True calls lbc_reactive_stats_noalloc (partials in the library's
+    static device array: no device allocation, same results)."""
     _check_pop(scalar_lattice, a, "a")
     _check_pop(scalar_lattice, b, "b")
     _check_set([("a", a), ("b", b)])
     nx, ny, nz = _extents(scalar_lattice, a)
     out = (ctypes.c_double * 8)()
-    check(lib().lbc_reactive_stats(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx, ny,
-                                   nz, float(k_react), float(ref_a), float(ref_b), out, _stream(stream)),
-          "lbc_reactive_stats")
+    fn = lib().lbc_reactive_stats_noalloc if no_alloc else lib().lbc_reactive_stats
+    check(fn(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx, ny, nz, float(k_react),
+             float(ref_a), float(ref_b), out, _stream(stream)),
+          "lbc_reactive_stats_noalloc" if no_alloc else "lbc_reactive_stats")
     return dict(zip(STAT_NAMES, out))
 
 

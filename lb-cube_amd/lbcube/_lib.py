@@ -36,6 +36,8 @@ SIGNATURES = {
     "lbc_fluid_diagnostics": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_reactive_stats": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double, _dp, c_vp],
                            c_int),
+    "lbc_reactive_stats_noalloc": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double, _dp,
+                                    c_vp], c_int),
     "lbc_max_speed": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, _dp, c_vp], c_int),
     "lbc_macro_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_vp, c_vp, c_vp], c_int),
     "lbc_scalar_fields": ([c_int, c_int, c_vp, c_u64, c_u64, c_u64, c_double, c_vp, c_vp], c_int),

@@ -116,3 +116,16 @@ def test_validation_fields_and_guard():
     assert L.lbc_macro_fields(3, 0, FAKE_A, 8, 8, 8, FAKE_B, 0x3000, None) == HIP_INVALID_VALUE
     assert L.lbc_scalar_fields(3, 0, None, 8, 8, 1, 0.0, FAKE_B, None) == HIP_INVALID_DEVICE_POINTER
     assert L.lbc_scalar_fields(0, 0, FAKE_A, 8, 8, 1, 0.0, FAKE_B, None) == HIP_INVALID_VALUE  # fluid lattice
+
+
+def test_validation_noalloc_stats():
+    """The no-allocation statistics entry point validates like lbc_reactive_stats
+    before touching the device."""
+    L = lbcube.lib()
+    out = (ctypes.c_double * 8)()
+    for fn in (L.lbc_reactive_stats, L.lbc_reactive_stats_noalloc):
+        assert fn(3, 0, None, FAKE_B, 8, 8, 1, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_DEVICE_POINTER
+        assert fn(3, 0, FAKE_A, FAKE_B, 8, 8, 1, 0.1, 0.0, 0.0, None, None) == HIP_INVALID_DEVICE_POINTER
+        assert fn(3, 0, FAKE_A, FAKE_B, 0, 8, 1, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
+        assert fn(4, 0, FAKE_A, FAKE_B, 8, 8, 0, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
+        assert fn(4, 0, FAKE_A, FAKE_B, 1 << 12, 1 << 10, 1 << 10, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
