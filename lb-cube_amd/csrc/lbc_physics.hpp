@@ -12,6 +12,7 @@
 // reach any result).
 #pragma once
 
+#include "lbc_constdiv.hpp"
 #include "lbc_lattice.hpp"
 
 namespace lbc {
@@ -428,12 +429,6 @@ __device__ __forceinline__ void collide_mrt_d3q19(R* f, R omega, Sink sink = {})
 
 // D3Q27 separable tensor-product MRT, lattice_mrt.hpp:206-426, 1023-1305.
 __host__ __device__ constexpr int t27(int a, int b, int c) { return (a * 3 + b) * 3 + c; }
-template <class R> __host__ __device__ constexpr R axis_basis(int order, int c) {
-    return order == 0 ? R{1} : (order == 1 ? R(c) : R{3} * R(c) * R(c) - R{2});
-}
-template <class R> __host__ __device__ constexpr R axis_norm(int order) {
-    return order == 0 ? R{3} : (order == 1 ? R{2} : R{6});
-}
 // Raw tensor moments carried by MRT rows 7..26, lattice_mrt.hpp:1092-1114.
 struct Rest27 {
     int a[20], b[20], c[20];
@@ -442,43 +437,42 @@ constexpr Rest27 kRest27 = {{1, 0, 1, 2, 2, 1, 0, 1, 0, 1, 2, 2, 0, 2, 1, 1, 2, 
                             {1, 1, 0, 1, 0, 2, 2, 0, 1, 1, 2, 0, 2, 1, 2, 1, 2, 1, 2, 2},
                             {0, 1, 1, 0, 1, 0, 1, 2, 2, 1, 0, 2, 2, 1, 1, 2, 1, 2, 2, 2}};
 
+// One separable pass of the forward transform over a line of three values at
+// c = -1, 0, +1 (lattice_mrt.hpp:1038-1087): the sums P0 = a + b + c,
+// P1 = -a + 0 b + c and P2 = a - 2 b + c in the reference's left-to-right
+// order.  The zero term of P1 is left out, and a - 2 b is one fused
+// multiply-add: 2 b is exact, so fma(-2, b, a) rounds like a + (-2 b).  The
+// reference's sums start at +0; starting at the first term instead changes
+// only the sign of a sum whose every term is -0 (the values are identical).
+template <class R> __device__ __forceinline__ void forward_line(R a, R b, R c, R& p0, R& p1, R& p2) {
+    p0 = (a + b) + c;
+    p1 = c - a;
+    if constexpr (sizeof(R) == 4) p2 = __builtin_fmaf(R(-2), b, a) + c;
+    else p2 = __builtin_fma(R(-2), b, a) + c;
+}
+
 template <class R> __device__ __forceinline__ void mrt_forward_d3q27(const R* f, R* mo) {
     R p[27], xt[27], xyt[27], raw[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) p[t27(D3Q27::cx[i] + 1, D3Q27::cy[i] + 1, D3Q27::cz[i] + 1)] = f[i];
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
+    for (int iy = 0; iy < 3; ++iy)
 #pragma unroll
-        for (int iy = 0; iy < 3; ++iy)
-#pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                R v{};
-#pragma unroll
-                for (int ix = 0; ix < 3; ++ix) v += axis_basis<R>(ax, ix - 1) * p[t27(ix, iy, iz)];
-                xt[t27(ax, iy, iz)] = v;
-            }
+        for (int iz = 0; iz < 3; ++iz)
+            forward_line<R>(p[t27(0, iy, iz)], p[t27(1, iy, iz)], p[t27(2, iy, iz)], xt[t27(0, iy, iz)],
+                            xt[t27(1, iy, iz)], xt[t27(2, iy, iz)]);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-        for (int ay = 0; ay < 3; ++ay)
-#pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                R v{};
-#pragma unroll
-                for (int iy = 0; iy < 3; ++iy) v += axis_basis<R>(ay, iy - 1) * xt[t27(ax, iy, iz)];
-                xyt[t27(ax, ay, iz)] = v;
-            }
+        for (int iz = 0; iz < 3; ++iz)
+            forward_line<R>(xt[t27(ax, 0, iz)], xt[t27(ax, 1, iz)], xt[t27(ax, 2, iz)], xyt[t27(ax, 0, iz)],
+                            xyt[t27(ax, 1, iz)], xyt[t27(ax, 2, iz)]);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
         for (int ay = 0; ay < 3; ++ay)
-#pragma unroll
-            for (int az = 0; az < 3; ++az) {
-                R v{};
-#pragma unroll
-                for (int iz = 0; iz < 3; ++iz) v += axis_basis<R>(az, iz - 1) * xyt[t27(ax, ay, iz)];
-                raw[t27(ax, ay, az)] = v;
-            }
+            forward_line<R>(xyt[t27(ax, ay, 0)], xyt[t27(ax, ay, 1)], xyt[t27(ax, ay, 2)], raw[t27(ax, ay, 0)],
+                            raw[t27(ax, ay, 1)], raw[t27(ax, ay, 2)]);
     const R r200 = raw[t27(2, 0, 0)], r020 = raw[t27(0, 2, 0)], r002 = raw[t27(0, 0, 2)];
     mo[0] = raw[t27(0, 0, 0)];
     mo[1] = raw[t27(1, 0, 0)];
@@ -491,6 +485,30 @@ template <class R> __device__ __forceinline__ void mrt_forward_d3q27(const R* f,
     for (int k = 0; k < 20; ++k) mo[7 + k] = raw[t27(kRest27.a[k], kRest27.b[k], kRest27.c[k])];
 }
 
+// One separable pass of the inverse over a line of three moments (orders 0,
+// 1, 2) to the values at c = -1, 0, +1 (lattice_mrt.hpp:1165-1205): the
+// reference's term raw * P(c) / norm, norms 3, 2, 6, summed over the orders
+// left to right.  Each quotient is formed once and equals the reference's IEEE
+// quotient bit for bit: v0 / 3 and v2 / 3 by the correctly rounded constant
+// division of lbc_constdiv.hpp, v1 / 2 exactly, (-2 v2) / 6 = -(v2 / 3) (the
+// same real number, so the same rounding), and v2 / 6 as (v2 / 3) / 2, which
+// is exact while v2 / 6 is a normal number; below that (|v2| < 3 * 2^-125 in
+// float, 3 * 2^-1021 in double) the IEEE division is taken.  The P1 term at
+// c = 0 is a zero and is left out; the sums start at their first term (the
+// sign of an all -0 sum aside, the same values as starting at +0).
+template <class R> __device__ __forceinline__ void inverse_line(R v0, R v1, R v2, R& m1, R& z0, R& p1) {
+    const R t0 = div_const<R, 3>(v0);
+    const R h1 = v1 / R{2};
+    const R t2 = div_const<R, 3>(v2);
+    constexpr R tiny = sizeof(R) == 4 ? R(0x1p-123) : R(0x1p-1019);  // >= 3 * 2^-125 resp. 3 * 2^-1021
+    R s2 = t2 / R{2};
+    if (__builtin_fabs(v2) < tiny) [[unlikely]]
+        s2 = v2 / R{6};
+    m1 = (t0 - h1) + s2;
+    z0 = t0 - t2;
+    p1 = (t0 + h1) + s2;
+}
+
 template <class R, class Sink = NoSink>
 __device__ __forceinline__ void mrt_inverse_d3q27(const R* mo, R* f, Sink sink = {}) {
     R raw[27], zi[27], yzi[27], pt[27];
@@ -498,44 +516,29 @@ __device__ __forceinline__ void mrt_inverse_d3q27(const R* mo, R* f, Sink sink =
     raw[t27(1, 0, 0)] = mo[1];
     raw[t27(0, 1, 0)] = mo[2];
     raw[t27(0, 0, 1)] = mo[3];
-    raw[t27(2, 0, 0)] = (mo[4] + mo[5]) / R{3};
-    raw[t27(0, 2, 0)] = (R{2} * mo[4] - mo[5] + R{3} * mo[6]) / R{6};
-    raw[t27(0, 0, 2)] = (R{2} * mo[4] - mo[5] - R{3} * mo[6]) / R{6};
+    raw[t27(2, 0, 0)] = div_const<R, 3>(mo[4] + mo[5]);
+    raw[t27(0, 2, 0)] = div_const<R, 6>(R{2} * mo[4] - mo[5] + R{3} * mo[6]);
+    raw[t27(0, 0, 2)] = div_const<R, 6>(R{2} * mo[4] - mo[5] - R{3} * mo[6]);
 #pragma unroll
     for (int k = 0; k < 20; ++k) raw[t27(kRest27.a[k], kRest27.b[k], kRest27.c[k])] = mo[7 + k];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
         for (int ay = 0; ay < 3; ++ay)
-#pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                R v{};
-#pragma unroll
-                for (int az = 0; az < 3; ++az) v += raw[t27(ax, ay, az)] * axis_basis<R>(az, iz - 1) / axis_norm<R>(az);
-                zi[t27(ax, ay, iz)] = v;
-            }
+            inverse_line<R>(raw[t27(ax, ay, 0)], raw[t27(ax, ay, 1)], raw[t27(ax, ay, 2)], zi[t27(ax, ay, 0)],
+                            zi[t27(ax, ay, 1)], zi[t27(ax, ay, 2)]);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-        for (int iy = 0; iy < 3; ++iy)
+        for (int iz = 0; iz < 3; ++iz)
+            inverse_line<R>(zi[t27(ax, 0, iz)], zi[t27(ax, 1, iz)], zi[t27(ax, 2, iz)], yzi[t27(ax, 0, iz)],
+                            yzi[t27(ax, 1, iz)], yzi[t27(ax, 2, iz)]);
 #pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                R v{};
+    for (int iy = 0; iy < 3; ++iy)
 #pragma unroll
-                for (int ay = 0; ay < 3; ++ay) v += zi[t27(ax, ay, iz)] * axis_basis<R>(ay, iy - 1) / axis_norm<R>(ay);
-                yzi[t27(ax, iy, iz)] = v;
-            }
-#pragma unroll
-    for (int ix = 0; ix < 3; ++ix)
-#pragma unroll
-        for (int iy = 0; iy < 3; ++iy)
-#pragma unroll
-            for (int iz = 0; iz < 3; ++iz) {
-                R v{};
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) v += yzi[t27(ax, iy, iz)] * axis_basis<R>(ax, ix - 1) / axis_norm<R>(ax);
-                pt[t27(ix, iy, iz)] = v;
-            }
+        for (int iz = 0; iz < 3; ++iz)
+            inverse_line<R>(yzi[t27(0, iy, iz)], yzi[t27(1, iy, iz)], yzi[t27(2, iy, iz)], pt[t27(0, iy, iz)],
+                            pt[t27(1, iy, iz)], pt[t27(2, iy, iz)]);
 #pragma unroll
     for (int i = 0; i < 27; ++i) {
         f[i] = pt[t27(D3Q27::cx[i] + 1, D3Q27::cy[i] + 1, D3Q27::cz[i] + 1)];
@@ -550,14 +553,11 @@ __device__ __forceinline__ void collide_mrt_d3q27(R* f, R s_nu, Sink sink = {}) 
     const Macro<R> mac = macro_state<D3Q27, R>(f);
     R m[27], fe[27], eq[27];
     mrt_forward_d3q27<R>(f, m);
-    const R rho = mac.rho, ux = mac.ux, uy = mac.uy, uz = mac.uz;
-    const R usq = ux * ux + uy * uy + uz * uz;
+    // compute_equilibrium_moments_d3q27 (lattice_mrt.hpp:1242-1268): the
+    // second-order feq of each population, then the forward transform
+    const R usq = vel_sq<D3Q27, R>(mac);
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        const R cx = R(D3Q27::cx[i]), cy = R(D3Q27::cy[i]), cz = R(D3Q27::cz[i]);
-        const R cu = cx * ux + cy * uy + cz * uz;
-        fe[i] = R(D3Q27::w[i]) * rho * (R{1} + R{3} * cu + R{4.5} * cu * cu - R{1.5} * usq);
-    }
+    for (int i = 0; i < 27; ++i) fe[i] = feq<D3Q27, R>(i, mac, usq);
     mrt_forward_d3q27<R>(fe, eq);
     const R s_b = R{1.19}, s_e = R{1.1}, s_eps = R{1.2}, s_q = R{1.2};
 #pragma unroll
