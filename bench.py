@@ -78,6 +78,80 @@ def copy_ceiling(lbcube, torch, nbytes, reps=20):
     return nbytes / (ms * 1e-3) / 1e9
 
 
+def _events_ms(torch, fn, steps, warmup):
+    """Average milliseconds of fn() over `steps` calls, timed with events on
+    the current stream after `warmup` untimed calls."""
+    for _ in range(warmup):
+        fn()
+    stream = torch.cuda.current_stream()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def pattern_ceiling(lbcube, torch, u, steps=20, warmup=3):
+    """GB/s of the pull-streaming step alone (lbc_pull_stream: the headline
+    kernel's loads, stores, addressing and residency, no collision) on the
+    headline's D3Q19 FP32 buffers: the ceiling of its access pattern."""
+    from lbcube import init as lbinit
+
+    bufs = [lbinit.equilibrium("D3Q19", None, u, "f32")]
+    bufs.append(torch.empty_like(bufs[0]))
+
+    def step():
+        lbcube.pull_stream("D3Q19", bufs[0], bufs[1])
+        bufs.reverse()
+
+    ms = _events_ms(torch, step, steps, warmup)
+    cells = bufs[0][0].numel()
+    del bufs
+    torch.cuda.empty_cache()
+    return 152 * cells / (ms * 1e-3) / 1e9, ms
+
+
+def pitched_variants(lbcube, torch, u, omega, steps, warmup, pads=(256, 1024, 4096, 16384)):
+    """D3Q19 TRT FP32 and the pull-stream step on buffers whose population
+    planes are padded by `pad` elements (lbc_fluid_step_pitched): the
+    reference's contiguous layout puts all 38 streams on a common 64 MiB
+    stride at 256^3.  Reported beside the headline, which keeps the
+    contiguous layout."""
+    from lbcube import init as lbinit
+
+    a0 = lbinit.equilibrium("D3Q19", None, u, "f32")
+    cells = a0[0].numel()
+    ext = tuple(reversed(a0.shape[1:]))
+    out = {}
+    for pad in pads:
+        pitch = cells + pad
+        bufs = [torch.empty(19 * pitch, dtype=torch.float32, device="cuda") for _ in range(2)]
+        bufs[0].view(19, pitch)[:, :cells].copy_(a0.view(19, cells))
+
+        def step():
+            lbcube.fluid_step_pitched("D3Q19", "TRT", bufs[0], bufs[1], ext, pitch, omega)
+            bufs.reverse()
+
+        def pull():
+            lbcube.pull_stream("D3Q19", bufs[0], bufs[1], extents=ext, plane_pitch=pitch)
+            bufs.reverse()
+
+        ms = _events_ms(torch, step, steps, warmup)
+        ms_pull = _events_ms(torch, pull, steps, warmup)
+        out[f"pad{pad}"] = {"ms_per_step": ms, "mlups": cells / (ms * 1e-3) / 1e6,
+                            "achieved_gbs": 152 * cells / (ms * 1e-3) / 1e9,
+                            "pattern_gbs": 152 * cells / (ms_pull * 1e-3) / 1e9}
+        print(f"# trt_f32 planes padded by {pad}: {out[f'pad{pad}']['mlups']:.0f} MLUPS, "
+              f"pull-stream {out[f'pad{pad}']['pattern_gbs']:.0f} GB/s", file=sys.stderr, flush=True)
+        del bufs
+    del a0
+    torch.cuda.empty_cache()
+    return out
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_D3Q19_TRT_f32_256.md")
 
 
@@ -463,7 +537,7 @@ def main():
                                           "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
-    ap.add_argument("--extras", default="adjoint,reactive_2d,adr_3d,ensemble,lattices,large",
+    ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large",
                     help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -494,9 +568,12 @@ def main():
               file=sys.stderr, flush=True)
     head = variants["trt_f32"]
     copy_gbs = copy_ceiling(lbcube, torch, 152 * n ** 3)
-    extras = {}
+    pattern_gbs, pattern_ms = pattern_ceiling(lbcube, torch, u)
+    print(f"# pull-stream ceiling: {pattern_gbs:.0f} GB/s ({pattern_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     chosen = set() if args.no_extras else set(args.extras.split(","))
-    if chosen:
+    pitched = pitched_variants(lbcube, torch, u, omega, args.steps, args.warmup) if "pitched" in chosen else None
+    extras = {}
+    if chosen - {"pitched"}:
         del u
         torch.cuda.empty_cache()
         if "adjoint" in chosen:
@@ -541,11 +618,16 @@ def main():
                      "traffic": measured_traffic() if (n == 256 and block is None) else None,
                      "algorithmic_bytes": 152 * n ** 3,
                      "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "bytes_per_cell": 152,
-                     "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
+                     "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4),
+                     "pattern_ceiling_gbs": round(pattern_gbs, 1),
+                     "frac_of_pattern": round(head["achieved_gbs"] / pattern_gbs, 4),
+                     "pattern_kernel": "pull_stream_kernel<D3Q19, F32> (lbc_pull_stream, same buffers)"},
         "cpu_baseline": cpu,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
+    if pitched:
+        out["pitched_planes"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in pitched.items()}
     if "trt_f16fast" in variants:
         out["fp16_over_fp32"] = round(variants["trt_f16fast"]["mlups"] / head["mlups"], 3)
     if extras:

@@ -79,6 +79,28 @@ int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, c
                       uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x, uint32_t block_y,
                       uint32_t block_z, void* stream);
 
+/* lbc_fluid_step_ex on pitched population buffers: plane i of `current` and
+ * `next` starts plane_pitch elements after plane i - 1 (plane_pitch = 0: the
+ * contiguous [Q][Z][Y][X] layout of the reference, pitch = nx * ny * nz;
+ * otherwise at least the cell count and at most 2^51).  Cells inside a plane
+ * keep the reference's order, and elements past the cells of a plane are
+ * neither read nor written.  Padding the planes moves the Q streams of a
+ * launch off a common power-of-two stride.  Results equal those of the
+ * contiguous layout bit for bit.  No counterpart in the reference. */
+int lbc_fluid_step_pitched(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
+                           uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_pitch, double omega,
+                           uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
+
+/* The pull-streaming step alone: next[i][x] = current[i][x - c_i] (periodic)
+ * for every population of a fluid lattice, the step of BGK at omega = 0 with
+ * the collision left out.  Same addressing, store form, launch shape and
+ * residency as the collide-stream kernels, so its rate is the ceiling of
+ * their memory access pattern (reported by the benchmark beside the spec
+ * peak).  plane_pitch as for lbc_fluid_step_pitched.  Validation as
+ * lbc_fluid_step.  No counterpart in the reference. */
+int lbc_pull_stream(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny, uint64_t nz,
+                    uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
+
 /* Reference concentrations of scalar fields.  With LBC_F16 / LBC_BF16
  * storage a scalar field's populations are stored as g_i - w_i c_ref, with
  * c_ref the field's reference concentration (w_i alone is the fluid's rest

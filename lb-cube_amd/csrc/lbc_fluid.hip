@@ -23,10 +23,12 @@ namespace lbc {
 // LBC_FLAG_FAST: BGK, TRT and MRT on every lattice; RLBM always evaluates the
 // reference's order.  WIDE: 64-bit per-lane addresses, for planes of more than
 // 4 GiB.
+// `plane`: elements from one population plane to the next (the cell count for
+// the reference's contiguous [Q][Z][Y][X] buffers; larger for pitched ones).
 template <class L, int CT, int ST, bool FAST, int MAXT, bool WIDE>
 __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                           typename Storage<ST>::S* __restrict__ nxt, Cells c,
-                                                          typename Storage<ST>::R omega,
+                                                          size_t plane, typename Storage<ST>::R omega,
                                                           typename Storage<ST>::R omega_minus) {
     using St = Storage<ST>;
     using R = typename St::R;
@@ -35,7 +37,6 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
-    const size_t plane = c.count;
     const Pull pull(dst, q.x, q.y, q.z, nx, ny, nz);
     R f[L::Q];
     if constexpr (FAST) {
@@ -59,20 +60,42 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     }
 }
 
+// The pull-streaming step alone: next[i][x] = cur[i][x - c_i] for every
+// population, with the fluid kernel's addressing, plane order, store form and
+// launch shape, and no collision (the identity collision of BGK at omega = 0).
+// Its rate is the bandwidth ceiling of the collide-stream kernels' access
+// pattern: the same 2 Q streams with the x-shifted pulls, without arithmetic.
+template <class L, int ST, int MAXT, bool WIDE>
+__global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storage<ST>::S* __restrict__ cur,
+                                                           typename Storage<ST>::S* __restrict__ nxt, Cells c,
+                                                           size_t plane) {
+    const uint32_t block = renumber_blocks<L::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
+    const uint32_t dst = block * blockDim.x + threadIdx.x;
+    if (dst >= c.count) return;
+    const CellCoord q = cell_coord(dst, c);
+    const Pull pull(dst, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
+    typename Storage<ST>::S v[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) v[i] = get(at<WIDE>(plane_ptr(cur, i, plane), pull.src(L::cx[i], L::cy[i], L::cz[i])));
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) put(at<WIDE>(plane_ptr(nxt, i, plane), dst), v[i]);
+}
+
 // Kernels that run faster with residency capped at kCappedWavesPerCu (measured
-// on MI355X): 3D lattices with FP32 storage or FP16 in the deviation form,
-// except the reference-order D3Q27 MRT (VALU-bound).  Left uncapped: BF16
-// (its software rounding makes the step VALU-heavier: 71.0k against 74.4k
-// MLUPS capped), FP64 (unchanged by the cap) and RLBM (not measured).
+// on MI355X): 3D lattices with FP32 storage or FP16 in the deviation form
+// (the reference-order D3Q27 MRT included since its constant divisions made it
+// memory-bound: +0.9% at 256^3, +2.7% at 384^3).  Left uncapped: BF16 (its
+// software rounding makes the step VALU-heavier: 71.0k against 74.4k MLUPS
+// capped), FP64 (unchanged by the cap) and RLBM (not measured).
 template <class L, int CT, int ST, bool FAST> constexpr bool capped_residency() {
     if (L::D != 3 || ST == kF64 || ST == kBF16 || CT == kRLBM) return false;
     if (kOffsetStorage<ST> && !FAST) return false;
-    if (L::Q == 27 && CT == kMRT && !FAST) return false;
     return true;
 }
 
 template <class L, int CT, int ST, bool FAST>
-hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double omega, hipStream_t stream) {
+hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, double omega,
+                        hipStream_t stream) {
     using St = Storage<ST>;
     using R = typename St::R;
     const R w = static_cast<R>(omega);
@@ -80,7 +103,7 @@ hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double
     const uint32_t lds = capped_residency<L, CT, ST, FAST>() ? capped_lds(s.threads()) : 0;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const typename St::S*>(cur),
-                           static_cast<typename St::S*>(nxt), s.cells, w, wm);
+                           static_cast<typename St::S*>(nxt), s.cells, static_cast<size_t>(plane), w, wm);
     };
     const bool wide = wide_planes<typename St::S>(s);
     if (s.threads() <= 256)
@@ -91,31 +114,92 @@ hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, double
 }
 
 template <class L, int CT, bool FAST>
-hipError_t dispatch_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, double omega,
-                            hipStream_t st) {
+hipError_t dispatch_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, uint64_t plane,
+                            double omega, hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_fluid<L, CT, kF32, FAST>(cur, nxt, s, omega, st);
-        case kF64: return launch_fluid<L, CT, kF64, FAST>(cur, nxt, s, omega, st);
-        case kF16: return launch_fluid<L, CT, kF16, FAST>(cur, nxt, s, omega, st);
-        case kBF16: return launch_fluid<L, CT, kBF16, FAST>(cur, nxt, s, omega, st);
+        case kF32: return launch_fluid<L, CT, kF32, FAST>(cur, nxt, s, plane, omega, st);
+        case kF64: return launch_fluid<L, CT, kF64, FAST>(cur, nxt, s, plane, omega, st);
+        case kF16: return launch_fluid<L, CT, kF16, FAST>(cur, nxt, s, plane, omega, st);
+        case kBF16: return launch_fluid<L, CT, kBF16, FAST>(cur, nxt, s, plane, omega, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <class L>
 hipError_t dispatch_collision(int collision, int storage, bool fast, const void* cur, void* nxt,
-                              const LaunchShape& s, double omega, hipStream_t st) {
+                              const LaunchShape& s, uint64_t plane, double omega, hipStream_t st) {
     switch (collision) {
         case kBGK:
-            return fast ? dispatch_storage<L, kBGK, true>(storage, cur, nxt, s, omega, st)
-                        : dispatch_storage<L, kBGK, false>(storage, cur, nxt, s, omega, st);
+            return fast ? dispatch_storage<L, kBGK, true>(storage, cur, nxt, s, plane, omega, st)
+                        : dispatch_storage<L, kBGK, false>(storage, cur, nxt, s, plane, omega, st);
         case kTRT:
-            return fast ? dispatch_storage<L, kTRT, true>(storage, cur, nxt, s, omega, st)
-                        : dispatch_storage<L, kTRT, false>(storage, cur, nxt, s, omega, st);
+            return fast ? dispatch_storage<L, kTRT, true>(storage, cur, nxt, s, plane, omega, st)
+                        : dispatch_storage<L, kTRT, false>(storage, cur, nxt, s, plane, omega, st);
         case kMRT:
-            return fast ? dispatch_storage<L, kMRT, true>(storage, cur, nxt, s, omega, st)
-                        : dispatch_storage<L, kMRT, false>(storage, cur, nxt, s, omega, st);
-        case kRLBM: return dispatch_storage<L, kRLBM, false>(storage, cur, nxt, s, omega, st);
+            return fast ? dispatch_storage<L, kMRT, true>(storage, cur, nxt, s, plane, omega, st)
+                        : dispatch_storage<L, kMRT, false>(storage, cur, nxt, s, plane, omega, st);
+        case kRLBM: return dispatch_storage<L, kRLBM, false>(storage, cur, nxt, s, plane, omega, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// The pull-streaming step shares the residency of the TRT step it bounds.
+template <class L, int ST>
+hipError_t launch_stream(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, hipStream_t stream) {
+    using S = typename Storage<ST>::S;
+    const uint32_t lds = capped_residency<L, kTRT, ST, kOffsetStorage<ST>>() ? capped_lds(s.threads()) : 0;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const S*>(cur), static_cast<S*>(nxt),
+                           s.cells, static_cast<size_t>(plane));
+    };
+    const bool wide = wide_planes<S>(s);
+    if (s.threads() <= 256)
+        wide ? go(pull_stream_kernel<L, ST, 256, true>) : go(pull_stream_kernel<L, ST, 256, false>);
+    else
+        wide ? go(pull_stream_kernel<L, ST, 1024, true>) : go(pull_stream_kernel<L, ST, 1024, false>);
+    return hipGetLastError();
+}
+
+template <class L>
+hipError_t stream_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, uint64_t plane,
+                          hipStream_t st) {
+    switch (storage) {
+        case kF32: return launch_stream<L, kF32>(cur, nxt, s, plane, st);
+        case kF64: return launch_stream<L, kF64>(cur, nxt, s, plane, st);
+        case kF16: return launch_stream<L, kF16>(cur, nxt, s, plane, st);
+        case kBF16: return launch_stream<L, kBF16>(cur, nxt, s, plane, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Plane pitch of a call in elements: 0 selects the contiguous layout (pitch =
+// cells); otherwise at least the cell count, and at most 2^51 (27 planes of
+// 8-byte values stay below 2^59 bytes).
+hipError_t plane_pitch(const LaunchShape& s, uint64_t requested, uint64_t& plane) {
+    const uint64_t cells = s.cells.count;
+    plane = requested == 0 ? cells : requested;
+    if (plane < cells || plane > (1ull << 51)) return hipErrorInvalidValue;
+    return hipSuccess;
+}
+
+hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
+                       uint64_t nx, uint64_t ny, uint64_t nz, uint64_t pitch, double omega, uint32_t block_x,
+                       uint32_t block_y, uint32_t block_z, void* stream) {
+    if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current == next) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    uint64_t plane = 0;
+    e = plane_pitch(s, pitch, plane);
+    if (e != hipSuccess) return e;
+    const bool fast = (flags & LBC_FLAG_FAST) != 0;
+    const hipStream_t st = as_stream(stream);
+    switch (lattice) {
+        case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, fast, current, next, s, plane, omega, st);
+        case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, fast, current, next, s, plane, omega, st);
+        case kD3Q27: return dispatch_collision<D3Q27>(collision, storage, fast, current, next, s, plane, omega, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -125,19 +209,35 @@ hipError_t dispatch_collision(int collision, int storage, bool fast, const void*
 extern "C" int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, const void* current,
                                  void* next, uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x,
                                  uint32_t block_y, uint32_t block_z, void* stream) {
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, 0, omega, block_x,
+                            block_y, block_z, stream);
+}
+
+extern "C" int lbc_fluid_step_pitched(int lattice, int collision, int storage, uint32_t flags, const void* current,
+                                      void* next, uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_pitch,
+                                      double omega, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                      void* stream) {
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, plane_pitch, omega,
+                            block_x, block_y, block_z, stream);
+}
+
+extern "C" int lbc_pull_stream(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
+                               uint64_t nz, uint64_t plane_pitch, uint32_t block_x, uint32_t block_y,
+                               uint32_t block_z, void* stream) {
     using namespace lbc;
     if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (current == next) return hipErrorInvalidValue;
-    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
-    const bool fast = (flags & LBC_FLAG_FAST) != 0;
+    uint64_t plane = 0;
+    e = lbc::plane_pitch(s, plane_pitch, plane);
+    if (e != hipSuccess) return e;
     const hipStream_t st = as_stream(stream);
     switch (lattice) {
-        case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, fast, current, next, s, omega, st);
-        case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, fast, current, next, s, omega, st);
-        case kD3Q27: return dispatch_collision<D3Q27>(collision, storage, fast, current, next, s, omega, st);
+        case kD2Q9: return stream_storage<D2Q9>(storage, current, next, s, plane, st);
+        case kD3Q19: return stream_storage<D3Q19>(storage, current, next, s, plane, st);
+        case kD3Q27: return stream_storage<D3Q27>(storage, current, next, s, plane, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -122,8 +122,8 @@ inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 // 256^3 ran 4.7% faster (39.1k against 37.3k MLUPS), 512^3 ~10%, D3Q27 TRT
 // FP32 and FP16 (deviation form) ~4%; 20 waves per CU gave the same, 24
 // less, 12 lost ground.  Kernels whose collision keeps the VALU busy
-// (reference-order FP16/BF16, reference-order D3Q27 MRT) lose from it, as did
-// the 2D fused reactive step, so the launchers apply it per kernel.
+// (reference-order FP16/BF16) lose from it, as did the 2D fused reactive
+// step, so the launchers apply it per kernel.
 constexpr uint32_t kCappedWavesPerCu = 16;
 
 // LDS per CU and per block of the current device (queried once per device).

@@ -110,6 +110,55 @@ def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fas
                                   bx, by, bz, _stream(stream)), "lbc_fluid_step_ex")
 
 
+def _check_flat(lattice, t, name, extents, pitch):
+    if not (t.is_cuda and t.is_contiguous()) or t.dtype not in STORAGE:
+        raise ValueError(f"{name} must be a contiguous CUDA tensor of a storage dtype")
+    cells = extents[0] * extents[1] * extents[2]
+    plane = pitch if pitch else cells
+    if t.numel() < (Q[lattice] - 1) * plane + cells:
+        raise ValueError(f"{name} holds {t.numel()} elements; {Q[lattice]} planes of pitch {plane} need more")
+
+
+def fluid_step_pitched(lattice, collision, cur, nxt, extents, plane_pitch, omega, block=None, stream=None,
+                       fast=False):
+    """lbc_fluid_step_pitched on flat buffers: plane i at element i * plane_pitch
+    (0: the contiguous layout); extents = (nx, ny, nz)."""
+    nx, ny, nz = (int(v) for v in extents)
+    if DIM[lattice] == 2:
+        nz = 1
+    for t, n in ((cur, "cur"), (nxt, "nxt")):
+        _check_flat(lattice, t, n, (nx, ny, nz), plane_pitch)
+    if cur.dtype != nxt.dtype or cur.device != nxt.device:
+        raise ValueError("cur and nxt must share dtype and device")
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_fluid_step_pitched(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
+                                       FLAG_FAST if fast else 0, cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
+                                       int(plane_pitch), float(omega), bx, by, bz, _stream(stream)),
+          "lbc_fluid_step_pitched")
+
+
+def pull_stream(lattice, cur, nxt, extents=None, plane_pitch=0, block=None, stream=None):
+    """The pull-streaming step alone (lbc_pull_stream).  Population tensors of
+    shape (Q, ...) for the contiguous layout, or flat buffers with `extents`
+    and `plane_pitch`."""
+    if extents is None:
+        _check_pop(lattice, cur, "cur")
+        _check_pop(lattice, nxt, "nxt")
+        _check_set([("cur", cur), ("nxt", nxt)])
+        nx, ny, nz = _extents(lattice, cur)
+    else:
+        nx, ny, nz = (int(v) for v in extents)
+        if DIM[lattice] == 2:
+            nz = 1
+        for t, n in ((cur, "cur"), (nxt, "nxt")):
+            _check_flat(lattice, t, n, (nx, ny, nz), plane_pitch)
+        if cur.dtype != nxt.dtype or cur.device != nxt.device:
+            raise ValueError("cur and nxt must share dtype and device")
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_pull_stream(LATTICES[lattice], STORAGE[cur.dtype], cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
+                                int(plane_pitch), bx, by, bz, _stream(stream)), "lbc_pull_stream")
+
+
 def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None,
                 ref=0.0):
     """Passive-scalar ADR step. Replaces launch_scalar_step_gpu.  `ref`: the

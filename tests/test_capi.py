@@ -129,3 +129,21 @@ def test_validation_noalloc_stats():
         assert fn(3, 0, FAKE_A, FAKE_B, 0, 8, 1, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
         assert fn(4, 0, FAKE_A, FAKE_B, 8, 8, 0, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
         assert fn(4, 0, FAKE_A, FAKE_B, 1 << 12, 1 << 10, 1 << 10, 0.1, 0.0, 0.0, out, None) == HIP_INVALID_VALUE
+
+
+
+def test_validation_pitched_and_pull_stream():
+    """The pitched step and the pull-stream step validate in the reference's
+    order, then reject a plane pitch below the cell count."""
+    L = lbcube.lib()
+    for fn in (lambda *a: L.lbc_fluid_step_pitched(1, 1, 0, 0, *a[:2], *a[2:5], a[5], 1.0, *a[6:], None),
+               lambda *a: L.lbc_pull_stream(1, 0, *a[:2], *a[2:5], a[5], *a[6:], None)):
+        assert fn(None, FAKE_B, 8, 8, 8, 0, 256, 1, 1) == HIP_INVALID_DEVICE_POINTER
+        assert fn(FAKE_A, FAKE_B, 8, 8, 8, 0, 0, 1, 1) == HIP_INVALID_CONFIGURATION
+        assert fn(FAKE_A, FAKE_B, 8, 0, 8, 0, 256, 1, 1) == HIP_INVALID_VALUE
+        assert fn(FAKE_A, FAKE_A, 8, 8, 8, 0, 256, 1, 1) == HIP_INVALID_VALUE
+        assert fn(FAKE_A, FAKE_B, 8, 8, 8, 511, 256, 1, 1) == HIP_INVALID_VALUE  # pitch below 512 cells
+        assert fn(FAKE_A, FAKE_B, 8, 8, 8, (1 << 51) + 1, 256, 1, 1) == HIP_INVALID_VALUE
+    assert L.lbc_fluid_step_pitched(1, 1, 0, 2, FAKE_A, FAKE_B, 8, 8, 8, 0, 1.0, 256, 1, 1, None) == \
+        HIP_INVALID_VALUE  # unknown flag
+    assert L.lbc_pull_stream(4, 0, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE  # scalar lattice

@@ -127,7 +127,7 @@ def velocity(lattice, f):
     return torch.stack(comps)
 
 
-@pytest.mark.parametrize("lattice,n", [("D3Q19", 256), ("D3Q27", 384)])
+@pytest.mark.parametrize("lattice,n", [("D3Q19", 256), ("D3Q19", 384), ("D3Q27", 256), ("D3Q27", 384)])
 def test_reduced_storage_error_against_fp64(gpu, lattice, n):
     """TRT Taylor-Green at Re = 1600 (u0 = 0.05, seeded divergence-free
     perturbation of rms 0.01 u0), 2000 steps: every storage format against the
@@ -438,3 +438,61 @@ def test_reactive_mrt_fast_against_fp64(gpu):
     # the species are 0 or 1 apart from the mixing layer: storing g - w/2
     # instead of g halves the magnitude FP16 rounds
     assert errs[("f16", 0.5)] < errs[("f16", 0.0)]
+
+
+def c3_statistics(storage, fast, steps=2000, every=50, n=2048, perturbation=0.0):
+    """The 2048^2 reactive-mixing configuration: D2Q9 MRT double shear layer
+    (tanh profile of thickness 80, u0 = 0.04, sinusoid plus seeded white noise
+    of amplitude 1e-3), species A and B in complementary tanh-smoothed half
+    planes, Sc = 1 (tau_s = tau = 0.6), Da = k delta / u0 = 10 (k = 0.005),
+    one fused step per time step; the reactive statistics every `every` steps
+    (compute_reactive_stats, include/lattice_io.hpp:256-338, at the reference's
+    cadence of src/simulate_reaction.cpp:556-570).  `perturbation`: relative
+    random change of the initial A (the FP64 spread yardstick).  Returns rows of
+    (segregation intensity, true reaction rate, mean A)."""
+    u = lbinit.double_shear_layer(n, u0=0.04, delta=80.0)
+    ca, cb = lbinit.tanh_half_planes(n, delta=80.0)
+    if perturbation:
+        g = torch.Generator(device="cpu").manual_seed(7)
+        ca = ca * (1 + perturbation * torch.randn(ca.shape, generator=g, dtype=torch.float64).to(ca.device))
+    k = 10.0 * 0.04 / 80.0
+    f = lbinit.equilibrium("D2Q9", None, u, storage)
+    a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage)
+    b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage)
+    ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    rows = []
+    for s in range(0, steps + 1):
+        if s:
+            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast)
+            f, ft, a, at, b, bt = ft, f, at, a, bt, b
+        if s % every == 0:
+            st = lbcube.reactive_stats("D2Q5", a, b, k)
+            rows.append((st["segregation_intensity"], st["true_reaction_rate"], st["mean_A"]))
+    return np.array(rows)
+
+
+def test_c3_statistics_fp32_against_fp64(gpu):
+    """The 2048^2 double-shear-layer run, 2000 steps: segregation intensity,
+    true reaction rate and mean A every 50 steps, FP32 (reference order and
+    deviation form) against FP64.  Yardstick: an FP64 run whose initial A is
+    changed by 1e-7 (relative), which moves the statistics by ~1e-10 only: the
+    2048^2 layer stays smooth over these 2000 steps.  Bounds: the deviation
+    form within 1e-4 relative.  The reference order in FP32 gains species mass
+    (its D2Q5 weights rounded to binary sum to 1 + 3e-8: mean A +1e-4 after
+    2000 steps), which the reaction rate k<AB> of the thin interfaces feels
+    most (measured 5.6e-4 at step 2000, growing linearly); its bounds are
+    segregation 5e-5, rate 1e-3 and mean A 2e-4."""
+    print("case C3 2048^2 statistics, FP32 vs FP64", flush=True)
+    ref = c3_statistics("f64", False)
+    spread = np.abs(c3_statistics("f64", False, perturbation=1e-7) - ref) / np.abs(ref)
+    print(f"FP64 perturbed by 1e-7: max relative spread (segregation, rate, mean A) {spread.max(axis=0)}", flush=True)
+    for fast in (False, True):
+        got = c3_statistics("f32", fast)
+        err = np.abs(got - ref) / np.abs(ref)
+        print(f"C3 FP32 fast={fast}: max relative error (segregation, rate, mean A) {err.max(axis=0)}; "
+              f"at the last sample {err[-1]}", flush=True)
+        if fast:
+            assert np.all(err <= 1e-4), (fast, err.max(axis=0))
+        else:
+            assert np.all(err <= np.array([5e-5, 1e-3, 2e-4])), (fast, err.max(axis=0))
+
