@@ -70,7 +70,13 @@ enum lbc_flags {
      * f - M^-1 S M (f - feq) with separable transforms.  Up to half (D3Q27
      * MRT: a quarter) of the VALU work of the reference-order form; results
      * differ from it by rounding only (bounded by the tests). */
-    LBC_FLAG_FAST = 1u
+    LBC_FLAG_FAST = 1u,
+    /* lbc_reactive_gradient with LBC_F32 storage only: the forward (and its
+     * checkpoints) in FP32 as without the flag, the adjoint state and its
+     * arithmetic in FP64 on that FP32 trajectory; the gradients are rounded
+     * to FP32.  Ignored with LBC_F64 storage; rejected by the step entry
+     * points. */
+    LBC_FLAG_F64_ADJOINT = 2u
 };
 
 /* lbc_fluid_step with flags (0 = lbc_fluid_step exactly).  Unknown flag bits
@@ -167,7 +173,8 @@ int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_latt
  * or LBC_F64.  flags: 0 (forward =
  * lbc_reactive_fused_step) or LBC_FLAG_FAST (forward =
  * lbc_reactive_fused_step_ex(..., LBC_FLAG_FAST, ...)); the adjoint sweep is
- * the same for both.  The caller passes a
+ * the same for both; with LBC_F32, LBC_FLAG_F64_ADJOINT carries the adjoint
+ * in FP64 on the FP32 trajectory.  The caller passes a
  * device workspace of at least lbc_reactive_gradient_workspace() bytes
  * (checkpoints plus two segment buffers); no device memory is allocated.  block_x * block_y must be a multiple of 64 (whole
  * wavefronts), else hipErrorInvalidConfiguration.  Synchronizes `stream`.

@@ -106,9 +106,11 @@ __device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R& al, R
     by += lam[0] * R(D2Q9::w[0]) * (-R(3) * uy);
 }
 
-template <class R, int CT, int MAXT>
+// RF: the forward states' type (the trajectory); R: the adjoint's state and
+// arithmetic (R = double with RF = float: the FP64 adjoint of an FP32 run).
+template <class RF, class R, int CT, int MAXT>
 __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
-    const R* __restrict__ f, const R* __restrict__ a, const R* __restrict__ b, const R* __restrict__ nuf1,
+    const RF* __restrict__ f, const RF* __restrict__ a, const RF* __restrict__ b, const R* __restrict__ nuf1,
     const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
     R* __restrict__ nub0, Cells c, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
     // XCD-aware block order as in the forward kernels (the reverse pulls share
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         for (int i = 0; i < 9; ++i) {
             const uint32_t src = pull.src(D2Q9::cx[i], D2Q9::cy[i], 0);
             const uint32_t rev = pull.src(-D2Q9::cx[i], -D2Q9::cy[i], 0);
-            fl[i] = plane_ptr(f, i, plane)[src];
+            fl[i] = static_cast<R>(plane_ptr(f, i, plane)[src]);
             lf[i] = plane_ptr(nuf1, i, plane)[rev];
         }
         // BGK, TRT and MRT conserve rho and j, so the velocity the forward
@@ -141,8 +143,8 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         for (int j = 0; j < 5; ++j) {
             const uint32_t src = pull.src(D2Q5::cx[j], D2Q5::cy[j], 0);
             const uint32_t rev = pull.src(-D2Q5::cx[j], -D2Q5::cy[j], 0);
-            ga[j] = plane_ptr(a, j, plane)[src];
-            gb[j] = plane_ptr(b, j, plane)[src];
+            ga[j] = static_cast<R>(plane_ptr(a, j, plane)[src]);
+            gb[j] = static_cast<R>(plane_ptr(b, j, plane)[src]);
             la[j] = plane_ptr(nua1, j, plane)[rev];
             lb[j] = plane_ptr(nub1, j, plane)[rev];
         }
@@ -261,9 +263,10 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
     }
 }
 
-// Lambda[i][y] = nu[i][y + c_i] (+ shift): the adjoint of stored populations.
-template <class R, class L>
-__global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict__ nu, R* __restrict__ lam, Cells c,
+// Lambda[i][y] = nu[i][y + c_i] (+ shift): the adjoint of stored populations,
+// rounded to the gradient's type RO.
+template <class R, class RO, class L>
+__global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict__ nu, RO* __restrict__ lam, Cells c,
                                                             R shift) {
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict_
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const uint32_t rev = pull.src(-L::cx[i], -L::cy[i], 0);
-        lam[i * plane + cell] = plane_ptr(nu, i, plane)[rev] + shift;
+        lam[i * plane + cell] = static_cast<RO>(plane_ptr(nu, i, plane)[rev] + shift);
     }
 }
 
@@ -333,7 +336,7 @@ inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, in
     L.off_seg = off;
     off = align_up(off + 2 * L.seg_slots * L.state_elems * es);  // two recomputed segments
     L.off_nu = off;
-    off = align_up(off + 2 * L.state_elems * es);  // adjoint ping-pong
+    off = align_up(off + 2 * L.state_elems * sizeof(double));  // adjoint ping-pong (FP64 or FP32 elements)
     L.off_part = off;
     off = align_up(off + L.blocks * sizeof(double));
     L.off_red = off;
@@ -366,7 +369,9 @@ struct SweepStreams {
     }
 };
 
-template <class R, int CT>
+// R: the forward's storage and arithmetic; RA: the adjoint's (R, or double for
+// the FP64 adjoint of an FP32 forward).
+template <class R, class RA, int CT>
 hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, const LaunchShape& s, int steps,
                         int interval,
                         double omega_d, double omega_c_d, double k_d, double* J_out, R* gf, R* ga, R* gb,
@@ -376,15 +381,18 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     const size_t cells = L.cells, se = L.state_elems;
     R* ckpt = reinterpret_cast<R*>(ws + L.off_ckpt);
     R* seg = reinterpret_cast<R*>(ws + L.off_seg);
-    R* nu0 = reinterpret_cast<R*>(ws + L.off_nu);
-    R* nu1 = nu0 + se;
+    RA* nu0 = reinterpret_cast<RA*>(ws + L.off_nu);
+    RA* nu1 = nu0 + se;
     double* part = reinterpret_cast<double*>(ws + L.off_part);
     double* red = reinterpret_cast<double*>(ws + L.off_red);
-    const R omega = static_cast<R>(omega_d), omega_c = static_cast<R>(omega_c_d), k = static_cast<R>(k_d);
-    const R wm = CT == kTRT ? trt_omega_minus<R>(omega) : R(0);
+    // the forward's parameters as it rounds them, widened for the adjoint
+    const RA omega = static_cast<RA>(static_cast<R>(omega_d)), omega_c = static_cast<RA>(static_cast<R>(omega_c_d));
+    const RA k = static_cast<RA>(static_cast<R>(k_d));
+    const RA wm = CT == kTRT ? static_cast<RA>(trt_omega_minus<R>(static_cast<R>(omega_d))) : RA(0);
+    const uint32_t forward_flags = flags & static_cast<uint32_t>(LBC_FLAG_FAST);
     hipError_t e;
     auto step_on = [&](hipStream_t on, State<R> in, State<R> out) {
-        return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, flags, in.f, out.f, in.a,
+        return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, forward_flags, in.f, out.f, in.a,
                                                                   out.a, in.b, out.b, s.nx, s.ny, 1, omega_d,
                                                                   omega_c_d, k_d, 0.0, 0.0, s.block.x, s.block.y, 1,
                                                                   on));
@@ -430,11 +438,11 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     }
     const hipStream_t rs = overlap ? ss.side : st;
     // adjoint initial condition: dJ/da_T = -1/N everywhere
-    const R inv_n = R(1) / static_cast<R>(cells);
+    const RA inv_n = RA(1) / static_cast<RA>(cells);
     const unsigned fb = static_cast<unsigned>(std::min<size_t>(4096, (se + 255) / 256));
-    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1, 9 * cells, R(0));
-    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1 + 9 * cells, 5 * cells, -inv_n);
-    hipLaunchKernelGGL((fill_kernel<R>), dim3(fb), dim3(256), 0, st, nu1 + 14 * cells, 5 * cells, R(0));
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1, 9 * cells, RA(0));
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + 9 * cells, 5 * cells, -inv_n);
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + 14 * cells, 5 * cells, RA(0));
     hipLaunchKernelGGL((zero_double_kernel<R>), dim3(4), dim3(256), 0, st, part, L.blocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // Reverse sweep.  Segment sgp's states 1..len-1 are recomputed into
@@ -467,12 +475,13 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
         if (overlap && (e = hipStreamWaitEvent(st, ss.recomputed[sgp & 1], 0)) != hipSuccess) return e;
         auto state_at = [&](int t) { return state_in(sgp, t); };
         for (int t = len - 1; t >= 0; --t) {
-            State<R> fw(state_at(t), cells), in(nu1, cells), out(nu0, cells);
+            State<R> fw(state_at(t), cells);
+            State<RA> in(nu1, cells), out(nu0, cells);
             if (s.threads() <= 256)
-                hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a, fw.b,
-                                   in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k, part);
+                hipLaunchKernelGGL((reactive_adjoint_kernel<R, RA, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a,
+                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k, part);
             else
-                hipLaunchKernelGGL((reactive_adjoint_kernel<R, CT, 1024>), s.grid, s.block, 0, st, fw.f, fw.a,
+                hipLaunchKernelGGL((reactive_adjoint_kernel<R, RA, CT, 1024>), s.grid, s.block, 0, st, fw.f, fw.a,
                                    fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k,
                                    part);
             if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -483,10 +492,10 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
         if (!overlap && sgp >= 1 && (e = recompute(sgp - 1)) != hipSuccess) return e;
     }
     // adjoints of the stored initial populations
-    State<R> fin(nu1, cells);
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.cells, R(0));
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.cells, inv_n);
-    hipLaunchKernelGGL((reverse_pull_kernel<R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.cells, R(0));
+    State<RA> fin(nu1, cells);
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.cells, RA(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.cells, inv_n);
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.cells, RA(0));
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, part, L.blocks, red + 2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double h[3];
@@ -538,15 +547,15 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags,
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
     if (collision != kBGK && collision != kTRT && collision != kMRT) return hipErrorInvalidValue;
-    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST | LBC_FLAG_F64_ADJOINT)) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
     const GradLayout L = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid);
     if (workspace_bytes < L.total) return hipErrorInvalidValue;
     char* ws = static_cast<char*>(workspace);
     const hipStream_t st = as_stream(stream);
-    auto run = [&](auto tag, auto ct) {
+    auto run = [&](auto tag, auto adjoint_tag, auto ct) {
         using R = decltype(tag);
-        return run_gradient<R, decltype(ct)::value>(
+        return run_gradient<R, decltype(adjoint_tag), decltype(ct)::value>(
             flags, static_cast<const R*>(f0), static_cast<const R*>(a0), static_cast<const R*>(b0), s, steps, interval,
             omega, omega_c, k_react, loss, static_cast<R*>(grad_f0), static_cast<R*>(grad_a0),
             static_cast<R*>(grad_b0), dloss_domega_c, ws, st);
@@ -555,6 +564,11 @@ extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags,
     using TRT = std::integral_constant<int, kTRT>;
     using MRT = std::integral_constant<int, kMRT>;
     if (storage == kF64)
-        return collision == kBGK ? run(0.0, BGK{}) : collision == kTRT ? run(0.0, TRT{}) : run(0.0, MRT{});
-    return collision == kBGK ? run(0.0f, BGK{}) : collision == kTRT ? run(0.0f, TRT{}) : run(0.0f, MRT{});
+        return collision == kBGK ? run(0.0, 0.0, BGK{}) : collision == kTRT ? run(0.0, 0.0, TRT{})
+                                                          : run(0.0, 0.0, MRT{});
+    if (flags & LBC_FLAG_F64_ADJOINT)
+        return collision == kBGK ? run(0.0f, 0.0, BGK{}) : collision == kTRT ? run(0.0f, 0.0, TRT{})
+                                                           : run(0.0f, 0.0, MRT{});
+    return collision == kBGK ? run(0.0f, 0.0f, BGK{}) : collision == kTRT ? run(0.0f, 0.0f, TRT{})
+                                                       : run(0.0f, 0.0f, MRT{});
 }

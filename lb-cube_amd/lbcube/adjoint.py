@@ -29,10 +29,15 @@ def workspace_bytes(dtype, nx, ny, steps, interval, block):
     return out.value
 
 
+FLAG_F64_ADJOINT = 2
+
+
 def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, interval=64, block=None,
-                      workspace=None, stream=None, fast=False):
+                      workspace=None, stream=None, fast=False, adjoint_f64=False):
     """(loss, dloss/df0, dloss/da0, dloss/db0, dloss/domega_c) by the discrete adjoint;
-    fast=True differentiates the forward of reactive_fused_step(..., fast=True)."""
+    fast=True differentiates the forward of reactive_fused_step(..., fast=True);
+    adjoint_f64=True (float32 inputs) carries the adjoint in float64 on the
+    float32 trajectory (LBC_FLAG_F64_ADJOINT)."""
     _check_pop("D2Q9", f0, "f0")
     _check_pop("D2Q5", a0, "a0")
     _check_pop("D2Q5", b0, "b0")
@@ -46,7 +51,8 @@ def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, int
         workspace = torch.empty(nbytes, dtype=torch.uint8, device=f0.device)
     gf, ga, gb = torch.empty_like(f0), torch.empty_like(a0), torch.empty_like(b0)
     loss, dw = ctypes.c_double(), ctypes.c_double()
-    check(lib().lbc_reactive_gradient(_COLL[collision], _STORE[f0.dtype], 1 if fast else 0, f0.data_ptr(),
+    flags = (1 if fast else 0) | (FLAG_F64_ADJOINT if adjoint_f64 else 0)
+    check(lib().lbc_reactive_gradient(_COLL[collision], _STORE[f0.dtype], flags, f0.data_ptr(),
                                       a0.data_ptr(), b0.data_ptr(),
                                       nx, ny, int(steps), int(interval), float(omega), float(omega_c),
                                       float(k_react), ctypes.byref(loss), gf.data_ptr(), ga.data_ptr(),
@@ -77,11 +83,11 @@ class ReactiveLoss(torch.autograd.Function):
     """loss = <C>(T) of the reactive loop; differentiable in (f0, a0, b0, tau_s)."""
 
     @staticmethod
-    def forward(ctx, f0, a0, b0, tau_s, collision, steps, omega, k_react, interval, fast):
+    def forward(ctx, f0, a0, b0, tau_s, collision, steps, omega, k_react, interval, fast, adjoint_f64=False):
         omega_c = 1.0 / float(tau_s)
         loss, gf, ga, gb, dw = reactive_gradient(collision, f0.detach().contiguous(), a0.detach().contiguous(),
                                                  b0.detach().contiguous(), steps, omega, omega_c, k_react, interval,
-                                                 fast=fast)
+                                                 fast=fast, adjoint_f64=adjoint_f64)
         dtau = -dw * omega_c * omega_c
         ctx.save_for_backward(gf, ga, gb, torch.tensor(dtau, dtype=torch.float64, device=f0.device))
         return torch.tensor(loss, dtype=f0.dtype, device=f0.device)
@@ -90,13 +96,13 @@ class ReactiveLoss(torch.autograd.Function):
     def backward(ctx, grad):
         gf, ga, gb, dtau = ctx.saved_tensors
         return (grad * gf, grad * ga, grad * gb, (grad.double() * dtau).to(grad.dtype), None, None, None, None, None,
-                None)
+                None, None)
 
 
 def reactive_loss(f0, a0, b0, tau_s, collision="BGK", steps=4096, omega=1.0 / 0.6, k_react=0.05, interval=64,
-                  fast=False):
+                  fast=False, adjoint_f64=False):
     tau = tau_s if torch.is_tensor(tau_s) else torch.tensor(float(tau_s), dtype=torch.float64, device=f0.device)
-    return ReactiveLoss.apply(f0, a0, b0, tau, collision, steps, omega, k_react, interval, fast)
+    return ReactiveLoss.apply(f0, a0, b0, tau, collision, steps, omega, k_react, interval, fast, adjoint_f64)
 
 
 def equilibrium_torch(rho, u):
