@@ -29,23 +29,24 @@ BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16")
 
 
 def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False,
-                 state=None):
+                 state=None, mfma=False):
     """MLUPS of `steps` fluid steps from the equilibrium of velocity field u
-    (or from `state`, a population tensor the call takes over)."""
+    (or from `state`, a population tensor the call takes over); mfma: the
+    deviation-form MRT's transforms on the matrix cores."""
     from lbcube import init as lbinit
 
     a = lbinit.equilibrium(lattice, None, u, storage) if state is None else state
     del state
     b = torch.empty_like(a)
     for _ in range(warmup):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast)
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
         a, b = b, a
     stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     for _ in range(steps):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast)
+        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
         a, b = b, a
     end.record(stream)
     torch.cuda.synchronize()
@@ -94,24 +95,26 @@ def _events_ms(torch, fn, steps, warmup):
     return start.elapsed_time(end) / steps
 
 
-def pattern_ceiling(lbcube, torch, u, steps=20, warmup=3):
-    """GB/s of the pull-streaming step alone (lbc_pull_stream: the headline
-    kernel's loads, stores, addressing and residency, no collision) on the
-    headline's D3Q19 FP32 buffers: the ceiling of its access pattern."""
+def pattern_ceiling(lbcube, torch, u, steps=20, warmup=3, lattice="D3Q19", storage="f32", shift=True):
+    """GB/s of the pull-streaming step alone (lbc_pull_stream: the fluid
+    kernels' loads, stores, addressing and residency, no collision) on the
+    lattice's buffers (default: the headline's D3Q19 FP32): the ceiling of
+    its access pattern.  shift=False: the same plane streams unshifted
+    (lbc_plane_copy)."""
     from lbcube import init as lbinit
 
-    bufs = [lbinit.equilibrium("D3Q19", None, u, "f32")]
+    bufs = [lbinit.equilibrium(lattice, None, u, storage)]
     bufs.append(torch.empty_like(bufs[0]))
 
     def step():
-        lbcube.pull_stream("D3Q19", bufs[0], bufs[1])
+        lbcube.pull_stream(lattice, bufs[0], bufs[1], shift=shift)
         bufs.reverse()
 
     ms = _events_ms(torch, step, steps, warmup)
     cells = bufs[0][0].numel()
     del bufs
     torch.cuda.empty_cache()
-    return 152 * cells / (ms * 1e-3) / 1e9, ms
+    return BYTES_PER_CELL[(lattice, storage)] * cells / (ms * 1e-3) / 1e9, ms
 
 
 def pitched_variants(lbcube, torch, u, omega, steps, warmup, pads=(256, 1024, 4096, 16384)):
@@ -171,8 +174,9 @@ def measured_traffic():
 
 def lattice_sweep(lbcube, torch, steps=10, warmup=3):
     """Taylor-Green Re = 1600 on D3Q19 and D3Q27 at 256^3 and 384^3, TRT and
-    MRT, FP32 storage and FP16 deviation storage (deviation-form TRT; MRT in
-    both forms)."""
+    MRT, FP32 storage and FP16 / BF16 deviation storage (deviation-form TRT;
+    MRT in both forms, and the deviation form with its transforms on the
+    matrix cores), with the pull-stream pattern ceilings of FP32 and FP16."""
     from lbcube import init as lbinit
 
     out = {}
@@ -180,14 +184,23 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
         omega = 1.0 / lbinit.tgv_re1600_tau(n)
         for lattice in ("D3Q19", "D3Q27"):
-            combos = [("TRT", "f32", False), ("TRT", "f16", True), ("MRT", "f32", False), ("MRT", "f32", True),
-                      ("MRT", "f16", True)]
-            for coll, storage, fast in combos:
-                key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}"
-                r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast)
+            combos = [("TRT", "f32", False, False), ("TRT", "f16", True, False), ("MRT", "f32", False, False),
+                      ("MRT", "f32", True, False), ("MRT", "f16", True, False), ("MRT", "f16", True, True),
+                      ("MRT", "bf16", True, False), ("MRT", "bf16", True, True)]
+            for coll, storage, fast, mfma in combos:
+                key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}{'mfma' if mfma else ''}"
+                r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast,
+                                 mfma=mfma)
                 out[key] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                 print(f"# {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s", file=sys.stderr,
                       flush=True)
+            for storage in ("f32", "f16"):
+                key = f"{lattice}_{n}_pull_{storage}"
+                gbs, ms = pattern_ceiling(lbcube, torch, u, steps=steps, warmup=warmup, lattice=lattice,
+                                          storage=storage)
+                out[key] = {"ms_per_step": round(ms, 4), "achieved_gbs": round(gbs, 1),
+                            "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+                print(f"# {key}: pull-stream pattern {gbs:.0f} GB/s", file=sys.stderr, flush=True)
         del u
         torch.cuda.empty_cache()
     return out
@@ -570,6 +583,8 @@ def main():
     copy_gbs = copy_ceiling(lbcube, torch, 152 * n ** 3)
     pattern_gbs, pattern_ms = pattern_ceiling(lbcube, torch, u)
     print(f"# pull-stream ceiling: {pattern_gbs:.0f} GB/s ({pattern_ms:.4f} ms/step)", file=sys.stderr, flush=True)
+    planes_gbs, planes_ms = pattern_ceiling(lbcube, torch, u, shift=False)
+    print(f"# unshifted 38-plane copy: {planes_gbs:.0f} GB/s ({planes_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     chosen = set() if args.no_extras else set(args.extras.split(","))
     pitched = pitched_variants(lbcube, torch, u, omega, args.steps, args.warmup) if "pitched" in chosen else None
     extras = {}
@@ -621,7 +636,9 @@ def main():
                      "copy_ceiling_gbs": round(copy_gbs, 1), "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4),
                      "pattern_ceiling_gbs": round(pattern_gbs, 1),
                      "frac_of_pattern": round(head["achieved_gbs"] / pattern_gbs, 4),
-                     "pattern_kernel": "pull_stream_kernel<D3Q19, F32> (lbc_pull_stream, same buffers)"},
+                     "pattern_kernel": "pull_stream_kernel<D3Q19, F32> (lbc_pull_stream, same buffers)",
+                     "plane_streams_ceiling_gbs": round(planes_gbs, 1),
+                     "plane_streams_kernel": "pull_stream_kernel<D3Q19, F32, unshifted> (lbc_plane_copy, same buffers)"},
         "cpu_baseline": cpu,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},

@@ -76,7 +76,17 @@ enum lbc_flags {
      * arithmetic in FP64 on that FP32 trajectory; the gradients are rounded
      * to FP32.  Ignored with LBC_F64 storage; rejected by the step entry
      * points. */
-    LBC_FLAG_F64_ADJOINT = 2u
+    LBC_FLAG_F64_ADJOINT = 2u,
+    /* lbc_fluid_step_ex / _pitched with LBC_FLAG_FAST, MRT, D3Q19 or D3Q27
+     * and LBC_F16 or LBC_BF16 storage only: the deviation-form MRT with its
+     * two moment transforms (M h and M^T p') on the matrix cores.  The
+     * relaxed non-equilibrium moments p' are rounded to the storage format
+     * before the second product (an error relative to the update); results
+     * agree with LBC_FLAG_FAST alone to that rounding (bounded by the tests).
+     * hipErrorInvalidValue for any other lattice, collision, storage or
+     * without LBC_FLAG_FAST; hipErrorInvalidConfiguration unless the block
+     * holds a whole number of 64-wide wavefronts. */
+    LBC_FLAG_MFMA = 4u
 };
 
 /* lbc_fluid_step with flags (0 = lbc_fluid_step exactly).  Unknown flag bits
@@ -106,6 +116,16 @@ int lbc_fluid_step_pitched(int lattice, int collision, int storage, uint32_t fla
  * lbc_fluid_step.  No counterpart in the reference. */
 int lbc_pull_stream(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny, uint64_t nz,
                     uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
+
+/* The same 2 Q plane streams without the pull shifts: next[i][x] =
+ * current[i][x] for every population plane, with lbc_pull_stream's
+ * addressing, store form, launch shape and residency.  Its rate against
+ * lbc_pull_stream's separates what the shifted reads cost from what the
+ * number of concurrent plane streams costs (lbc_stream_copy: two streams).
+ * Arguments and validation as lbc_pull_stream.  No counterpart in the
+ * reference. */
+int lbc_plane_copy(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny, uint64_t nz,
+                   uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
 
 /* Reference concentrations of scalar fields.  With LBC_F16 / LBC_BF16
  * storage a scalar field's populations are stored as g_i - w_i c_ref, with

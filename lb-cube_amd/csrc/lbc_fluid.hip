@@ -14,6 +14,7 @@
 #include "../../include/lbcube.h"
 #include "lbc_fast.hpp"
 #include "lbc_launch.hpp"
+#include "lbc_mfma.hpp"
 #include "lbc_physics.hpp"
 #include "lbc_storage.hpp"
 
@@ -45,8 +46,9 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             const uint32_t src = pull.src(L::cx[i], L::cy[i], L::cz[i]);
             f[i] = St::load_dev(at<WIDE>(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
-        collide_dev<L, CT, R>(f, omega, omega_minus,
-                              [&](int i, R v) { St::store_dev(at<WIDE>(plane_ptr(nxt, i, plane), dst), v, R(L::w[i])); });
+        collide_dev<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
+            St::store_dev(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v, R(L::w[i]));
+        });
     } else {
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
@@ -55,8 +57,65 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             f[i] = St::load(at<WIDE>(plane_ptr(cur, i, plane), src), off);
         }
         collide_cell<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
-            St::store(at<WIDE>(plane_ptr(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+            St::store(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
         });
+    }
+}
+
+// LBC_FLAG_MFMA: the deviation-form MRT of D3Q19 / D3Q27 with both moment
+// transforms on the matrix cores (lbc_mfma.hpp), FP16 / BF16 storage.  The
+// launcher guarantees blocks of whole waves; a wave that starts past the
+// domain's end leaves at once, and the lanes of a partial last wave take part
+// in the products with zero populations and store nothing.
+template <int ST> __device__ __forceinline__ float decode16(uint32_t bits) {
+    if constexpr (ST == kF16) return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(bits)));
+    else return __builtin_bit_cast(float, bits << 16);
+}
+
+template <class L, int ST, int MAXT, bool WIDE>
+__global__ __launch_bounds__(MAXT) void fluid_step_mfma_kernel(const uint16_t* __restrict__ cur,
+                                                               uint16_t* __restrict__ nxt, Cells c, size_t plane,
+                                                               float omega) {
+    using St = Storage<ST>;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t dst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dst - lane >= c.count) return;  // the whole wave lies past the domain
+    const bool live = dst < c.count;
+    uint32_t hw[16];  // the cell's stored deviations in pairs (population 2v in the low half), zero-padded
+#pragma unroll
+    for (int v = 0; v < 16; ++v) hw[v] = 0u;
+    if (live) {
+        const CellCoord q = cell_coord(dst, c);
+        const Pull pull(dst, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
+#pragma unroll
+        for (int i = 0; i < L::Q; ++i) {
+            const uint32_t raw = get(at<WIDE>(plane_ptr(cur, i, plane), pull.src(L::cx[i], L::cy[i], L::cz[i])));
+            hw[i / 2] |= raw << (16 * (i % 2));
+        }
+    }
+    const MfmaOperands& T = mfma_operands<L, ST>();
+    uint32_t a[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) a[v] = T.a[0][lane][v];
+    float m[32];
+    mfma_product<ST>(a, hw, m);  // m = M h
+    float p[32];
+    if constexpr (L::Q == 19) mfma_relax_d3q19<ST>(m, p, omega);
+    else mfma_relax_d3q27<ST>(m, p, omega);
+    uint32_t pw[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) pw[v] = pack2<ST>(p[2 * v], p[2 * v + 1]);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) a[v] = T.a[1][lane][v];
+    float out[32];
+    mfma_product<ST>(a, pw, out);  // out = M^T p'
+    if (!live) return;
+    constexpr float inv_scale = 1.0f / kMfmaScale<ST>;
+    const uint32_t cell = fresh_lane_value(dst);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+        const float h = decode16<ST>((hw[i / 2] >> (16 * (i % 2))) & 0xFFFFu);
+        St::store_dev(at<WIDE>(plane_ptr_opaque(nxt, i, plane), cell), fmadd(-inv_scale, out[i], h), 0.0f);
     }
 }
 
@@ -65,7 +124,9 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
 // launch shape, and no collision (the identity collision of BGK at omega = 0).
 // Its rate is the bandwidth ceiling of the collide-stream kernels' access
 // pattern: the same 2 Q streams with the x-shifted pulls, without arithmetic.
-template <class L, int ST, int MAXT, bool WIDE>
+// SHIFT = false: the same 2 Q plane streams read without the shifts
+// (next[i][x] = cur[i][x]), the ceiling of the plane streams themselves.
+template <class L, int ST, int MAXT, bool WIDE, bool SHIFT = true>
 __global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                            typename Storage<ST>::S* __restrict__ nxt, Cells c,
                                                            size_t plane) {
@@ -76,9 +137,10 @@ __global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storag
     const Pull pull(dst, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
     typename Storage<ST>::S v[L::Q];
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) v[i] = get(at<WIDE>(plane_ptr(cur, i, plane), pull.src(L::cx[i], L::cy[i], L::cz[i])));
+    for (int i = 0; i < L::Q; ++i)
+        v[i] = get(at<WIDE>(plane_ptr(cur, i, plane), SHIFT ? pull.src(L::cx[i], L::cy[i], L::cz[i]) : dst));
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) put(at<WIDE>(plane_ptr(nxt, i, plane), dst), v[i]);
+    for (int i = 0; i < L::Q; ++i) put(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v[i]);
 }
 
 // Kernels that run faster with residency capped at kCappedWavesPerCu (measured
@@ -143,8 +205,40 @@ hipError_t dispatch_collision(int collision, int storage, bool fast, const void*
     }
 }
 
-// The pull-streaming step shares the residency of the TRT step it bounds.
 template <class L, int ST>
+hipError_t launch_fluid_mfma(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, double omega,
+                             hipStream_t stream) {
+    const float w = static_cast<float>(omega);
+    const uint32_t lds = capped_lds(s.threads());
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const uint16_t*>(cur),
+                           static_cast<uint16_t*>(nxt), s.cells, static_cast<size_t>(plane), w);
+    };
+    const bool wide = wide_planes<uint16_t>(s);
+    if (s.threads() <= 256)
+        wide ? go(fluid_step_mfma_kernel<L, ST, 256, true>) : go(fluid_step_mfma_kernel<L, ST, 256, false>);
+    else
+        wide ? go(fluid_step_mfma_kernel<L, ST, 1024, true>) : go(fluid_step_mfma_kernel<L, ST, 1024, false>);
+    return hipGetLastError();
+}
+
+// LBC_FLAG_MFMA: with LBC_FLAG_FAST, MRT, D3Q19 or D3Q27 and FP16 or BF16
+// storage only; blocks of whole waves.
+hipError_t dispatch_mfma(int lattice, int collision, int storage, uint32_t flags, const void* cur, void* nxt,
+                         const LaunchShape& s, uint64_t plane, double omega, hipStream_t st) {
+    if (!(flags & LBC_FLAG_FAST) || collision != kMRT) return hipErrorInvalidValue;
+    if (lattice != kD3Q19 && lattice != kD3Q27) return hipErrorInvalidValue;
+    if (storage != kF16 && storage != kBF16) return hipErrorInvalidValue;
+    if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;
+    if (lattice == kD3Q19)
+        return storage == kF16 ? launch_fluid_mfma<D3Q19, kF16>(cur, nxt, s, plane, omega, st)
+                               : launch_fluid_mfma<D3Q19, kBF16>(cur, nxt, s, plane, omega, st);
+    return storage == kF16 ? launch_fluid_mfma<D3Q27, kF16>(cur, nxt, s, plane, omega, st)
+                           : launch_fluid_mfma<D3Q27, kBF16>(cur, nxt, s, plane, omega, st);
+}
+
+// The pull-streaming step shares the residency of the TRT step it bounds.
+template <class L, int ST, bool SHIFT>
 hipError_t launch_stream(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, hipStream_t stream) {
     using S = typename Storage<ST>::S;
     const uint32_t lds = capped_residency<L, kTRT, ST, kOffsetStorage<ST>>() ? capped_lds(s.threads()) : 0;
@@ -154,20 +248,43 @@ hipError_t launch_stream(const void* cur, void* nxt, const LaunchShape& s, uint6
     };
     const bool wide = wide_planes<S>(s);
     if (s.threads() <= 256)
-        wide ? go(pull_stream_kernel<L, ST, 256, true>) : go(pull_stream_kernel<L, ST, 256, false>);
+        wide ? go(pull_stream_kernel<L, ST, 256, true, SHIFT>) : go(pull_stream_kernel<L, ST, 256, false, SHIFT>);
     else
-        wide ? go(pull_stream_kernel<L, ST, 1024, true>) : go(pull_stream_kernel<L, ST, 1024, false>);
+        wide ? go(pull_stream_kernel<L, ST, 1024, true, SHIFT>) : go(pull_stream_kernel<L, ST, 1024, false, SHIFT>);
     return hipGetLastError();
 }
 
-template <class L>
+template <class L, bool SHIFT>
 hipError_t stream_storage(int storage, const void* cur, void* nxt, const LaunchShape& s, uint64_t plane,
                           hipStream_t st) {
     switch (storage) {
-        case kF32: return launch_stream<L, kF32>(cur, nxt, s, plane, st);
-        case kF64: return launch_stream<L, kF64>(cur, nxt, s, plane, st);
-        case kF16: return launch_stream<L, kF16>(cur, nxt, s, plane, st);
-        case kBF16: return launch_stream<L, kBF16>(cur, nxt, s, plane, st);
+        case kF32: return launch_stream<L, kF32, SHIFT>(cur, nxt, s, plane, st);
+        case kF64: return launch_stream<L, kF64, SHIFT>(cur, nxt, s, plane, st);
+        case kF16: return launch_stream<L, kF16, SHIFT>(cur, nxt, s, plane, st);
+        case kBF16: return launch_stream<L, kBF16, SHIFT>(cur, nxt, s, plane, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t plane_pitch(const LaunchShape& s, uint64_t requested, uint64_t& plane);
+
+template <bool SHIFT>
+hipError_t stream_entry(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
+                        uint64_t nz, uint64_t pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                        void* stream) {
+    if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current == next) return hipErrorInvalidValue;
+    uint64_t plane = 0;
+    e = plane_pitch(s, pitch, plane);
+    if (e != hipSuccess) return e;
+    const hipStream_t st = as_stream(stream);
+    switch (lattice) {
+        case kD2Q9: return stream_storage<D2Q9, SHIFT>(storage, current, next, s, plane, st);
+        case kD3Q19: return stream_storage<D3Q19, SHIFT>(storage, current, next, s, plane, st);
+        case kD3Q27: return stream_storage<D3Q27, SHIFT>(storage, current, next, s, plane, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -190,12 +307,13 @@ hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, 
     hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (current == next) return hipErrorInvalidValue;
-    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST)) return hipErrorInvalidValue;
+    if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST | LBC_FLAG_MFMA)) return hipErrorInvalidValue;
     uint64_t plane = 0;
     e = plane_pitch(s, pitch, plane);
     if (e != hipSuccess) return e;
     const bool fast = (flags & LBC_FLAG_FAST) != 0;
     const hipStream_t st = as_stream(stream);
+    if (flags & LBC_FLAG_MFMA) return dispatch_mfma(lattice, collision, storage, flags, current, next, s, plane, omega, st);
     switch (lattice) {
         case kD2Q9: return dispatch_collision<D2Q9>(collision, storage, fast, current, next, s, plane, omega, st);
         case kD3Q19: return dispatch_collision<D3Q19>(collision, storage, fast, current, next, s, plane, omega, st);
@@ -224,22 +342,15 @@ extern "C" int lbc_fluid_step_pitched(int lattice, int collision, int storage, u
 extern "C" int lbc_pull_stream(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
                                uint64_t nz, uint64_t plane_pitch, uint32_t block_x, uint32_t block_y,
                                uint32_t block_z, void* stream) {
-    using namespace lbc;
-    if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
-    LaunchShape s;
-    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
-    if (e != hipSuccess) return e;
-    if (current == next) return hipErrorInvalidValue;
-    uint64_t plane = 0;
-    e = lbc::plane_pitch(s, plane_pitch, plane);
-    if (e != hipSuccess) return e;
-    const hipStream_t st = as_stream(stream);
-    switch (lattice) {
-        case kD2Q9: return stream_storage<D2Q9>(storage, current, next, s, plane, st);
-        case kD3Q19: return stream_storage<D3Q19>(storage, current, next, s, plane, st);
-        case kD3Q27: return stream_storage<D3Q27>(storage, current, next, s, plane, st);
-        default: return hipErrorInvalidValue;
-    }
+    return lbc::stream_entry<true>(lattice, storage, current, next, nx, ny, nz, plane_pitch, block_x, block_y,
+                                   block_z, stream);
+}
+
+extern "C" int lbc_plane_copy(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
+                              uint64_t nz, uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                              void* stream) {
+    return lbc::stream_entry<false>(lattice, storage, current, next, nx, ny, nz, plane_pitch, block_x, block_y,
+                                    block_z, stream);
 }
 
 extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const void* current, void* next, uint64_t nx,

@@ -93,6 +93,31 @@ template <class T> __device__ __forceinline__ T* plane_ptr(T* base, int i, size_
     return base + static_cast<size_t>(i) * plane;
 }
 
+// Base of population plane i for stores: the same address with the plane's
+// byte offset passed through v_readfirstlane (a no-op on a value already in
+// scalar registers).  The compiler otherwise sees the Q store addresses
+// base + i * plane + cell as one chain, strength-reduces it to a per-lane
+// 64-bit pointer stepped by the plane size, and spends two or three VALU
+// instructions per store on 64-bit address arithmetic instead of the scalar
+// base + 32-bit lane offset form.
+template <class T> __device__ __forceinline__ T* plane_ptr_opaque(T* base, int i, size_t plane) {
+    const uint64_t off = static_cast<uint64_t>(i) * plane * sizeof(T);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(off));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(off >> 32));
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + ((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// The same per-lane value, recomputed where it is used: an empty asm
+// statement (no instruction) that the optimizer must assume changes it, so
+// the stores' byte offsets are formed in the stores' own block as 32-bit
+// values (the scalar base + 32-bit offset addressing mode) rather than taken
+// over as a 64-bit value from an earlier block.
+__device__ __forceinline__ uint32_t fresh_lane_value(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Element `cell` of a plane.  Planes of at most 4 GiB (WIDE = false): uniform
 // base + zero-extended 32-bit byte offset, the form the global-load/store
 // "saddr" addressing mode takes directly (no per-lane 64-bit address

@@ -93,20 +93,27 @@ def _block(lattice, block, nx, ny, nz):
 
 
 FLAG_FAST = 1
+FLAG_MFMA = 4
 
 
-def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False):
+def _step_flags(fast, mfma):
+    return (FLAG_FAST if fast else 0) | (FLAG_MFMA if mfma else 0)
+
+
+def fluid_step(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False, mfma=False):
     """Fused pull-stream + collide: nxt <- step(cur). Replaces launch_step_gpu.
 
     fast=True selects the deviation-form FMA arithmetic for BGK, TRT and MRT
-    (LBC_FLAG_FAST); the default evaluates the reference's operation order."""
+    (LBC_FLAG_FAST); the default evaluates the reference's operation order.
+    mfma=True (with fast=True; D3Q19 / D3Q27 MRT, float16 / bfloat16 storage)
+    runs that form's moment transforms on the matrix cores (LBC_FLAG_MFMA)."""
     _check_pop(lattice, cur, "cur")
     _check_pop(lattice, nxt, "nxt")
     _check_set([("cur", cur), ("nxt", nxt)])
     nx, ny, nz = _extents(lattice, cur)
     bx, by, bz = _block(lattice, block, nx, ny, nz)
     check(lib().lbc_fluid_step_ex(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
-                                  FLAG_FAST if fast else 0, cur.data_ptr(), nxt.data_ptr(), nx, ny, nz, float(omega),
+                                  _step_flags(fast, mfma), cur.data_ptr(), nxt.data_ptr(), nx, ny, nz, float(omega),
                                   bx, by, bz, _stream(stream)), "lbc_fluid_step_ex")
 
 
@@ -120,7 +127,7 @@ def _check_flat(lattice, t, name, extents, pitch):
 
 
 def fluid_step_pitched(lattice, collision, cur, nxt, extents, plane_pitch, omega, block=None, stream=None,
-                       fast=False):
+                       fast=False, mfma=False):
     """lbc_fluid_step_pitched on flat buffers: plane i at element i * plane_pitch
     (0: the contiguous layout); extents = (nx, ny, nz)."""
     nx, ny, nz = (int(v) for v in extents)
@@ -132,15 +139,16 @@ def fluid_step_pitched(lattice, collision, cur, nxt, extents, plane_pitch, omega
         raise ValueError("cur and nxt must share dtype and device")
     bx, by, bz = _block(lattice, block, nx, ny, nz)
     check(lib().lbc_fluid_step_pitched(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
-                                       FLAG_FAST if fast else 0, cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
+                                       _step_flags(fast, mfma), cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
                                        int(plane_pitch), float(omega), bx, by, bz, _stream(stream)),
           "lbc_fluid_step_pitched")
 
 
-def pull_stream(lattice, cur, nxt, extents=None, plane_pitch=0, block=None, stream=None):
-    """The pull-streaming step alone (lbc_pull_stream).  Population tensors of
-    shape (Q, ...) for the contiguous layout, or flat buffers with `extents`
-    and `plane_pitch`."""
+def pull_stream(lattice, cur, nxt, extents=None, plane_pitch=0, block=None, stream=None, shift=True):
+    """The pull-streaming step alone (lbc_pull_stream), or with shift=False
+    the same plane streams copied in place (lbc_plane_copy).  Population
+    tensors of shape (Q, ...) for the contiguous layout, or flat buffers with
+    `extents` and `plane_pitch`."""
     if extents is None:
         _check_pop(lattice, cur, "cur")
         _check_pop(lattice, nxt, "nxt")
@@ -155,8 +163,9 @@ def pull_stream(lattice, cur, nxt, extents=None, plane_pitch=0, block=None, stre
         if cur.dtype != nxt.dtype or cur.device != nxt.device:
             raise ValueError("cur and nxt must share dtype and device")
     bx, by, bz = _block(lattice, block, nx, ny, nz)
-    check(lib().lbc_pull_stream(LATTICES[lattice], STORAGE[cur.dtype], cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
-                                int(plane_pitch), bx, by, bz, _stream(stream)), "lbc_pull_stream")
+    name = "lbc_pull_stream" if shift else "lbc_plane_copy"
+    check(getattr(lib(), name)(LATTICES[lattice], STORAGE[cur.dtype], cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
+                               int(plane_pitch), bx, by, bz, _stream(stream)), name)
 
 
 def scalar_step(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None, stream=None,

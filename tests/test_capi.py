@@ -83,6 +83,26 @@ def test_unknown_codes_rejected(bad):
     assert fluid(FAKE_A, FAKE_B, lattice=lattice, coll=coll, storage=storage) == HIP_INVALID_VALUE
 
 
+def fluid_ex(flags, lattice=1, coll=2, storage=2, block=(256, 1, 1)):
+    return lbcube.lib().lbc_fluid_step_ex(lattice, coll, storage, flags, FAKE_A, FAKE_B, 8, 8, 8, 1.0, *block, None)
+
+
+def test_mfma_flag_validation():
+    """LBC_FLAG_MFMA only with LBC_FLAG_FAST, MRT, D3Q19 / D3Q27 and 16-bit
+    storage, and only with blocks of whole 64-wide wavefronts."""
+    fast, mfma = 1, 4
+    assert fluid_ex(mfma) == HIP_INVALID_VALUE  # without the deviation form
+    assert fluid_ex(fast | mfma, coll=1) == HIP_INVALID_VALUE  # TRT
+    assert fluid_ex(fast | mfma, coll=3) == HIP_INVALID_VALUE  # RLBM
+    assert fluid_ex(fast | mfma, lattice=0) == HIP_INVALID_VALUE  # D2Q9
+    assert fluid_ex(fast | mfma, storage=0) == HIP_INVALID_VALUE  # FP32
+    assert fluid_ex(fast | mfma, storage=1) == HIP_INVALID_VALUE  # FP64
+    assert fluid_ex(fast | mfma, block=(100, 1, 1)) == HIP_INVALID_CONFIGURATION
+    assert fluid_ex(fast | mfma, lattice=2, storage=3, block=(8, 4, 1)) == HIP_INVALID_CONFIGURATION  # 32
+    assert fluid_ex(fast | 8) == HIP_INVALID_VALUE  # unknown bit
+    assert fluid_ex(fast | 2) == HIP_INVALID_VALUE  # the adjoint's flag
+
+
 def test_validation_scalar_and_reaction():
     L = lbcube.lib()
     assert L.lbc_scalar_step(0, 3, 0, FAKE_A, FAKE_B, None, 8, 8, 1, 1.0, 0.0, 0.0, 16, 8, 1, None) == \
@@ -137,7 +157,8 @@ def test_validation_pitched_and_pull_stream():
     order, then reject a plane pitch below the cell count."""
     L = lbcube.lib()
     for fn in (lambda *a: L.lbc_fluid_step_pitched(1, 1, 0, 0, *a[:2], *a[2:5], a[5], 1.0, *a[6:], None),
-               lambda *a: L.lbc_pull_stream(1, 0, *a[:2], *a[2:5], a[5], *a[6:], None)):
+               lambda *a: L.lbc_pull_stream(1, 0, *a[:2], *a[2:5], a[5], *a[6:], None),
+               lambda *a: L.lbc_plane_copy(1, 0, *a[:2], *a[2:5], a[5], *a[6:], None)):
         assert fn(None, FAKE_B, 8, 8, 8, 0, 256, 1, 1) == HIP_INVALID_DEVICE_POINTER
         assert fn(FAKE_A, FAKE_B, 8, 8, 8, 0, 0, 1, 1) == HIP_INVALID_CONFIGURATION
         assert fn(FAKE_A, FAKE_B, 8, 0, 8, 0, 256, 1, 1) == HIP_INVALID_VALUE
@@ -147,3 +168,5 @@ def test_validation_pitched_and_pull_stream():
     assert L.lbc_fluid_step_pitched(1, 1, 0, 2, FAKE_A, FAKE_B, 8, 8, 8, 0, 1.0, 256, 1, 1, None) == \
         HIP_INVALID_VALUE  # unknown flag
     assert L.lbc_pull_stream(4, 0, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE  # scalar lattice
+    assert L.lbc_plane_copy(4, 0, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE
+    assert L.lbc_plane_copy(1, 7, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE  # storage

@@ -1,5 +1,6 @@
-"""GPU: the pull-streaming step alone (lbc_pull_stream) and the fluid step on
-pitched population buffers (lbc_fluid_step_pitched).
+"""GPU: the pull-streaming step alone (lbc_pull_stream), the unshifted plane
+copy (lbc_plane_copy) and the fluid step on pitched population buffers
+(lbc_fluid_step_pitched).
 
   * lbc_pull_stream is the periodic shift next[i][x] = cur[i][x - c_i]
     (include/lattice_core.hpp:70-78) bit for bit, on odd extents, in every
@@ -81,6 +82,16 @@ def test_pull_stream_is_the_periodic_shift(gpu, lattice, storage):
     pad = planes[:, cells:]
     ref = bits(pitched(cur, pitch, 0x3C3C)).reshape(q, pitch)[:, cells:]
     assert np.array_equal(pad, ref)
+    # lbc_plane_copy: the same plane streams without the shifts (the identity)
+    same = torch.empty_like(cur)
+    lbcube.pull_stream(lattice, cur, same, shift=False)
+    fc, fn = pitched(cur, pitch, 0x5A5A), pitched(cur, pitch, 0x3C3C)
+    lbcube.pull_stream(lattice, fc, fn, extents=ext, plane_pitch=pitch, shift=False)
+    torch.cuda.synchronize()
+    assert np.array_equal(bits(same), bits(cur))
+    planes = bits(fn).reshape(q, pitch)
+    assert np.array_equal(planes[:, :cells].reshape(got.shape), bits(cur))
+    assert np.array_equal(planes[:, cells:], ref)
 
 
 CASES = [("D3Q19", "TRT", "f32", False), ("D3Q19", "TRT", "f32", True), ("D3Q19", "TRT", "f16", True),
