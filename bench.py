@@ -508,6 +508,47 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     del f, a, b, state
     torch.cuda.empty_cache()
     out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C, {n}^2 double shear layer, Da=10 (k = Da u0 / delta = 0.005)"
+    out.update(reactive_2d_large(lbcube, torch))
+    return out
+
+
+def reactive_2d_large(lbcube, torch, n=8192, steps=20):
+    """The same fused step on an 8192^2 double shear layer (16x the cells of
+    the 2048^2 line, ~0.9 ms per FP16 step against ~0.06): the rate of the
+    kernel when the fixed start and drain of a launch no longer count."""
+    from lbcube import init as lbinit
+
+    out = {}
+    delta = 80.0 * n / 2048
+    u = lbinit.double_shear_layer(n, u0=0.04, delta=delta)
+    ca, cb = lbinit.tanh_half_planes(n, delta=delta)
+    k = 10.0 * 0.04 / delta
+    for storage, fast, ref in (("f32", False, 0.0), ("f16", False, 0.5), ("f16", True, 0.5)):
+        state = [lbinit.equilibrium("D2Q9", None, u, storage)]
+        state.append(torch.empty_like(state[0]))
+        for c in (ca, cb):
+            state.append(lbinit.scalar_equilibrium("D2Q5", c, u, storage, ref))
+            state.append(torch.empty_like(state[-1]))
+
+        def fused(state=state, fast=fast, ref=ref):
+            f, ft, a, at, b, bt = state
+            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast,
+                                       ref_a=ref, ref_b=ref)
+            state[:] = [ft, f, at, a, bt, b]
+
+        for _ in range(2):
+            fused()
+        ms = _graph_ms(torch, fused, steps)
+        bpc = 38 * (4 if storage == "f32" else 2)
+        key = f"{storage}{'_fast' if fast else ''}_{n}"
+        out[key] = {"mlups": n * n / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * n * n / (ms * 1e-3) / 1e9,
+                    "bytes_per_cell": bpc}
+        print(f"# reactive 2D {key}: {out[key]['mlups']:.0f} MLUPS, {out[key]['achieved_gbs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+        del state
+        torch.cuda.empty_cache()
+    del u, ca, cb
+    torch.cuda.empty_cache()
     return out
 
 

@@ -208,6 +208,23 @@ int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void
                           void* workspace, uint64_t workspace_bytes, uint32_t block_x, uint32_t block_y,
                           void* stream);
 
+/* The same for the 3D reactive loop: a D3Q19 or D3Q27 fluid (fluid_lattice
+ * LBC_D3Q19 / LBC_D3Q27; BGK or TRT, else hipErrorInvalidValue) plus two D3Q7
+ * species, `steps` calls of lbc_reactive_fused_step(fluid_lattice,
+ * collision, LBC_D3Q7, ...) from f0, a0, b0; loss <A>(0) - <A>(T).  The
+ * workspace holds the checkpoints and two segment buffers of states of
+ * Q + 14 planes.  Flags, storage, checkpoints, streams, block rule and
+ * synchronisation as lbc_reactive_gradient.  No counterpart in the reference. */
+int lbc_reactive_gradient_workspace_3d(int fluid_lattice, int storage, uint64_t nx, uint64_t ny, uint64_t nz,
+                                       int steps, int interval, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                       uint64_t* bytes);
+int lbc_reactive_gradient_3d(int fluid_lattice, int collision, int storage, uint32_t flags, const void* f0,
+                             const void* a0, const void* b0, uint64_t nx, uint64_t ny, uint64_t nz, int steps,
+                             int interval, double omega, double omega_c, double k_react, double* loss,
+                             void* grad_f0, void* grad_a0, void* grad_b0, double* dloss_domega_c, void* workspace,
+                             uint64_t workspace_bytes, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                             void* stream);
+
 /* Ensembles of independent 2D reactive instances (D2Q9 BGK/TRT + two D2Q5
  * species, A + B -> C), each with its own extents, omega, omega_c and k, in
  * three packed buffers per time level (fluid, A, B) and advanced by one launch

@@ -1,6 +1,7 @@
-// Discrete adjoint of the 2D reactive loop: D2Q9 BGK, TRT or MRT fluid plus two
-// D2Q5 species reacting A + B -> C (the loop of src/simulate_reaction.cpp:516-546,
-// run by lbc_reactive_fused_step).  No counterpart in the reference.
+// Discrete adjoint of the reactive loop: a D2Q9 (BGK, TRT or MRT) or D3Q19 /
+// D3Q27 (BGK or TRT) fluid plus two D2Q5 / D3Q7 species reacting A + B -> C
+// (the loop of src/simulate_reaction.cpp:516-546, run by
+// lbc_reactive_fused_step).  No counterpart in the reference.
 //
 // Forward step n: f_{n+1} = collide(pull f_n); (a, b)_{n+1} = reaction step of
 // pulled (a, b)_n advected by u(f_{n+1}) at the same cell.  The adjoint keeps
@@ -25,6 +26,7 @@
 #include <cstdint>
 #include <cstring>
 #include <type_traits>
+#include <utility>
 
 #include "../../include/lbcube.h"
 #include "lbc_launch.hpp"
@@ -81,54 +83,62 @@ template <class R> __device__ __forceinline__ void reaction_grad(R a, R b, R k, 
 }
 
 // alpha = sum lam_q w_q P_q, beta = sum lam_q w_q grad_u P_q: the pieces of the
-// transposed equilibrium Jacobian (d feq_q / d f_i = w_q [P_q + G_q . (c_i - u)]).
-// sum_q w_q P_q = 1 exactly, but not with the weights rounded to binary (their
-// FP32 sum is 1 + 7e-9 for D2Q9), which would make a constant adjoint grow by
-// that much per step: the rest direction takes w_0 P_0 = 1 - sum_{q>0} w_q P_q.
-template <class R>
-__device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R& al, R& bx, R& by) {
-    const R uu = ux * ux + uy * uy;
-    al = bx = by = R(0);
+// transposed equilibrium Jacobian (d feq_q / d f_i = w_q [P_q + G_q . (c_i - u)],
+// P_q = 1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u, G_q = 3 c + 9 (c.u) c - 3 u; cs2 =
+// 1/3 for every fluid lattice).  sum_q w_q P_q = 1 exactly, but not with the
+// weights rounded to binary (their FP32 sum is 1 + 7e-9 for D2Q9), which would
+// make a constant adjoint grow by that much per step: the rest direction takes
+// w_0 P_0 = 1 - sum_{q>0} w_q P_q.
+template <class F, class R>
+__device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R uz, R& al, R& bx, R& by, R& bz) {
+    const R uu = F::D == 3 ? ux * ux + uy * uy + uz * uz : ux * ux + uy * uy;
+    al = bx = by = bz = R(0);
     R wp_moving = R(0);
 #pragma unroll
-    for (int q = 1; q < 9; ++q) {
-        const R w = R(D2Q9::w[q]);
-        const R cx = R(D2Q9::cx[q]), cy = R(D2Q9::cy[q]);
-        const R cu = cx * ux + cy * uy;
+    for (int q = 1; q < F::Q; ++q) {
+        const R w = R(F::w[q]);
+        const R cx = R(F::cx[q]), cy = R(F::cy[q]), cz = R(F::cz[q]);
+        const R cu = F::D == 3 ? cx * ux + cy * uy + cz * uz : cx * ux + cy * uy;
         const R wP = w * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu);
         wp_moving += wP;
         al += lam[q] * wP;
         bx += lam[q] * w * (R(3) * cx + R(9) * cu * cx - R(3) * ux);
         by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * uy);
+        if constexpr (F::D == 3) bz += lam[q] * w * (R(3) * cz + R(9) * cu * cz - R(3) * uz);
     }
     al += lam[0] * (R(1) - wp_moving);
-    bx += lam[0] * R(D2Q9::w[0]) * (-R(3) * ux);
-    by += lam[0] * R(D2Q9::w[0]) * (-R(3) * uy);
+    bx += lam[0] * R(F::w[0]) * (-R(3) * ux);
+    by += lam[0] * R(F::w[0]) * (-R(3) * uy);
+    if constexpr (F::D == 3) bz += lam[0] * R(F::w[0]) * (-R(3) * uz);
 }
 
-// RF: the forward states' type (the trajectory); R: the adjoint's state and
-// arithmetic (R = double with RF = float: the FP64 adjoint of an FP32 run).
-template <class RF, class R, int CT, int MAXT>
+// F, S: the fluid and species lattices; RF: the forward states' type (the
+// trajectory); R: the adjoint's state and arithmetic (R = double with RF =
+// float: the FP64 adjoint of an FP32 run).
+template <class F, class S, class RF, class R, int CT, int MAXT>
 __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
     const RF* __restrict__ f, const RF* __restrict__ a, const RF* __restrict__ b, const R* __restrict__ nuf1,
     const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
     R* __restrict__ nub0, Cells c, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
+    static_assert(F::D == S::D, "a fluid and a species lattice of one dimension");
+    static_assert(CT != kMRT || F::Q == 9, "the MRT adjoint covers D2Q9");
+    constexpr int QF = F::Q, QS = S::Q;
     // XCD-aware block order as in the forward kernels (the reverse pulls share
     // block-edge lines the same way); each block keeps its own dJ/d omega_c
     // slot, indexed by the dispatch id, so a slot always sums the same cells
-    const uint32_t block = renumber_blocks<2, R>() ? xcd_block() : blockIdx.x;
+    const uint32_t block = renumber_blocks<F::D, R>() ? xcd_block() : blockIdx.x;
     const uint32_t cell = block * blockDim.x + threadIdx.x;
     double acc = 0.0;
     if (cell < c.count) {
         const CellCoord q = cell_coord(cell, c);
-        const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
+        const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
         const size_t plane = c.count;
-        const Pull pull(cell, q.x, q.y, 0, nx, ny, 1);
-        R fl[9], lf[9];
+        const Pull pull(cell, q.x, q.y, q.z, nx, ny, nz);
+        R fl[QF], lf[QF];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            const uint32_t src = pull.src(D2Q9::cx[i], D2Q9::cy[i], 0);
-            const uint32_t rev = pull.src(-D2Q9::cx[i], -D2Q9::cy[i], 0);
+        for (int i = 0; i < QF; ++i) {
+            const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
+            const uint32_t rev = pull.src(-F::cx[i], -F::cy[i], -F::cz[i]);
             fl[i] = static_cast<R>(plane_ptr(f, i, plane)[src]);
             lf[i] = plane_ptr(nuf1, i, plane)[rev];
         }
@@ -136,31 +146,34 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         // scalar step read from the post-collision state equals that of the
         // pulled state up to rounding: the linearisation point is taken from
         // the pulled populations, without re-colliding them.
-        const Macro<R> m = macro_state<D2Q9, R>(fl);
+        const Macro<R> m = macro_state<F, R>(fl);
         const Macro<R>& mp = m;
-        R ga[5], gb[5], la[5], lb[5];
+        R ga[QS], gb[QS], la[QS], lb[QS];
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const uint32_t src = pull.src(D2Q5::cx[j], D2Q5::cy[j], 0);
-            const uint32_t rev = pull.src(-D2Q5::cx[j], -D2Q5::cy[j], 0);
+        for (int j = 0; j < QS; ++j) {
+            const uint32_t src = pull.src(S::cx[j], S::cy[j], S::cz[j]);
+            const uint32_t rev = pull.src(-S::cx[j], -S::cy[j], -S::cz[j]);
             ga[j] = static_cast<R>(plane_ptr(a, j, plane)[src]);
             gb[j] = static_cast<R>(plane_ptr(b, j, plane)[src]);
             la[j] = plane_ptr(nua1, j, plane)[rev];
             lb[j] = plane_ptr(nub1, j, plane)[rev];
         }
-        const R ca = concentration<D2Q5, R>(ga), cb = concentration<D2Q5, R>(gb);
+        const R ca = concentration<S, R>(ga), cb = concentration<S, R>(gb);
         R dsa, dsb;
         reaction_grad<R>(ca, cb, k, dsa, dsb);
-        const R inv_cs2 = R(3);
+        static_assert(1.0 / S::cs2 == 3.0 || 1.0 / S::cs2 == 4.0, "1 / cs2 of D2Q5 (3) or D3Q7 (4)");
+        const R inv_cs2 = R(1.0 / S::cs2);
         // Rest direction by remainder (w_0 e_0 = 1 - sum_{j>0} w_j e_j, and
         // likewise w_0), so a constant adjoint is carried without the 3e-8
-        // per-step growth of the D2Q5 weights rounded to FP32.
-        R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0, we_moving = 0, w_moving = 0;
+        // per-step growth of the species weights rounded to FP32.
+        R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, paz = 0, pbx = 0, pby = 0, pbz = 0, we_moving = 0,
+          w_moving = 0;
 #pragma unroll
-        for (int j = 1; j < 5; ++j) {
-            const R w = R(D2Q5::w[j]);
-            const R cx = R(D2Q5::cx[j]), cy = R(D2Q5::cy[j]);
-            const R we = w * (R(1) + (cx * mp.ux + cy * mp.uy) * inv_cs2);
+        for (int j = 1; j < QS; ++j) {
+            const R w = R(S::w[j]);
+            const R cx = R(S::cx[j]), cy = R(S::cy[j]), cz = R(S::cz[j]);
+            const R cu = S::D == 3 ? cx * mp.ux + cy * mp.uy + cz * mp.uz : cx * mp.ux + cy * mp.uy;
+            const R we = w * (R(1) + cu * inv_cs2);
             we_moving += we;
             w_moving += w;
             ea += la[j] * we;
@@ -171,6 +184,10 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
             pay += la[j] * w * cy;
             pbx += lb[j] * w * cx;
             pby += lb[j] * w * cy;
+            if constexpr (S::D == 3) {
+                paz += la[j] * w * cz;
+                pbz += lb[j] * w * cz;
+            }
             acc += static_cast<double>(la[j] * (ca * we - ga[j]) + lb[j] * (cb * we - gb[j]));
         }
         {
@@ -183,16 +200,26 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
         }
         const R sw = wa + wb;
 #pragma unroll
-        for (int j = 0; j < 5; ++j) {
+        for (int j = 0; j < QS; ++j) {
             put(plane_ptr(nua0, j, plane) + cell, (R(1) - omega_c) * la[j] + omega_c * ea + sw * dsa);
             put(plane_ptr(nub0, j, plane) + cell, (R(1) - omega_c) * lb[j] + omega_c * eb + sw * dsb);
         }
         const R lux = omega_c * inv_cs2 * (ca * pax + cb * pbx);
         const R luy = omega_c * inv_cs2 * (ca * pay + cb * pby);
+        const R luz = omega_c * inv_cs2 * (ca * paz + cb * pbz);
         const R irho = R(1) / mp.rho;
 #pragma unroll
-        for (int i = 0; i < 9; ++i)
-            lf[i] += (lux * (R(D2Q9::cx[i]) - mp.ux) + luy * (R(D2Q9::cy[i]) - mp.uy)) * irho;
+        for (int i = 0; i < QF; ++i) {
+            R t = lux * (R(F::cx[i]) - mp.ux) + luy * (R(F::cy[i]) - mp.uy);
+            if constexpr (F::D == 3) t += luz * (R(F::cz[i]) - mp.uz);
+            lf[i] += t * irho;
+        }
+        // d feq / d f applied as (c_i - u) . b
+        auto dev = [&](int i, R bx, R by, R bz) {
+            R t = bx * (R(F::cx[i]) - m.ux) + by * (R(F::cy[i]) - m.uy);
+            if constexpr (F::D == 3) t += bz * (R(F::cz[i]) - m.uz);
+            return t;
+        };
         if constexpr (CT == kMRT) {
             // f' = M^-1 [(I - S) M f + S m_eq(rho, j)], M^-1 = M^T D^-1
             // (collide_mrt_d2q9): J^T lam = M^T (I - S) mu + (d m_eq/d f)^T S mu
@@ -221,30 +248,26 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
                     t + grho + R(D2Q9::cx[i]) * gjx + R(D2Q9::cy[i]) * gjy);
             }
         } else if constexpr (CT == kBGK) {
-            R al, bx, by;
-            feq_transpose<R>(lf, m.ux, m.uy, al, bx, by);
+            R al, bx, by, bz;
+            feq_transpose<F, R>(lf, m.ux, m.uy, m.uz, al, bx, by, bz);
 #pragma unroll
-            for (int i = 0; i < 9; ++i)
-                put(plane_ptr(nuf0, i, plane) + cell,
-                    (R(1) - omega) * lf[i] +
-                        omega * (al + bx * (R(D2Q9::cx[i]) - m.ux) + by * (R(D2Q9::cy[i]) - m.uy)));
+            for (int i = 0; i < QF; ++i)
+                put(plane_ptr(nuf0, i, plane) + cell, (R(1) - omega) * lf[i] + omega * (al + dev(i, bx, by, bz)));
         } else {
-            R lp[9], lm[9];
+            R lp[QF], lm[QF];
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                lp[i] = R(0.5) * (lf[i] + lf[D2Q9::opp[i]]);
-                lm[i] = R(0.5) * (lf[i] - lf[D2Q9::opp[i]]);
+            for (int i = 0; i < QF; ++i) {
+                lp[i] = R(0.5) * (lf[i] + lf[F::opp[i]]);
+                lm[i] = R(0.5) * (lf[i] - lf[F::opp[i]]);
             }
-            R ap, bxp, byp, am, bxm, bym;
-            feq_transpose<R>(lp, m.ux, m.uy, ap, bxp, byp);
-            feq_transpose<R>(lm, m.ux, m.uy, am, bxm, bym);
+            R ap, bxp, byp, bzp, am, bxm, bym, bzm;
+            feq_transpose<F, R>(lp, m.ux, m.uy, m.uz, ap, bxp, byp, bzp);
+            feq_transpose<F, R>(lm, m.ux, m.uy, m.uz, am, bxm, bym, bzm);
 #pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const R dx = R(D2Q9::cx[i]) - m.ux, dy = R(D2Q9::cy[i]) - m.uy;
+            for (int i = 0; i < QF; ++i)
                 put(plane_ptr(nuf0, i, plane) + cell, lf[i] - omega * lp[i] - omega_minus * lm[i] +
-                                                          omega * (ap + bxp * dx + byp * dy) +
-                                                          omega_minus * (am + bxm * dx + bym * dy));
-            }
+                                                          omega * (ap + dev(i, bxp, byp, bzp)) +
+                                                          omega_minus * (am + dev(i, bxm, bym, bzm)));
         }
     }
     // d J / d omega_c: per-block partial added to this block's own slot, so
@@ -271,12 +294,12 @@ __global__ __launch_bounds__(1024) void reverse_pull_kernel(const R* __restrict_
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
     const CellCoord q = cell_coord(cell, c);
-    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny);
+    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
     const size_t plane = c.count;
-    const Pull pull(cell, q.x, q.y, 0, nx, ny, 1);
+    const Pull pull(cell, q.x, q.y, q.z, nx, ny, nz);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
-        const uint32_t rev = pull.src(-L::cx[i], -L::cy[i], 0);
+        const uint32_t rev = pull.src(-L::cx[i], -L::cy[i], -L::cz[i]);
         lam[i * plane + cell] = static_cast<RO>(plane_ptr(nu, i, plane)[rev] + shift);
     }
 }
@@ -322,10 +345,11 @@ struct GradLayout {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
-inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, int interval, dim3 grid) {
+// planes: population planes of one state (Q of the fluid + 2 Q of the species).
+inline GradLayout grad_layout(size_t es, size_t planes, uint64_t cells, int steps, int interval, dim3 grid) {
     GradLayout L{};
-    L.cells = nx * ny;
-    L.state_elems = 19 * L.cells;
+    L.cells = cells;
+    L.state_elems = planes * L.cells;
     L.es = es;
     L.nseg = (static_cast<size_t>(steps) + interval - 1) / interval;
     L.seg_slots = static_cast<size_t>(interval);
@@ -345,9 +369,11 @@ inline GradLayout grad_layout(size_t es, uint64_t nx, uint64_t ny, int steps, in
     return L;
 }
 
-template <class R> struct State {
+// One state of the loop in a workspace slot: the fluid's Q planes, then each
+// species' Q planes.
+template <class F, class S, class R> struct State {
     R *f, *a, *b;
-    State(R* base, size_t cells) : f(base), a(base + 9 * cells), b(base + 14 * cells) {}
+    State(R* base, size_t cells) : f(base), a(base + F::Q * cells), b(base + (F::Q + S::Q) * cells) {}
 };
 
 // The second stream of the reverse sweep and the events that order it
@@ -369,15 +395,20 @@ struct SweepStreams {
     }
 };
 
-// R: the forward's storage and arithmetic; RA: the adjoint's (R, or double for
-// the FP64 adjoint of an FP32 forward).
-template <class R, class RA, int CT>
+template <class F, class S> constexpr size_t kStatePlanes = F::Q + 2 * S::Q;
+
+// F, S: the fluid and species lattices; R: the forward's storage and
+// arithmetic; RA: the adjoint's (R, or double for the FP64 adjoint of an FP32
+// forward).
+template <class F, class S, class R, class RA, int CT>
 hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, const LaunchShape& s, int steps,
                         int interval,
                         double omega_d, double omega_c_d, double k_d, double* J_out, R* gf, R* ga, R* gb,
                         double* djdwc_out, char* ws, hipStream_t st) {
+    using St = State<F, S, R>;
+    using StA = State<F, S, RA>;
     const int storage = std::is_same<R, double>::value ? kF64 : kF32;
-    const GradLayout L = grad_layout(sizeof(R), s.nx, s.ny, steps, interval, s.grid);
+    const GradLayout L = grad_layout(sizeof(R), kStatePlanes<F, S>, s.cells.count, steps, interval, s.grid);
     const size_t cells = L.cells, se = L.state_elems;
     R* ckpt = reinterpret_cast<R*>(ws + L.off_ckpt);
     R* seg = reinterpret_cast<R*>(ws + L.off_seg);
@@ -391,28 +422,30 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     const RA wm = CT == kTRT ? static_cast<RA>(trt_omega_minus<R>(static_cast<R>(omega_d))) : RA(0);
     const uint32_t forward_flags = flags & static_cast<uint32_t>(LBC_FLAG_FAST);
     hipError_t e;
-    auto step_on = [&](hipStream_t on, State<R> in, State<R> out) {
-        return static_cast<hipError_t>(lbc_reactive_fused_step_ex(kD2Q9, CT, kD2Q5, storage, forward_flags, in.f, out.f, in.a,
-                                                                  out.a, in.b, out.b, s.nx, s.ny, 1, omega_d,
-                                                                  omega_c_d, k_d, 0.0, 0.0, s.block.x, s.block.y, 1,
-                                                                  on));
+    constexpr size_t QF = F::Q, QS = S::Q;
+    auto step_on = [&](hipStream_t on, St in, St out) {
+        return static_cast<hipError_t>(lbc_reactive_fused_step_ex(F::id, CT, S::id, storage, forward_flags, in.f,
+                                                                  out.f, in.a, out.a, in.b, out.b, s.nx, s.ny, s.nz,
+                                                                  omega_d, omega_c_d, k_d, 0.0, 0.0, s.block.x,
+                                                                  s.block.y, s.block.z, on));
     };
-    auto step = [&](State<R> in, State<R> out) { return step_on(st, in, out); };
+    auto step = [&](St in, St out) { return step_on(st, in, out); };
     // forward with checkpoints at the segment starts
-    if ((e = hipMemcpyAsync(ckpt, f0, 9 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(ckpt + 9 * cells, a0, 5 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+    if ((e = hipMemcpyAsync(ckpt, f0, QF * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ckpt + QF * cells, a0, QS * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) !=
+        hipSuccess)
         return e;
-    if ((e = hipMemcpyAsync(ckpt + 14 * cells, b0, 5 * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) !=
+    if ((e = hipMemcpyAsync(ckpt + (QF + QS) * cells, b0, QS * cells * sizeof(R), hipMemcpyDeviceToDevice, st)) !=
         hipSuccess)
         return e;
     R* last = nullptr;
     for (size_t sg = 0; sg < L.nseg; ++sg) {
         const int len = static_cast<int>(std::min<size_t>(interval, steps - sg * interval));
-        State<R> cur(ckpt + sg * se, cells);
+        St cur(ckpt + sg * se, cells);
         for (int t = 0; t < len; ++t) {
             const bool to_ckpt = (t == len - 1) && (sg + 1 < L.nseg);
             R* dst = to_ckpt ? ckpt + (sg + 1) * se : seg + ((t & 1) ? se : 0);
-            State<R> out(dst, cells);
+            St out(dst, cells);
             if ((e = step(cur, out)) != hipSuccess) return e;
             cur = out;
             last = dst;
@@ -420,9 +453,10 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     }
     // J = <A>(0) - <A>(T)
     // (fixed-order two-level sums: 512 per-block partials in red[512..1023], then one block)
-    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, a0, 5 * cells, red + 512);
+    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, a0, QS * cells, red + 512);
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red);
-    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, last + 9 * cells, 5 * cells, red + 512);
+    hipLaunchKernelGGL((partial_sum_kernel<R>), dim3(512), dim3(256), 0, st, last + QF * cells, QS * cells,
+                       red + 512);
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, red + 512, 512, red + 1);
     // The recomputes may overwrite the forward's scratch states from here on.
     // Segments shorter than kOverlapSteps stay on `stream`: two cross-stream
@@ -440,9 +474,9 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     // adjoint initial condition: dJ/da_T = -1/N everywhere
     const RA inv_n = RA(1) / static_cast<RA>(cells);
     const unsigned fb = static_cast<unsigned>(std::min<size_t>(4096, (se + 255) / 256));
-    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1, 9 * cells, RA(0));
-    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + 9 * cells, 5 * cells, -inv_n);
-    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + 14 * cells, 5 * cells, RA(0));
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1, QF * cells, RA(0));
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + QF * cells, QS * cells, -inv_n);
+    hipLaunchKernelGGL((fill_kernel<RA>), dim3(fb), dim3(256), 0, st, nu1 + (QF + QS) * cells, QS * cells, RA(0));
     hipLaunchKernelGGL((zero_double_kernel<R>), dim3(4), dim3(256), 0, st, part, L.blocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // Reverse sweep.  Segment sgp's states 1..len-1 are recomputed into
@@ -460,7 +494,7 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
     auto recompute = [&](size_t sgp) {
         hipError_t err = hipSuccess;
         for (int t = 0; t + 1 < seg_len(sgp) && err == hipSuccess; ++t)
-            err = step_on(rs, State<R>(state_in(sgp, t), cells), State<R>(state_in(sgp, t + 1), cells));
+            err = step_on(rs, St(state_in(sgp, t), cells), St(state_in(sgp, t + 1), cells));
         return err == hipSuccess && overlap ? hipEventRecord(ss.recomputed[sgp & 1], ss.side) : err;
     };
     if ((e = recompute(L.nseg - 1)) != hipSuccess) return e;
@@ -475,14 +509,15 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
         if (overlap && (e = hipStreamWaitEvent(st, ss.recomputed[sgp & 1], 0)) != hipSuccess) return e;
         auto state_at = [&](int t) { return state_in(sgp, t); };
         for (int t = len - 1; t >= 0; --t) {
-            State<R> fw(state_at(t), cells);
-            State<RA> in(nu1, cells), out(nu0, cells);
+            St fw(state_at(t), cells);
+            StA in(nu1, cells), out(nu0, cells);
             if (s.threads() <= 256)
-                hipLaunchKernelGGL((reactive_adjoint_kernel<R, RA, CT, 256>), s.grid, s.block, 0, st, fw.f, fw.a,
-                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k, part);
+                hipLaunchKernelGGL((reactive_adjoint_kernel<F, S, R, RA, CT, 256>), s.grid, s.block, 0, st, fw.f,
+                                   fw.a, fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k,
+                                   part);
             else
-                hipLaunchKernelGGL((reactive_adjoint_kernel<R, RA, CT, 1024>), s.grid, s.block, 0, st, fw.f, fw.a,
-                                   fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k,
+                hipLaunchKernelGGL((reactive_adjoint_kernel<F, S, R, RA, CT, 1024>), s.grid, s.block, 0, st, fw.f,
+                                   fw.a, fw.b, in.f, in.a, in.b, out.f, out.a, out.b, s.cells, omega, wm, omega_c, k,
                                    part);
             if ((e = hipGetLastError()) != hipSuccess) return e;
             std::swap(nu0, nu1);
@@ -492,10 +527,10 @@ hipError_t run_gradient(uint32_t flags, const R* f0, const R* a0, const R* b0, c
         if (!overlap && sgp >= 1 && (e = recompute(sgp - 1)) != hipSuccess) return e;
     }
     // adjoints of the stored initial populations
-    State<RA> fin(nu1, cells);
-    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q9>), s.grid, s.block, 0, st, fin.f, gf, s.cells, RA(0));
-    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q5>), s.grid, s.block, 0, st, fin.a, ga, s.cells, inv_n);
-    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, D2Q5>), s.grid, s.block, 0, st, fin.b, gb, s.cells, RA(0));
+    StA fin(nu1, cells);
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, F>), s.grid, s.block, 0, st, fin.f, gf, s.cells, RA(0));
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, S>), s.grid, s.block, 0, st, fin.a, ga, s.cells, inv_n);
+    hipLaunchKernelGGL((reverse_pull_kernel<RA, R, S>), s.grid, s.block, 0, st, fin.b, gb, s.cells, RA(0));
     hipLaunchKernelGGL((sum_kernel<double>), dim3(1), dim3(256), 0, st, part, L.blocks, red + 2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     double h[3];
@@ -515,60 +550,122 @@ namespace lbc {
 constexpr uint32_t kMaxAdjointCells = 1u << 29;
 }  // namespace lbc
 
-extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
-                                               uint32_t block_x, uint32_t block_y, uint64_t* bytes) {
-    using namespace lbc;
+namespace lbc {
+
+// The pairs the adjoint covers: D2Q9 + D2Q5 (BGK, TRT, MRT), D3Q19 / D3Q27 +
+// D3Q7 (BGK, TRT).
+inline bool gradient_pair(int fluid, int collision) {
+    if (fluid == kD2Q9) return collision == kBGK || collision == kTRT || collision == kMRT;
+    if (fluid == kD3Q19 || fluid == kD3Q27) return collision == kBGK || collision == kTRT;
+    return false;
+}
+
+inline size_t state_planes(int fluid) {
+    return fluid == kD2Q9 ? kStatePlanes<D2Q9, D2Q5> : fluid == kD3Q19 ? kStatePlanes<D3Q19, D3Q7>
+                                                                       : kStatePlanes<D3Q27, D3Q7>;
+}
+
+hipError_t gradient_workspace(int fluid, int storage, uint64_t nx, uint64_t ny, uint64_t nz, int steps, int interval,
+                              uint32_t block_x, uint32_t block_y, uint32_t block_z, uint64_t* bytes) {
     if (bytes == nullptr) return hipErrorInvalidValue;
+    if (fluid != kD2Q9 && fluid != kD3Q19 && fluid != kD3Q27) return hipErrorInvalidValue;
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
     LaunchShape s;
-    const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
+    const hipError_t e = make_shape(fluid == kD2Q9, nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (s.cells.count > kMaxAdjointCells) return hipErrorInvalidValue;
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;
-    *bytes = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid).total;
+    *bytes = grad_layout(storage == kF64 ? 8 : 4, state_planes(fluid), s.cells.count, steps, interval, s.grid).total;
     return hipSuccess;
 }
 
-extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void* f0, const void* a0,
-                                     const void* b0, uint64_t nx, uint64_t ny, int steps, int interval, double omega, double omega_c,
-                                     double k_react, double* loss, void* grad_f0, void* grad_a0, void* grad_b0,
-                                     double* dloss_domega_c, void* workspace, uint64_t workspace_bytes,
-                                     uint32_t block_x, uint32_t block_y, void* stream) {
-    using namespace lbc;
+hipError_t gradient(int fluid, int collision, int storage, uint32_t flags, const void* f0, const void* a0,
+                    const void* b0, uint64_t nx, uint64_t ny, uint64_t nz, int steps, int interval, double omega,
+                    double omega_c, double k_react, double* loss, void* grad_f0, void* grad_a0, void* grad_b0,
+                    double* dloss_domega_c, void* workspace, uint64_t workspace_bytes, uint32_t block_x,
+                    uint32_t block_y, uint32_t block_z, void* stream) {
     if (f0 == nullptr || a0 == nullptr || b0 == nullptr || grad_f0 == nullptr || grad_a0 == nullptr ||
         grad_b0 == nullptr || workspace == nullptr)
         return hipErrorInvalidDevicePointer;
     if (loss == nullptr || dloss_domega_c == nullptr) return hipErrorInvalidValue;
+    if (fluid != kD2Q9 && fluid != kD3Q19 && fluid != kD3Q27) return hipErrorInvalidValue;
     LaunchShape s;
-    const hipError_t e = make_shape(true, nx, ny, 1, block_x, block_y, 1, s);
+    const hipError_t e = make_shape(fluid == kD2Q9, nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (s.cells.count > kMaxAdjointCells) return hipErrorInvalidValue;
     if (s.threads() % 64 != 0) return hipErrorInvalidConfiguration;  // whole waves: the block reduction
     if (storage != kF32 && storage != kF64) return hipErrorInvalidValue;
-    if (collision != kBGK && collision != kTRT && collision != kMRT) return hipErrorInvalidValue;
+    if (!gradient_pair(fluid, collision)) return hipErrorInvalidValue;
     if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST | LBC_FLAG_F64_ADJOINT)) return hipErrorInvalidValue;
     if (steps <= 0 || interval <= 0) return hipErrorInvalidValue;
-    const GradLayout L = grad_layout(storage == kF64 ? 8 : 4, nx, ny, steps, interval, s.grid);
+    const GradLayout L =
+        grad_layout(storage == kF64 ? 8 : 4, state_planes(fluid), s.cells.count, steps, interval, s.grid);
     if (workspace_bytes < L.total) return hipErrorInvalidValue;
     char* ws = static_cast<char*>(workspace);
     const hipStream_t st = as_stream(stream);
-    auto run = [&](auto tag, auto adjoint_tag, auto ct) {
+    auto run = [&](auto fs, auto tag, auto adjoint_tag, auto ct) {
+        using Pair = decltype(fs);
         using R = decltype(tag);
-        return run_gradient<R, decltype(adjoint_tag), decltype(ct)::value>(
-            flags, static_cast<const R*>(f0), static_cast<const R*>(a0), static_cast<const R*>(b0), s, steps, interval,
-            omega, omega_c, k_react, loss, static_cast<R*>(grad_f0), static_cast<R*>(grad_a0),
+        return run_gradient<typename Pair::first_type, typename Pair::second_type, R, decltype(adjoint_tag),
+                            decltype(ct)::value>(
+            flags, static_cast<const R*>(f0), static_cast<const R*>(a0), static_cast<const R*>(b0), s, steps,
+            interval, omega, omega_c, k_react, loss, static_cast<R*>(grad_f0), static_cast<R*>(grad_a0),
             static_cast<R*>(grad_b0), dloss_domega_c, ws, st);
     };
     using BGK = std::integral_constant<int, kBGK>;
     using TRT = std::integral_constant<int, kTRT>;
     using MRT = std::integral_constant<int, kMRT>;
-    if (storage == kF64)
-        return collision == kBGK ? run(0.0, 0.0, BGK{}) : collision == kTRT ? run(0.0, 0.0, TRT{})
-                                                          : run(0.0, 0.0, MRT{});
-    if (flags & LBC_FLAG_F64_ADJOINT)
-        return collision == kBGK ? run(0.0f, 0.0, BGK{}) : collision == kTRT ? run(0.0f, 0.0, TRT{})
-                                                           : run(0.0f, 0.0, MRT{});
-    return collision == kBGK ? run(0.0f, 0.0f, BGK{}) : collision == kTRT ? run(0.0f, 0.0f, TRT{})
-                                                       : run(0.0f, 0.0f, MRT{});
+    // storage and adjoint precision: FP64; FP32; FP32 with the FP64 adjoint
+    auto by_precision = [&](auto fs, auto ct) {
+        if (storage == kF64) return run(fs, 0.0, 0.0, ct);
+        if (flags & LBC_FLAG_F64_ADJOINT) return run(fs, 0.0f, 0.0, ct);
+        return run(fs, 0.0f, 0.0f, ct);
+    };
+    auto by_collision = [&](auto fs) {
+        if (collision == kBGK) return by_precision(fs, BGK{});
+        if (collision == kTRT) return by_precision(fs, TRT{});
+        if constexpr (std::is_same<typename decltype(fs)::first_type, D2Q9>::value) return by_precision(fs, MRT{});
+        return hipErrorInvalidValue;
+    };
+    if (fluid == kD2Q9) return by_collision(std::pair<D2Q9, D2Q5>{});
+    if (fluid == kD3Q19) return by_collision(std::pair<D3Q19, D3Q7>{});
+    return by_collision(std::pair<D3Q27, D3Q7>{});
+}
+
+}  // namespace lbc
+
+extern "C" int lbc_reactive_gradient_workspace(int storage, uint64_t nx, uint64_t ny, int steps, int interval,
+                                               uint32_t block_x, uint32_t block_y, uint64_t* bytes) {
+    return lbc::gradient_workspace(lbc::kD2Q9, storage, nx, ny, 1, steps, interval, block_x, block_y, 1, bytes);
+}
+
+extern "C" int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void* f0, const void* a0,
+                                     const void* b0, uint64_t nx, uint64_t ny, int steps, int interval, double omega,
+                                     double omega_c, double k_react, double* loss, void* grad_f0, void* grad_a0,
+                                     void* grad_b0, double* dloss_domega_c, void* workspace, uint64_t workspace_bytes,
+                                     uint32_t block_x, uint32_t block_y, void* stream) {
+    return lbc::gradient(lbc::kD2Q9, collision, storage, flags, f0, a0, b0, nx, ny, 1, steps, interval, omega,
+                         omega_c, k_react, loss, grad_f0, grad_a0, grad_b0, dloss_domega_c, workspace,
+                         workspace_bytes, block_x, block_y, 1, stream);
+}
+
+extern "C" int lbc_reactive_gradient_workspace_3d(int fluid_lattice, int storage, uint64_t nx, uint64_t ny,
+                                                  uint64_t nz, int steps, int interval, uint32_t block_x,
+                                                  uint32_t block_y, uint32_t block_z, uint64_t* bytes) {
+    if (fluid_lattice != lbc::kD3Q19 && fluid_lattice != lbc::kD3Q27) return hipErrorInvalidValue;
+    return lbc::gradient_workspace(fluid_lattice, storage, nx, ny, nz, steps, interval, block_x, block_y, block_z,
+                                   bytes);
+}
+
+extern "C" int lbc_reactive_gradient_3d(int fluid_lattice, int collision, int storage, uint32_t flags,
+                                        const void* f0, const void* a0, const void* b0, uint64_t nx, uint64_t ny,
+                                        uint64_t nz, int steps, int interval, double omega, double omega_c,
+                                        double k_react, double* loss, void* grad_f0, void* grad_a0, void* grad_b0,
+                                        double* dloss_domega_c, void* workspace, uint64_t workspace_bytes,
+                                        uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
+    if (fluid_lattice != lbc::kD3Q19 && fluid_lattice != lbc::kD3Q27) return hipErrorInvalidValue;
+    return lbc::gradient(fluid_lattice, collision, storage, flags, f0, a0, b0, nx, ny, nz, steps, interval, omega,
+                         omega_c, k_react, loss, grad_f0, grad_a0, grad_b0, dloss_domega_c, workspace,
+                         workspace_bytes, block_x, block_y, block_z, stream);
 }

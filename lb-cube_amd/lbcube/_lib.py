@@ -49,6 +49,11 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_uint64)], c_int),
     "lbc_reactive_gradient": ([c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_u64, c_u64, c_int, c_int, c_double, c_double,
                                c_double, _dp, c_vp, c_vp, c_vp, _dp, c_vp, c_u64, c_u32, c_u32, c_vp], c_int),
+    "lbc_reactive_gradient_workspace_3d": ([c_int, c_int, c_u64, c_u64, c_u64, c_int, c_int, c_u32, c_u32, c_u32,
+                                            ctypes.POINTER(c_u64)], c_int),
+    "lbc_reactive_gradient_3d": ([c_int, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_int, c_int,
+                                  c_double, c_double, c_double, _dp, c_vp, c_vp, c_vp, _dp, c_vp, c_u64, c_u32, c_u32,
+                                  c_u32, c_vp], c_int),
     "lbc_ensemble_create": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)], c_int),
     "lbc_ensemble_layout": ([c_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), c_vp, c_vp,
                              c_vp], c_int),

@@ -1081,31 +1081,40 @@ template <class R> void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
     dsb = -a * k * (k * a * dphi + e) / (den * den);
 }
 
-template <class R, int CT>
+// The fluid lattice F (D2Q9, D3Q19, D3Q27; all cs2 = 1/3) and the species
+// lattice S (D2Q5, D3Q7) of one reactive pair; MRT for D2Q9 only.
+template <class F, class S, class R, int CT>
 void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, const R* nua1, const R* nub1, R* nuf0,
                            R* nua0, R* nub0, Grid g, R omega, R omega_c, R k, long double* djdw) {
-    using F = D2Q9;
-    using S = D2Q5;
+    static_assert(F::D == S::D, "a fluid and a species lattice of one dimension");
+    constexpr int D = F::D, QF = F::Q, QS = S::Q;
     const i64 N = g.cells();
     const R wm = CT == C_TRT ? omega_minus<R>(omega) : R(0);
     const R cs2 = static_cast<R>(S::cs2);
     long double acc = 0;
+    for (i64 z = 0; z < g.nz; ++z)
     for (i64 y = 0; y < g.ny; ++y) {
         for (i64 x = 0; x < g.nx; ++x) {
-            const i64 cell = y * g.nx + x;
-            R fl[9], fp[9], lf[9];
-            for (int i = 0; i < 9; ++i) {
-                fl[i] = f[i * N + pull(y, F::c[2 * i + 1], g.ny) * g.nx + pull(x, F::c[2 * i], g.nx)];
+            const i64 cell = (z * g.ny + y) * g.nx + x;
+            // source of a population with velocity c (sign: +1 the pull, -1 the reverse pull)
+            auto at = [&](const int* c, int sign) {
+                const i64 sx = pull(x, sign * c[0], g.nx), sy = pull(y, sign * c[1], g.ny);
+                const i64 sz = D == 3 ? pull(z, sign * c[2], g.nz) : 0;
+                return (sz * g.ny + sy) * g.nx + sx;
+            };
+            R fl[QF], fp[QF], lf[QF];
+            for (int i = 0; i < QF; ++i) {
+                fl[i] = f[i * N + at(&F::c[i * D], 1)];
                 // reverse pull: the adjoint of the population stored at this cell
-                lf[i] = nuf1[i * N + pull(y, -F::c[2 * i + 1], g.ny) * g.nx + pull(x, -F::c[2 * i], g.nx)];
+                lf[i] = nuf1[i * N + at(&F::c[i * D], -1)];
                 fp[i] = fl[i];
             }
             collide_cell<F, CT, R>(fp, omega);
             const Macro<F, R> mp = macro_state<F, R>(fp);  // what the scalar step sees
-            R ga[5], gb[5], la[5], lb[5];
-            for (int j = 0; j < 5; ++j) {
-                const i64 src = j * N + pull(y, S::c[2 * j + 1], g.ny) * g.nx + pull(x, S::c[2 * j], g.nx);
-                const i64 rsrc = j * N + pull(y, -S::c[2 * j + 1], g.ny) * g.nx + pull(x, -S::c[2 * j], g.nx);
+            R ga[QS], gb[QS], la[QS], lb[QS];
+            for (int j = 0; j < QS; ++j) {
+                const i64 src = j * N + at(&S::c[j * D], 1);
+                const i64 rsrc = j * N + at(&S::c[j * D], -1);
                 ga[j] = a[src];
                 gb[j] = b[src];
                 la[j] = nua1[rsrc];
@@ -1114,50 +1123,64 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
             const R ca = concentration<S, R>(ga), cb = concentration<S, R>(gb);
             R dsa, dsb;
             reaction_grad<R>(ca, cb, k, dsa, dsb);
-            R ea = 0, eb = 0, wa = 0, wb = 0, pax = 0, pay = 0, pbx = 0, pby = 0;
-            for (int j = 0; j < 5; ++j) {
+            R ea = 0, eb = 0, wa = 0, wb = 0, pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
+            for (int j = 0; j < QS; ++j) {
                 const R w = static_cast<R>(S::w[j]);
-                const R cx = static_cast<R>(S::c[2 * j]), cy = static_cast<R>(S::c[2 * j + 1]);
-                const R e = R(1) + (cx * mp.u[0] + cy * mp.u[1]) / cs2;
+                R cu = 0;
+                for (int d = 0; d < D; ++d) cu += static_cast<R>(S::c[j * D + d]) * mp.u[d];
+                const R e = R(1) + cu / cs2;
                 ea += la[j] * w * e;
                 eb += lb[j] * w * e;
                 wa += la[j] * w;
                 wb += lb[j] * w;
-                pax += la[j] * w * cx;
-                pay += la[j] * w * cy;
-                pbx += lb[j] * w * cx;
-                pby += lb[j] * w * cy;
+                for (int d = 0; d < D; ++d) {
+                    pa[d] += la[j] * w * static_cast<R>(S::c[j * D + d]);
+                    pb[d] += lb[j] * w * static_cast<R>(S::c[j * D + d]);
+                }
                 acc += static_cast<long double>(la[j]) * (scalar_feq<S, R>(j, ca, mp.u) - ga[j]) +
                        static_cast<long double>(lb[j]) * (scalar_feq<S, R>(j, cb, mp.u) - gb[j]);
             }
             const R sw = wa + wb;
-            for (int j = 0; j < 5; ++j) {
+            for (int j = 0; j < QS; ++j) {
                 nua0[j * N + cell] = (R(1) - omega_c) * la[j] + omega_c * ea + sw * dsa;
                 nub0[j * N + cell] = (R(1) - omega_c) * lb[j] + omega_c * eb + sw * dsb;
             }
-            const R lux = omega_c / cs2 * (ca * pax + cb * pbx);
-            const R luy = omega_c / cs2 * (ca * pay + cb * pby);
+            R lu[3];
+            for (int d = 0; d < D; ++d) lu[d] = omega_c / cs2 * (ca * pa[d] + cb * pb[d]);
             // velocity coupling: d u' / d fp_i = (c_i - u') / rho'
-            for (int i = 0; i < 9; ++i) {
-                const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
-                lf[i] += (lux * (cx - mp.u[0]) + luy * (cy - mp.u[1])) / mp.rho;
+            for (int i = 0; i < QF; ++i) {
+                R t = 0;
+                for (int d = 0; d < D; ++d) t += lu[d] * (static_cast<R>(F::c[i * D + d]) - mp.u[d]);
+                lf[i] += t / mp.rho;
             }
-            // transposed collision Jacobian at the pre-collision state
+            // transposed collision Jacobian at the pre-collision state:
+            // d feq_q / d f_i = w_q [P_q + G_q . (c_i - u)], P_q = feq_q / (w_q rho),
+            // G_q = dP_q / du = 3 c_q + 9 (c_q . u) c_q - 3 u (cs2 = 1/3)
             const Macro<F, R> m = macro_state<F, R>(fl);
-            const R uu = m.u[0] * m.u[0] + m.u[1] * m.u[1];
-            auto moments = [&](const R* lam, R& al, R& bx, R& by) {
-                al = bx = by = 0;
-                for (int q = 0; q < 9; ++q) {
+            R uu = 0;
+            for (int d = 0; d < D; ++d) uu += m.u[d] * m.u[d];
+            auto moments = [&](const R* lam, R& al, R* bd) {
+                al = 0;
+                for (int d = 0; d < D; ++d) bd[d] = 0;
+                for (int q = 0; q < QF; ++q) {
                     const R w = static_cast<R>(F::w[q]);
-                    const R cx = static_cast<R>(F::c[2 * q]), cy = static_cast<R>(F::c[2 * q + 1]);
-                    const R cu = cx * m.u[0] + cy * m.u[1];
+                    R cu = 0;
+                    for (int d = 0; d < D; ++d) cu += static_cast<R>(F::c[q * D + d]) * m.u[d];
                     const R P = R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu;
                     al += lam[q] * w * P;
-                    bx += lam[q] * w * (R(3) * cx + R(9) * cu * cx - R(3) * m.u[0]);
-                    by += lam[q] * w * (R(3) * cy + R(9) * cu * cy - R(3) * m.u[1]);
+                    for (int d = 0; d < D; ++d) {
+                        const R cd = static_cast<R>(F::c[q * D + d]);
+                        bd[d] += lam[q] * w * (R(3) * cd + R(9) * cu * cd - R(3) * m.u[d]);
+                    }
                 }
             };
+            auto dev = [&](int i, const R* bd) {
+                R t = 0;
+                for (int d = 0; d < D; ++d) t += bd[d] * (static_cast<R>(F::c[i * D + d]) - m.u[d]);
+                return t;
+            };
             if constexpr (CT == C_MRT) {
+                static_assert(std::is_same<F, D2Q9>::value, "the MRT adjoint covers D2Q9");
                 // f' = M^-1 [(I - S) M f + S m_eq(rho, j)] with M^-1 = M^T D^-1
                 // (collide_mrt_d2q9, lattice_mrt.hpp:548-603), so
                 // J^T lam = M^T (I - S) mu + (d m_eq / d f)^T S mu, mu = D^-1 M lam,
@@ -1167,7 +1190,7 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                                                {0, -2, 0, 2, 0, 1, -1, -1, 1},  {0, 0, 1, 0, -1, 1, 1, -1, -1},
                                                {0, 0, -2, 0, 2, 1, 1, -1, -1},  {0, 1, -1, 1, -1, 0, 0, 0, 0},
                                                {0, 0, 0, 0, 0, 1, -1, 1, -1}};
-                const R s[9] = {R(0), R{1.63}, R{1.14}, R(0), R{1.92}, R(0), R{1.92}, omega, omega};
+                const R sr[9] = {R(0), R{1.63}, R{1.14}, R(0), R{1.92}, R(0), R{1.92}, omega, omega};
                 R mu[9], nm[9], ne[9];
                 for (int r = 0; r < 9; ++r) {
                     R acc_r = 0, norm = 0;
@@ -1176,8 +1199,8 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                         norm += R(rows[r][i] * rows[r][i]);
                     }
                     mu[r] = acc_r / norm;
-                    nm[r] = (R(1) - s[r]) * mu[r];
-                    ne[r] = s[r] * mu[r];
+                    nm[r] = (R(1) - sr[r]) * mu[r];
+                    ne[r] = sr[r] * mu[r];
                 }
                 const R ux = m.u[0], uy = m.u[1];
                 const R grho = ne[1] * (R(-2) - R(3) * uu) + ne[2] * (R(1) + R(3) * uu) - ne[7] * (ux * ux - uy * uy) -
@@ -1191,81 +1214,140 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                     nuf0[i * N + cell] = t + grho + cx * gjx + cy * gjy;
                 }
             } else if constexpr (CT == C_BGK) {
-                R al, bx, by;
-                moments(lf, al, bx, by);
-                for (int i = 0; i < 9; ++i) {
-                    const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
-                    nuf0[i * N + cell] =
-                        (R(1) - omega) * lf[i] + omega * (al + bx * (cx - m.u[0]) + by * (cy - m.u[1]));
-                }
+                R al, bd[3];
+                moments(lf, al, bd);
+                for (int i = 0; i < QF; ++i) nuf0[i * N + cell] = (R(1) - omega) * lf[i] + omega * (al + dev(i, bd));
             } else {
-                R lp[9], lm[9];
-                for (int i = 0; i < 9; ++i) {
+                R lp[QF], lm[QF];
+                for (int i = 0; i < QF; ++i) {
                     lp[i] = R(0.5) * (lf[i] + lf[F::opp[i]]);
                     lm[i] = R(0.5) * (lf[i] - lf[F::opp[i]]);
                 }
-                R ap, bxp, byp, am, bxm, bym;
-                moments(lp, ap, bxp, byp);
-                moments(lm, am, bxm, bym);
-                for (int i = 0; i < 9; ++i) {
-                    const R cx = static_cast<R>(F::c[2 * i]), cy = static_cast<R>(F::c[2 * i + 1]);
-                    const R dx = cx - m.u[0], dy = cy - m.u[1];
-                    nuf0[i * N + cell] = lf[i] - omega * lp[i] - wm * lm[i] +
-                                         omega * (ap + bxp * dx + byp * dy) + wm * (am + bxm * dx + bym * dy);
-                }
+                R ap, bp[3], am, bm[3];
+                moments(lp, ap, bp);
+                moments(lm, am, bm);
+                for (int i = 0; i < QF; ++i)
+                    nuf0[i * N + cell] = lf[i] - omega * lp[i] - wm * lm[i] + omega * (ap + dev(i, bp)) +
+                                         wm * (am + dev(i, bm));
             }
         }
     }
     *djdw += acc;
 }
 
-template <class R> R mean_conc(const R* g, i64 N) {
+template <class S, class R> R mean_conc(const R* g, i64 N) {
     long double s = 0;
-    for (i64 j = 0; j < 5 * N; ++j) s += g[j];
+    for (i64 j = 0; j < S::Q * N; ++j) s += g[j];
     return static_cast<R>(s / N);
 }
 
 // J = <C>(T) = <A>(0) - <A>(T).  Runs the forward (all states kept), then the
 // adjoint; returns J, dJ/d(f0, a0, b0) (as adjoints of the stored initial
 // populations) and dJ/d(omega_c).
-template <class R, int CT>
+template <class F, class S, class R, int CT>
 void reactive_gradient(const R* f0, const R* a0, const R* b0, Grid g, R omega, R omega_c, R k, int steps, R* J,
                        R* gf, R* ga, R* gb, R* djdwc) {
+    constexpr int D = F::D;
     const i64 N = g.cells();
-    const size_t nf = 9 * N, ns = 5 * N;
-    std::vector<R> F((steps + 1) * nf), A((steps + 1) * ns), B((steps + 1) * ns);
-    std::copy(f0, f0 + nf, F.begin());
+    const size_t nf = F::Q * N, ns = S::Q * N;
+    std::vector<R> Fs((steps + 1) * nf), A((steps + 1) * ns), B((steps + 1) * ns);
+    std::copy(f0, f0 + nf, Fs.begin());
     std::copy(a0, a0 + ns, A.begin());
     std::copy(b0, b0 + ns, B.begin());
     constexpr int DT = std::is_same<R, double>::value ? T_F64 : T_F32;
     for (int n = 0; n < steps; ++n) {
-        fluid_step<D2Q9, CT, DT>(&F[n * nf], &F[(n + 1) * nf], g, omega, 1);
-        scalar_like_step<D2Q9, D2Q5, DT>(&A[n * ns], &A[(n + 1) * ns], &B[n * ns], &B[(n + 1) * ns],
-                                         &F[(n + 1) * nf], g, omega_c, k, true, 1);
+        fluid_step<F, CT, DT>(&Fs[n * nf], &Fs[(n + 1) * nf], g, omega, 1);
+        scalar_like_step<F, S, DT>(&A[n * ns], &A[(n + 1) * ns], &B[n * ns], &B[(n + 1) * ns], &Fs[(n + 1) * nf], g,
+                                   omega_c, k, true, 1);
     }
-    *J = mean_conc<R>(&A[0], N) - mean_conc<R>(&A[steps * ns], N);
+    *J = mean_conc<S, R>(&A[0], N) - mean_conc<S, R>(&A[steps * ns], N);
     std::vector<R> nf1(nf, R(0)), na1(ns, R(-1) / static_cast<R>(N)), nb1(ns, R(0));
     std::vector<R> nf0(nf), na0(ns), nb0(ns);
     long double dj = 0;
     for (int n = steps - 1; n >= 0; --n) {
-        reactive_adjoint_step<R, CT>(&F[n * nf], &A[n * ns], &B[n * ns], nf1.data(), na1.data(), nb1.data(),
-                                     nf0.data(), na0.data(), nb0.data(), g, omega, omega_c, k, &dj);
+        reactive_adjoint_step<F, S, R, CT>(&Fs[n * nf], &A[n * ns], &B[n * ns], nf1.data(), na1.data(), nb1.data(),
+                                           nf0.data(), na0.data(), nb0.data(), g, omega, omega_c, k, &dj);
         nf1.swap(nf0);
         na1.swap(na0);
         nb1.swap(nb0);
     }
-    for (i64 y = 0; y < g.ny; ++y)
-        for (i64 x = 0; x < g.nx; ++x) {
-            for (int i = 0; i < 9; ++i)
-                gf[i * N + y * g.nx + x] =
-                    nf1[i * N + pull(y, -D2Q9::c[2 * i + 1], g.ny) * g.nx + pull(x, -D2Q9::c[2 * i], g.nx)];
-            for (int j = 0; j < 5; ++j) {
-                const i64 r = j * N + pull(y, -D2Q5::c[2 * j + 1], g.ny) * g.nx + pull(x, -D2Q5::c[2 * j], g.nx);
-                ga[j * N + y * g.nx + x] = na1[r] + R(1) / static_cast<R>(N);
-                gb[j * N + y * g.nx + x] = nb1[r];
+    for (i64 z = 0; z < g.nz; ++z)
+        for (i64 y = 0; y < g.ny; ++y)
+            for (i64 x = 0; x < g.nx; ++x) {
+                const i64 cell = (z * g.ny + y) * g.nx + x;
+                auto rev = [&](const int* c) {
+                    const i64 sz = D == 3 ? pull(z, -c[2], g.nz) : 0;
+                    return (sz * g.ny + pull(y, -c[1], g.ny)) * g.nx + pull(x, -c[0], g.nx);
+                };
+                for (int i = 0; i < F::Q; ++i) gf[i * N + cell] = nf1[i * N + rev(&F::c[i * D])];
+                for (int j = 0; j < S::Q; ++j) {
+                    const i64 r = j * N + rev(&S::c[j * D]);
+                    ga[j * N + cell] = na1[r] + R(1) / static_cast<R>(N);
+                    gb[j * N + cell] = nb1[r];
+                }
             }
-        }
     *djdwc = static_cast<R>(dj);
+}
+
+// Entry helpers of the adjoint: the collisions the adjoint covers for a
+// fluid/species pair (BGK, TRT; MRT for D2Q9), dispatched at run time.
+template <class F, class S>
+int gradient_entry(int ct, int dt, const void* f0, const void* a0, const void* b0, Grid g, double omega,
+                   double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb, double* djdwc) {
+    auto run = [&](auto tag, auto ctc) -> int {
+        using R = decltype(tag);
+        constexpr int CT = decltype(ctc)::value;
+        R j, w;
+        reactive_gradient<F, S, R, CT>(static_cast<const R*>(f0), static_cast<const R*>(a0),
+                                       static_cast<const R*>(b0), g, static_cast<R>(omega), static_cast<R>(omega_c),
+                                       static_cast<R>(k), steps, &j, static_cast<R*>(gf), static_cast<R*>(ga),
+                                       static_cast<R*>(gb), &w);
+        *J = j;
+        *djdwc = w;
+        return 0;
+    };
+    auto by_ct = [&](auto tag) -> int {
+        if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
+        if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
+        if constexpr (std::is_same<F, D2Q9>::value) {
+            if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
+        }
+        return -1;
+    };
+    if (dt == T_F64) return by_ct(double{});
+    if (dt == T_F32) return by_ct(float{});
+    return -1;
+}
+
+template <class F, class S>
+int adjoint_step_entry(int ct, int dt, const void* f, const void* a, const void* b, const void* nuf1,
+                       const void* nua1, const void* nub1, void* nuf0, void* nua0, void* nub0, Grid g, double omega,
+                       double omega_c, double k, double* djdw) {
+    long double acc = 0;
+    auto run = [&](auto tag, auto ctc) -> int {
+        using R = decltype(tag);
+        constexpr int CT = decltype(ctc)::value;
+        reactive_adjoint_step<F, S, R, CT>(
+            static_cast<const R*>(f), static_cast<const R*>(a), static_cast<const R*>(b),
+            static_cast<const R*>(nuf1), static_cast<const R*>(nua1), static_cast<const R*>(nub1),
+            static_cast<R*>(nuf0), static_cast<R*>(nua0), static_cast<R*>(nub0), g, static_cast<R>(omega),
+            static_cast<R>(omega_c), static_cast<R>(k), &acc);
+        return 0;
+    };
+    auto by_ct = [&](auto tag) -> int {
+        if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
+        if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
+        if constexpr (std::is_same<F, D2Q9>::value) {
+            if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
+        }
+        return -1;
+    };
+    int rc;
+    if (dt == T_F64) rc = by_ct(double{});
+    else if (dt == T_F80) rc = by_ct((long double){});
+    else return -1;
+    *djdw = static_cast<double>(acc);
+    return rc;
 }
 
 }  // namespace orc
@@ -1518,41 +1600,20 @@ int orc_f16_roundtrip(const float* in, uint16_t* h, float* back, long long n, in
 int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const void* b0, long long nx, long long ny,
                           double omega, double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb,
                           double* djdwc) {
-    const Grid g{nx, ny, 1};
-    if (dt == T_F64) {
-        double j, w;
-        if (ct == C_BGK)
-            reactive_gradient<double, C_BGK>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
-                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
-        else if (ct == C_TRT)
-            reactive_gradient<double, C_TRT>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
-                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
-        else if (ct == C_MRT)
-            reactive_gradient<double, C_MRT>((const double*)f0, (const double*)a0, (const double*)b0, g, omega,
-                                             omega_c, k, steps, &j, (double*)gf, (double*)ga, (double*)gb, &w);
-        else
-            return -1;
-        *J = j;
-        *djdwc = w;
-    } else if (dt == T_F32) {
-        float j, w;
-        if (ct == C_BGK)
-            reactive_gradient<float, C_BGK>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
-                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
-        else if (ct == C_TRT)
-            reactive_gradient<float, C_TRT>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
-                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
-        else if (ct == C_MRT)
-            reactive_gradient<float, C_MRT>((const float*)f0, (const float*)a0, (const float*)b0, g, (float)omega,
-                                            (float)omega_c, (float)k, steps, &j, (float*)gf, (float*)ga, (float*)gb, &w);
-        else
-            return -1;
-        *J = j;
-        *djdwc = w;
-    } else {
-        return -1;
-    }
-    return 0;
+    return gradient_entry<D2Q9, D2Q5>(ct, dt, f0, a0, b0, Grid{nx, ny, 1}, omega, omega_c, k, steps, J, gf, ga, gb,
+                                      djdwc);
+}
+
+// The same for the 3D pairs (D3Q19 or D3Q27 fluid + two D3Q7 species; BGK, TRT).
+int orc_reactive_gradient_3d(int fluid, int ct, int dt, const void* f0, const void* a0, const void* b0,
+                             long long nx, long long ny, long long nz, double omega, double omega_c, double k,
+                             int steps, double* J, void* gf, void* ga, void* gb, double* djdwc) {
+    const Grid g{nx, ny, nz};
+    if (fluid == L_D3Q19)
+        return gradient_entry<D3Q19, D3Q7>(ct, dt, f0, a0, b0, g, omega, omega_c, k, steps, J, gf, ga, gb, djdwc);
+    if (fluid == L_D3Q27)
+        return gradient_entry<D3Q27, D3Q7>(ct, dt, f0, a0, b0, g, omega, omega_c, k, steps, J, gf, ga, gb, djdwc);
+    return -1;
 }
 
 // One step of the discrete adjoint with arbitrary adjoint inputs (collision-cell
@@ -1560,24 +1621,22 @@ int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const 
 int orc_reactive_adjoint_step(int ct, int dt, const void* f, const void* a, const void* b, const void* nuf1,
                               const void* nua1, const void* nub1, void* nuf0, void* nua0, void* nub0, long long nx,
                               long long ny, double omega, double omega_c, double k, double* djdw) {
-    if (ct != C_BGK && ct != C_TRT && ct != C_MRT) return -1;  // the adjoint covers BGK, TRT, MRT
-    const Grid g = make_grid(L_D2Q9, nx, ny, 1);
-    long double acc = 0;
-    auto run = [&](auto tag) {
-        using R = decltype(tag);
-        ORC_CT_SWITCH(ct, (reactive_adjoint_step<R, CT>(
-                              static_cast<const R*>(f), static_cast<const R*>(a), static_cast<const R*>(b),
-                              static_cast<const R*>(nuf1), static_cast<const R*>(nua1), static_cast<const R*>(nub1),
-                              static_cast<R*>(nuf0), static_cast<R*>(nua0), static_cast<R*>(nub0), g,
-                              static_cast<R>(omega), static_cast<R>(omega_c), static_cast<R>(k), &acc)));
-        return 0;
-    };
-    int rc;
-    if (dt == T_F64) rc = run(double{});
-    else if (dt == T_F80) rc = run((long double){});
-    else return -1;
-    *djdw = static_cast<double>(acc);
-    return rc;
+    return adjoint_step_entry<D2Q9, D2Q5>(ct, dt, f, a, b, nuf1, nua1, nub1, nuf0, nua0, nub0, Grid{nx, ny, 1},
+                                          omega, omega_c, k, djdw);
+}
+
+int orc_reactive_adjoint_step_3d(int fluid, int ct, int dt, const void* f, const void* a, const void* b,
+                                 const void* nuf1, const void* nua1, const void* nub1, void* nuf0, void* nua0,
+                                 void* nub0, long long nx, long long ny, long long nz, double omega, double omega_c,
+                                 double k, double* djdw) {
+    const Grid g{nx, ny, nz};
+    if (fluid == L_D3Q19)
+        return adjoint_step_entry<D3Q19, D3Q7>(ct, dt, f, a, b, nuf1, nua1, nub1, nuf0, nua0, nub0, g, omega,
+                                               omega_c, k, djdw);
+    if (fluid == L_D3Q27)
+        return adjoint_step_entry<D3Q27, D3Q7>(ct, dt, f, a, b, nuf1, nua1, nub1, nuf0, nua0, nub0, g, omega,
+                                               omega_c, k, djdw);
+    return -1;
 }
 
 int orc_phi_slope(int dt, const double* z, double* phi, double* dphi, long long n) {

@@ -68,6 +68,8 @@ def lib() -> ctypes.CDLL:
             "orc_reaction_grad": [i, vp, vp, d, vp, vp, ll],
             "orc_phi_slope": [i, vp, vp, vp, ll],
             "orc_reactive_adjoint_step": [i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, ll, ll, d, d, d, dp],
+            "orc_reactive_gradient_3d": [i, i, i, vp, vp, vp, ll, ll, ll, d, d, d, i, dp, vp, vp, vp, dp],
+            "orc_reactive_adjoint_step_3d": [i, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, ll, ll, ll, d, d, d, dp],
             "orc_max_threads": [],
         }
         for name, args in sig.items():
@@ -214,16 +216,23 @@ def fluid_diagnostics(lattice, dtype, f, wide=False):
     return np.array(out[:])
 
 
-def reactive_adjoint_step(collision, dtype, f, a, b, nuf1, nua1, nub1, omega, omega_c, k):
-    """One step of the oracle's discrete adjoint (D2Q9 + 2 D2Q5) from arbitrary
-    adjoint inputs at the collision cell; returns (nuf0, nua0, nub0, djdw)."""
+def reactive_adjoint_step(collision, dtype, f, a, b, nuf1, nua1, nub1, omega, omega_c, k, lattice="D2Q9"):
+    """One step of the oracle's discrete adjoint (D2Q9 + 2 D2Q5, or D3Q19 /
+    D3Q27 + 2 D3Q7) from arbitrary adjoint inputs at the collision cell;
+    returns (nuf0, nua0, nub0, djdw)."""
     arrs = [np.ascontiguousarray(x, dtype=NP_STORAGE[dtype]) for x in (f, a, b, nuf1, nua1, nub1)]
     outs = [np.empty_like(arrs[0]), np.empty_like(arrs[1]), np.empty_like(arrs[2])]
-    nx, ny, _ = _dims("D2Q9", arrs[0].shape)
+    nx, ny, nz = _dims(lattice, arrs[0].shape)
     dw = (ctypes.c_double * 1)()
-    _check(lib().orc_reactive_adjoint_step(COLLISIONS[collision], DTYPES[dtype], *(_p(x) for x in arrs),
-                                           *(_p(x) for x in outs), nx, ny, float(omega), float(omega_c), float(k),
-                                           dw), "reactive_adjoint_step")
+    if lattice == "D2Q9":
+        _check(lib().orc_reactive_adjoint_step(COLLISIONS[collision], DTYPES[dtype], *(_p(x) for x in arrs),
+                                               *(_p(x) for x in outs), nx, ny, float(omega), float(omega_c),
+                                               float(k), dw), "reactive_adjoint_step")
+    else:
+        _check(lib().orc_reactive_adjoint_step_3d(LATTICES[lattice], COLLISIONS[collision], DTYPES[dtype],
+                                                  *(_p(x) for x in arrs), *(_p(x) for x in outs), nx, ny, nz,
+                                                  float(omega), float(omega_c), float(k), dw),
+               "reactive_adjoint_step_3d")
     return outs[0], outs[1], outs[2], dw[0]
 
 
@@ -308,18 +317,25 @@ WEIGHTS = {
 }
 
 
-def reactive_gradient(collision, dtype, f0, a0, b0, omega, omega_c, k, steps):
-    """J = <A>(0) - <A>(T) of the 2D reactive loop and its discrete adjoint.
+def reactive_gradient(collision, dtype, f0, a0, b0, omega, omega_c, k, steps, lattice="D2Q9"):
+    """J = <A>(0) - <A>(T) of the reactive loop (D2Q9 + 2 D2Q5, or D3Q19 /
+    D3Q27 + 2 D3Q7) and its discrete adjoint.
 
     Returns (J, dJ/df0, dJ/da0, dJ/db0, dJ/domega_c).  Project-written CPU adjoint
     (test infrastructure): the reference has no adjoint."""
     f0, a0, b0 = (np.ascontiguousarray(x) for x in (f0, a0, b0))
     gf, ga, gb = np.empty_like(f0), np.empty_like(a0), np.empty_like(b0)
     J, dw = ctypes.c_double(), ctypes.c_double()
-    ny, nx = f0.shape[1:]
-    _check(lib().orc_reactive_gradient(COLLISIONS[collision], DTYPES[dtype], _p(f0), _p(a0), _p(b0), nx, ny,
-                                       float(omega), float(omega_c), float(k), int(steps), ctypes.byref(J), _p(gf),
-                                       _p(ga), _p(gb), ctypes.byref(dw)), "reactive_gradient")
+    nx, ny, nz = _dims(lattice, f0.shape)
+    if lattice == "D2Q9":
+        _check(lib().orc_reactive_gradient(COLLISIONS[collision], DTYPES[dtype], _p(f0), _p(a0), _p(b0), nx, ny,
+                                           float(omega), float(omega_c), float(k), int(steps), ctypes.byref(J),
+                                           _p(gf), _p(ga), _p(gb), ctypes.byref(dw)), "reactive_gradient")
+    else:
+        _check(lib().orc_reactive_gradient_3d(LATTICES[lattice], COLLISIONS[collision], DTYPES[dtype], _p(f0), _p(a0),
+                                              _p(b0), nx, ny, nz, float(omega), float(omega_c), float(k),
+                                              int(steps), ctypes.byref(J), _p(gf), _p(ga), _p(gb),
+                                              ctypes.byref(dw)), "reactive_gradient_3d")
     return J.value, gf, ga, gb, dw.value
 
 

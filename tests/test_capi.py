@@ -57,6 +57,32 @@ def test_validation_order_fluid():
     assert fluid(FAKE_A, FAKE_B, nx=1 << 31, ny=1 << 17, nz=1 << 16, block=(2, 4, 2)) == HIP_INVALID_VALUE
 
 
+def test_gradient_3d_validation():
+    """The 3D adjoint: D3Q19 / D3Q27 fluids only, BGK or TRT, whole waves."""
+    L = lbcube.lib()
+    out = ctypes.c_uint64()
+    assert L.lbc_reactive_gradient_workspace_3d(1, 0, 16, 16, 16, 8, 4, 256, 1, 1, ctypes.byref(out)) == 0
+    small = out.value
+    assert L.lbc_reactive_gradient_workspace_3d(2, 0, 16, 16, 16, 8, 4, 256, 1, 1, ctypes.byref(out)) == 0
+    assert out.value > small  # D3Q27 states hold 41 planes against D3Q19's 33
+    assert L.lbc_reactive_gradient_workspace_3d(1, 0, 16, 16, 16, 8, 4, 100, 1, 1, ctypes.byref(out)) == \
+        HIP_INVALID_CONFIGURATION
+    for fluid in (0, 3, 4):
+        assert L.lbc_reactive_gradient_workspace_3d(fluid, 0, 16, 16, 16, 8, 4, 256, 1, 1, ctypes.byref(out)) == \
+            HIP_INVALID_VALUE
+    loss, dw = ctypes.c_double(), ctypes.c_double()
+
+    def grad(fluid, coll, block=(256, 1, 1)):
+        return L.lbc_reactive_gradient_3d(fluid, coll, 0, 0, FAKE_A, FAKE_B, 0x3000, 16, 16, 16, 8, 4, 1.5, 1.2, 0.1,
+                                          ctypes.byref(loss), 0x4000, 0x5000, 0x6000, ctypes.byref(dw), 0x7000, 1 << 40,
+                                          *block, None)
+
+    assert grad(1, 2) == HIP_INVALID_VALUE  # MRT
+    assert grad(2, 3) == HIP_INVALID_VALUE  # RLBM
+    assert grad(0, 0) == HIP_INVALID_VALUE  # a 2D fluid
+    assert grad(1, 0, block=(96, 1, 1)) == HIP_INVALID_CONFIGURATION
+
+
 def test_gradient_needs_whole_waves():
     out = ctypes.c_uint64()
     lib = lbcube.lib()
