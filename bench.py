@@ -346,6 +346,53 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
                                  "adjoint_over_forward": (t_all - t_fwd) / t_fwd}}
 
 
+def adjoint_3d_ratio(torch, n=128, steps=256, interval=64):
+    """Differentiated 3D reactive run (D3Q19 TRT + two D3Q7, A+B->C), FP32 with
+    the fast forward: a sheared layer with species on either side of a
+    smoothed plane; time of the forward steps (graph replay) against the full
+    gradient (forward with checkpoints, segment recompute, adjoint sweep)."""
+    import lbcube
+    from lbcube import adjoint as ADJ
+    from lbcube import init as lbinit
+
+    r = torch.arange(n, dtype=torch.float64, device="cuda")
+    z, y, x = torch.meshgrid(r, r, r, indexing="ij")
+    u = torch.stack([0.04 * torch.tanh((y - n / 2) / 4.0), 0.004 * torch.sin(2 * math.pi * x / n),
+                     0.004 * torch.cos(2 * math.pi * z / n)])
+    ca = 0.5 * (1 + torch.tanh((x - n / 2) / 3.0))
+    f0 = lbinit.equilibrium("D3Q19", None, u, "f32")
+    a0 = lbinit.scalar_equilibrium("D3Q7", ca, u, "f32")
+    b0 = lbinit.scalar_equilibrium("D3Q7", 1.0 - ca, u, "f32")
+    del z, y, x, u, ca
+    omega, omega_c, k = 1 / 0.6, 1 / 0.8, 0.05
+    st = [f0.clone(), torch.empty_like(f0), a0.clone(), torch.empty_like(a0), b0.clone(), torch.empty_like(b0)]
+
+    def step():
+        lbcube.reactive_fused_step("D3Q19", "TRT", "D3Q7", st[0], st[1], st[2], st[3], st[4], st[5], omega, omega_c,
+                                   k, fast=True)
+        st[:] = [st[1], st[0], st[3], st[2], st[5], st[4]]
+
+    t_fwd = _graph_ms(torch, step, steps, chunk=64) * steps
+    del st
+    block = lbcube.default_block("D3Q19", n, n, n)
+    ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block, "D3Q19", n), dtype=torch.uint8,
+                     device="cuda")
+    ADJ.reactive_gradient("TRT", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws, fast=True,
+                          lattice="D3Q19")
+    out = {}
+    t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("TRT", f0, a0, b0, steps, omega, omega_c, k,
+                                                                     interval, workspace=ws, fast=True,
+                                                                     lattice="D3Q19")))
+    ws_gb = ws.numel() / 1e9
+    del ws, f0, a0, b0
+    torch.cuda.empty_cache()
+    return {"workload": f"D3Q19 TRT + 2xD3Q7 A+B->C, {n}^3, {steps} steps, FP32 (fast forward), checkpoint every "
+                        f"{interval}",
+            "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
+            "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
+            "workspace_gb": ws_gb, "loss": out["r"][0]}
+
+
 def ensemble_rate(torch, steps=1000):
     """512 independent 2D reactive instances (D2Q9 TRT + 2xD2Q5), edges
     log-uniform in 64..1024, FP16 storage with FP32 deviation-form arithmetic,
@@ -634,6 +681,11 @@ def main():
         torch.cuda.empty_cache()
         if "adjoint" in chosen:
             extras["adjoint"] = adjoint_ratio(torch)
+            extras["adjoint_3d"] = adjoint_3d_ratio(torch)
+            a3 = extras["adjoint_3d"]
+            print(f"# adjoint 3D: forward {a3['forward_ms_per_step']:.3f} ms/step, gradient "
+                  f"{a3['gradient_ms_per_step']:.3f} ms/step, adjoint / forward {a3['adjoint_over_forward']:.2f}",
+                  file=sys.stderr, flush=True)
             print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
         if "reactive_2d" in chosen:
             extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)

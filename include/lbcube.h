@@ -209,7 +209,7 @@ int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void
                           void* stream);
 
 /* The same for the 3D reactive loop: a D3Q19 or D3Q27 fluid (fluid_lattice
- * LBC_D3Q19 / LBC_D3Q27; BGK or TRT, else hipErrorInvalidValue) plus two D3Q7
+ * LBC_D3Q19 / LBC_D3Q27; BGK, TRT or MRT, else hipErrorInvalidValue) plus two D3Q7
  * species, `steps` calls of lbc_reactive_fused_step(fluid_lattice,
  * collision, LBC_D3Q7, ...) from f0, a0, b0; loss <A>(0) - <A>(T).  The
  * workspace holds the checkpoints and two segment buffers of states of

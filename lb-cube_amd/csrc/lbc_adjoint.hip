@@ -1,5 +1,5 @@
-// Discrete adjoint of the reactive loop: a D2Q9 (BGK, TRT or MRT) or D3Q19 /
-// D3Q27 (BGK or TRT) fluid plus two D2Q5 / D3Q7 species reacting A + B -> C
+// Discrete adjoint of the reactive loop: a D2Q9, D3Q19 or D3Q27 fluid (BGK,
+// TRT or MRT) plus two D2Q5 / D3Q7 species reacting A + B -> C
 // (the loop of src/simulate_reaction.cpp:516-546, run by
 // lbc_reactive_fused_step).  No counterpart in the reference.
 //
@@ -29,6 +29,7 @@
 #include <utility>
 
 #include "../../include/lbcube.h"
+#include "lbc_fast.hpp"
 #include "lbc_launch.hpp"
 #include "lbc_physics.hpp"
 #include "lbc_storage.hpp"
@@ -112,6 +113,87 @@ __device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R uz, R&
     if constexpr (F::D == 3) bz += lam[0] * R(F::w[0]) * (-R(3) * uz);
 }
 
+// w = M^-1 S M lam for the D3Q27 MRT (collide_mrt_d3q27: f' = f - M^-1 S M
+// (f - feq)), by the separable transforms of its deviation form: three 1D
+// passes of {1, c, 3c^2 - 2}, the rates divided by the row norms (the three
+// diagonal second moments relaxed in the reference's combinations X + Y + Z,
+// 2X - Y - Z, Y - Z, a symmetric 3 x 3 block), three inverse passes.  The
+// map is symmetric (orthogonal rows, rates diagonal in the reference's
+// basis), so it is also its own transpose.
+template <class R> __device__ __forceinline__ void mrt27_operator(const R* lam, R* w, R s_nu) {
+    using L = D3Q27;
+    R p[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)] = lam[i];
+    auto fwd = [](R& lo, R& mid, R& hi) {
+        const R s = lo + hi;
+        const R m0 = s + mid, m1 = hi - lo, m2 = s - R(2) * mid;
+        lo = m0;
+        mid = m1;
+        hi = m2;
+    };
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(j, 0, k)], p[g27(j, 1, k)], p[g27(j, 2, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fwd(p[g27(j, k, 0)], p[g27(j, k, 1)], p[g27(j, k, 2)]);
+    constexpr double n[3] = {3.0, 2.0, 6.0};
+    const R s_b = R(1.19), s_e = R(1.1), s_eps = R(1.2), s_q = R(1.2);
+    {
+        const R X = p[g27(2, 0, 0)], Y = p[g27(0, 2, 0)], Z = p[g27(0, 0, 2)];
+        const R dm4 = s_b * (X + Y + Z), dm5 = s_nu * (R(2) * X - Y - Z), dm6 = s_nu * (Y - Z);
+        constexpr double k = 1.0 / 54.0;
+        p[g27(2, 0, 0)] = (dm4 + dm5) * R(k / 3.0);
+        p[g27(0, 2, 0)] = (R(3) * dm6 + R(2) * dm4 - dm5) * R(k / 6.0);
+        p[g27(0, 0, 2)] = (R(-3) * dm6 + R(2) * dm4 - dm5) * R(k / 6.0);
+    }
+    p[g27(0, 0, 0)] = p[g27(1, 0, 0)] = p[g27(0, 1, 0)] = p[g27(0, 0, 1)] = R(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int order = a + b + c;
+                const int nonzero = (a > 0) + (b > 0) + (c > 0);
+                if (order <= 1 || (order == 2 && nonzero == 1)) continue;
+                R rate;
+                if (order == 2) rate = s_nu;
+                else if (order == 3 && nonzero == 2) rate = s_e;
+                else if (order == 3) rate = s_eps;
+                else rate = s_q;
+                p[g27(a, b, c)] *= rate * R(1.0 / (n[a] * n[b] * n[c]));
+            }
+    auto inv = [](R& lo, R& mid, R& hi) {
+        const R m0 = lo, m1 = mid, m2 = hi;
+        const R t = m0 + m2;
+        lo = t - m1;
+        mid = m0 - R(2) * m2;
+        hi = t + m1;
+    };
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(j, k, 0)], p[g27(j, k, 1)], p[g27(j, k, 2)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(j, 0, k)], p[g27(j, 1, k)], p[g27(j, 2, k)]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) inv(p[g27(0, j, k)], p[g27(1, j, k)], p[g27(2, j, k)]);
+#pragma unroll
+    for (int i = 0; i < 27; ++i) w[i] = p[g27(L::cx[i] + 1, L::cy[i] + 1, L::cz[i] + 1)];
+}
+
 // F, S: the fluid and species lattices; RF: the forward states' type (the
 // trajectory); R: the adjoint's state and arithmetic (R = double with RF =
 // float: the FP64 adjoint of an FP32 run).
@@ -121,7 +203,7 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
     const R* __restrict__ nua1, const R* __restrict__ nub1, R* __restrict__ nuf0, R* __restrict__ nua0,
     R* __restrict__ nub0, Cells c, R omega, R omega_minus, R omega_c, R k, double* __restrict__ djdw) {
     static_assert(F::D == S::D, "a fluid and a species lattice of one dimension");
-    static_assert(CT != kMRT || F::Q == 9, "the MRT adjoint covers D2Q9");
+
     constexpr int QF = F::Q, QS = S::Q;
     // XCD-aware block order as in the forward kernels (the reverse pulls share
     // block-edge lines the same way); each block keeps its own dJ/d omega_c
@@ -220,7 +302,56 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
             if constexpr (F::D == 3) t += bz * (R(F::cz[i]) - m.uz);
             return t;
         };
-        if constexpr (CT == kMRT) {
+        if constexpr (CT == kMRT && QF == 19) {
+            // collide_mrt_d3q19: J^T lam = M^T (I - S) mu + (d m_eq/d f)^T S mu,
+            // mu = D^-1 M lam, with the reference's equilibria written in (rho,
+            // j) (rho u_a u_b = j_a j_b / rho): rows 1, 2 (energy), 4/6/8
+            // (-2/3 j), 9-13 (shear) depend on (rho, j); ghosts are zero.
+            constexpr const Basis<19>& B = kD3Q19Basis;
+            const R rates[19] = {R(0), R(1), R(1), R(0), R(1), R(0), R(1), R(0), R(1), omega,
+                                 omega, omega, omega, omega, R(1), R(1), R(1), R(1), R(1)};
+            R nm[19], ne[19];
+#pragma unroll
+            for (int r = 0; r < 19; ++r) {
+                R acc_r = R(0);
+#pragma unroll
+                for (int i = 0; i < 19; ++i)
+                    if (B.m[r][i] != 0) acc_r += R(B.m[r][i]) * lf[i];
+                const R mu = acc_r / R(B.norm[r]);
+                ne[r] = rates[r] * mu;
+                nm[r] = mu - ne[r];
+            }
+            const R ux = m.ux, uy = m.uy, uz = m.uz, uu = ux * ux + uy * uy + uz * uz;
+            const R grho = ne[1] * (R(-11) - R(19) * uu) + ne[2] * (R(3) + R(5.5) * uu) -
+                           ne[9] * R(3) * (R(2) * ux * ux - uy * uy - uz * uz) - ne[10] * (uy * uy - uz * uz) -
+                           ne[11] * ux * uy - ne[12] * uy * uz - ne[13] * ux * uz;
+            const R two_thirds = R(2) / R(3);
+            const R gjx = R(38) * ux * ne[1] - R(11) * ux * ne[2] - two_thirds * ne[4] + R(12) * ux * ne[9] +
+                          uy * ne[11] + uz * ne[13];
+            const R gjy = R(38) * uy * ne[1] - R(11) * uy * ne[2] - two_thirds * ne[6] - R(6) * uy * ne[9] +
+                          R(2) * uy * ne[10] + ux * ne[11] + uz * ne[12];
+            const R gjz = R(38) * uz * ne[1] - R(11) * uz * ne[2] - two_thirds * ne[8] - R(6) * uz * ne[9] -
+                          R(2) * uz * ne[10] + uy * ne[12] + ux * ne[13];
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                R t = R(0);
+#pragma unroll
+                for (int r = 0; r < 19; ++r)
+                    if (B.m[r][i] != 0) t += R(B.m[r][i]) * nm[r];
+                put(plane_ptr(nuf0, i, plane) + cell, t + grho + R(F::cx[i]) * gjx + R(F::cy[i]) * gjy +
+                                                          R(F::cz[i]) * gjz);
+            }
+        } else if constexpr (CT == kMRT && QF == 27) {
+            // collide_mrt_d3q27: f' = f - M^-1 S M (f - feq) with M^-1 S M
+            // symmetric, so J^T lam = lam - w + (d feq/d f)^T w, w = M^-1 S M lam.
+            R w[27];
+            mrt27_operator<R>(lf, w, omega);
+            R al, bx, by, bz;
+            feq_transpose<F, R>(w, m.ux, m.uy, m.uz, al, bx, by, bz);
+#pragma unroll
+            for (int i = 0; i < 27; ++i)
+                put(plane_ptr(nuf0, i, plane) + cell, lf[i] - w[i] + al + dev(i, bx, by, bz));
+        } else if constexpr (CT == kMRT) {
             // f' = M^-1 [(I - S) M f + S m_eq(rho, j)], M^-1 = M^T D^-1
             // (collide_mrt_d2q9): J^T lam = M^T (I - S) mu + (d m_eq/d f)^T S mu
             // with mu = D^-1 M lam and d/df_i = d/drho + c_ix d/djx + c_iy d/djy.
@@ -552,12 +683,11 @@ constexpr uint32_t kMaxAdjointCells = 1u << 29;
 
 namespace lbc {
 
-// The pairs the adjoint covers: D2Q9 + D2Q5 (BGK, TRT, MRT), D3Q19 / D3Q27 +
-// D3Q7 (BGK, TRT).
+// The pairs the adjoint covers: D2Q9 + D2Q5, D3Q19 / D3Q27 + D3Q7, each with
+// BGK, TRT or MRT.
 inline bool gradient_pair(int fluid, int collision) {
-    if (fluid == kD2Q9) return collision == kBGK || collision == kTRT || collision == kMRT;
-    if (fluid == kD3Q19 || fluid == kD3Q27) return collision == kBGK || collision == kTRT;
-    return false;
+    if (fluid != kD2Q9 && fluid != kD3Q19 && fluid != kD3Q27) return false;
+    return collision == kBGK || collision == kTRT || collision == kMRT;
 }
 
 inline size_t state_planes(int fluid) {
@@ -625,8 +755,7 @@ hipError_t gradient(int fluid, int collision, int storage, uint32_t flags, const
     auto by_collision = [&](auto fs) {
         if (collision == kBGK) return by_precision(fs, BGK{});
         if (collision == kTRT) return by_precision(fs, TRT{});
-        if constexpr (std::is_same<typename decltype(fs)::first_type, D2Q9>::value) return by_precision(fs, MRT{});
-        return hipErrorInvalidValue;
+        return by_precision(fs, MRT{});
     };
     if (fluid == kD2Q9) return by_collision(std::pair<D2Q9, D2Q5>{});
     if (fluid == kD3Q19) return by_collision(std::pair<D3Q19, D3Q7>{});

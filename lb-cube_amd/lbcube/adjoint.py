@@ -1,7 +1,7 @@
 """Gradients of the reactive loop through the HIP discrete adjoint.
 
-The loop is D2Q9 (BGK, TRT or MRT) + two D2Q5 species, or D3Q19 / D3Q27 (BGK
-or TRT) + two D3Q7 species, with A + B -> C, stepped by
+The loop is D2Q9 + two D2Q5 species, or D3Q19 / D3Q27 + two D3Q7 species (BGK,
+TRT or MRT), with A + B -> C, stepped by
 lbc_reactive_fused_step; the loss is the domain-mean product <C>(T) =
 <A>(0) - <A>(T).  `reactive_gradient` calls lbc_reactive_gradient (2D) or
 lbc_reactive_gradient_3d (forward with checkpoints every `interval` steps,
@@ -45,7 +45,7 @@ def reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c, k_react, int
                       workspace=None, stream=None, fast=False, adjoint_f64=False, lattice="D2Q9"):
     """(loss, dloss/df0, dloss/da0, dloss/db0, dloss/domega_c) by the discrete adjoint;
     lattice: the fluid lattice (D2Q9 with D2Q5 species; D3Q19 or D3Q27 with
-    D3Q7 species, BGK or TRT); fast=True differentiates the forward of
+    D3Q7 species); fast=True differentiates the forward of
     reactive_fused_step(..., fast=True); adjoint_f64=True (float32 inputs)
     carries the adjoint in float64 on the float32 trajectory
     (LBC_FLAG_F64_ADJOINT)."""

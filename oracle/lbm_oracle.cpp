@@ -1081,8 +1081,29 @@ template <class R> void reaction_grad(R a, R b, R k, R& dsa, R& dsb) {
     dsb = -a * k * (k * a * dphi + e) / (den * den);
 }
 
+// Rows of the D3Q19 moment basis (M[r][i] = moments of the unit population
+// e_i) and their norms sum_i M[r][i]^2, for the MRT adjoint.
+struct D3q19Rows {
+    double m[19][19];
+    double norm[19];
+};
+template <class R> D3q19Rows d3q19_rows() {
+    D3q19Rows b{};
+    for (int i = 0; i < 19; ++i) {
+        R e[19] = {}, mo[19];
+        e[i] = R(1);
+        moments_d3q19<R>(e, mo);
+        for (int r = 0; r < 19; ++r) b.m[r][i] = static_cast<double>(mo[r]);
+    }
+    for (int r = 0; r < 19; ++r) {
+        b.norm[r] = 0;
+        for (int i = 0; i < 19; ++i) b.norm[r] += b.m[r][i] * b.m[r][i];
+    }
+    return b;
+}
+
 // The fluid lattice F (D2Q9, D3Q19, D3Q27; all cs2 = 1/3) and the species
-// lattice S (D2Q5, D3Q7) of one reactive pair; MRT for D2Q9 only.
+// lattice S (D2Q5, D3Q7) of one reactive pair; BGK, TRT and MRT.
 template <class F, class S, class R, int CT>
 void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, const R* nua1, const R* nub1, R* nuf0,
                            R* nua0, R* nub0, Grid g, R omega, R omega_c, R k, long double* djdw) {
@@ -1179,8 +1200,53 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                 for (int d = 0; d < D; ++d) t += bd[d] * (static_cast<R>(F::c[i * D + d]) - m.u[d]);
                 return t;
             };
-            if constexpr (CT == C_MRT) {
-                static_assert(std::is_same<F, D2Q9>::value, "the MRT adjoint covers D2Q9");
+            if constexpr (CT == C_MRT && F::Q == 19) {
+                // collide_mrt_d3q19: f' = M^-1 [(I - S) M f + S m_eq(rho, j)]
+                // with an orthogonal basis and diagonal S, so M^-1 (I - S) M is
+                // symmetric and J^T lam = M^-1 (I - S) M lam + (d m_eq/d f)^T S mu,
+                // mu = D^-1 M lam; d m_eq / d f_i = d/drho + c_i . d/dj of the
+                // equilibria written in (rho, j): rho u_a u_b = j_a j_b / rho.
+                static const D3q19Rows B = d3q19_rows<R>();
+                const R rates[19] = {R{0}, R{1}, R{1}, R{0}, R{1}, R{0}, R{1}, R{0}, R{1}, omega,
+                                     omega, omega, omega, omega, R{1}, R{1}, R{1}, R{1}, R{1}};
+                R mo[19], keep[19], back[19], ne[19];
+                moments_d3q19<R>(lf, mo);
+                for (int r = 0; r < 19; ++r) {
+                    keep[r] = (R(1) - rates[r]) * mo[r];
+                    ne[r] = rates[r] * mo[r] / static_cast<R>(B.norm[r]);
+                }
+                populations_d3q19<R>(keep, back);
+                const R ux = m.u[0], uy = m.u[1], uz = m.u[2];
+                const R grho = ne[1] * (R(-11) - R(19) * uu) + ne[2] * (R(3) + R(5.5) * uu) -
+                               ne[9] * R(3) * (R(2) * ux * ux - uy * uy - uz * uz) - ne[10] * (uy * uy - uz * uz) -
+                               ne[11] * ux * uy - ne[12] * uy * uz - ne[13] * ux * uz;
+                const R gj[3] = {R(38) * ux * ne[1] - R(11) * ux * ne[2] - R(2) / R(3) * ne[4] + R(12) * ux * ne[9] +
+                                     uy * ne[11] + uz * ne[13],
+                                 R(38) * uy * ne[1] - R(11) * uy * ne[2] - R(2) / R(3) * ne[6] - R(6) * uy * ne[9] +
+                                     R(2) * uy * ne[10] + ux * ne[11] + uz * ne[12],
+                                 R(38) * uz * ne[1] - R(11) * uz * ne[2] - R(2) / R(3) * ne[8] - R(6) * uz * ne[9] -
+                                     R(2) * uz * ne[10] + uy * ne[12] + ux * ne[13]};
+                for (int i = 0; i < 19; ++i) {
+                    R t = grho;
+                    for (int d = 0; d < 3; ++d) t += static_cast<R>(F::c[i * 3 + d]) * gj[d];
+                    nuf0[i * N + cell] = back[i] + t;
+                }
+            } else if constexpr (CT == C_MRT && F::Q == 27) {
+                // collide_mrt_d3q27: f' = f - M^-1 S M (f - feq), M^-1 S M symmetric
+                // (orthogonal basis, diagonal S), so J^T lam = lam - w + (d feq/d f)^T w
+                // with w = M^-1 S M lam.
+                const R s_b = R{1.19}, s_e = R{1.1}, s_eps = R{1.2}, s_q = R{1.2};
+                const R rates[27] = {R{0}, R{0}, R{0}, R{0}, s_b, omega, omega, omega, omega, omega, s_e, s_e, s_e,
+                                     s_e, s_e, s_e, s_eps, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q, s_q};
+                R mo[27], w[27];
+                moments_d3q27<R>(lf, mo);
+                for (int r = 0; r < 27; ++r) mo[r] *= rates[r];
+                populations_d3q27<R>(mo, w);
+                R al, bd[3];
+                moments(w, al, bd);
+                for (int i = 0; i < 27; ++i) nuf0[i * N + cell] = lf[i] - w[i] + al + dev(i, bd);
+            } else if constexpr (CT == C_MRT) {
+                static_assert(std::is_same<F, D2Q9>::value, "the MRT adjoint covers D2Q9, D3Q19 and D3Q27");
                 // f' = M^-1 [(I - S) M f + S m_eq(rho, j)] with M^-1 = M^T D^-1
                 // (collide_mrt_d2q9, lattice_mrt.hpp:548-603), so
                 // J^T lam = M^T (I - S) mu + (d m_eq / d f)^T S mu, mu = D^-1 M lam,
@@ -1289,8 +1355,8 @@ void reactive_gradient(const R* f0, const R* a0, const R* b0, Grid g, R omega, R
     *djdwc = static_cast<R>(dj);
 }
 
-// Entry helpers of the adjoint: the collisions the adjoint covers for a
-// fluid/species pair (BGK, TRT; MRT for D2Q9), dispatched at run time.
+// Entry helpers of the adjoint: the collisions the adjoint covers (BGK, TRT,
+// MRT) for a fluid/species pair, dispatched at run time.
 template <class F, class S>
 int gradient_entry(int ct, int dt, const void* f0, const void* a0, const void* b0, Grid g, double omega,
                    double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb, double* djdwc) {
@@ -1309,9 +1375,7 @@ int gradient_entry(int ct, int dt, const void* f0, const void* a0, const void* b
     auto by_ct = [&](auto tag) -> int {
         if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
         if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
-        if constexpr (std::is_same<F, D2Q9>::value) {
-            if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
-        }
+        if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
         return -1;
     };
     if (dt == T_F64) return by_ct(double{});
@@ -1337,9 +1401,7 @@ int adjoint_step_entry(int ct, int dt, const void* f, const void* a, const void*
     auto by_ct = [&](auto tag) -> int {
         if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
         if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
-        if constexpr (std::is_same<F, D2Q9>::value) {
-            if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
-        }
+        if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
         return -1;
     };
     int rc;
@@ -1604,7 +1666,7 @@ int orc_reactive_gradient(int ct, int dt, const void* f0, const void* a0, const 
                                       djdwc);
 }
 
-// The same for the 3D pairs (D3Q19 or D3Q27 fluid + two D3Q7 species; BGK, TRT).
+// The same for the 3D pairs (D3Q19 or D3Q27 fluid + two D3Q7 species; BGK, TRT, MRT).
 int orc_reactive_gradient_3d(int fluid, int ct, int dt, const void* f0, const void* a0, const void* b0,
                              long long nx, long long ny, long long nz, double omega, double omega_c, double k,
                              int steps, double* J, void* gf, void* ga, void* gb, double* djdwc) {

@@ -1,5 +1,5 @@
 """GPU: the discrete adjoint of the 3D reactive loop (lbc_reactive_gradient_3d):
-a D3Q19 or D3Q27 fluid (BGK, TRT) with two D3Q7 species, A + B -> C.
+a D3Q19 or D3Q27 fluid (BGK, TRT, MRT) with two D3Q7 species, A + B -> C.
 
   * against the oracle's FP64 CPU adjoint of the same loop (tight tolerance;
     checkpoint intervals 1, 5 and 8 over 17 steps, so the segment recompute
@@ -28,7 +28,7 @@ def dev(x, gpu, dtype=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(x)).to(gpu).to(dtype)
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
 @pytest.mark.parametrize("interval", [1, 5, 8])
 def test_adjoint_3d_matches_oracle_fp64(gpu, lattice, coll, interval):
@@ -89,32 +89,33 @@ def sheared_state(n, lattice, gpu):
     return dev(f, gpu), dev(a, gpu), dev(b, gpu)
 
 
+@pytest.mark.parametrize("coll", ["TRT", "MRT"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
-def test_adjoint_3d_central_differences(gpu, lattice):
+def test_adjoint_3d_central_differences(gpu, lattice, coll):
     """40^3, 200 steps, checkpoints every 32: directional derivatives along 6
     random directions (all populations) and dJ/d omega_c against FP64
     central differences of the forward loop."""
-    print(f"case adjoint 3d central differences {lattice}", flush=True)
+    print(f"case adjoint 3d central differences {lattice} {coll}", flush=True)
     n, steps, omega, omega_c, k = 40, 200, 1 / 0.6, 1 / 0.8, 0.05
     f, a, b = sheared_state(n, lattice, gpu)
-    J, gf, ga, gb, dw = ADJ.reactive_gradient("TRT", f, a, b, steps, omega, omega_c, k, interval=32, lattice=lattice)
-    assert abs(J - ADJ.loss_of("TRT", f, a, b, steps, omega, omega_c, k, lattice=lattice)) <= 1e-14
+    J, gf, ga, gb, dw = ADJ.reactive_gradient(coll, f, a, b, steps, omega, omega_c, k, interval=32, lattice=lattice)
+    assert abs(J - ADJ.loss_of(coll, f, a, b, steps, omega, omega_c, k, lattice=lattice)) <= 1e-14
     gen = torch.Generator(device="cpu").manual_seed(7)
     worst = 0.0
     for trial in range(6):
         vf, va, vb = (torch.randn(x.shape, generator=gen, dtype=torch.float64).to(gpu) * 1e-3 for x in (f, a, b))
         h = 1e-3
-        jp = ADJ.loss_of("TRT", f + h * vf, a + h * va, b + h * vb, steps, omega, omega_c, k, lattice=lattice)
-        jm = ADJ.loss_of("TRT", f - h * vf, a - h * va, b - h * vb, steps, omega, omega_c, k, lattice=lattice)
+        jp = ADJ.loss_of(coll, f + h * vf, a + h * va, b + h * vb, steps, omega, omega_c, k, lattice=lattice)
+        jm = ADJ.loss_of(coll, f - h * vf, a - h * va, b - h * vb, steps, omega, omega_c, k, lattice=lattice)
         fd = (jp - jm) / (2 * h)
         ad = ((gf * vf).sum() + (ga * va).sum() + (gb * vb).sum()).item()
         err = abs(fd - ad) / abs(fd)
         worst = max(worst, err)
-        print(f"{lattice} direction {trial}: fd {fd:.10e} adjoint {ad:.10e} ({err:.1e})", flush=True)
+        print(f"{lattice} {coll} direction {trial}: fd {fd:.10e} adjoint {ad:.10e} ({err:.1e})", flush=True)
     hw = 1e-5
-    fdw = (ADJ.loss_of("TRT", f, a, b, steps, omega, omega_c + hw, k, lattice=lattice) -
-           ADJ.loss_of("TRT", f, a, b, steps, omega, omega_c - hw, k, lattice=lattice)) / (2 * hw)
-    print(f"{lattice} omega_c: fd {fdw:.10e} adjoint {dw:.10e}; worst direction {worst:.1e}", flush=True)
+    fdw = (ADJ.loss_of(coll, f, a, b, steps, omega, omega_c + hw, k, lattice=lattice) -
+           ADJ.loss_of(coll, f, a, b, steps, omega, omega_c - hw, k, lattice=lattice)) / (2 * hw)
+    print(f"{lattice} {coll} omega_c: fd {fdw:.10e} adjoint {dw:.10e}; worst direction {worst:.1e}", flush=True)
     assert worst <= 1e-6 and abs(fdw - dw) <= 1e-6 * abs(fdw)
 
 
