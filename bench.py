@@ -24,7 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "lb-cube_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E spec peak
-BYTES_PER_CELL = {("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16"): 76, ("D3Q19", "f64"): 304,
+BYTES_PER_CELL = {("D2Q9", "f32"): 72, ("D2Q9", "f16"): 36,
+                  ("D3Q19", "f32"): 152, ("D3Q19", "f16"): 76, ("D3Q19", "bf16"): 76, ("D3Q19", "f64"): 304,
                   ("D3Q27", "f32"): 216, ("D3Q27", "f16"): 108, ("D3Q27", "bf16"): 108, ("D3Q27", "f64"): 432}
 
 
@@ -555,6 +556,15 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     del f, a, b, state
     torch.cuda.empty_cache()
     out["workload"] = f"D2Q9 MRT + 2xD2Q5 A+B->C, {n}^2 double shear layer, Da=10 (k = Da u0 / delta = 0.005)"
+    # the D2Q9 plane streams of the same domain, with and without the pull
+    # shifts, in both element sizes: what the shifted 2- and 4-byte loads cost
+    for storage in ("f32", "f16"):
+        for shift in (True, False):
+            gbs, ms = pattern_ceiling(lbcube, torch, u, steps=50, warmup=5, lattice="D2Q9", storage=storage,
+                                      shift=shift)
+            key = f"d2q9_{'pull' if shift else 'plane_copy'}_{storage}"
+            out[key] = {"achieved_gbs": round(gbs, 1), "ms_per_step": round(ms, 4)}
+            print(f"# reactive 2D {key}: {gbs:.0f} GB/s", file=sys.stderr, flush=True)
     out.update(reactive_2d_large(lbcube, torch))
     return out
 
