@@ -565,7 +565,28 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
             key = f"d2q9_{'pull' if shift else 'plane_copy'}_{storage}"
             out[key] = {"achieved_gbs": round(gbs, 1), "ms_per_step": round(ms, 4)}
             print(f"# reactive 2D {key}: {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    out["d2q9_plane_copy_by_row_length"] = plane_copy_by_row_length(lbcube, torch)
     out.update(reactive_2d_large(lbcube, torch))
+    return out
+
+
+def plane_copy_by_row_length(lbcube, torch, cells=1 << 22, reps=50):
+    """GB/s of the unshifted D2Q9 plane copy (lbc_plane_copy) at a fixed cell
+    count with rows of 256 to 8,192 cells, FP32 and FP16."""
+    out = {}
+    for storage, dt in (("f32", torch.float32), ("f16", torch.float16)):
+        for nx in (256, 1024, 2048, 8192):
+            bufs = [torch.zeros((9, cells // nx, nx), dtype=dt, device="cuda")]
+            bufs.append(torch.empty_like(bufs[0]))
+
+            def step():
+                lbcube.pull_stream("D2Q9", bufs[0], bufs[1], shift=False)
+                bufs.reverse()
+
+            ms = _events_ms(torch, step, reps, 5)
+            out[f"{storage}_nx{nx}"] = round(2 * 9 * bufs[0].element_size() * cells / (ms * 1e-3) / 1e9, 1)
+            del bufs
+    print(f"# reactive 2D plane copy by row length: {out}", file=sys.stderr, flush=True)
     return out
 
 
