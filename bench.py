@@ -30,31 +30,41 @@ BYTES_PER_CELL = {("D2Q9", "f32"): 72, ("D2Q9", "f16"): 36,
 
 
 def time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, block=None, fast=False,
-                 state=None, mfma=False):
+                 state=None, mfma=False, layout="planes"):
     """MLUPS of `steps` fluid steps from the equilibrium of velocity field u
     (or from `state`, a population tensor the call takes over); mfma: the
-    deviation-form MRT's transforms on the matrix cores."""
+    deviation-form MRT's transforms on the matrix cores; layout "slabs": the
+    populations stored slab by slab, [Z][Q][Y][X] (lbc_fluid_step_strided),
+    converted before the warm-up and back after the timed steps."""
     from lbcube import init as lbinit
 
     a = lbinit.equilibrium(lattice, None, u, storage) if state is None else state
     del state
+    cells = a[0].numel()
+    step = lbcube.fluid_step
+    if layout == "slabs":
+        a = lbcube.to_slabs(lattice, a)
+        step = lbcube.fluid_step_slabs
     b = torch.empty_like(a)
     for _ in range(warmup):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
+        step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
         a, b = b, a
     stream = torch.cuda.current_stream()
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     for _ in range(steps):
-        lbcube.fluid_step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
+        step(lattice, coll, a, b, omega, block=block, fast=fast, mfma=mfma)
         a, b = b, a
     end.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / steps
-    cells = a[0].numel()
     mlups = cells / (ms * 1e-3) / 1e6
     gbs = BYTES_PER_CELL[(lattice, storage)] * cells / (ms * 1e-3) / 1e9
+    if layout == "slabs":
+        del b
+        a = lbcube.from_slabs(lattice, a)
+        b = None
     diag = lbcube.fluid_diagnostics(lattice, a)
     del a, b
     torch.cuda.empty_cache()
@@ -96,23 +106,32 @@ def _events_ms(torch, fn, steps, warmup):
     return start.elapsed_time(end) / steps
 
 
-def pattern_ceiling(lbcube, torch, u, steps=20, warmup=3, lattice="D3Q19", storage="f32", shift=True):
+def pattern_ceiling(lbcube, torch, u, steps=20, warmup=3, lattice="D3Q19", storage="f32", shift=True,
+                    layout="planes"):
     """GB/s of the pull-streaming step alone (lbc_pull_stream: the fluid
     kernels' loads, stores, addressing and residency, no collision) on the
     lattice's buffers (default: the headline's D3Q19 FP32): the ceiling of
     its access pattern.  shift=False: the same plane streams unshifted
-    (lbc_plane_copy)."""
+    (lbc_plane_copy).  layout "slabs": the [Z][Q][Y][X] order
+    (lbc_pull_stream_strided)."""
     from lbcube import init as lbinit
 
     bufs = [lbinit.equilibrium(lattice, None, u, storage)]
+    cells = bufs[0][0].numel()
+    if layout == "slabs":
+        bufs[0] = lbcube.to_slabs(lattice, bufs[0])
     bufs.append(torch.empty_like(bufs[0]))
 
     def step():
-        lbcube.pull_stream(lattice, bufs[0], bufs[1], shift=shift)
+        if layout == "slabs":
+            ext = (bufs[0].shape[3], bufs[0].shape[2], bufs[0].shape[0])
+            lbcube.pull_stream_strided(lattice, bufs[0], bufs[1], ext, *lbcube.slab_strides(lattice, ext),
+                                       shift=shift)
+        else:
+            lbcube.pull_stream(lattice, bufs[0], bufs[1], shift=shift)
         bufs.reverse()
 
     ms = _events_ms(torch, step, steps, warmup)
-    cells = bufs[0][0].numel()
     del bufs
     torch.cuda.empty_cache()
     return BYTES_PER_CELL[(lattice, storage)] * cells / (ms * 1e-3) / 1e9, ms
@@ -666,7 +685,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", default="trt_f32,trt_f16fast,trt_f32fast,trt_f16,bgk_f32,bgk_f16fast,mrt_f32,"
-                                          "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast")
+                                          "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast,trt_f32_slabs,"
+                                          "trt_f32fast_slabs,trt_f16fast_slabs,mrt_f16fast_slabs",
+                    help="coll_storage[fast][_slabs]: _slabs stores the populations slab by slab, [Z][Q][Y][X]")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
     ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large",
@@ -691,11 +712,11 @@ def main():
     block = tuple(int(v) for v in args.block.split(",")) if args.block else None
     variants = {}
     for name in args.variants.split(","):
-        coll, storage = name.split("_")
+        coll, storage, *layout = name.split("_")
         fast = storage.endswith("fast")
         storage = storage[:-4] if fast else storage
         variants[name] = time_variant(lbcube, torch, "D3Q19", coll.upper(), storage, u, omega, args.steps,
-                                      args.warmup, block=block, fast=fast)
+                                      args.warmup, block=block, fast=fast, layout=layout[0] if layout else "planes")
         print(f"# {name}: {variants[name]['mlups']:.0f} MLUPS, {variants[name]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
     head = variants["trt_f32"]
@@ -704,6 +725,10 @@ def main():
     print(f"# pull-stream ceiling: {pattern_gbs:.0f} GB/s ({pattern_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     planes_gbs, planes_ms = pattern_ceiling(lbcube, torch, u, shift=False)
     print(f"# unshifted 38-plane copy: {planes_gbs:.0f} GB/s ({planes_ms:.4f} ms/step)", file=sys.stderr, flush=True)
+    slab_pattern_gbs, _ = pattern_ceiling(lbcube, torch, u, layout="slabs")
+    slab_planes_gbs, _ = pattern_ceiling(lbcube, torch, u, shift=False, layout="slabs")
+    print(f"# slab order [Z][Q][Y][X]: pull-stream {slab_pattern_gbs:.0f} GB/s, unshifted copy {slab_planes_gbs:.0f} "
+          "GB/s", file=sys.stderr, flush=True)
     chosen = set() if args.no_extras else set(args.extras.split(","))
     pitched = pitched_variants(lbcube, torch, u, omega, args.steps, args.warmup) if "pitched" in chosen else None
     extras = {}
@@ -767,6 +792,14 @@ def main():
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                      for k, v in variants.items()},
     }
+    if "trt_f32_slabs" in variants:
+        sl = variants["trt_f32_slabs"]
+        out["slab_order"] = {
+            "layout": "[Z][Q][Y][X]: the 19 planes of each xy-slab side by side (lbc_fluid_step_strided)",
+            "kernel": "fluid_step_kernel<D3Q19, TRT, F32>", "mlups": round(sl["mlups"], 1),
+            "achieved_gbs": round(sl["achieved_gbs"], 1), "frac": round(sl["roofline_frac"], 4),
+            "pattern_ceiling_gbs": round(slab_pattern_gbs, 1), "plane_streams_ceiling_gbs": round(slab_planes_gbs, 1),
+            "over_reference_layout": round(sl["mlups"] / head["mlups"], 4)}
     if pitched:
         out["pitched_planes"] = {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in pitched.items()}
     if "trt_f16fast" in variants:

@@ -127,6 +127,47 @@ int lbc_pull_stream(int lattice, int storage, const void* current, void* next, u
 int lbc_plane_copy(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny, uint64_t nz,
                    uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
 
+/* Strided population layouts.  Population i of cell (x, y, z) lies at element
+ *   i * plane_stride + z * slab_stride + y * nx + x
+ * of a buffer.  Two families are accepted:
+ *  - slab_stride 0 or nx * ny: the reference's [Q][Z][Y][X] planes, with
+ *    plane_stride 0 (the cell count) or at least the cell count (pitched, as
+ *    lbc_fluid_step_pitched);
+ *  - a larger slab_stride: the planes stored slab by slab, with plane_stride
+ *    0 (= nx * ny) or at least nx * ny and slab_stride at least Q times
+ *    plane_stride; plane_stride = nx * ny and slab_stride = Q * nx * ny is the
+ *    dense [Z][Q][Y][X] order, in which the Q planes of one xy-slab lie
+ *    side by side.
+ * Anything else, or a layout whose per-plane offsets reach 2^32 elements
+ * ((nz - 1) * slab_stride + nx * ny > 2^32 - 1), -> hipErrorInvalidValue.
+ * Elements outside the cells are neither read nor written. */
+
+/* lbc_fluid_step_ex on a strided layout.  Every cell's result equals the
+ * contiguous layout's bit for bit; only the placement differs.  The dense
+ * [Z][Q][Y][X] order keeps a launch's 2 Q streams within Q * nx * ny elements
+ * of each other (the reference's planes put them a whole plane apart), which
+ * the HBM serves at a higher rate.  No counterpart in the reference: a
+ * caller that holds the reference's planes converts once with lbc_relayout
+ * and steps in the slab order. */
+int lbc_fluid_step_strided(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
+                           uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_stride, uint64_t slab_stride,
+                           double omega, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream);
+
+/* lbc_pull_stream (shift != 0) or lbc_plane_copy (shift == 0) on a strided
+ * layout: the access-pattern ceilings of lbc_fluid_step_strided.  No
+ * counterpart in the reference. */
+int lbc_pull_stream_strided(int lattice, int storage, int shift, const void* current, void* next, uint64_t nx,
+                            uint64_t ny, uint64_t nz, uint64_t plane_stride, uint64_t slab_stride, uint32_t block_x,
+                            uint32_t block_y, uint32_t block_z, void* stream);
+
+/* Copies a population set from one strided layout to another (both as
+ * above; element size from `storage`), e.g. the reference's planes to the
+ * [Z][Q][Y][X] order and back.  src == dst -> hipErrorInvalidValue; the two
+ * buffers must not overlap.  No counterpart in the reference. */
+int lbc_relayout(int lattice, int storage, const void* src, uint64_t src_plane_stride, uint64_t src_slab_stride,
+                 void* dst, uint64_t dst_plane_stride, uint64_t dst_slab_stride, uint64_t nx, uint64_t ny,
+                 uint64_t nz, void* stream);
+
 /* Reference concentrations of scalar fields.  With LBC_F16 / LBC_BF16
  * storage a scalar field's populations are stored as g_i - w_i c_ref, with
  * c_ref the field's reference concentration (w_i alone is the fluid's rest

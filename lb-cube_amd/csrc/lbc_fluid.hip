@@ -38,7 +38,8 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
-    const Pull pull(dst, q.x, q.y, q.z, nx, ny, nz);
+    const uint32_t cell = dst + static_cast<uint32_t>(q.z) * c.zgap;
+    const Pull pull(cell, q.x, q.y, q.z, nx, ny, nz, c.zgap);
     R f[L::Q];
     if constexpr (FAST) {
 #pragma unroll
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             f[i] = St::load_dev(at<WIDE>(plane_ptr(cur, i, plane), src), R(L::w[i]));
         }
         collide_dev<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
-            St::store_dev(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v, R(L::w[i]));
+            St::store_dev(at<WIDE>(plane_ptr_opaque(nxt, i, plane), cell), v, R(L::w[i]));
         });
     } else {
 #pragma unroll
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
             f[i] = St::load(at<WIDE>(plane_ptr(cur, i, plane), src), off);
         }
         collide_cell<L, CT, R>(f, omega, omega_minus, [&](int i, R v) {
-            St::store(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
+            St::store(at<WIDE>(plane_ptr_opaque(nxt, i, plane), cell), v, kOffsetStorage<ST> ? R(L::w[i]) : R(0));
         });
     }
 }
@@ -84,9 +85,12 @@ __global__ __launch_bounds__(MAXT) void fluid_step_mfma_kernel(const uint16_t* _
     uint32_t hw[16];  // the cell's stored deviations in pairs (population 2v in the low half), zero-padded
 #pragma unroll
     for (int v = 0; v < 16; ++v) hw[v] = 0u;
+    uint32_t off = 0u;  // the cell's element offset in its planes
     if (live) {
         const CellCoord q = cell_coord(dst, c);
-        const Pull pull(dst, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
+        off = dst + static_cast<uint32_t>(q.z) * c.zgap;
+        const Pull pull(off, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz),
+                        c.zgap);
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
             const uint32_t raw = get(at<WIDE>(plane_ptr(cur, i, plane), pull.src(L::cx[i], L::cy[i], L::cz[i])));
@@ -111,7 +115,7 @@ __global__ __launch_bounds__(MAXT) void fluid_step_mfma_kernel(const uint16_t* _
     mfma_product<ST>(a, pw, out);  // out = M^T p'
     if (!live) return;
     constexpr float inv_scale = 1.0f / kMfmaScale<ST>;
-    const uint32_t cell = fresh_lane_value(dst);
+    const uint32_t cell = fresh_lane_value(off);
 #pragma unroll
     for (int i = 0; i < L::Q; ++i) {
         const float h = decode16<ST>((hw[i / 2] >> (16 * (i % 2))) & 0xFFFFu);
@@ -134,13 +138,15 @@ __global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storag
     const uint32_t dst = block * blockDim.x + threadIdx.x;
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
-    const Pull pull(dst, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
+    const uint32_t cell = dst + static_cast<uint32_t>(q.z) * c.zgap;
+    const Pull pull(cell, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz),
+                    c.zgap);
     typename Storage<ST>::S v[L::Q];
 #pragma unroll
     for (int i = 0; i < L::Q; ++i)
-        v[i] = get(at<WIDE>(plane_ptr(cur, i, plane), SHIFT ? pull.src(L::cx[i], L::cy[i], L::cz[i]) : dst));
+        v[i] = get(at<WIDE>(plane_ptr(cur, i, plane), SHIFT ? pull.src(L::cx[i], L::cy[i], L::cz[i]) : cell));
 #pragma unroll
-    for (int i = 0; i < L::Q; ++i) put(at<WIDE>(plane_ptr_opaque(nxt, i, plane), dst), v[i]);
+    for (int i = 0; i < L::Q; ++i) put(at<WIDE>(plane_ptr_opaque(nxt, i, plane), cell), v[i]);
 }
 
 // Kernels that run faster with residency capped at kCappedWavesPerCu (measured
@@ -155,6 +161,17 @@ template <class L, int CT, int ST, bool FAST> constexpr bool capped_residency() 
     return true;
 }
 
+// Resident waves per CU where the residency is capped: kCappedWavesPerCu,
+// except the D3Q19 step in FP16 with the deviation-form arithmetic, which
+// keeps more waves busy per byte moved: at 256^3, 24 waves per CU gave TRT
+// 74.5-75.4k MLUPS against 70.4-74.6k at 16 (81.5-81.9k against 79.4-79.8k
+// with the planes stored slab by slab) and MRT 75.1-75.8k against 73.8-74.0k
+// (81.3-81.8k against 77.2-77.6k); 28 gave the same, 32 less.  D3Q27 in FP16
+// (50.7-50.8k at 24 against 51.4-51.7k) and every FP32 step stay at 16.
+template <class L, int CT, int ST, bool FAST> constexpr uint32_t capped_waves() {
+    return (L::Q == 19 && ST == kF16 && FAST) ? 24u : kCappedWavesPerCu;
+}
+
 template <class L, int CT, int ST, bool FAST>
 hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, double omega,
                         hipStream_t stream) {
@@ -162,7 +179,8 @@ hipError_t launch_fluid(const void* cur, void* nxt, const LaunchShape& s, uint64
     using R = typename St::R;
     const R w = static_cast<R>(omega);
     const R wm = CT == kTRT ? trt_omega_minus<R>(w) : R(0);
-    const uint32_t lds = capped_residency<L, CT, ST, FAST>() ? capped_lds(s.threads()) : 0;
+    const uint32_t lds =
+        capped_residency<L, CT, ST, FAST>() ? capped_lds(s.threads(), capped_waves<L, CT, ST, FAST>()) : 0;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const typename St::S*>(cur),
                            static_cast<typename St::S*>(nxt), s.cells, static_cast<size_t>(plane), w, wm);
@@ -241,7 +259,9 @@ hipError_t dispatch_mfma(int lattice, int collision, int storage, uint32_t flags
 template <class L, int ST, bool SHIFT>
 hipError_t launch_stream(const void* cur, void* nxt, const LaunchShape& s, uint64_t plane, hipStream_t stream) {
     using S = typename Storage<ST>::S;
-    const uint32_t lds = capped_residency<L, kTRT, ST, kOffsetStorage<ST>>() ? capped_lds(s.threads()) : 0;
+    constexpr bool kFast = kOffsetStorage<ST>;
+    const uint32_t lds =
+        capped_residency<L, kTRT, ST, kFast>() ? capped_lds(s.threads(), capped_waves<L, kTRT, ST, kFast>()) : 0;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const S*>(cur), static_cast<S*>(nxt),
                            s.cells, static_cast<size_t>(plane));
@@ -266,19 +286,19 @@ hipError_t stream_storage(int storage, const void* cur, void* nxt, const LaunchS
     }
 }
 
-hipError_t plane_pitch(const LaunchShape& s, uint64_t requested, uint64_t& plane);
+hipError_t plane_layout(int lattice, LaunchShape& s, uint64_t plane_stride, uint64_t slab_stride, uint64_t& plane);
 
 template <bool SHIFT>
 hipError_t stream_entry(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
-                        uint64_t nz, uint64_t pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z,
-                        void* stream) {
+                        uint64_t nz, uint64_t pitch, uint64_t slab_stride, uint32_t block_x, uint32_t block_y,
+                        uint32_t block_z, void* stream) {
     if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
     hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
     if (e != hipSuccess) return e;
     if (current == next) return hipErrorInvalidValue;
     uint64_t plane = 0;
-    e = plane_pitch(s, pitch, plane);
+    e = plane_layout(lattice, s, pitch, slab_stride, plane);
     if (e != hipSuccess) return e;
     const hipStream_t st = as_stream(stream);
     switch (lattice) {
@@ -289,19 +309,91 @@ hipError_t stream_entry(int lattice, int storage, const void* current, void* nex
     }
 }
 
-// Plane pitch of a call in elements: 0 selects the contiguous layout (pitch =
-// cells); otherwise at least the cell count, and at most 2^51 (27 planes of
-// 8-byte values stay below 2^59 bytes).
-hipError_t plane_pitch(const LaunchShape& s, uint64_t requested, uint64_t& plane) {
+// Plane layout of a call, in elements.  Population i of cell (x, y, z) lies
+// at i * plane_stride + z * slab_stride + y * nx + x.
+//  - slab_stride 0 or nx ny: the reference's planes, [Q][Z][Y][X];
+//    plane_stride 0 (the cell count) or at least the cell count (pitched).
+//  - a larger slab_stride: the planes stored slab by slab, [Z][Q][Y][X] when
+//    dense (plane_stride 0, i.e. nx ny, and slab_stride Q nx ny);
+//    plane_stride at least nx ny and slab_stride at least Q plane_stride, so
+//    no two (population, slab) pairs share an element.
+// A lane's offset within its plane, (nz - 1) slab_stride + nx ny - 1, must stay
+// below 2^32 elements; plane_stride at most 2^51 (27 planes of 8-byte values
+// stay below 2^59 bytes).
+hipError_t plane_layout(int lattice, LaunchShape& s, uint64_t plane_stride, uint64_t slab_stride, uint64_t& plane) {
     const uint64_t cells = s.cells.count;
-    plane = requested == 0 ? cells : requested;
-    if (plane < cells || plane > (1ull << 51)) return hipErrorInvalidValue;
+    const uint64_t slab = static_cast<uint64_t>(s.cells.nx) * s.cells.ny;
+    const uint64_t q = lattice == kD2Q9 ? 9 : lattice == kD3Q19 ? 19 : lattice == kD3Q27 ? 27 : 0;
+    if (q == 0) return hipErrorInvalidValue;
+    if (slab_stride == 0 || slab_stride == slab) {
+        plane = plane_stride == 0 ? cells : plane_stride;
+        if (plane < cells || plane > (1ull << 51)) return hipErrorInvalidValue;
+        s.cells.zgap = 0;
+        return hipSuccess;
+    }
+    plane = plane_stride == 0 ? slab : plane_stride;
+    if (plane < slab || plane > (1ull << 51) || slab_stride / q < plane) return hipErrorInvalidValue;
+    if (slab_stride - slab >= (1ull << 32)) return hipErrorInvalidValue;
+    if (s.cells.nz > 1 && slab_stride > (0xFFFFFFFFull - slab) / (s.cells.nz - 1)) return hipErrorInvalidValue;
+    s.cells.zgap = static_cast<uint32_t>(slab_stride - slab);
     return hipSuccess;
 }
 
+// Copies every population of every cell from one plane layout to another
+// (e.g. the reference's [Q][Z][Y][X] planes to slab-by-slab [Z][Q][Y][X] and
+// back): one work-item per cell, x fastest, so both sides are read and written
+// in runs of consecutive elements.  64-bit offsets: a layout conversion, not a
+// step.
+template <class T>
+__global__ __launch_bounds__(256) void relayout_kernel(const T* __restrict__ src, T* __restrict__ dst, Cells c,
+                                                       int q, uint64_t src_plane, uint64_t src_slab,
+                                                       uint64_t dst_plane, uint64_t dst_slab) {
+    const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= c.count) return;
+    const CellCoord p = cell_coord(cell, c);
+    const uint64_t in_slab = static_cast<uint64_t>(p.y) * c.nx + static_cast<uint64_t>(p.x);
+    const uint64_t from = static_cast<uint64_t>(p.z) * src_slab + in_slab;
+    const uint64_t to = static_cast<uint64_t>(p.z) * dst_slab + in_slab;
+    for (int i = 0; i < q; ++i)
+        dst[static_cast<uint64_t>(i) * dst_plane + to] = src[static_cast<uint64_t>(i) * src_plane + from];
+}
+
+hipError_t relayout_entry(int lattice, int storage, const void* src, uint64_t src_plane, uint64_t src_slab,
+                          void* dst, uint64_t dst_plane, uint64_t dst_slab, uint64_t nx, uint64_t ny, uint64_t nz,
+                          void* stream) {
+    if (src == nullptr || dst == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 256, 1, 1, s);
+    if (e != hipSuccess) return e;
+    if (src == dst) return hipErrorInvalidValue;
+    uint64_t sp = 0, dp = 0;
+    e = plane_layout(lattice, s, src_plane, src_slab, sp);
+    if (e != hipSuccess) return e;
+    const uint64_t slab = static_cast<uint64_t>(s.cells.nx) * s.cells.ny;
+    const uint64_t ss = slab + s.cells.zgap;
+    e = plane_layout(lattice, s, dst_plane, dst_slab, dp);
+    if (e != hipSuccess) return e;
+    const uint64_t ds = slab + s.cells.zgap;
+    const int q = lattice == kD2Q9 ? 9 : lattice == kD3Q19 ? 19 : 27;
+    const hipStream_t st = as_stream(stream);
+    auto go = [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(relayout_kernel<T>, s.grid, s.block, 0, st, static_cast<const T*>(src),
+                           static_cast<T*>(dst), s.cells, q, sp, ss, dp, ds);
+        return hipGetLastError();
+    };
+    switch (storage) {
+        case kF32: return go(uint32_t{});
+        case kF64: return go(uint64_t{});
+        case kF16:
+        case kBF16: return go(uint16_t{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
-                       uint64_t nx, uint64_t ny, uint64_t nz, uint64_t pitch, double omega, uint32_t block_x,
-                       uint32_t block_y, uint32_t block_z, void* stream) {
+                       uint64_t nx, uint64_t ny, uint64_t nz, uint64_t pitch, uint64_t slab_stride, double omega,
+                       uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
     if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
     hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
@@ -309,7 +401,7 @@ hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, 
     if (current == next) return hipErrorInvalidValue;
     if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST | LBC_FLAG_MFMA)) return hipErrorInvalidValue;
     uint64_t plane = 0;
-    e = plane_pitch(s, pitch, plane);
+    e = plane_layout(lattice, s, pitch, slab_stride, plane);
     if (e != hipSuccess) return e;
     const bool fast = (flags & LBC_FLAG_FAST) != 0;
     const hipStream_t st = as_stream(stream);
@@ -327,7 +419,7 @@ hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, 
 extern "C" int lbc_fluid_step_ex(int lattice, int collision, int storage, uint32_t flags, const void* current,
                                  void* next, uint64_t nx, uint64_t ny, uint64_t nz, double omega, uint32_t block_x,
                                  uint32_t block_y, uint32_t block_z, void* stream) {
-    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, 0, omega, block_x,
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, 0, 0, omega, block_x,
                             block_y, block_z, stream);
 }
 
@@ -335,21 +427,39 @@ extern "C" int lbc_fluid_step_pitched(int lattice, int collision, int storage, u
                                       void* next, uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_pitch,
                                       double omega, uint32_t block_x, uint32_t block_y, uint32_t block_z,
                                       void* stream) {
-    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, plane_pitch, omega,
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, plane_pitch, 0, omega,
                             block_x, block_y, block_z, stream);
+}
+
+extern "C" int lbc_fluid_step_strided(int lattice, int collision, int storage, uint32_t flags, const void* current,
+                                      void* next, uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_stride,
+                                      uint64_t slab_stride, double omega, uint32_t block_x, uint32_t block_y,
+                                      uint32_t block_z, void* stream) {
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, plane_stride, slab_stride,
+                            omega, block_x, block_y, block_z, stream);
 }
 
 extern "C" int lbc_pull_stream(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
                                uint64_t nz, uint64_t plane_pitch, uint32_t block_x, uint32_t block_y,
                                uint32_t block_z, void* stream) {
-    return lbc::stream_entry<true>(lattice, storage, current, next, nx, ny, nz, plane_pitch, block_x, block_y,
+    return lbc::stream_entry<true>(lattice, storage, current, next, nx, ny, nz, plane_pitch, 0, block_x, block_y,
                                    block_z, stream);
+}
+
+extern "C" int lbc_pull_stream_strided(int lattice, int storage, int shift, const void* current, void* next,
+                                       uint64_t nx, uint64_t ny, uint64_t nz, uint64_t plane_stride,
+                                       uint64_t slab_stride, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                       void* stream) {
+    return shift ? lbc::stream_entry<true>(lattice, storage, current, next, nx, ny, nz, plane_stride, slab_stride,
+                                           block_x, block_y, block_z, stream)
+                 : lbc::stream_entry<false>(lattice, storage, current, next, nx, ny, nz, plane_stride, slab_stride,
+                                            block_x, block_y, block_z, stream);
 }
 
 extern "C" int lbc_plane_copy(int lattice, int storage, const void* current, void* next, uint64_t nx, uint64_t ny,
                               uint64_t nz, uint64_t plane_pitch, uint32_t block_x, uint32_t block_y, uint32_t block_z,
                               void* stream) {
-    return lbc::stream_entry<false>(lattice, storage, current, next, nx, ny, nz, plane_pitch, block_x, block_y,
+    return lbc::stream_entry<false>(lattice, storage, current, next, nx, ny, nz, plane_pitch, 0, block_x, block_y,
                                     block_z, stream);
 }
 
@@ -358,4 +468,11 @@ extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const voi
                               uint32_t block_z, void* stream) {
     return lbc_fluid_step_ex(lattice, collision, storage, 0u, current, next, nx, ny, nz, omega, block_x, block_y,
                              block_z, stream);
+}
+
+extern "C" int lbc_relayout(int lattice, int storage, const void* src, uint64_t src_plane_stride,
+                            uint64_t src_slab_stride, void* dst, uint64_t dst_plane_stride, uint64_t dst_slab_stride,
+                            uint64_t nx, uint64_t ny, uint64_t nz, void* stream) {
+    return lbc::relayout_entry(lattice, storage, src, src_plane_stride, src_slab_stride, dst, dst_plane_stride,
+                               dst_slab_stride, nx, ny, nz, stream);
 }

@@ -40,9 +40,14 @@ __host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
 // kernels map work-item p of the launch to cell p, so every wave covers 64
 // consecutive cells whatever nx and the block shape are (no idle lanes at row
 // ends beyond the last wave of the domain).
+// zgap: elements between the end of one xy-slab of a population plane and the
+// start of the next (0 in the reference's [Q][Z][Y][X] planes; (Q - 1) nx ny
+// when the planes are stored slab by slab, [Z][Q][Y][X]).  Cell (x, y, z)
+// lies at element z (nx ny + zgap) + y nx + x of its population plane.
 struct Cells {
     uint32_t nx, ny, nz, count;
     FastDiv by_nx, by_ny;
+    uint32_t zgap;
 };
 
 struct CellCoord {
@@ -76,10 +81,16 @@ constexpr uint64_t kMaxExtent = (1ull << 31) - 1;
 // HIP's limit on blocks per launch along x.
 constexpr uint64_t kMaxBlocks = (1ull << 31) - 1;
 
+// Elements a lane's offset spans within a population plane: the cell count,
+// or more when the slabs are spaced apart (zgap).
+inline uint64_t lane_extent(const Cells& c) {
+    return static_cast<uint64_t>(c.count) + static_cast<uint64_t>(c.nz - 1) * c.zgap;
+}
+
 // Population planes of more than 4 GiB take 64-bit per-lane addresses (the
 // kernels' WIDE instantiation); smaller planes keep the 32-bit byte offsets.
 template <class S> bool wide_planes(const LaunchShape& s) {
-    return static_cast<uint64_t>(s.cells.count) * sizeof(S) > (1ull << 32);
+    return lane_extent(s.cells) * sizeof(S) > (1ull << 32);
 }
 
 // Validation order of src/lattice_cuda.cu:488-503: pointers, block, 2D
@@ -108,7 +119,7 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
     s.nz = static_cast<int>(nz);
     s.cells = Cells{static_cast<uint32_t>(nx), static_cast<uint32_t>(ny), static_cast<uint32_t>(nz),
                     static_cast<uint32_t>(count), make_fastdiv(static_cast<uint32_t>(nx)),
-                    make_fastdiv(static_cast<uint32_t>(ny))};
+                    make_fastdiv(static_cast<uint32_t>(ny)), 0u};
     return hipSuccess;
 }
 
@@ -147,10 +158,10 @@ inline LdsLimits lds_limits() {
 }
 
 // Dynamic LDS bytes per block that hold a launch of `threads`-wide blocks to
-// kCappedWavesPerCu resident waves per CU (0: no cap, e.g. blocks of more
-// than 8 waves, or a device whose limits could not be read).
-inline uint32_t capped_lds(uint32_t threads) {
-    const uint32_t blocks = kCappedWavesPerCu / ((threads + 63) / 64);
+// `waves` resident waves per CU (0: no cap, e.g. blocks of more than half
+// that many waves, or a device whose limits could not be read).
+inline uint32_t capped_lds(uint32_t threads, uint32_t waves = kCappedWavesPerCu) {
+    const uint32_t blocks = waves / ((threads + 63) / 64);
     if (blocks < 2) return 0;
     const LdsLimits l = lds_limits();
     const uint32_t lds = l.per_cu / blocks;

@@ -140,20 +140,25 @@ template <int ST> constexpr bool kOffsetStorage = (ST == kF16 || ST == kBF16);
 // x + 1 (offset +1, or 1 - nx at the last column); y and z likewise in units
 // of a row and of an xy-plane.  Six selects per cell, then each population's
 // source is the destination plus up to three of them (no multiplies).  The
-// sums are taken modulo 2^32, where every source lands back in [0, cells)
-// (cells < 2^32).
+// sums are taken modulo 2^32, where every source lands back in [0, extent)
+// (the plane's lane extent, below 2^32).
 struct Pull {
     uint32_t dst;
     uint32_t xm, xp, ym, yp, zm, zp;
-    __device__ __forceinline__ Pull(uint32_t cell, int x, int y, int z, int nx, int ny, int nz) : dst(cell) {
+    // `cell`: the destination's element offset in its plane; `zgap`: elements
+    // between consecutive xy-slabs of a plane beyond the slab itself (0 for
+    // contiguous planes), so a z step is nx ny + zgap elements.
+    __device__ __forceinline__ Pull(uint32_t cell, int x, int y, int z, int nx, int ny, int nz, uint32_t zgap = 0u)
+        : dst(cell) {
         const uint32_t row = static_cast<uint32_t>(nx);
         const uint32_t slab = row * static_cast<uint32_t>(ny);
+        const uint32_t zstep = slab + zgap;
         xm = x == 0 ? row - 1u : ~0u;
         xp = x + 1 == nx ? 1u - row : 1u;
         ym = y == 0 ? slab - row : 0u - row;
         yp = y + 1 == ny ? row - slab : row;
-        zm = z == 0 ? slab * static_cast<uint32_t>(nz - 1) : 0u - slab;
-        zp = z + 1 == nz ? 0u - slab * static_cast<uint32_t>(nz - 1) : slab;
+        zm = z == 0 ? zstep * static_cast<uint32_t>(nz - 1) : 0u - zstep;
+        zp = z + 1 == nz ? 0u - zstep * static_cast<uint32_t>(nz - 1) : zstep;
     }
     // source cell of a population with velocity (cx, cy, cz)
     __device__ __forceinline__ uint32_t src(int cx, int cy, int cz) const {

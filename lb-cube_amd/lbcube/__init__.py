@@ -144,6 +144,109 @@ def fluid_step_pitched(lattice, collision, cur, nxt, extents, plane_pitch, omega
           "lbc_fluid_step_pitched")
 
 
+def slab_strides(lattice, extents):
+    """(plane_stride, slab_stride) of the dense [Z][Q][Y][X] order: the Q
+    population planes of each xy-slab side by side."""
+    nx, ny = int(extents[0]), int(extents[1])
+    return nx * ny, Q[lattice] * nx * ny
+
+
+def _check_strided(lattice, t, name, extents, plane_stride, slab_stride):
+    if not (t.is_cuda and t.is_contiguous()) or t.dtype not in STORAGE:
+        raise ValueError(f"{name} must be a contiguous CUDA tensor of a storage dtype")
+    nx, ny, nz = extents
+    slab = nx * ny
+    if slab_stride in (0, slab):
+        plane, step = (plane_stride or slab * nz), slab
+    else:
+        plane, step = (plane_stride or slab), slab_stride
+    need = (Q[lattice] - 1) * plane + (nz - 1) * step + slab
+    if t.numel() < need:
+        raise ValueError(f"{name} holds {t.numel()} elements; the layout needs {need}")
+
+
+def _strided_args(lattice, cur, nxt, extents, plane_stride, slab_stride):
+    nx, ny, nz = (int(v) for v in extents)
+    if DIM[lattice] == 2:
+        nz = 1
+    for t, n in ((cur, "cur"), (nxt, "nxt")):
+        _check_strided(lattice, t, n, (nx, ny, nz), int(plane_stride), int(slab_stride))
+    if cur.dtype != nxt.dtype or cur.device != nxt.device:
+        raise ValueError("cur and nxt must share dtype and device")
+    return nx, ny, nz
+
+
+def fluid_step_strided(lattice, collision, cur, nxt, extents, plane_stride, slab_stride, omega, block=None,
+                       stream=None, fast=False, mfma=False):
+    """lbc_fluid_step_strided on flat buffers: population i of cell (x, y, z)
+    at element i * plane_stride + z * slab_stride + y * nx + x (slab_strides()
+    gives the dense [Z][Q][Y][X] order); extents = (nx, ny, nz)."""
+    nx, ny, nz = _strided_args(lattice, cur, nxt, extents, plane_stride, slab_stride)
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_fluid_step_strided(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
+                                       _step_flags(fast, mfma), cur.data_ptr(), nxt.data_ptr(), nx, ny, nz,
+                                       int(plane_stride), int(slab_stride), float(omega), bx, by, bz,
+                                       _stream(stream)), "lbc_fluid_step_strided")
+
+
+def fluid_step_slabs(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False, mfma=False):
+    """fluid_step on populations stored slab by slab: tensors of shape
+    (nz, Q, ny, nx) (3D lattices)."""
+    if DIM[lattice] != 3:
+        raise ValueError("the slab order applies to 3D lattices")
+    for t, n in ((cur, "cur"), (nxt, "nxt")):
+        if t.dim() != 4 or t.shape[1] != Q[lattice]:
+            raise ValueError(f"{n} must have shape (nz, {Q[lattice]}, ny, nx)")
+    if cur.shape != nxt.shape:
+        raise ValueError("cur and nxt must share a shape")
+    ext = (cur.shape[3], cur.shape[2], cur.shape[0])
+    fluid_step_strided(lattice, collision, cur, nxt, ext, *slab_strides(lattice, ext), omega, block=block,
+                       stream=stream, fast=fast, mfma=mfma)
+
+
+def relayout(lattice, src, dst, extents, src_strides=(0, 0), dst_strides=(0, 0), stream=None):
+    """lbc_relayout: copy a population set between two strided layouts
+    ((plane_stride, slab_stride) pairs; (0, 0) is the reference's planes)."""
+    nx, ny, nz = (int(v) for v in extents)
+    if DIM[lattice] == 2:
+        nz = 1
+    _check_strided(lattice, src, "src", (nx, ny, nz), *(int(v) for v in src_strides))
+    _check_strided(lattice, dst, "dst", (nx, ny, nz), *(int(v) for v in dst_strides))
+    if src.dtype != dst.dtype or src.device != dst.device:
+        raise ValueError("src and dst must share dtype and device")
+    check(lib().lbc_relayout(LATTICES[lattice], STORAGE[src.dtype], src.data_ptr(), int(src_strides[0]),
+                             int(src_strides[1]), dst.data_ptr(), int(dst_strides[0]), int(dst_strides[1]), nx, ny,
+                             nz, _stream(stream)), "lbc_relayout")
+
+
+def to_slabs(lattice, f, stream=None):
+    """Populations of shape (Q, nz, ny, nx) -> a new tensor of shape
+    (nz, Q, ny, nx) in the slab order (lbc_relayout)."""
+    _check_pop(lattice, f, "f")
+    nx, ny, nz = _extents(lattice, f)
+    out = torch.empty((nz, Q[lattice], ny, nx), dtype=f.dtype, device=f.device)
+    relayout(lattice, f, out, (nx, ny, nz), (0, 0), slab_strides(lattice, (nx, ny)), stream=stream)
+    return out
+
+
+def from_slabs(lattice, h, stream=None):
+    """The inverse of to_slabs: (nz, Q, ny, nx) -> (Q, nz, ny, nx)."""
+    nz, q, ny, nx = h.shape
+    out = torch.empty((q, nz, ny, nx), dtype=h.dtype, device=h.device)
+    relayout(lattice, h, out, (nx, ny, nz), slab_strides(lattice, (nx, ny)), (0, 0), stream=stream)
+    return out
+
+
+def pull_stream_strided(lattice, cur, nxt, extents, plane_stride, slab_stride, block=None, stream=None, shift=True):
+    """lbc_pull_stream_strided: the pull-streaming step (shift=True) or the
+    unshifted plane copy on a strided layout."""
+    nx, ny, nz = _strided_args(lattice, cur, nxt, extents, plane_stride, slab_stride)
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_pull_stream_strided(LATTICES[lattice], STORAGE[cur.dtype], int(bool(shift)), cur.data_ptr(),
+                                        nxt.data_ptr(), nx, ny, nz, int(plane_stride), int(slab_stride), bx, by, bz,
+                                        _stream(stream)), "lbc_pull_stream_strided")
+
+
 def pull_stream(lattice, cur, nxt, extents=None, plane_pitch=0, block=None, stream=None, shift=True):
     """The pull-streaming step alone (lbc_pull_stream), or with shift=False
     the same plane streams copied in place (lbc_plane_copy).  Population

@@ -28,6 +28,11 @@ SIGNATURES = {
                                 c_u32, c_u32, c_vp], c_int),
     "lbc_pull_stream": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_plane_copy": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_fluid_step_strided": ([c_int, c_int, c_int, c_u32, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_u64, c_double,
+                                c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_pull_stream_strided": ([c_int, c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_u64, c_u32, c_u32,
+                                 c_u32, c_vp], c_int),
+    "lbc_relayout": ([c_int, c_int, c_vp, c_u64, c_u64, c_vp, c_u64, c_u64, c_u64, c_u64, c_u64, c_vp], c_int),
     "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double,
                          c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reaction_ab_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double,

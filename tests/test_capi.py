@@ -197,3 +197,35 @@ def test_validation_pitched_and_pull_stream():
     assert L.lbc_pull_stream(4, 0, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE  # scalar lattice
     assert L.lbc_plane_copy(4, 0, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE
     assert L.lbc_plane_copy(1, 7, FAKE_A, FAKE_B, 8, 8, 8, 0, 256, 1, 1, None) == HIP_INVALID_VALUE  # storage
+
+
+def test_validation_strided_layouts():
+    """Strided layouts on an 8^3 D3Q19 domain (64-cell slabs, 512 cells): the
+    reference's planes with a pitch below the cell count, slab-spaced planes
+    closer than a slab, slabs closer than Q planes, per-plane offsets of 2^32
+    elements and more, and a scalar lattice are rejected before any launch;
+    so are aliasing buffers and unknown storage in lbc_relayout."""
+    L = lbcube.lib()
+    step = lambda a, b, plane, slab, lattice=1, nz=8: L.lbc_fluid_step_strided(  # noqa: E731
+        lattice, 1, 0, 0, a, b, 8, 8, nz, plane, slab, 1.0, 256, 1, 1, None)
+    pull = lambda a, b, plane, slab, lattice=1, nz=8: L.lbc_pull_stream_strided(  # noqa: E731
+        lattice, 0, 1, a, b, 8, 8, nz, plane, slab, 256, 1, 1, None)
+    relayout_src = lambda a, b, plane, slab, lattice=1, nz=8: L.lbc_relayout(  # noqa: E731
+        lattice, 0, a, plane, slab, b, 0, 0, 8, 8, nz, None)
+    relayout_dst = lambda a, b, plane, slab, lattice=1, nz=8: L.lbc_relayout(  # noqa: E731
+        lattice, 0, a, 0, 0, b, plane, slab, 8, 8, nz, None)
+    for fn in (step, pull, relayout_src, relayout_dst):
+        assert fn(None, FAKE_B, 0, 0) == HIP_INVALID_DEVICE_POINTER
+        assert fn(FAKE_A, FAKE_A, 0, 0) == HIP_INVALID_VALUE
+        assert fn(FAKE_A, FAKE_B, 511, 64) == HIP_INVALID_VALUE  # reference planes, pitch below 512 cells
+        assert fn(FAKE_A, FAKE_B, 63, 19 * 64) == HIP_INVALID_VALUE  # planes closer than a slab
+        assert fn(FAKE_A, FAKE_B, 0, 19 * 64 - 1) == HIP_INVALID_VALUE  # slabs closer than 19 planes
+        assert fn(FAKE_A, FAKE_B, 100, 19 * 100 - 1) == HIP_INVALID_VALUE
+        assert fn(FAKE_A, FAKE_B, (1 << 51) + 1, 1 << 60) == HIP_INVALID_VALUE
+        # (nz - 1) * slab_stride + 64 must stay below 2^32 elements
+        assert fn(FAKE_A, FAKE_B, 0, (0xFFFFFFFF - 64) // 7 + 1) == HIP_INVALID_VALUE
+        assert fn(FAKE_A, FAKE_B, 0, 0, lattice=4) == HIP_INVALID_VALUE  # scalar lattice
+        assert fn(FAKE_A, FAKE_B, 0, 1 << 33, nz=1) == HIP_INVALID_VALUE  # slab gap of 2^32 or more
+    assert L.lbc_relayout(1, 7, FAKE_A, 0, 0, FAKE_B, 0, 19 * 64, 8, 8, 8, None) == HIP_INVALID_VALUE  # storage
+    assert L.lbc_fluid_step_strided(1, 1, 0, 8, FAKE_A, FAKE_B, 8, 8, 8, 0, 19 * 64, 1.0, 256, 1, 1, None) == \
+        HIP_INVALID_VALUE  # unknown flag
