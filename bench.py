@@ -480,15 +480,30 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
         lbcube.reactive_fused_step("D3Q19", "TRT", "D3Q7", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
         state[:] = [ft, f, at, a, bt, b]
 
+    # the fused step with the three population sets in the slab order
+    # (lbc_reactive_fused_step_tiled with tiles of ny rows: each xy-slab holds
+    # its Q planes back to back), converted once before the timing
+    slabs = [lbcube.to_tiles(lat, x, n) for lat, x in (("D3Q19", f), ("D3Q7", a), ("D3Q7", b))]
+    sstate = [slabs[0], torch.empty_like(slabs[0]), slabs[1], torch.empty_like(slabs[1]), slabs[2],
+              torch.empty_like(slabs[2])]
+    del slabs
+
+    def fused_slabs(fast):
+        f, ft, a, at, b, bt = sstate
+        lbcube.reactive_fused_step_tiled("D3Q19", "TRT", "D3Q7", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
+        sstate[:] = [ft, f, at, a, bt, b]
+
     out = {"workload": f"D3Q19 TGV {n}^3 (u0 = 0.05) + two D3Q7 species (C = 0.5 +- 0.2 sin), FP32"}
     for name, fn, bpc in (("scalar_step", scalar, 132), ("reaction_step", reaction, 188),
-                          ("fused_trt", lambda: fused(False), 264), ("fused_trt_fast", lambda: fused(True), 264)):
+                          ("fused_trt", lambda: fused(False), 264), ("fused_trt_fast", lambda: fused(True), 264),
+                          ("fused_trt_slabs", lambda: fused_slabs(False), 264),
+                          ("fused_trt_fast_slabs", lambda: fused_slabs(True), 264)):
         ms = _graph_ms(torch, fn, steps, chunk=10)
         out[name] = {"mlups": cells / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * cells / (ms * 1e-3) / 1e9,
                      "bytes_per_cell": bpc, "ms_per_step": ms}
         print(f"# adr 3d {name}: {out[name]['mlups']:.0f} MLUPS, {out[name]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
-    del state, f, ft, a, at, b, bt
+    del state, f, ft, a, at, b, bt, sstate
     torch.cuda.empty_cache()
     return out
 
@@ -586,6 +601,60 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
             print(f"# reactive 2D {key}: {gbs:.0f} GB/s", file=sys.stderr, flush=True)
     out["d2q9_plane_copy_by_row_length"] = plane_copy_by_row_length(lbcube, torch)
     out.update(reactive_2d_large(lbcube, torch))
+    out["tiles"] = reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps)
+    return out
+
+
+def reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps, row_set=(2, 4, 8, 32)):
+    """The same fused 2048^2 step with the three population sets stored in
+    tiles of R rows (lbc_reactive_fused_step_tiled: each tile holds the Q
+    planes of its rows back to back), FP32 and FP16 in both arithmetic forms,
+    graph-replayed like the lines above; and the FP32 loop with the reactive
+    statistics every 50 steps in tiles of 4 rows (lbc_reactive_stats_tiled).
+    The conversion from the reference's planes (lbc_relayout_tiled) happens
+    once, before the timing."""
+    from lbcube import init as lbinit
+
+    out = {}
+    for storage, fast, ref in (("f32", False, 0.0), ("f32", True, 0.0), ("f16", False, 0.5), ("f16", True, 0.5)):
+        planes = [lbinit.equilibrium("D2Q9", None, u, storage), lbinit.scalar_equilibrium("D2Q5", ca, u, storage, ref),
+                  lbinit.scalar_equilibrium("D2Q5", cb, u, storage, ref)]
+        es = planes[0].element_size()
+        for rows in row_set:
+            f, a, b = (lbcube.to_tiles(lat, x, rows) for lat, x in zip(("D2Q9", "D2Q5", "D2Q5"), planes))
+            state = [f, torch.empty_like(f), a, torch.empty_like(a), b, torch.empty_like(b)]
+
+            def fused(state=state, fast=fast, ref=ref):
+                f, ft, a, at, b, bt = state
+                lbcube.reactive_fused_step_tiled("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k,
+                                                 fast=fast, ref_a=ref, ref_b=ref)
+                state[:] = [ft, f, at, a, bt, b]
+
+            for _ in range(4):
+                fused()
+            ms = _graph_ms(torch, fused, steps)
+            key = f"{storage}{'_fast' if fast else ''}_rows{rows}"
+            out[key] = {"mlups": n * n / (ms * 1e-3) / 1e6, "achieved_gbs": 38 * es * n * n / (ms * 1e-3) / 1e9,
+                        "bytes_per_cell": 38 * es}
+            if storage == "f32" and not fast and rows == 4:
+                def loop(state=state):
+                    for s in range(1, steps + 1):
+                        fused(state)
+                        if s % 50 == 0:
+                            lbcube.reactive_stats_tiled("D2Q5", state[2], state[4], k)
+
+                ms_loop = _time(torch, loop) / steps
+                out[key]["loop_with_stats_every_50_mlups"] = n * n / (ms_loop * 1e-3) / 1e6
+            del f, a, b, state
+        del planes
+        torch.cuda.empty_cache()
+    best = {}
+    for key, v in out.items():
+        form = key.rsplit("_rows", 1)[0]
+        if form not in best or v["mlups"] > out[best[form]]["mlups"]:
+            best[form] = key
+    print("# reactive 2D in tiles: " + ", ".join(f"{k} {out[v]['mlups']:.0f} MLUPS ({v})" for k, v in best.items()),
+          file=sys.stderr, flush=True)
     return out
 
 

@@ -168,6 +168,48 @@ int lbc_relayout(int lattice, int storage, const void* src, uint64_t src_plane_s
                  void* dst, uint64_t dst_plane_stride, uint64_t dst_slab_stride, uint64_t nx, uint64_t ny,
                  uint64_t nz, void* stream);
 
+/* Tiles.  A population buffer in tiles of tile_rows rows holds, tile after
+ * tile, the Q planes of the tile's cells back to back: population i of cell
+ * (x, y, z) lies at
+ *   t * Q * T + i * T + ((z * ny + y) * nx + x - t * T)
+ * with T = tile_rows * nx the cells of a tile and t the cell's tile:
+ *  - 3D: tile_rows = ny, t = z: the slab order [Z][Q][Y][X]
+ *    (lbc_fluid_step_strided with plane_stride nx * ny and slab_stride
+ *    Q * nx * ny);
+ *  - 2D: tile_rows divides ny, t = y / tile_rows: [ny / tile_rows][Q][tile_rows][X].
+ * tile_rows = 0 selects the reference's planes.  Other values, or buffers of
+ * 2^32 elements or more (Q times the cells), -> hipErrorInvalidValue.  The
+ * kernels give every cell the result of the reference's planes bit for bit;
+ * tiles keep the planes a step reads within a few tiles of each other, which
+ * the HBM serves faster than planes a whole domain apart.  No counterpart in
+ * the reference. */
+
+/* lbc_relayout between the reference's planes (rows 0) and tiles, or between
+ * two tilings, of any lattice's populations (fluid or species). */
+int lbc_relayout_tiled(int lattice, int storage, const void* src, uint64_t src_tile_rows, void* dst,
+                       uint64_t dst_tile_rows, uint64_t nx, uint64_t ny, uint64_t nz, void* stream);
+
+/* lbc_fluid_step_ex on populations in tiles (fluid lattices). */
+int lbc_fluid_step_tiled(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
+                         uint64_t nx, uint64_t ny, uint64_t nz, uint64_t tile_rows, double omega, uint32_t block_x,
+                         uint32_t block_y, uint32_t block_z, void* stream);
+
+/* lbc_reactive_fused_step_ex with the fluid and both species in tiles of the
+ * same rows (each buffer with its own lattice's Q per tile). */
+int lbc_reactive_fused_step_tiled(int fluid_lattice, int collision, int scalar_lattice, int storage, uint32_t flags,
+                                  const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
+                                  const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                                  uint64_t tile_rows, double omega, double omega_c, double k_react,
+                                  double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
+                                  uint32_t block_z, void* stream);
+
+/* lbc_reactive_stats of species in tiles: the cells are summed in the same
+ * order, so the results equal lbc_reactive_stats' on the reference's planes
+ * bit for bit. */
+int lbc_reactive_stats_tiled(int scalar_lattice, int storage, const void* species_a, const void* species_b,
+                             uint64_t nx, uint64_t ny, uint64_t nz, uint64_t tile_rows, double k_react,
+                             double reference_a, double reference_b, double* out8, void* stream);
+
 /* Reference concentrations of scalar fields.  With LBC_F16 / LBC_BF16
  * storage a scalar field's populations are stored as g_i - w_i c_ref, with
  * c_ref the field's reference concentration (w_i alone is the fluid's rest

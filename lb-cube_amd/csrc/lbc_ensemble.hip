@@ -72,11 +72,11 @@ __global__ __launch_bounds__(THREADS) void ensemble_kernel(const typename Storag
     const Pull pull(p, x, static_cast<int>(y), 0, static_cast<int>(in.nx), static_cast<int>(in.ny), 1);
     if constexpr (FAST)
         reactive_cell_fast<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off,
-                                               cb + in.s_off, nb + in.s_off, pull, in.plane, omega, wm, wc, k, ra,
-                                               rb);
+                                               cb + in.s_off, nb + in.s_off, pull, pull, in.plane, omega, wm, wc, k,
+                                               ra, rb);
     else
         reactive_cell<D2Q9, CT, D2Q5, ST>(cf + in.f_off, nf + in.f_off, ca + in.s_off, na + in.s_off, cb + in.s_off,
-                                          nb + in.s_off, pull, in.plane, omega, wm, wc, k, ra, rb);
+                                          nb + in.s_off, pull, pull, in.plane, omega, wm, wc, k, ra, rb);
 }
 
 }  // namespace lbc

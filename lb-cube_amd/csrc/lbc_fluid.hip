@@ -33,13 +33,14 @@ __global__ __launch_bounds__(MAXT) void fluid_step_kernel(const typename Storage
                                                           typename Storage<ST>::R omega_minus) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t block = renumber_blocks<L::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
+    // tiled 2D grids keep the dispatch order (as the fused reactive kernel)
+    const uint32_t block =
+        (renumber_blocks<L::D, typename Storage<ST>::S>() && c.tiles == 1) ? xcd_block() : blockIdx.x;
     const uint32_t dst = block * blockDim.x + threadIdx.x;
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
-    const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
-    const uint32_t cell = dst + static_cast<uint32_t>(q.z) * c.zgap;
-    const Pull pull(cell, q.x, q.y, q.z, nx, ny, nz, c.zgap);
+    const Pull pull = locate<L::D>(dst, q, c, c.zgap);
+    const uint32_t cell = pull.dst;
     R f[L::Q];
     if constexpr (FAST) {
 #pragma unroll
@@ -88,9 +89,8 @@ __global__ __launch_bounds__(MAXT) void fluid_step_mfma_kernel(const uint16_t* _
     uint32_t off = 0u;  // the cell's element offset in its planes
     if (live) {
         const CellCoord q = cell_coord(dst, c);
-        off = dst + static_cast<uint32_t>(q.z) * c.zgap;
-        const Pull pull(off, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz),
-                        c.zgap);
+        const Pull pull = locate<L::D>(dst, q, c, c.zgap);
+        off = pull.dst;
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) {
             const uint32_t raw = get(at<WIDE>(plane_ptr(cur, i, plane), pull.src(L::cx[i], L::cy[i], L::cz[i])));
@@ -134,13 +134,13 @@ template <class L, int ST, int MAXT, bool WIDE, bool SHIFT = true>
 __global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storage<ST>::S* __restrict__ cur,
                                                            typename Storage<ST>::S* __restrict__ nxt, Cells c,
                                                            size_t plane) {
-    const uint32_t block = renumber_blocks<L::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
+    const uint32_t block =
+        (renumber_blocks<L::D, typename Storage<ST>::S>() && c.tiles == 1) ? xcd_block() : blockIdx.x;
     const uint32_t dst = block * blockDim.x + threadIdx.x;
     if (dst >= c.count) return;
     const CellCoord q = cell_coord(dst, c);
-    const uint32_t cell = dst + static_cast<uint32_t>(q.z) * c.zgap;
-    const Pull pull(cell, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz),
-                    c.zgap);
+    const Pull pull = locate<L::D>(dst, q, c, c.zgap);
+    const uint32_t cell = pull.dst;
     typename Storage<ST>::S v[L::Q];
 #pragma unroll
     for (int i = 0; i < L::Q; ++i)
@@ -339,47 +339,39 @@ hipError_t plane_layout(int lattice, LaunchShape& s, uint64_t plane_stride, uint
     return hipSuccess;
 }
 
-// Copies every population of every cell from one plane layout to another
-// (e.g. the reference's [Q][Z][Y][X] planes to slab-by-slab [Z][Q][Y][X] and
-// back): one work-item per cell, x fastest, so both sides are read and written
-// in runs of consecutive elements.  64-bit offsets: a layout conversion, not a
-// step.
+// One side of a layout conversion: population i of cell (x, y, z) at element
+// i * plane + (z ny + y) nx + x + t * zgap, with t = z in 3D and y / rows in
+// 2D (the tile of Cells).
+struct LayoutSide {
+    uint64_t plane, zgap;
+    FastDiv by_rows;
+};
+
+// Copies every population of every cell from one layout to another (e.g. the
+// reference's [Q][Z][Y][X] planes to the slab order [Z][Q][Y][X], or to 2D
+// tiles of rows, and back): one work-item per cell, x fastest, so both sides
+// are read and written in runs of consecutive elements.  64-bit offsets: a
+// layout conversion, not a step.
 template <class T>
 __global__ __launch_bounds__(256) void relayout_kernel(const T* __restrict__ src, T* __restrict__ dst, Cells c,
-                                                       int q, uint64_t src_plane, uint64_t src_slab,
-                                                       uint64_t dst_plane, uint64_t dst_slab) {
+                                                       int q, LayoutSide from, LayoutSide to) {
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
     const CellCoord p = cell_coord(cell, c);
-    const uint64_t in_slab = static_cast<uint64_t>(p.y) * c.nx + static_cast<uint64_t>(p.x);
-    const uint64_t from = static_cast<uint64_t>(p.z) * src_slab + in_slab;
-    const uint64_t to = static_cast<uint64_t>(p.z) * dst_slab + in_slab;
+    const bool two_d = c.nz == 1;
+    const uint64_t ts = two_d ? fast_div(static_cast<uint32_t>(p.y), from.by_rows) : static_cast<uint64_t>(p.z);
+    const uint64_t td = two_d ? fast_div(static_cast<uint32_t>(p.y), to.by_rows) : static_cast<uint64_t>(p.z);
+    const uint64_t a = cell + ts * from.zgap, b = cell + td * to.zgap;
     for (int i = 0; i < q; ++i)
-        dst[static_cast<uint64_t>(i) * dst_plane + to] = src[static_cast<uint64_t>(i) * src_plane + from];
+        dst[static_cast<uint64_t>(i) * to.plane + b] = src[static_cast<uint64_t>(i) * from.plane + a];
 }
 
-hipError_t relayout_entry(int lattice, int storage, const void* src, uint64_t src_plane, uint64_t src_slab,
-                          void* dst, uint64_t dst_plane, uint64_t dst_slab, uint64_t nx, uint64_t ny, uint64_t nz,
-                          void* stream) {
-    if (src == nullptr || dst == nullptr) return hipErrorInvalidDevicePointer;
-    LaunchShape s;
-    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 256, 1, 1, s);
-    if (e != hipSuccess) return e;
-    if (src == dst) return hipErrorInvalidValue;
-    uint64_t sp = 0, dp = 0;
-    e = plane_layout(lattice, s, src_plane, src_slab, sp);
-    if (e != hipSuccess) return e;
-    const uint64_t slab = static_cast<uint64_t>(s.cells.nx) * s.cells.ny;
-    const uint64_t ss = slab + s.cells.zgap;
-    e = plane_layout(lattice, s, dst_plane, dst_slab, dp);
-    if (e != hipSuccess) return e;
-    const uint64_t ds = slab + s.cells.zgap;
-    const int q = lattice == kD2Q9 ? 9 : lattice == kD3Q19 ? 19 : 27;
-    const hipStream_t st = as_stream(stream);
+hipError_t launch_relayout(int storage, const void* src, void* dst, const LaunchShape& s, int q, LayoutSide from,
+                           LayoutSide to, hipStream_t st) {
     auto go = [&](auto tag) {
         using T = decltype(tag);
         hipLaunchKernelGGL(relayout_kernel<T>, s.grid, s.block, 0, st, static_cast<const T*>(src),
-                           static_cast<T*>(dst), s.cells, q, sp, ss, dp, ds);
+                           static_cast<T*>(dst), s.cells, q, from, to);
         return hipGetLastError();
     };
     switch (storage) {
@@ -391,9 +383,47 @@ hipError_t relayout_entry(int lattice, int storage, const void* src, uint64_t sr
     }
 }
 
+// A layout side from strides (lbc_relayout) or from rows per tile
+// (lbc_relayout_tiled: rows = 0 is the reference's planes).
+hipError_t strided_side(int lattice, const LaunchShape& shape, uint64_t plane_stride, uint64_t slab_stride,
+                        LayoutSide& side) {
+    LaunchShape s = shape;
+    uint64_t plane = 0;
+    const hipError_t e = plane_layout(lattice, s, plane_stride, slab_stride, plane);
+    side = LayoutSide{plane, s.cells.zgap, s.cells.by_rows};
+    return e;
+}
+
+hipError_t tiled_side(int lattice, const LaunchShape& shape, uint64_t rows, LayoutSide& side) {
+    LaunchShape s = shape;
+    uint64_t plane = 0;
+    const int q = q_of(lattice);
+    if (q == 0) return hipErrorInvalidValue;
+    const hipError_t e = tile_layout(is_2d(lattice), rows, q, s, plane);
+    side = LayoutSide{plane, s.cells.zgap, s.cells.by_rows};
+    return e;
+}
+
+template <class SideFn>
+hipError_t relayout_entry(int lattice, int storage, const void* src, void* dst, uint64_t nx, uint64_t ny,
+                          uint64_t nz, void* stream, SideFn sides) {
+    if (src == nullptr || dst == nullptr) return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, 256, 1, 1, s);
+    if (e != hipSuccess) return e;
+    if (src == dst) return hipErrorInvalidValue;
+    LayoutSide from{}, to{};
+    e = sides(s, from, to);
+    if (e != hipSuccess) return e;
+    return launch_relayout(storage, src, dst, s, q_of(lattice), from, to, as_stream(stream));
+}
+
+// tiled: dense tiles of `tile_rows` rows (tile_layout); otherwise the strided
+// layout (pitch, slab_stride) of plane_layout.
 hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, const void* current, void* next,
                        uint64_t nx, uint64_t ny, uint64_t nz, uint64_t pitch, uint64_t slab_stride, double omega,
-                       uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
+                       uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream, bool tiled = false,
+                       uint64_t tile_rows = 0) {
     if (current == nullptr || next == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
     hipError_t e = make_shape(is_2d(lattice), nx, ny, nz, block_x, block_y, block_z, s);
@@ -401,7 +431,12 @@ hipError_t fluid_entry(int lattice, int collision, int storage, uint32_t flags, 
     if (current == next) return hipErrorInvalidValue;
     if (flags & ~static_cast<uint32_t>(LBC_FLAG_FAST | LBC_FLAG_MFMA)) return hipErrorInvalidValue;
     uint64_t plane = 0;
-    e = plane_layout(lattice, s, pitch, slab_stride, plane);
+    if (tiled) {
+        if (q_of(lattice) == 0 || lattice == kD2Q5 || lattice == kD3Q7) return hipErrorInvalidValue;
+        e = tile_layout(is_2d(lattice), tile_rows, q_of(lattice), s, plane);
+    } else {
+        e = plane_layout(lattice, s, pitch, slab_stride, plane);
+    }
     if (e != hipSuccess) return e;
     const bool fast = (flags & LBC_FLAG_FAST) != 0;
     const hipStream_t st = as_stream(stream);
@@ -473,6 +508,28 @@ extern "C" int lbc_fluid_step(int lattice, int collision, int storage, const voi
 extern "C" int lbc_relayout(int lattice, int storage, const void* src, uint64_t src_plane_stride,
                             uint64_t src_slab_stride, void* dst, uint64_t dst_plane_stride, uint64_t dst_slab_stride,
                             uint64_t nx, uint64_t ny, uint64_t nz, void* stream) {
-    return lbc::relayout_entry(lattice, storage, src, src_plane_stride, src_slab_stride, dst, dst_plane_stride,
-                               dst_slab_stride, nx, ny, nz, stream);
+    return lbc::relayout_entry(lattice, storage, src, dst, nx, ny, nz, stream,
+                               [&](const lbc::LaunchShape& s, lbc::LayoutSide& from, lbc::LayoutSide& to) {
+                                   hipError_t e = lbc::strided_side(lattice, s, src_plane_stride, src_slab_stride, from);
+                                   if (e != hipSuccess) return e;
+                                   return lbc::strided_side(lattice, s, dst_plane_stride, dst_slab_stride, to);
+                               });
+}
+
+extern "C" int lbc_relayout_tiled(int lattice, int storage, const void* src, uint64_t src_tile_rows, void* dst,
+                                  uint64_t dst_tile_rows, uint64_t nx, uint64_t ny, uint64_t nz, void* stream) {
+    return lbc::relayout_entry(lattice, storage, src, dst, nx, ny, nz, stream,
+                               [&](const lbc::LaunchShape& s, lbc::LayoutSide& from, lbc::LayoutSide& to) {
+                                   hipError_t e = lbc::tiled_side(lattice, s, src_tile_rows, from);
+                                   if (e != hipSuccess) return e;
+                                   return lbc::tiled_side(lattice, s, dst_tile_rows, to);
+                               });
+}
+
+extern "C" int lbc_fluid_step_tiled(int lattice, int collision, int storage, uint32_t flags, const void* current,
+                                    void* next, uint64_t nx, uint64_t ny, uint64_t nz, uint64_t tile_rows,
+                                    double omega, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                    void* stream) {
+    return lbc::fluid_entry(lattice, collision, storage, flags, current, next, nx, ny, nz, 0, 0, omega, block_x,
+                            block_y, block_z, stream, true, tile_rows);
 }

@@ -8,8 +8,10 @@
 
 namespace lbc {
 
-// Fluid step + A + B -> C step of one cell: `pull` locates the cell and its
-// periodic sources, `plane` is the element stride between population planes;
+// Fluid step + A + B -> C step of one cell: `pf` locates the cell and its
+// periodic sources in the fluid planes and `ps` in the species planes (the
+// same pull unless the buffers are stored in tiles, whose gaps differ with the
+// lattices' Q), `plane` is the element stride between population planes;
 // WIDE selects 64-bit per-lane addresses (planes of more than 4 GiB).
 // The reaction step of the reference reads the fluid populations that the
 // fluid step has just written
@@ -23,18 +25,18 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
                                               const typename Storage<ST>::S* __restrict__ cur_a,
                                               typename Storage<ST>::S* __restrict__ nxt_a,
                                               const typename Storage<ST>::S* __restrict__ cur_b,
-                                              typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
-                                              size_t plane, typename Storage<ST>::R omega,
+                                              typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pf,
+                                              const Pull& ps, size_t plane, typename Storage<ST>::R omega,
                                               typename Storage<ST>::R omega_minus, typename Storage<ST>::R omega_c,
                                               typename Storage<ST>::R k_react, typename Storage<ST>::R ref_a,
                                               typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t cell = pull.dst;
+    const uint32_t cell = pf.dst, scell = ps.dst;
     R f[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
-        const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
+        const uint32_t src = pf.src(F::cx[i], F::cy[i], F::cz[i]);
         f[i] = St::load(at<WIDE>(plane_ptr(cur_f, i, plane), src), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     }
     collide_cell<F, CT, R>(f, omega, omega_minus, [&](int i, R v) {
@@ -48,7 +50,7 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
+        const uint32_t src = ps.src(S::cx[i], S::cy[i], S::cz[i]);
         ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
         gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
@@ -59,8 +61,8 @@ __device__ __forceinline__ void reactive_cell(const typename Storage<ST>::S* __r
     collide_scalar<S, R>(gb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), scell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), scell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
@@ -75,18 +77,18 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
                                                    const typename Storage<ST>::S* __restrict__ cur_a,
                                                    typename Storage<ST>::S* __restrict__ nxt_a,
                                                    const typename Storage<ST>::S* __restrict__ cur_b,
-                                                   typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pull,
-                                                   size_t plane, typename Storage<ST>::R omega,
+                                                   typename Storage<ST>::S* __restrict__ nxt_b, const Pull& pf,
+                                                   const Pull& ps, size_t plane, typename Storage<ST>::R omega,
                                                    typename Storage<ST>::R omega_minus,
                                                    typename Storage<ST>::R omega_c, typename Storage<ST>::R k_react,
                                                    typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t cell = pull.dst;
+    const uint32_t cell = pf.dst, scell = ps.dst;
     R d[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i) {
-        const uint32_t src = pull.src(F::cx[i], F::cy[i], F::cz[i]);
+        const uint32_t src = pf.src(F::cx[i], F::cy[i], F::cz[i]);
         d[i] = St::load_dev(at<WIDE>(plane_ptr(cur_f, i, plane), src), R(F::w[i]));
     }
     collide_dev<F, CT, R>(d, omega, omega_minus,
@@ -97,7 +99,7 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
+        const uint32_t src = ps.src(S::cx[i], S::cy[i], S::cz[i]);
         ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
         gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
     }
@@ -108,8 +110,8 @@ __device__ __forceinline__ void reactive_cell_fast(const typename Storage<ST>::S
     collide_scalar_fast<S, R>(gb, cb, m.ux, m.uy, m.uz, omega_c, source);
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), scell), ga[i], R(S::w[i]) * ref_a);
+        St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), scell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 

@@ -77,22 +77,28 @@ __global__ __launch_bounds__(kStatThreads) void fluid_diag_kernel(const typename
     block_reduce<3>(v, op, partial + blockIdx.x * kStatSlots);
 }
 
+// The species may be stored in tiles (lbc_reactive_stats_tiled): cell c lies
+// at element c + (c / tile) zgap of its plane, `plane` elements from one plane
+// to the next (the reference's planes: plane = cells, zgap = 0).  The cells
+// are visited in the same order either way, so the sums are the same bit for
+// bit.
 template <class S, int ST>
 __global__ __launch_bounds__(kStatThreads) void reactive_stats_kernel(const typename Storage<ST>::S* __restrict__ a,
                                                                       const typename Storage<ST>::S* __restrict__ b,
-                                                                      uint32_t cells, typename Storage<ST>::R ref_a,
+                                                                      uint32_t cells, size_t plane, uint32_t zgap,
+                                                                      FastDiv by_tile, typename Storage<ST>::R ref_a,
                                                                       typename Storage<ST>::R ref_b, double* partial) {
     using St = Storage<ST>;
     using R = typename St::R;
     double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     constexpr int op[5] = {0, 0, 0, 0, 0};
-    const size_t plane = cells;
     for (uint32_t c = blockIdx.x * kStatThreads + threadIdx.x; c < cells; c += kStatBlocks * kStatThreads) {
+        const size_t e = c + static_cast<size_t>(fast_div(c, by_tile)) * zgap;
         R ga[S::Q], gb[S::Q];
 #pragma unroll
         for (int i = 0; i < S::Q; ++i) {
-            ga[i] = St::load(plane_ptr(a, i, plane) + c, R(S::w[i]) * ref_a);
-            gb[i] = St::load(plane_ptr(b, i, plane) + c, R(S::w[i]) * ref_b);
+            ga[i] = St::load(plane_ptr(a, i, plane) + e, R(S::w[i]) * ref_a);
+            gb[i] = St::load(plane_ptr(b, i, plane) + e, R(S::w[i]) * ref_b);
         }
         const double ca = static_cast<double>(concentration<S, R>(ga));
         const double cb = static_cast<double>(concentration<S, R>(gb));
@@ -328,8 +334,8 @@ hipError_t run_scalar_fields(const void* g, uint32_t cells, double ref, double* 
 // no_alloc: the partial sums go to the static scratch array instead of the
 // workspace allocated on first use (lbc_reactive_stats_noalloc).
 template <class S, int ST>
-hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
-                              double* out, hipStream_t st, bool no_alloc) {
+hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, size_t plane, uint32_t zgap, double k,
+                              double ra, double rb, double* out, hipStream_t st, bool no_alloc) {
     std::lock_guard<std::mutex> lock(stats_mutex);
     hipError_t e;
     double* ws = no_alloc ? scratch_address(e) : workspace(e);
@@ -337,7 +343,8 @@ hipError_t run_reactive_stats(const void* a, const void* b, uint32_t cells, doub
     using R = typename Storage<ST>::R;
     hipLaunchKernelGGL((reactive_stats_kernel<S, ST>), dim3(kStatBlocks), dim3(kStatThreads), 0, st,
                        static_cast<const typename Storage<ST>::S*>(a), static_cast<const typename Storage<ST>::S*>(b),
-                       cells, static_cast<R>(ra), static_cast<R>(rb), ws);
+                       cells, plane, zgap, make_fastdiv(static_cast<uint32_t>(plane < cells ? plane : cells)),
+                       static_cast<R>(ra), static_cast<R>(rb), ws);
     hipLaunchKernelGGL((finish_kernel<5>), dim3(1), dim3(kStatThreads), 0, st, ws, ws + kStatBlocks * kStatSlots,
                        cells, k, static_cast<double>(Eps<R>::v));
     e = hipGetLastError();
@@ -382,35 +389,39 @@ hipError_t scalar_fields_storage(int storage, const void* g, uint32_t cells, dou
 }
 
 template <class S>
-hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, double k, double ra, double rb,
-                         double* out, hipStream_t st, bool no_alloc) {
+hipError_t stats_storage(int storage, const void* a, const void* b, uint32_t cells, size_t plane, uint32_t zgap,
+                         double k, double ra, double rb, double* out, hipStream_t st, bool no_alloc) {
     switch (storage) {
-        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, k, ra, rb, out, st, no_alloc);
-        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, k, ra, rb, out, st, no_alloc);
-        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, k, ra, rb, out, st, no_alloc);
-        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, k, ra, rb, out, st, no_alloc);
+        case kF32: return run_reactive_stats<S, kF32>(a, b, cells, plane, zgap, k, ra, rb, out, st, no_alloc);
+        case kF64: return run_reactive_stats<S, kF64>(a, b, cells, plane, zgap, k, ra, rb, out, st, no_alloc);
+        case kF16: return run_reactive_stats<S, kF16>(a, b, cells, plane, zgap, k, ra, rb, out, st, no_alloc);
+        case kBF16: return run_reactive_stats<S, kBF16>(a, b, cells, plane, zgap, k, ra, rb, out, st, no_alloc);
         default: return hipErrorInvalidValue;
     }
 }
 
+// tile_rows: 0 for the reference's planes, else the species' dense tiles
+// (tile_layout).
 hipError_t reactive_stats_entry(int scalar_lattice, int storage, const void* species_a, const void* species_b,
                                 uint64_t nx, uint64_t ny, uint64_t nz, double k_react, double reference_a,
-                                double reference_b, double* out8, void* stream, bool no_alloc) {
+                                double reference_b, double* out8, void* stream, bool no_alloc,
+                                uint64_t tile_rows = 0) {
     if (species_a == nullptr || species_b == nullptr || out8 == nullptr) return hipErrorInvalidDevicePointer;
     LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
+    hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, kStatThreads, 1, 1, s);
     if (e != hipSuccess) return e;
     const uint32_t cells = static_cast<uint32_t>(s.nx) * s.ny * s.nz;
+    if (scalar_lattice != kD2Q5 && scalar_lattice != kD3Q7) return hipErrorInvalidValue;
+    uint64_t plane = cells;
+    e = tile_layout(is_2d(scalar_lattice), tile_rows, q_of(scalar_lattice), s, plane);
+    if (e != hipSuccess) return e;
+    const uint32_t zgap = s.cells.zgap;
     const hipStream_t st = as_stream(stream);
-    switch (scalar_lattice) {
-        case kD2Q5:
-            return stats_storage<D2Q5>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
-                                       st, no_alloc);
-        case kD3Q7:
-            return stats_storage<D3Q7>(storage, species_a, species_b, cells, k_react, reference_a, reference_b, out8,
-                                       st, no_alloc);
-        default: return hipErrorInvalidValue;
-    }
+    if (scalar_lattice == kD2Q5)
+        return stats_storage<D2Q5>(storage, species_a, species_b, cells, plane, zgap, k_react, reference_a,
+                                   reference_b, out8, st, no_alloc);
+    return stats_storage<D3Q7>(storage, species_a, species_b, cells, plane, zgap, k_react, reference_a, reference_b,
+                               out8, st, no_alloc);
 }
 
 }  // namespace lbc
@@ -437,6 +448,14 @@ extern "C" int lbc_reactive_stats(int scalar_lattice, int storage, const void* s
                                   double reference_b, double* out8, void* stream) {
     return lbc::reactive_stats_entry(scalar_lattice, storage, species_a, species_b, nx, ny, nz, k_react, reference_a,
                                      reference_b, out8, stream, false);
+}
+
+extern "C" int lbc_reactive_stats_tiled(int scalar_lattice, int storage, const void* species_a,
+                                        const void* species_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                                        uint64_t tile_rows, double k_react, double reference_a, double reference_b,
+                                        double* out8, void* stream) {
+    return lbc::reactive_stats_entry(scalar_lattice, storage, species_a, species_b, nx, ny, nz, k_react, reference_a,
+                                     reference_b, out8, stream, false, tile_rows);
 }
 
 extern "C" int lbc_reactive_stats_noalloc(int scalar_lattice, int storage, const void* species_a,

@@ -237,6 +237,93 @@ def from_slabs(lattice, h, stream=None):
     return out
 
 
+def _tiled_extents(lattice, t, name):
+    """(nx, ny, nz, rows) of a tiled population tensor: (tiles, Q, rows, nx)
+    in 2D, (nz, Q, ny, nx) in 3D (the slab order: a tile is an xy-slab)."""
+    if not (t.is_cuda and t.is_contiguous()) or t.dtype not in STORAGE:
+        raise ValueError(f"{name} must be a contiguous CUDA tensor of a storage dtype")
+    if t.dim() != 4 or t.shape[1] != Q[lattice]:
+        raise ValueError(f"{name} must have shape (tiles, {Q[lattice]}, rows, nx)")
+    tiles, _, rows, nx = (int(v) for v in t.shape)
+    if DIM[lattice] == 2:
+        return nx, tiles * rows, 1, rows
+    return nx, rows, tiles, rows
+
+
+def to_tiles(lattice, f, rows=None, stream=None):
+    """Populations of shape (Q, [nz,] ny, nx) -> a new tensor in tiles
+    (lbc_relayout_tiled): 2D (ny / rows, Q, rows, nx), each tile holding the Q
+    planes of its rows back to back; 3D (nz, Q, ny, nx), the slab order
+    (rows = ny)."""
+    _check_pop(lattice, f, "f")
+    nx, ny, nz = _extents(lattice, f)
+    if DIM[lattice] == 3:
+        rows = ny if rows is None else int(rows)
+        shape = (nz, Q[lattice], ny, nx)
+    else:
+        rows = int(rows)
+        if rows <= 0 or ny % rows:
+            raise ValueError(f"rows must divide ny = {ny}")
+        shape = (ny // rows, Q[lattice], rows, nx)
+    out = torch.empty(shape, dtype=f.dtype, device=f.device)
+    check(lib().lbc_relayout_tiled(LATTICES[lattice], STORAGE[f.dtype], f.data_ptr(), 0, out.data_ptr(), rows, nx,
+                                   ny, nz, _stream(stream)), "lbc_relayout_tiled")
+    return out
+
+
+def from_tiles(lattice, t, stream=None):
+    """The inverse of to_tiles: back to the reference's planes (Q, [nz,] ny, nx)."""
+    nx, ny, nz, rows = _tiled_extents(lattice, t, "t")
+    shape = (Q[lattice], ny, nx) if DIM[lattice] == 2 else (Q[lattice], nz, ny, nx)
+    out = torch.empty(shape, dtype=t.dtype, device=t.device)
+    check(lib().lbc_relayout_tiled(LATTICES[lattice], STORAGE[t.dtype], t.data_ptr(), rows, out.data_ptr(), 0, nx,
+                                   ny, nz, _stream(stream)), "lbc_relayout_tiled")
+    return out
+
+
+def fluid_step_tiled(lattice, collision, cur, nxt, omega, block=None, stream=None, fast=False, mfma=False):
+    """fluid_step on populations in tiles (to_tiles): lbc_fluid_step_tiled."""
+    nx, ny, nz, rows = _tiled_extents(lattice, cur, "cur")
+    if nxt.shape != cur.shape or nxt.dtype != cur.dtype:
+        raise ValueError("cur and nxt must share shape and dtype")
+    _tiled_extents(lattice, nxt, "nxt")
+    bx, by, bz = _block(lattice, block, nx, ny, nz)
+    check(lib().lbc_fluid_step_tiled(LATTICES[lattice], COLLISIONS[collision], STORAGE[cur.dtype],
+                                     _step_flags(fast, mfma), cur.data_ptr(), nxt.data_ptr(), nx, ny, nz, rows,
+                                     float(omega), bx, by, bz, _stream(stream)), "lbc_fluid_step_tiled")
+
+
+def reactive_fused_step_tiled(fluid_lattice, collision, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b,
+                              omega, omega_c, k_react, block=None, stream=None, fast=False, ref_a=0.0, ref_b=0.0):
+    """reactive_fused_step on the three population sets in tiles of the same
+    rows (to_tiles): lbc_reactive_fused_step_tiled."""
+    nx, ny, nz, rows = _tiled_extents(scalar_lattice, cur_a, "cur_a")
+    for t, n, lat in ((cur_f, "cur_f", fluid_lattice), (nxt_f, "nxt_f", fluid_lattice), (nxt_a, "nxt_a", scalar_lattice),
+                      (cur_b, "cur_b", scalar_lattice), (nxt_b, "nxt_b", scalar_lattice)):
+        if _tiled_extents(lat, t, n) != (nx, ny, nz, rows) or t.dtype != cur_a.dtype:
+            raise ValueError(f"{n} must match cur_a's tiles and dtype")
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_reactive_fused_step_tiled(LATTICES[fluid_lattice], COLLISIONS[collision], LATTICES[scalar_lattice],
+                                              STORAGE[cur_f.dtype], 1 if fast else 0, cur_f.data_ptr(),
+                                              nxt_f.data_ptr(), cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(),
+                                              nxt_b.data_ptr(), nx, ny, nz, rows, float(omega), float(omega_c),
+                                              float(k_react), float(ref_a), float(ref_b), bx, by, bz,
+                                              _stream(stream)), "lbc_reactive_fused_step_tiled")
+
+
+def reactive_stats_tiled(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=0.0):
+    """reactive_stats of species in tiles (to_tiles): lbc_reactive_stats_tiled;
+    the same sums in the same order, so the same results bit for bit."""
+    nx, ny, nz, rows = _tiled_extents(scalar_lattice, a, "a")
+    if _tiled_extents(scalar_lattice, b, "b") != (nx, ny, nz, rows) or b.dtype != a.dtype:
+        raise ValueError("b must match a's tiles and dtype")
+    out = (ctypes.c_double * 8)()
+    check(lib().lbc_reactive_stats_tiled(LATTICES[scalar_lattice], STORAGE[a.dtype], a.data_ptr(), b.data_ptr(), nx,
+                                         ny, nz, rows, float(k_react), float(ref_a), float(ref_b), out,
+                                         _stream(stream)), "lbc_reactive_stats_tiled")
+    return dict(zip(STAT_NAMES, out))
+
+
 def pull_stream_strided(lattice, cur, nxt, extents, plane_stride, slab_stride, block=None, stream=None, shift=True):
     """lbc_pull_stream_strided: the pull-streaming step (shift=True) or the
     unshifted plane copy on a strided layout."""

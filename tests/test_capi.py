@@ -227,5 +227,39 @@ def test_validation_strided_layouts():
         assert fn(FAKE_A, FAKE_B, 0, 0, lattice=4) == HIP_INVALID_VALUE  # scalar lattice
         assert fn(FAKE_A, FAKE_B, 0, 1 << 33, nz=1) == HIP_INVALID_VALUE  # slab gap of 2^32 or more
     assert L.lbc_relayout(1, 7, FAKE_A, 0, 0, FAKE_B, 0, 19 * 64, 8, 8, 8, None) == HIP_INVALID_VALUE  # storage
+
+
+def test_validation_tiles():
+    """Tiles: 2D rows must divide ny, 3D rows must be ny (the slab order);
+    Q times the cells must stay below 2^32; scalar lattices are no fluid;
+    checks come after the reference's pointer / block / extent order."""
+    L = lbcube.lib()
+    fluid = lambda lattice, nx, ny, nz, rows, a=FAKE_A: L.lbc_fluid_step_tiled(  # noqa: E731
+        lattice, 1, 0, 0, a, FAKE_B, nx, ny, nz, rows, 1.0, 256, 1, 1, None)
+    fused = lambda fl, sl, nx, ny, nz, rows: L.lbc_reactive_fused_step_tiled(  # noqa: E731
+        fl, 1, sl, 0, 0, FAKE_A, FAKE_B, 0x3000, 0x4000, 0x5000, 0x6000, nx, ny, nz, rows, 1.0, 1.0, 0.1, 0.0, 0.0,
+        256, 1, 1, None)
+    out = (ctypes.c_double * 8)()
+    stats = lambda sl, nx, ny, nz, rows: L.lbc_reactive_stats_tiled(  # noqa: E731
+        sl, 0, FAKE_A, FAKE_B, nx, ny, nz, rows, 0.1, 0.0, 0.0, out, None)
+    relayout = lambda lattice, nx, ny, nz, r0, r1: L.lbc_relayout_tiled(  # noqa: E731
+        lattice, 0, FAKE_A, r0, FAKE_B, r1, nx, ny, nz, None)
+    assert fluid(0, 16, 12, 1, 4, a=None) == HIP_INVALID_DEVICE_POINTER
+    assert fluid(0, 16, 12, 1, 5) == HIP_INVALID_VALUE  # 5 does not divide 12
+    assert fluid(0, 16, 12, 1, 24) == HIP_INVALID_VALUE
+    assert fluid(1, 8, 8, 8, 4) == HIP_INVALID_VALUE  # 3D tiles are whole slabs
+    assert fluid(3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a scalar lattice
+    assert fluid(1, 1 << 12, 1 << 10, 1 << 7, 1 << 10) == HIP_INVALID_VALUE  # 19 x 2^29 elements
+    assert fused(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
+    assert fused(1, 4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert fused(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # scalar lattice as the fluid
+    assert stats(3, 16, 12, 1, 5) == HIP_INVALID_VALUE
+    assert stats(4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert stats(0, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a fluid lattice
+    assert relayout(0, 16, 12, 1, 0, 5) == HIP_INVALID_VALUE
+    assert relayout(4, 8, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert relayout(7, 8, 8, 8, 0, 8) == HIP_INVALID_VALUE  # unknown lattice
+    assert L.lbc_relayout_tiled(0, 0, FAKE_A, 0, FAKE_A, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # src == dst
+    assert L.lbc_relayout_tiled(0, 9, FAKE_A, 0, FAKE_B, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # storage
     assert L.lbc_fluid_step_strided(1, 1, 0, 8, FAKE_A, FAKE_B, 8, 8, 8, 0, 19 * 64, 1.0, 256, 1, 1, None) == \
         HIP_INVALID_VALUE  # unknown flag

@@ -1,6 +1,8 @@
-"""GPU: strided population layouts (lbc_fluid_step_strided, lbc_pull_stream_strided,
-lbc_relayout), above all the slab order [Z][Q][Y][X], in which the Q planes of
-each xy-slab lie side by side.
+"""GPU: strided and tiled population layouts (lbc_fluid_step_strided,
+lbc_pull_stream_strided, lbc_relayout; lbc_fluid_step_tiled,
+lbc_reactive_fused_step_tiled, lbc_reactive_stats_tiled, lbc_relayout_tiled),
+above all the slab order [Z][Q][Y][X], in which the Q planes of each xy-slab
+lie side by side, and 2D tiles of rows [ny / R][Q][R][X].
 
 The layout changes where a population is stored, never its value:
   * the reference-order step in the slab order equals the oracle bit for bit
@@ -12,7 +14,12 @@ The layout changes where a population is stored, never its value:
   * lbc_relayout equals torch's permutation of the axes, and the pull step in
     the slab order is the pull permutation;
   * at 256^3 (the benchmark's domain) the slab order's TRT step equals the
-    contiguous one after 4 steps.
+    contiguous one after 4 steps;
+  * 2D tiles (1, 4, a ragged divisor and all rows per tile): the D2Q9 step
+    equals the oracle bit for bit; the fused fluid + A + B -> C step (2D and
+    3D, both arithmetic forms, every storage) and the reactive statistics
+    equal their runs on the reference's planes bit for bit, also on the
+    2048^2 domain of the 2D reactive benchmark.
 """
 from __future__ import annotations
 
@@ -23,7 +30,7 @@ import torch
 import cases
 import lbcube
 from oracle import oracle as O
-from test_gpu_parity import COLLS, SHAPE, STORES, TORCH_DT, to_dev, to_host
+from test_gpu_parity import COLLS, SHAPE, STORES, TORCH_DT, _species, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
@@ -169,4 +176,125 @@ def test_slab_order_at_256_cubed(gpu):
         want = run_planes("D3Q19", "TRT", f, 1 / 0.50382, 4, fast=fast)
         assert torch.equal(got, want), fast
         del got, want
+    torch.cuda.empty_cache()
+
+
+def test_relayout_tiled_is_the_tile_permutation(gpu):
+    """2D tiles of R rows: tile t holds rows tR..tR+R-1 of every plane; 3D
+    tiles are the xy-slabs.  Fluid and species lattices, every storage."""
+    for lattice, shape in (("D2Q9", (12, 7)), ("D2Q5", (12, 7)), ("D3Q19", (3, 4, 5)), ("D3Q7", (3, 4, 5))):
+        for storage in STORES:
+            q = lbcube.Q[lattice]
+            f = to_dev(cases.random_perturbed("D2Q9" if q in (9, 5) else "D3Q19", shape, storage, seed=3)[:q],
+                       storage, gpu)
+            for rows in ((1, 3, 4, 12) if len(shape) == 2 else (shape[1],)):
+                t = lbcube.to_tiles(lattice, f, rows)
+                if len(shape) == 2:
+                    want = f.reshape(q, shape[0] // rows, rows, shape[1]).permute(1, 0, 2, 3).contiguous()
+                else:
+                    want = f.permute(1, 0, 2, 3).contiguous()
+                assert torch.equal(bits(t), bits(want)), (lattice, storage, rows)
+                assert torch.equal(bits(lbcube.from_tiles(lattice, t)), bits(f))
+
+
+def run_tiled_fluid(lattice, coll, f, omega, steps, rows, fast=False, block=None):
+    a = lbcube.to_tiles(lattice, f, rows)
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        lbcube.fluid_step_tiled(lattice, coll, a, b, omega, fast=fast, block=block)
+        a, b = b, a
+    return lbcube.from_tiles(lattice, a)
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("coll", COLLS)
+def test_tiled_d2q9_step_matches_oracle(gpu, coll, storage):
+    """Odd nx, ny = 12 in tiles of 1, 3, 4 and 12 rows: every y step across a
+    tile boundary and the periodic wrap across all tiles."""
+    print(f"case tiles D2Q9 {coll} {storage}", flush=True)
+    f0 = cases.random_perturbed("D2Q9", (12, 13), storage, seed=11)
+    omega = 1.0 / 0.62
+    want = O.fluid_run("D2Q9", coll, storage, f0, omega, 6)
+    f = to_dev(f0, storage, gpu)
+    for rows in (1, 3, 4, 12):
+        got = to_host(run_tiled_fluid("D2Q9", coll, f, omega, 6, rows), storage)
+        assert np.array_equal(got, want), rows
+    if coll != "RLBM":
+        for rows in (1, 4):
+            got = run_tiled_fluid("D2Q9", coll, f, omega, 5, rows, fast=True, block=(64, 1, 1))
+            assert torch.equal(bits(got), bits(run_planes("D2Q9", coll, f, omega, 5, fast=True, block=(64, 1, 1))))
+
+
+def run_fused(fl, coll, sl, f, a, b, steps, fast, refs, rows=None, block=None):
+    """`steps` fused steps on the reference's planes (rows None) or in tiles."""
+    omega, omega_c, k = 1.3, 1.05, 0.02
+    if rows is not None:
+        f, a, b = lbcube.to_tiles(fl, f, rows), lbcube.to_tiles(sl, a, rows), lbcube.to_tiles(sl, b, rows)
+        step = lbcube.reactive_fused_step_tiled
+    else:
+        f, a, b = f.clone(), a.clone(), b.clone()
+        step = lbcube.reactive_fused_step
+    ft, at, bt = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+    for _ in range(steps):
+        step(fl, coll, sl, f, ft, a, at, b, bt, omega, omega_c, k, fast=fast, ref_a=refs[0], ref_b=refs[1],
+             block=block)
+        f, ft, a, at, b, bt = ft, f, at, a, bt, b
+    if rows is not None:
+        return lbcube.from_tiles(fl, f), lbcube.from_tiles(sl, a), lbcube.from_tiles(sl, b)
+    return f, a, b
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
+def test_tiled_fused_step_matches_planes(gpu, pair, storage):
+    fl, sl = pair
+    shape = (12, 13) if fl == "D2Q9" else (4, 6, 7)
+    refs = (0.5, 0.25) if storage in ("f16", "bf16") else (0.0, 0.0)
+    f = to_dev(cases.random_perturbed(fl, shape, storage, seed=2), storage, gpu)
+    a0, b0 = _species(sl, shape, storage, seed=8, refs=refs)
+    a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+    row_set = (1, 4, 12) if fl == "D2Q9" else (shape[1],)
+    for coll in ("BGK", "TRT", "MRT", "RLBM"):
+        for fast in ((False, True) if coll != "RLBM" else (False,)):
+            print(f"case tiles fused {fl} {coll} {storage} fast={fast}", flush=True)
+            want = run_fused(fl, coll, sl, f, a, b, 4, fast, refs)
+            for rows in row_set:
+                got = run_fused(fl, coll, sl, f, a, b, 4, fast, refs, rows=rows)
+                for x, y in zip(got, want):
+                    assert torch.equal(bits(x), bits(y)), (coll, fast, rows)
+
+
+@pytest.mark.parametrize("pair", [("D2Q5", (12, 13), (1, 4, 12)), ("D3Q7", (4, 6, 7), (6,))])
+def test_tiled_reactive_stats_equal_planes(gpu, pair):
+    sl, shape, row_set = pair
+    for storage in STORES:
+        refs = (0.5, 0.25) if storage in ("f16", "bf16") else (0.0, 0.0)
+        a0, b0 = _species(sl, shape, storage, seed=5, refs=refs)
+        a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+        want = lbcube.reactive_stats(sl, a, b, 0.03, ref_a=refs[0], ref_b=refs[1])
+        for rows in row_set:
+            got = lbcube.reactive_stats_tiled(sl, lbcube.to_tiles(sl, a, rows), lbcube.to_tiles(sl, b, rows), 0.03,
+                                              ref_a=refs[0], ref_b=refs[1])
+            assert got == want, (storage, rows)
+
+
+def test_tiled_fused_step_at_2048_squared(gpu):
+    """The 2D reactive benchmark's domain: 3 fused MRT steps of a seeded
+    state, FP32 reference order and FP16 deviation form, in tiles of 4 and 64
+    rows, equal the reference's planes bit for bit."""
+    print("case tiles fused 2048^2", flush=True)
+    from lbcube import init as lbinit
+    n = 2048
+    rho, u = lbinit.taylor_green_2d(n, n, 0.04)
+    ca, cb = lbinit.tanh_half_planes(n, delta=40.0)
+    for storage, fast, refs in (("f32", False, (0.0, 0.0)), ("f16", True, (0.5, 0.5))):
+        f = lbinit.equilibrium("D2Q9", rho, u, storage)
+        a = lbinit.scalar_equilibrium("D2Q5", ca, u, storage, refs[0])
+        b = lbinit.scalar_equilibrium("D2Q5", cb, u, storage, refs[1])
+        want = run_fused("D2Q9", "MRT", "D2Q5", f, a, b, 3, fast, refs)
+        for rows in (4, 64):
+            got = run_fused("D2Q9", "MRT", "D2Q5", f, a, b, 3, fast, refs, rows=rows)
+            for x, y in zip(got, want):
+                assert torch.equal(bits(x), bits(y)), (storage, rows)
+        del f, a, b, want, got
     torch.cuda.empty_cache()
