@@ -14,6 +14,12 @@
 //   -csv FILE              statistics file (default reaction_diagnostics.csv)
 //   -vtk_dir DIR           folder of the reaction_<step>.vtk snapshots (default .)
 //   -no_vtk                no snapshots
+//   -tiles R               the fused step with the three population sets in
+//                          tiles of R rows (R divides ny; lbc_reactive_fused_step_tiled,
+//                          statistics by lbc_reactive_stats_tiled): the same
+//                          results bit for bit as -fused, a faster HBM pattern;
+//                          converted into the idle ping-pong buffers for
+//                          snapshots and -dump
 // Rows: step,mean_A,mean_B,var_A,var_B,covariance,segregation_intensity,
 // true_reaction_rate,mixed_reaction_rate with %.17g (include/lattice_io.hpp:352-368),
 // computed on the device with FP64 accumulation.  Snapshots: C_A, C_B,
@@ -51,6 +57,7 @@ struct Config {
     std::string load, dump, csv = "reaction_diagnostics.csv";
     std::string vtk_dir = ".";
     bool vtk = true;
+    std::size_t tiles = 0;
 };
 
 void check(hipError_t e, const char* what) {
@@ -86,8 +93,10 @@ Config parse(int argc, char** argv) {
         else if (a == "-csv") c.csv = value(i);
         else if (a == "-vtk_dir") c.vtk_dir = value(i);
         else if (a == "-no_vtk") c.vtk = false;
+        else if (a == "-tiles") c.tiles = size_value(i);
         else throw std::invalid_argument("unknown argument: " + a);
     }
+    if (c.tiles != 0 && c.ny % c.tiles != 0) throw std::invalid_argument("-tiles must divide -ny");
     if (c.steps < 0) throw std::invalid_argument("-steps must be non-negative");
     if (c.output_frequency <= 0) throw std::invalid_argument("-out_freq must be greater than zero");
     if (c.tau <= 0.5 || c.tau_c <= 0.5) throw std::invalid_argument("relaxation times must be greater than 0.5");
@@ -133,9 +142,9 @@ template <class L> void read_raw(lbm::HostLattice<L, Real>& m, const std::string
         throw std::runtime_error("cannot read " + path);
 }
 
-void write_raw(const Device& d, const std::string& path) {
-    std::vector<Real> host(d.count);
-    check(hipMemcpy(host.data(), d.cur, d.count * sizeof(Real), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+void write_raw(const Real* device, std::size_t count, const std::string& path) {
+    std::vector<Real> host(count);
+    check(hipMemcpy(host.data(), device, count * sizeof(Real), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     std::ofstream out(path, std::ios::binary);
     if (!out.write(reinterpret_cast<const char*>(host.data()), host.size() * sizeof(Real)))
         throw std::runtime_error("cannot write " + path);
@@ -162,7 +171,8 @@ int main(int argc, char** argv) {
                "mixed_reaction_rate\n";
         std::cout << "LB-Cube reactive mixing (MI355X)\nGrid: " << c.nx << " x " << c.ny << "\nSteps: " << c.steps
                   << "\nOutput frequency: " << c.output_frequency << "\nOmega: " << omega << "\nOmega_c: " << omega_c
-                  << "\nk_react: " << c.k_react << "\nBackend: GPU" << (c.fused ? " (fused step)" : "") << '\n'
+                  << "\nk_react: " << c.k_react << "\nBackend: GPU"
+                  << (c.tiles ? " (fused step, tiles)" : c.fused ? " (fused step)" : "") << '\n'
                   << std::flush;
 
         Device f(fluid.population_count()), a(species_a.population_count()), b(species_b.population_count());
@@ -170,11 +180,39 @@ int main(int argc, char** argv) {
         check(hipMemcpy(a.cur, species_a.current(), a.count * sizeof(Real), hipMemcpyHostToDevice), "hipMemcpy H2D");
         check(hipMemcpy(b.cur, species_b.current(), b.count * sizeof(Real), hipMemcpyHostToDevice), "hipMemcpy H2D");
 
+        const int fluid_code = lbm::lattice_code<FluidLattice>(), scalar_code = lbm::lattice_code<ScalarLattice>();
+        // -tiles: each population set into tiles of c.tiles rows, through its idle buffer
+        auto relayout = [&](int code, Device& d, std::size_t from, std::size_t to) {
+            check(static_cast<hipError_t>(lbc_relayout_tiled(code, LBC_F64, d.cur, from, d.next, to, c.nx, c.ny, 1,
+                                                             nullptr)),
+                  "lbc_relayout_tiled");
+        };
+        if (c.tiles != 0) {
+            relayout(fluid_code, f, 0, c.tiles);
+            relayout(scalar_code, a, 0, c.tiles);
+            relayout(scalar_code, b, 0, c.tiles);
+            std::swap(f.cur, f.next);
+            std::swap(a.cur, a.next);
+            std::swap(b.cur, b.next);
+        }
+        // A set in the reference's planes: its current buffer, or with tiles its
+        // planes converted into the idle buffer (the next step overwrites it).
+        auto planes = [&](int code, Device& d) -> const Real* {
+            if (c.tiles == 0) return d.cur;
+            relayout(code, d, c.tiles, 0);
+            return d.next;
+        };
+
         auto log = [&](int s) {
             double out[8];
-            check(static_cast<hipError_t>(lbc_reactive_stats(lbm::lattice_code<ScalarLattice>(), LBC_F64, a.cur, b.cur, c.nx, c.ny,
-                                                             1, c.k_react, 0.0, 0.0, out, nullptr)),
-                  "lbc_reactive_stats");
+            if (c.tiles != 0)
+                check(static_cast<hipError_t>(lbc_reactive_stats_tiled(scalar_code, LBC_F64, a.cur, b.cur, c.nx, c.ny, 1,
+                                                                       c.tiles, c.k_react, 0.0, 0.0, out, nullptr)),
+                      "lbc_reactive_stats_tiled");
+            else
+                check(static_cast<hipError_t>(lbc_reactive_stats(scalar_code, LBC_F64, a.cur, b.cur, c.nx, c.ny, 1,
+                                                                 c.k_react, 0.0, 0.0, out, nullptr)),
+                      "lbc_reactive_stats");
             char row[400];
             std::snprintf(row, sizeof(row), "%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g\n", s, out[0], out[1],
                           out[2], out[3], out[4], out[5], out[6], out[7]);
@@ -185,9 +223,9 @@ int main(int argc, char** argv) {
             if (!c.vtk) return;
             char name[40];
             std::snprintf(name, sizeof(name), "/reaction_%06d.vtk", s);
-            const auto fields = lbc_vtk::fluid_fields(staging, lbm::lattice_code<FluidLattice>(), f.cur, c.nx, c.ny, 1);
-            const auto ca = lbc_vtk::concentration(staging, staging.ca, lbm::lattice_code<ScalarLattice>(), a.cur, c.nx, c.ny, 1);
-            const auto cb = lbc_vtk::concentration(staging, staging.cb, lbm::lattice_code<ScalarLattice>(), b.cur, c.nx, c.ny, 1);
+            const auto fields = lbc_vtk::fluid_fields(staging, fluid_code, planes(fluid_code, f), c.nx, c.ny, 1);
+            const auto ca = lbc_vtk::concentration(staging, staging.ca, scalar_code, planes(scalar_code, a), c.nx, c.ny, 1);
+            const auto cb = lbc_vtk::concentration(staging, staging.cb, scalar_code, planes(scalar_code, b), c.nx, c.ny, 1);
             lbc_vtk::write_reactive(c.vtk_dir + name, "LB-Cube reactive mixing step " + std::to_string(s), false,
                                     fields, ca, cb, c.k_react, c.nx, c.ny);
         };
@@ -195,7 +233,13 @@ int main(int argc, char** argv) {
         log(0);
         const auto start = std::chrono::steady_clock::now();
         for (int s = 1; s <= c.steps; ++s) {
-            if (c.fused) {
+            if (c.tiles != 0) {
+                check(static_cast<hipError_t>(lbc_reactive_fused_step_tiled(
+                          fluid_code, LBC_BGK, scalar_code, LBC_F64, 0u, f.cur, f.next, a.cur, a.next, b.cur, b.next,
+                          c.nx, c.ny, 1, c.tiles, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
+                      "lbc_reactive_fused_step_tiled");
+                std::swap(f.cur, f.next);
+            } else if (c.fused) {
                 check(static_cast<hipError_t>(lbc_reactive_fused_step(
                           lbm::lattice_code<FluidLattice>(), LBC_BGK, lbm::lattice_code<ScalarLattice>(), LBC_F64, f.cur, f.next, a.cur, a.next, b.cur,
                           b.next, c.nx, c.ny, 1, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
@@ -223,9 +267,9 @@ int main(int argc, char** argv) {
                   << static_cast<double>(c.nx) * c.ny * c.steps / elapsed.count() / 1e6 << " MLUPS\n"
                   << std::flush;
         if (!c.dump.empty()) {
-            write_raw(f, c.dump + ".f");
-            write_raw(a, c.dump + ".a");
-            write_raw(b, c.dump + ".b");
+            write_raw(planes(fluid_code, f), f.count, c.dump + ".f");
+            write_raw(planes(scalar_code, a), a.count, c.dump + ".a");
+            write_raw(planes(scalar_code, b), b.count, c.dump + ".b");
         }
         return EXIT_SUCCESS;
     } catch (const std::exception& error) {

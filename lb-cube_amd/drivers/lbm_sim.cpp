@@ -22,6 +22,11 @@
 //   -csv FILE              diagnostics file (default diagnostics.csv)
 //   -vtk_dir DIR           folder of the lbm_<step>.vtk snapshots (default .)
 //   -no_vtk                no snapshots
+//   -slabs                 keep the device populations in the slab order
+//                          [Z][Q][Y][X] (lbc_fluid_step_tiled; same results bit
+//                          for bit, a faster HBM pattern); they are converted
+//                          into the idle ping-pong buffer for diagnostics,
+//                          snapshots and -dump
 // Diagnostics rows: step,total_mass,total_kinetic_energy,max_velocity_magnitude
 // with %.17g (include/lattice_io.hpp:182-234), summed on the device in FP64.
 // Snapshots: density, velocity_magnitude, velocity (write_vtk,
@@ -60,6 +65,7 @@ struct Config {
     std::string load, dump, csv = "diagnostics.csv";
     std::string vtk_dir = ".";
     bool vtk = true;
+    bool slabs = false;
 };
 
 void check(hipError_t e, const char* what) {
@@ -95,6 +101,7 @@ Config parse(int argc, char** argv) {
         else if (a == "-csv") c.csv = value(i);
         else if (a == "-vtk_dir") c.vtk_dir = value(i);
         else if (a == "-no_vtk") c.vtk = false;
+        else if (a == "-slabs") c.slabs = true;
         else throw std::invalid_argument("unknown argument: " + a);
     }
     if (c.steps < 0) throw std::invalid_argument("-steps must be non-negative");
@@ -126,10 +133,11 @@ int main(int argc, char** argv) {
         const Config c = parse(argc, argv);
         const Real omega = 1.0 / c.tau;
         hipError_t (*launch)(const Real*, Real*, const Config&, Real) = nullptr;
+        int collision_code = LBC_BGK;
         if (c.collision == "bgk") launch = step<lbm::CollisionType::BGK>;
-        else if (c.collision == "trt") launch = step<lbm::CollisionType::TRT>;
-        else if (c.collision == "mrt") launch = step<lbm::CollisionType::MRT>;
-        else if (c.collision == "rlbm") launch = step<lbm::CollisionType::RLBM>;
+        else if (c.collision == "trt") launch = step<lbm::CollisionType::TRT>, collision_code = LBC_TRT;
+        else if (c.collision == "mrt") launch = step<lbm::CollisionType::MRT>, collision_code = LBC_MRT;
+        else if (c.collision == "rlbm") launch = step<lbm::CollisionType::RLBM>, collision_code = LBC_RLBM;
         else throw std::invalid_argument("-collision must be bgk, trt, mrt or rlbm");
 
         lbm::HostLattice<Lattice, Real> memory{c.nx, c.ny, c.nz};
@@ -155,10 +163,26 @@ int main(int argc, char** argv) {
         check(hipMalloc(reinterpret_cast<void**>(&current), count * sizeof(Real)), "hipMalloc current populations");
         check(hipMalloc(reinterpret_cast<void**>(&next), count * sizeof(Real)), "hipMalloc next populations");
         check(hipMemcpy(current, memory.current(), count * sizeof(Real), hipMemcpyHostToDevice), "hipMemcpy H2D");
+        const int code = lbm::lattice_code<Lattice>();
+        if (c.slabs) {  // into the slab order, through the idle buffer
+            check(static_cast<hipError_t>(lbc_relayout_tiled(code, LBC_F64, current, 0, next, c.ny, c.nx, c.ny, c.nz,
+                                                             nullptr)),
+                  "lbc_relayout_tiled");
+            std::swap(current, next);
+        }
+        // The populations in the reference's planes: `current` itself, or its
+        // slab order converted into the idle buffer (the next step overwrites it).
+        auto planes = [&]() -> const Real* {
+            if (!c.slabs) return current;
+            check(static_cast<hipError_t>(lbc_relayout_tiled(code, LBC_F64, current, c.ny, next, 0, c.nx, c.ny, c.nz,
+                                                             nullptr)),
+                  "lbc_relayout_tiled");
+            return next;
+        };
 
         auto log = [&](int s) {
             double out[3];
-            check(static_cast<hipError_t>(lbc_fluid_diagnostics(lbm::lattice_code<Lattice>(), LBC_F64, current, c.nx, c.ny, c.nz, out,
+            check(static_cast<hipError_t>(lbc_fluid_diagnostics(code, LBC_F64, planes(), c.nx, c.ny, c.nz, out,
                                                                 nullptr)),
                   "lbc_fluid_diagnostics");
             char row[160];
@@ -169,14 +193,19 @@ int main(int argc, char** argv) {
         auto snapshot = [&](int s) {
             if (!c.vtk) return;
             lbc_vtk::write_fluid(c.vtk_dir, static_cast<std::size_t>(s),
-                                 lbc_vtk::fluid_fields(staging, lbm::lattice_code<Lattice>(), current, c.nx, c.ny, c.nz), c.nx, c.ny,
+                                 lbc_vtk::fluid_fields(staging, code, planes(), c.nx, c.ny, c.nz), c.nx, c.ny,
                                  c.nz, Lattice::D);
         };
         snapshot(0);
         log(0);
         const auto start = std::chrono::steady_clock::now();
         for (int s = 1; s <= c.steps; ++s) {
-            check(launch(current, next, c, omega), "launch_step_gpu");
+            if (c.slabs)
+                check(static_cast<hipError_t>(lbc_fluid_step_tiled(code, collision_code, LBC_F64, 0u, current, next,
+                                                                   c.nx, c.ny, c.nz, c.ny, omega, 256, 1, 1, nullptr)),
+                      "lbc_fluid_step_tiled");
+            else
+                check(launch(current, next, c, omega), "launch_step_gpu");
             std::swap(current, next);
             if (s % c.output_frequency == 0) {
                 snapshot(s);
@@ -191,7 +220,7 @@ int main(int argc, char** argv) {
                   << " MLUPS\n"
                   << std::flush;
         if (!c.dump.empty()) {
-            check(hipMemcpy(memory.current(), current, count * sizeof(Real), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            check(hipMemcpy(memory.current(), planes(), count * sizeof(Real), hipMemcpyDeviceToHost), "hipMemcpy D2H");
             std::ofstream out(c.dump, std::ios::binary);
             if (!out.write(reinterpret_cast<const char*>(memory.current()), count * sizeof(Real)))
                 throw std::runtime_error("cannot write " + c.dump);

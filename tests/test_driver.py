@@ -95,6 +95,7 @@ def test_run_matches_oracle(gpu, tmp_path, collision):
                                           (["-tau_c", "0.4"], "relaxation times must be greater than 0.5"),
                                           (["-k_react", "-1"], "-k_react must be non-negative"),
                                           (["-nx", "0"], "-nx must be greater than zero"),
+                                          (["-ny", "12", "-tiles", "5"], "-tiles must divide -ny"),
                                           (["-bogus"], "unknown argument: -bogus")])
 def test_reaction_argument_errors(tmp_path, args, message):
     assert os.path.exists(REACT_EXE), "build first: make -C lb-cube_amd"
@@ -469,3 +470,54 @@ def test_reaction_sim_own_initial_state(gpu, tmp_path, fused):
     assert np.array_equal(got[0], state[0])
     for g, w in zip(got[1:], state[1:]):
         assert np.max(np.abs(g - w)) <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("collision", ["trt", "mrt"])
+def test_lbm_sim_slab_order_equals_planes(gpu, tmp_path, collision):
+    """lbm_sim -slabs (the device populations in the slab order, diagnostics,
+    snapshots and the dump converted back): every diagnostics row, every VTK
+    field and the final populations equal the run on the reference's planes
+    byte for byte."""
+    print(f"case lbm_sim -slabs {collision}", flush=True)
+    outs = []
+    for extra in ([], ["-slabs"]):
+        d = tmp_path / ("slabs" if extra else "planes")
+        d.mkdir()
+        r = run("-nx", 12, "-ny", 10, "-nz", 8, "-steps", 20, "-out_freq", 5, "-collision", collision, "-dump",
+                d / "out.bin", "-csv", d / "d.csv", "-vtk_dir", d, *extra, cwd=d)
+        assert r.returncode == 0, r.stderr
+        outs.append(d)
+    planes, slabs = outs
+    assert (planes / "out.bin").read_bytes() == (slabs / "out.bin").read_bytes()
+    assert (planes / "d.csv").read_text() == (slabs / "d.csv").read_text()
+    vtks = sorted(p.name for p in planes.glob("*.vtk"))
+    assert len(vtks) == 5 and vtks == sorted(p.name for p in slabs.glob("*.vtk"))
+    for name in vtks:
+        assert (planes / name).read_text() == (slabs / name).read_text(), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tiles", [1, 4, 48])
+def test_reaction_sim_tiles_equal_fused(gpu, tmp_path, tiles):
+    """lbm_reaction_sim -tiles R (the fused step and the statistics on the
+    three population sets in tiles of R rows): every statistics row, every VTK
+    field and the final populations equal the -fused run on the reference's
+    planes byte for byte."""
+    print(f"case lbm_reaction_sim -tiles {tiles}", flush=True)
+    outs = []
+    for extra in (["-fused"], ["-tiles", tiles]):
+        d = tmp_path / ("tiles" if extra[0] == "-tiles" else "fused")
+        d.mkdir()
+        r = run("-nx", 40, "-ny", 48, "-steps", 60, "-out_freq", 20, "-dump", d / "out", "-csv", d / "s.csv",
+                "-vtk_dir", d, *extra, cwd=d, exe=REACT_EXE)
+        assert r.returncode == 0, r.stderr
+        outs.append(d)
+    fused, tiled = outs
+    for suffix in "fab":
+        assert (fused / f"out.{suffix}").read_bytes() == (tiled / f"out.{suffix}").read_bytes(), suffix
+    assert (fused / "s.csv").read_text() == (tiled / "s.csv").read_text()
+    vtks = sorted(p.name for p in fused.glob("*.vtk"))
+    assert len(vtks) == 4 and vtks == sorted(p.name for p in tiled.glob("*.vtk"))
+    for name in vtks:
+        assert (fused / name).read_text() == (tiled / name).read_text(), name
