@@ -196,7 +196,9 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
     """Taylor-Green Re = 1600 on D3Q19 and D3Q27 at 256^3 and 384^3, TRT and
     MRT, FP32 storage and FP16 / BF16 deviation storage (deviation-form TRT;
     MRT in both forms, and the deviation form with its transforms on the
-    matrix cores), with the pull-stream pattern ceilings of FP32 and FP16."""
+    matrix cores), with the pull-stream pattern ceilings of FP32 and FP16;
+    TRT FP32 and FP16 deviation form again in the slab order [Z][Q][Y][X]
+    (keys *_slabs)."""
     from lbcube import init as lbinit
 
     out = {}
@@ -204,13 +206,16 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
         omega = 1.0 / lbinit.tgv_re1600_tau(n)
         for lattice in ("D3Q19", "D3Q27"):
-            combos = [("TRT", "f32", False, False), ("TRT", "f16", True, False), ("MRT", "f32", False, False),
-                      ("MRT", "f32", True, False), ("MRT", "f16", True, False), ("MRT", "f16", True, True),
-                      ("MRT", "bf16", True, False), ("MRT", "bf16", True, True)]
-            for coll, storage, fast, mfma in combos:
-                key = f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}{'mfma' if mfma else ''}"
+            combos = [("TRT", "f32", False, False, "planes"), ("TRT", "f16", True, False, "planes"),
+                      ("MRT", "f32", False, False, "planes"), ("MRT", "f32", True, False, "planes"),
+                      ("MRT", "f16", True, False, "planes"), ("MRT", "f16", True, True, "planes"),
+                      ("MRT", "bf16", True, False, "planes"), ("MRT", "bf16", True, True, "planes"),
+                      ("TRT", "f32", False, False, "slabs"), ("TRT", "f16", True, False, "slabs")]
+            for coll, storage, fast, mfma, layout in combos:
+                key = (f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}{'mfma' if mfma else ''}"
+                       f"{'_slabs' if layout == 'slabs' else ''}")
                 r = time_variant(lbcube, torch, lattice, coll, storage, u, omega, steps, warmup, fast=fast,
-                                 mfma=mfma)
+                                 mfma=mfma, layout=layout)
                 out[key] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                 print(f"# {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s", file=sys.stderr,
                       flush=True)
@@ -493,8 +498,14 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
         lbcube.reactive_fused_step_tiled("D3Q19", "TRT", "D3Q7", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
         sstate[:] = [ft, f, at, a, bt, b]
 
+    def pattern(st):
+        lbcube.reactive_pull_stream("D3Q19", "D3Q7", *st, tiled=st is sstate)
+        st[:] = [st[1], st[0], st[3], st[2], st[5], st[4]]
+
     out = {"workload": f"D3Q19 TGV {n}^3 (u0 = 0.05) + two D3Q7 species (C = 0.5 +- 0.2 sin), FP32"}
-    for name, fn, bpc in (("scalar_step", scalar, 132), ("reaction_step", reaction, 188),
+    for name, fn, bpc in (("fused_pattern", lambda: pattern(state), 264),
+                          ("fused_pattern_slabs", lambda: pattern(sstate), 264),
+                          ("scalar_step", scalar, 132), ("reaction_step", reaction, 188),
                           ("fused_trt", lambda: fused(False), 264), ("fused_trt_fast", lambda: fused(True), 264),
                           ("fused_trt_slabs", lambda: fused_slabs(False), 264),
                           ("fused_trt_fast_slabs", lambda: fused_slabs(True), 264)):
@@ -608,7 +619,8 @@ def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
 def reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps, row_set=(2, 4, 8, 32)):
     """The same fused 2048^2 step with the three population sets stored in
     tiles of R rows (lbc_reactive_fused_step_tiled: each tile holds the Q
-    planes of its rows back to back), FP32 and FP16 in both arithmetic forms,
+    planes of its rows back to back), FP32 and FP16 in both arithmetic forms
+    and FP64,
     graph-replayed like the lines above; and the FP32 loop with the reactive
     statistics every 50 steps in tiles of 4 rows (lbc_reactive_stats_tiled).
     The conversion from the reference's planes (lbc_relayout_tiled) happens
@@ -616,7 +628,8 @@ def reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps, row_set=(2, 4, 8, 3
     from lbcube import init as lbinit
 
     out = {}
-    for storage, fast, ref in (("f32", False, 0.0), ("f32", True, 0.0), ("f16", False, 0.5), ("f16", True, 0.5)):
+    for storage, fast, ref in (("f32", False, 0.0), ("f32", True, 0.0), ("f16", False, 0.5), ("f16", True, 0.5),
+                               ("f64", False, 0.0)):
         planes = [lbinit.equilibrium("D2Q9", None, u, storage), lbinit.scalar_equilibrium("D2Q5", ca, u, storage, ref),
                   lbinit.scalar_equilibrium("D2Q5", cb, u, storage, ref)]
         es = planes[0].element_size()
@@ -648,8 +661,35 @@ def reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps, row_set=(2, 4, 8, 3
             del f, a, b, state
         del planes
         torch.cuda.empty_cache()
+    # the fused step's pattern ceiling (lbc_reactive_pull_stream: its loads,
+    # stores, addressing and block order without arithmetic) on the planes and
+    # in tiles of 4 and 8 rows
+    for storage in ("f32", "f16"):
+        planes = [lbinit.equilibrium("D2Q9", None, u, storage), lbinit.scalar_equilibrium("D2Q5", ca, u, storage),
+                  lbinit.scalar_equilibrium("D2Q5", cb, u, storage)]
+        es = planes[0].element_size()
+        for rows in (0, 4, 8):
+            cur = ([lbcube.to_tiles(lat, x, rows) for lat, x in zip(("D2Q9", "D2Q5", "D2Q5"), planes)] if rows
+                   else [x.clone() for x in planes])
+            bufs = [cur[0], torch.empty_like(cur[0]), cur[1], torch.empty_like(cur[1]), cur[2], torch.empty_like(cur[2])]
+
+            def pattern(bufs=bufs, rows=rows):
+                lbcube.reactive_pull_stream("D2Q9", "D2Q5", *bufs, tiled=bool(rows))
+                bufs[:] = [bufs[1], bufs[0], bufs[3], bufs[2], bufs[5], bufs[4]]
+
+            ms = _graph_ms(torch, pattern, steps)
+            key = f"pattern_{storage}_{'planes' if not rows else f'rows{rows}'}"
+            out[key] = {"achieved_gbs": 38 * es * n * n / (ms * 1e-3) / 1e9, "ms_per_step": ms}
+            del cur, bufs
+        del planes
+        torch.cuda.empty_cache()
+    print("# reactive 2D pattern ceilings (GB/s): " + ", ".join(f"{k[8:]} {v['achieved_gbs']:.0f}"
+                                                                for k, v in out.items() if k.startswith("pattern_")),
+          file=sys.stderr, flush=True)
     best = {}
     for key, v in out.items():
+        if key.startswith("pattern_"):
+            continue
         form = key.rsplit("_rows", 1)[0]
         if form not in best or v["mlups"] > out[best[form]]["mlups"]:
             best[form] = key

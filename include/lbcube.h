@@ -4,7 +4,9 @@
  * This is the drop-in boundary for LB-Cube's GPU launchers
  * (include/lattice_cuda.cuh of LB-Cube).  Every entry point:
  *   - takes device pointers to SoA population buffers laid out [Q][Z][Y][X]
- *     with x fastest (include/lattice_memory.hpp:67-272, src/lattice_cuda.cu:69-78),
+ *     with x fastest (include/lattice_memory.hpp:67-272, src/lattice_cuda.cu:69-78);
+ *     the *_pitched, *_strided and *_tiled extensions take the layouts they
+ *     document,
  *   - validates in the reference's order (src/lattice_cuda.cu:488-503):
  *       null pointer            -> hipErrorInvalidDevicePointer (17)
  *       any zero block extent   -> hipErrorInvalidConfiguration (9)
@@ -202,6 +204,19 @@ int lbc_reactive_fused_step_tiled(int fluid_lattice, int collision, int scalar_l
                                   uint64_t tile_rows, double omega, double omega_c, double k_react,
                                   double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
                                   uint32_t block_z, void* stream);
+
+/* The fused step's memory pattern alone (the ceiling of
+ * lbc_reactive_fused_step(_tiled) on the same buffers): every fluid and
+ * species population pulled from x - c_i (shift != 0) or read in place
+ * (shift == 0) and stored, with the fused kernel's addressing, block order,
+ * launch shape and residency, and no arithmetic.  tile_rows as for
+ * lbc_reactive_fused_step_tiled (0: the reference's planes); validation as
+ * that call.  No counterpart in the reference. */
+int lbc_reactive_pull_stream(int fluid_lattice, int scalar_lattice, int storage, int shift,
+                             const void* current_fluid, void* next_fluid, const void* current_a, void* next_a,
+                             const void* current_b, void* next_b, uint64_t nx, uint64_t ny, uint64_t nz,
+                             uint64_t tile_rows, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                             void* stream);
 
 /* lbc_reactive_stats of species in tiles: the cells are summed in the same
  * order, so the results equal lbc_reactive_stats' on the reference's planes

@@ -104,6 +104,49 @@ __global__ __launch_bounds__(MAXT) void reactive_fused_kernel(
                                           omega_c, k_react, ref_a, ref_b);
 }
 
+// The fused step's access pattern alone: every fluid and species population
+// pulled from x - c_i (SHIFT) or read in place, and stored, with the fused
+// kernel's addressing, block order, launch shape and residency and no
+// arithmetic.  Its rate is the ceiling of the fused step's memory pattern on
+// the same buffers (the reference's planes or tiles).
+template <class F, class S, int ST, int MAXT, bool WIDE, bool TILED, bool SHIFT>
+__global__ __launch_bounds__(MAXT) void reactive_pull_kernel(
+    const typename Storage<ST>::S* __restrict__ cur_f, typename Storage<ST>::S* __restrict__ nxt_f,
+    const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
+    const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b, Cells c,
+    uint32_t plane, uint32_t zgap_s) {
+    using T = typename Storage<ST>::S;
+    const uint32_t block = (!TILED && renumber_blocks<F::D, T>()) ? xcd_block() : blockIdx.x;
+    const uint32_t p = block * blockDim.x + threadIdx.x;
+    if (p >= c.count) return;
+    const CellCoord q = cell_coord(p, c);
+    Pull pf(p, q.x, q.y, q.z, static_cast<int>(c.nx), static_cast<int>(c.ny), static_cast<int>(c.nz));
+    Pull ps = pf;
+    if constexpr (TILED) {
+        pf = locate<F::D>(p, q, c, c.zgap);
+        ps = locate<F::D>(p, q, c, zgap_s);
+    }
+    const size_t pl = TILED ? plane : c.count;
+    T v[F::Q];
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i)
+        v[i] = get(at<WIDE>(plane_ptr(cur_f, i, pl), SHIFT ? pf.src(F::cx[i], F::cy[i], F::cz[i]) : pf.dst));
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) put(at<WIDE>(plane_ptr(nxt_f, i, pl), pf.dst), v[i]);
+    T va[S::Q], vb[S::Q];
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        const uint32_t src = SHIFT ? ps.src(S::cx[i], S::cy[i], S::cz[i]) : ps.dst;
+        va[i] = get(at<WIDE>(plane_ptr(cur_a, i, pl), src));
+        vb[i] = get(at<WIDE>(plane_ptr(cur_b, i, pl), src));
+    }
+#pragma unroll
+    for (int i = 0; i < S::Q; ++i) {
+        put(at<WIDE>(plane_ptr(nxt_a, i, pl), ps.dst), va[i]);
+        put(at<WIDE>(plane_ptr(nxt_b, i, pl), ps.dst), vb[i]);
+    }
+}
+
 // Residency cap (kCappedWavesPerCu, lbc_launch.hpp), measured on the 256^3
 // TGV with two D3Q7 species.  FP32: scalar step 41.2k -> 45.0k MLUPS,
 // reaction step 30.9k -> 31.8k, fused TRT step 21.0k -> 21.3k (both forms).
@@ -227,6 +270,45 @@ hipError_t fused_by_collision(int collision, int storage, bool fast, const void*
 #undef LBC_FUSED
 }
 
+template <class F, class S, int ST, bool SHIFT>
+hipError_t launch_reactive_pull(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
+                                const LaunchShape& s, hipStream_t stream) {
+    using T = typename Storage<ST>::S;
+    const bool tiled = s.cells.zgap != 0;
+    const uint32_t plane = tiled ? s.cells.zgap / (F::Q - 1) : s.cells.count;
+    const uint32_t zgap_s = tiled ? (S::Q - 1) * plane : 0u;
+    const uint32_t lds = capped_residency_adr<F::D, ST, true>() ? capped_lds(s.threads()) : 0;
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(cf), static_cast<T*>(nf),
+                           static_cast<const T*>(ca), static_cast<T*>(na), static_cast<const T*>(cb),
+                           static_cast<T*>(nb), s.cells, plane, zgap_s);
+    };
+    const bool wide = wide_planes<T>(s);
+    auto pick = [&](auto tl) {
+        constexpr bool TL = decltype(tl)::value;
+        if (s.threads() <= 256)
+            wide ? args(reactive_pull_kernel<F, S, ST, 256, true, TL, SHIFT>)
+                 : args(reactive_pull_kernel<F, S, ST, 256, false, TL, SHIFT>);
+        else
+            wide ? args(reactive_pull_kernel<F, S, ST, 1024, true, TL, SHIFT>)
+                 : args(reactive_pull_kernel<F, S, ST, 1024, false, TL, SHIFT>);
+    };
+    tiled ? pick(std::true_type{}) : pick(std::false_type{});
+    return hipGetLastError();
+}
+
+template <class F, class S, bool SHIFT>
+hipError_t reactive_pull_by_storage(int storage, const void* cf, void* nf, const void* ca, void* na, const void* cb,
+                                    void* nb, const LaunchShape& s, hipStream_t st) {
+    switch (storage) {
+        case kF32: return launch_reactive_pull<F, S, kF32, SHIFT>(cf, nf, ca, na, cb, nb, s, st);
+        case kF64: return launch_reactive_pull<F, S, kF64, SHIFT>(cf, nf, ca, na, cb, nb, s, st);
+        case kF16: return launch_reactive_pull<F, S, kF16, SHIFT>(cf, nf, ca, na, cb, nb, s, st);
+        case kBF16: return launch_reactive_pull<F, S, kBF16, SHIFT>(cf, nf, ca, na, cb, nb, s, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace lbc
 
 extern "C" int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
@@ -337,4 +419,38 @@ extern "C" int lbc_reactive_fused_step(int fluid_lattice, int collision, int sca
     return lbc_reactive_fused_step_ex(fluid_lattice, collision, scalar_lattice, storage, 0u, current_fluid,
                                       next_fluid, current_a, next_a, current_b, next_b, nx, ny, nz, omega, omega_c,
                                       k_react, reference_a, reference_b, block_x, block_y, block_z, stream);
+}
+
+extern "C" int lbc_reactive_pull_stream(int fluid_lattice, int scalar_lattice, int storage, int shift,
+                                        const void* current_fluid, void* next_fluid, const void* current_a,
+                                        void* next_a, const void* current_b, void* next_b, uint64_t nx, uint64_t ny,
+                                        uint64_t nz, uint64_t tile_rows, uint32_t block_x, uint32_t block_y,
+                                        uint32_t block_z, void* stream) {
+    using namespace lbc;
+    if (current_fluid == nullptr || next_fluid == nullptr || current_a == nullptr || next_a == nullptr ||
+        current_b == nullptr || next_b == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_fluid == next_fluid || current_a == next_a || current_b == next_b || next_a == next_b)
+        return hipErrorInvalidValue;
+    const int qf = q_of(fluid_lattice);
+    if (qf == 0 || fluid_lattice == kD2Q5 || fluid_lattice == kD3Q7) return hipErrorInvalidValue;
+    uint64_t plane = 0;
+    e = tile_layout(is_2d(scalar_lattice), tile_rows, qf, s, plane);
+    if (e != hipSuccess) return e;
+    const hipStream_t st = as_stream(stream);
+    auto go = [&](auto f, auto sc) -> hipError_t {
+        using F = decltype(f);
+        using S = decltype(sc);
+        return shift ? reactive_pull_by_storage<F, S, true>(storage, current_fluid, next_fluid, current_a, next_a,
+                                                           current_b, next_b, s, st)
+                     : reactive_pull_by_storage<F, S, false>(storage, current_fluid, next_fluid, current_a, next_a,
+                                                            current_b, next_b, s, st);
+    };
+    if (fluid_lattice == kD2Q9 && scalar_lattice == kD2Q5) return go(D2Q9{}, D2Q5{});
+    if (fluid_lattice == kD3Q19 && scalar_lattice == kD3Q7) return go(D3Q19{}, D3Q7{});
+    if (fluid_lattice == kD3Q27 && scalar_lattice == kD3Q7) return go(D3Q27{}, D3Q7{});
+    return hipErrorInvalidValue;
 }

@@ -311,6 +311,34 @@ def reactive_fused_step_tiled(fluid_lattice, collision, scalar_lattice, cur_f, n
                                               _stream(stream)), "lbc_reactive_fused_step_tiled")
 
 
+def reactive_pull_stream(fluid_lattice, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, tiled=False,
+                         shift=True, block=None, stream=None):
+    """lbc_reactive_pull_stream: the fused step's access pattern without
+    arithmetic (the ceiling of reactive_fused_step on the same buffers), on
+    the reference's planes or (tiled=True) on tensors from to_tiles."""
+    if tiled:
+        nx, ny, nz, rows = _tiled_extents(scalar_lattice, cur_a, "cur_a")
+        sets = ((cur_f, "cur_f", fluid_lattice), (nxt_f, "nxt_f", fluid_lattice), (nxt_a, "nxt_a", scalar_lattice),
+                (cur_b, "cur_b", scalar_lattice), (nxt_b, "nxt_b", scalar_lattice))
+        for t, n, lat in sets:
+            if _tiled_extents(lat, t, n) != (nx, ny, nz, rows) or t.dtype != cur_a.dtype:
+                raise ValueError(f"{n} must match cur_a's tiles and dtype")
+    else:
+        rows = 0
+        _check_pop(fluid_lattice, cur_f, "cur_f")
+        _check_pop(fluid_lattice, nxt_f, "nxt_f")
+        for t, n in ((cur_a, "cur_a"), (nxt_a, "nxt_a"), (cur_b, "cur_b"), (nxt_b, "nxt_b")):
+            _check_pop(scalar_lattice, t, n)
+        _check_set([("cur_f", cur_f), ("nxt_f", nxt_f), ("cur_a", cur_a), ("nxt_a", nxt_a), ("cur_b", cur_b),
+                    ("nxt_b", nxt_b)])
+        nx, ny, nz = _extents(scalar_lattice, cur_a)
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_reactive_pull_stream(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_f.dtype],
+                                         int(bool(shift)), cur_f.data_ptr(), nxt_f.data_ptr(), cur_a.data_ptr(),
+                                         nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(), nx, ny, nz, rows, bx,
+                                         by, bz, _stream(stream)), "lbc_reactive_pull_stream")
+
+
 def reactive_stats_tiled(scalar_lattice, a, b, k_react, stream=None, ref_a=0.0, ref_b=0.0):
     """reactive_stats of species in tiles (to_tiles): lbc_reactive_stats_tiled;
     the same sums in the same order, so the same results bit for bit."""

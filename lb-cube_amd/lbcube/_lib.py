@@ -39,6 +39,8 @@ SIGNATURES = {
     "lbc_reactive_fused_step_tiled": ([c_int, c_int, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64,
                                        c_u64, c_u64, c_u64, c_double, c_double, c_double, c_double, c_double, c_u32,
                                        c_u32, c_u32, c_vp], c_int),
+    "lbc_reactive_pull_stream": ([c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64,
+                                  c_u64, c_u64, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_stats_tiled": ([c_int, c_int, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_double, c_double, c_double,
                                   _dp, c_vp], c_int),
     "lbc_scalar_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double, c_double, c_double,

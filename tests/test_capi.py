@@ -261,5 +261,13 @@ def test_validation_tiles():
     assert relayout(7, 8, 8, 8, 0, 8) == HIP_INVALID_VALUE  # unknown lattice
     assert L.lbc_relayout_tiled(0, 0, FAKE_A, 0, FAKE_A, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # src == dst
     assert L.lbc_relayout_tiled(0, 9, FAKE_A, 0, FAKE_B, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # storage
+    pull = lambda fl, sl, nx, ny, nz, rows, st=0, a=FAKE_A: L.lbc_reactive_pull_stream(  # noqa: E731
+        fl, sl, st, 1, a, FAKE_B, 0x3000, 0x4000, 0x5000, 0x6000, nx, ny, nz, rows, 256, 1, 1, None)
+    assert pull(0, 3, 16, 12, 1, 4, a=None) == HIP_INVALID_DEVICE_POINTER
+    assert pull(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
+    assert pull(1, 4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert pull(3, 3, 16, 12, 1, 0) == HIP_INVALID_VALUE  # scalar lattice as the fluid
+    assert pull(1, 3, 8, 8, 8, 0) == HIP_INVALID_VALUE  # mismatched pair
+    assert pull(0, 3, 16, 12, 1, 4, st=9) == HIP_INVALID_VALUE  # storage
     assert L.lbc_fluid_step_strided(1, 1, 0, 8, FAKE_A, FAKE_B, 8, 8, 8, 0, 19 * 64, 1.0, 256, 1, 1, None) == \
         HIP_INVALID_VALUE  # unknown flag

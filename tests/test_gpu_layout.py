@@ -298,3 +298,36 @@ def test_tiled_fused_step_at_2048_squared(gpu):
                 assert torch.equal(bits(x), bits(y)), (storage, rows)
         del f, a, b, want, got
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5", (12, 13), (0, 4)), ("D3Q19", "D3Q7", (4, 6, 7), (0, 6)),
+                                  ("D3Q27", "D3Q7", (4, 6, 7), (0, 6))])
+def test_reactive_pull_stream_is_the_pull_permutation(gpu, pair):
+    """lbc_reactive_pull_stream, the fused step's pattern ceiling: on the
+    reference's planes and in tiles, every fluid and species population moves
+    by the pull permutation (shift) or stays (no shift)."""
+    fl, sl, shape, row_set = pair
+    qf, qs = lbcube.Q[fl], lbcube.Q[sl]
+    n = int(np.prod(shape))
+    ids = [(1.0 + torch.arange(q * n, dtype=torch.float64, device=gpu) + off).reshape((q,) + shape)
+           for q, off in ((qf, 0), (qs, 1e6), (qs, 2e6))]
+    dims = tuple(range(len(shape)))
+
+    def rolled(x, lattice):
+        vel = lbcube.VELOCITIES[lattice]
+        return torch.stack([torch.roll(x[i], shifts=tuple(reversed(c[:len(shape)])), dims=dims)
+                            for i, c in enumerate(vel)])
+
+    want = [rolled(ids[0], fl), rolled(ids[1], sl), rolled(ids[2], sl)]
+    for rows in row_set:
+        for shift in (True, False):
+            if rows:
+                cur = [lbcube.to_tiles(lat, x, rows) for lat, x in zip((fl, sl, sl), ids)]
+            else:
+                cur = [x.clone() for x in ids]
+            nxt = [torch.empty_like(x) for x in cur]
+            lbcube.reactive_pull_stream(fl, sl, cur[0], nxt[0], cur[1], nxt[1], cur[2], nxt[2], tiled=bool(rows),
+                                        shift=shift)
+            got = [lbcube.from_tiles(lat, x) if rows else x for lat, x in zip((fl, sl, sl), nxt)]
+            for g, w, x in zip(got, want, ids):
+                assert torch.equal(g, w if shift else x), (rows, shift)
