@@ -212,7 +212,8 @@ template <class F, int CT, class S, int ST, bool FAST>
 hipError_t launch_fused(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
                         const LaunchShape& s, double omega, double omega_c, double k, double ra, double rb,
                         hipStream_t stream) {
-    const bool tiled = s.cells.zgap != 0;
+    // one 2D tile holding every row is the reference's planes
+    const bool tiled = s.cells.zgap != 0 && (F::D == 3 || s.cells.tiles > 1);
     const uint32_t plane = tiled ? s.cells.zgap / (F::Q - 1) : s.cells.count;  // a tile's cells, or all cells
     const uint32_t zgap_s = tiled ? (S::Q - 1) * plane : 0u;
     using St = Storage<ST>;
@@ -274,7 +275,7 @@ template <class F, class S, int ST, bool SHIFT>
 hipError_t launch_reactive_pull(const void* cf, void* nf, const void* ca, void* na, const void* cb, void* nb,
                                 const LaunchShape& s, hipStream_t stream) {
     using T = typename Storage<ST>::S;
-    const bool tiled = s.cells.zgap != 0;
+    const bool tiled = s.cells.zgap != 0 && (F::D == 3 || s.cells.tiles > 1);
     const uint32_t plane = tiled ? s.cells.zgap / (F::Q - 1) : s.cells.count;
     const uint32_t zgap_s = tiled ? (S::Q - 1) * plane : 0u;
     const uint32_t lds = capped_residency_adr<F::D, ST, true>() ? capped_lds(s.threads()) : 0;
