@@ -519,6 +519,69 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
     return out
 
 
+def shard_rate(lbcube, torch, u, omega, steps=50, warmup=5, gpus=8):
+    """The steps at one GPU's share of a z-slab (2D: y-slab) decomposition
+    over `gpus` GPUs: D3Q19 TRT at 256 x 256 x (256 / gpus) (the first slabs
+    of the headline's velocity field), FP32 and FP16 deviation form, on the
+    reference's planes and in the slab order; and the fused 2D reactive MRT
+    step at 2048 x (2048 / gpus).  Set against each step, the time the halo
+    exchange of that shard would take on one xGMI link (~153 GB/s): per face
+    the populations entering across it (D3Q19: 5 per cell of the face; D2Q9
+    + 2 D2Q5: 3 + 1 + 1 per cell of the edge row)."""
+    from lbcube import init as lbinit
+
+    out = {}
+    n = u.shape[1]
+    nz = n // gpus
+    us = u[:, :nz].contiguous()
+    link = 153e9
+    for storage, fast in (("f32", False), ("f16", True)):
+        for layout in ("planes", "slabs"):
+            r = time_variant(lbcube, torch, "D3Q19", "TRT", storage, us, omega, steps, warmup, fast=fast,
+                             layout=layout)
+            es = 4 if storage == "f32" else 2
+            halo_us = 5 * n * n * es / link * 1e6  # one face
+            key = f"d3q19_{n}x{n}x{nz}_trt_{storage}{'fast' if fast else ''}_{layout}"
+            out[key] = {"mlups": r["mlups"], "achieved_gbs": r["achieved_gbs"], "roofline_frac": r["roofline_frac"],
+                        "step_us": r["ms_per_step"] * 1e3, "halo_face_us_one_link": halo_us}
+            print(f"# shard {key}: {r['mlups']:.0f} MLUPS, {r['achieved_gbs']:.0f} GB/s, step "
+                  f"{r['ms_per_step'] * 1e3:.1f} us, halo per face {halo_us:.1f} us", file=sys.stderr, flush=True)
+    del us
+    m = 2048
+    ny = m // gpus
+    uu = lbinit.double_shear_layer(m, u0=0.04, delta=80.0)[:, :ny].contiguous()
+    ca, cb = (c[:ny].contiguous() for c in lbinit.tanh_half_planes(m, delta=80.0))
+    k = 10.0 * 0.04 / 80.0
+    for storage, fast, ref in (("f32", False, 0.0), ("f16", True, 0.5)):
+        st = [lbinit.equilibrium("D2Q9", None, uu, storage)]
+        st.append(torch.empty_like(st[0]))
+        for c in (ca, cb):
+            st.append(lbinit.scalar_equilibrium("D2Q5", c, uu, storage, ref))
+            st.append(torch.empty_like(st[-1]))
+
+        def fused(st=st, fast=fast, ref=ref):
+            f, ft, a, at, b, bt = st
+            lbcube.reactive_fused_step("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k, fast=fast,
+                                       ref_a=ref, ref_b=ref)
+            st[:] = [ft, f, at, a, bt, b]
+
+        for _ in range(4):
+            fused()
+        ms = _graph_ms(torch, fused, 100)
+        es = st[0].element_size()
+        halo_us = 5 * m * es / link * 1e6  # one edge row
+        key = f"c3_{m}x{ny}_mrt_{storage}{'fast' if fast else ''}"
+        out[key] = {"mlups": m * ny / (ms * 1e-3) / 1e6, "achieved_gbs": 38 * es * m * ny / (ms * 1e-3) / 1e9,
+                    "step_us": ms * 1e3, "halo_face_us_one_link": halo_us}
+        print(f"# shard {key}: {out[key]['mlups']:.0f} MLUPS, step {ms * 1e3:.1f} us, halo per face {halo_us:.2f} us",
+              file=sys.stderr, flush=True)
+        del st
+    torch.cuda.empty_cache()
+    out["note"] = (f"one GPU's share of a {gpus}-way slab decomposition; periodic in the slab direction here "
+                   "(the halo exchange is estimated, not run: one GPU)")
+    return out
+
+
 def reactive_2d_rate(lbcube, torch, n=2048, steps=100):
     """2048^2 D2Q9 MRT double shear layer + two D2Q5 species (Da = 10), FP32
     and FP64: the fused single-pass step (152 / 304 B per cell), the reference's
@@ -799,7 +862,7 @@ def main():
                     help="coll_storage[fast][_slabs]: _slabs stores the populations slab by slab, [Z][Q][Y][X]")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
-    ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large",
+    ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large,shard",
                     help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -866,6 +929,10 @@ def main():
             extras["lattices"] = lattice_sweep(lbcube, torch)
         if "large" in chosen:
             extras["large"] = large_domain_rate(lbcube, torch)
+        if "shard" in chosen:
+            u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
+            extras["shard"] = shard_rate(lbcube, torch, u, omega)
+            del u
         u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
     cpu = None
     if not args.no_cpu_baseline:
