@@ -151,9 +151,13 @@ def slab_strides(lattice, extents):
     return nx * ny, Q[lattice] * nx * ny
 
 
-def _check_strided(lattice, t, name, extents, plane_stride, slab_stride):
+def _check_storage(t, name):
     if not (t.is_cuda and t.is_contiguous()) or t.dtype not in STORAGE:
         raise ValueError(f"{name} must be a contiguous CUDA tensor of a storage dtype")
+
+
+def _check_strided(lattice, t, name, extents, plane_stride, slab_stride):
+    _check_storage(t, name)
     nx, ny, nz = extents
     slab = nx * ny
     if slab_stride in (0, slab):
@@ -240,8 +244,7 @@ def from_slabs(lattice, h, stream=None):
 def _tiled_extents(lattice, t, name):
     """(nx, ny, nz, rows) of a tiled population tensor: (tiles, Q, rows, nx)
     in 2D, (nz, Q, ny, nx) in 3D (the slab order: a tile is an xy-slab)."""
-    if not (t.is_cuda and t.is_contiguous()) or t.dtype not in STORAGE:
-        raise ValueError(f"{name} must be a contiguous CUDA tensor of a storage dtype")
+    _check_storage(t, name)
     if t.dim() != 4 or t.shape[1] != Q[lattice]:
         raise ValueError(f"{name} must have shape (tiles, {Q[lattice]}, rows, nx)")
     tiles, _, rows, nx = (int(v) for v in t.shape)

@@ -29,8 +29,13 @@ def cpu_wrappers(monkeypatch):
         if t.dim() != lbcube.DIM[lattice] + 1 or t.shape[0] != lbcube.Q[lattice]:
             raise ValueError(f"{name}: bad population shape")
 
+    def check_storage(t, name):  # _check_storage without the CUDA placement
+        if not t.is_contiguous() or t.dtype not in lbcube.STORAGE:
+            raise ValueError(f"{name} must be a contiguous tensor of a storage dtype")
+
     stub = _NoKernel()
     monkeypatch.setattr(lbcube, "_check_pop", check_pop)
+    monkeypatch.setattr(lbcube, "_check_storage", check_storage)
     monkeypatch.setattr(lbcube, "lib", lambda: stub)
     monkeypatch.setattr(adjoint, "_check_pop", check_pop)
     monkeypatch.setattr(adjoint, "lib", lambda: stub)
@@ -93,3 +98,36 @@ def test_gradient_species_shape(cpu_wrappers):
     with pytest.raises(ValueError, match="storage format"):
         adjoint.reactive_gradient("BGK", f, pops(5, (64, 64)), pops(5, (64, 64), torch.float64), 8, 1.0, 1.0, 0.1,
                                   block=(256, 1))
+
+
+def test_strided_and_tiled_layout_mismatches(cpu_wrappers):
+    """The strided, slab-order and tiled wrappers refuse buffers too small for
+    their layout, pairs of different shape or dtype, tiles of another height,
+    and rows that do not divide ny, before any kernel is reached."""
+    q = 19
+    with pytest.raises(ValueError, match="the layout needs"):  # 19 planes of 4 x 4 in slabs of 4: 1216 elements
+        lbcube.fluid_step_strided("D3Q19", "TRT", torch.zeros(1215), torch.zeros(1216), (4, 4, 4), 16, 19 * 16, 1.0,
+                                  block=(64, 1, 1))
+    with pytest.raises(ValueError, match="share a shape"):
+        lbcube.fluid_step_slabs("D3Q19", "TRT", torch.zeros(4, q, 4, 4), torch.zeros(5, q, 4, 4), 1.0,
+                                block=(64, 1, 1))
+    with pytest.raises(ValueError, match="shape"):
+        lbcube.fluid_step_slabs("D3Q19", "TRT", torch.zeros(4, 9, 4, 4), torch.zeros(4, 9, 4, 4), 1.0,
+                                block=(64, 1, 1))
+    with pytest.raises(ValueError, match="rows must divide"):
+        lbcube.to_tiles("D2Q9", pops(9, (12, 8)), rows=5)
+    f, a = torch.zeros(3, 9, 4, 8), torch.zeros(3, 5, 4, 8)
+    with pytest.raises(ValueError, match="match cur_a"):  # a fluid in tiles of 2 rows against species in 4
+        lbcube.reactive_fused_step_tiled("D2Q9", "MRT", "D2Q5", torch.zeros(6, 9, 2, 8), torch.zeros(6, 9, 2, 8), a,
+                                         a.clone(), a.clone(), a.clone(), 1.0, 1.0, 0.1, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="match cur_a"):
+        lbcube.reactive_fused_step_tiled("D2Q9", "MRT", "D2Q5", f, f.clone(), a, a.clone(), a.clone(),
+                                         a.to(torch.float16), 1.0, 1.0, 0.1, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="must have shape"):
+        lbcube.reactive_fused_step_tiled("D2Q9", "MRT", "D2Q5", a, a.clone(), a, a.clone(), a.clone(), a.clone(),
+                                         1.0, 1.0, 0.1, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="match cur_a"):
+        lbcube.reactive_pull_stream("D2Q9", "D2Q5", f, f.clone(), a, a.clone(), a.clone(), torch.zeros(2, 5, 6, 8),
+                                    tiled=True, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="match a's tiles"):
+        lbcube.reactive_stats_tiled("D2Q5", a, torch.zeros(6, 5, 2, 8), 0.1)
