@@ -519,6 +519,41 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
     return out
 
 
+def tiles_3d_rate(lbcube, torch, n=512, steps=10, warmup=3, row_set=(512, 64, 16)):
+    """D3Q19 TRT at 512^3, FP32 and FP16 deviation form, on the reference's
+    planes and in 3D tiles of 512 (the slab order), 64 and 16 rows
+    (lbc_fluid_step_tiled), converted once before the timing."""
+    from lbcube import init as lbinit
+
+    u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
+    omega = 1.0 / lbinit.tgv_re1600_tau(n)
+    out = {"workload": f"D3Q19 TRT periodic TGV {n}^3, Re=1600, seeded perturbation"}
+    for storage, fast in (("f32", False), ("f16", True)):
+        r = time_variant(lbcube, torch, "D3Q19", "TRT", storage, u, omega, steps, warmup, fast=fast)
+        line = {"planes": r["mlups"]}
+        for rows in row_set:
+            a = lbcube.to_tiles("D3Q19", lbinit.equilibrium("D3Q19", None, u, storage), rows)
+            b = torch.empty_like(a)
+            bufs = [a, b]
+            del a, b
+
+            def step(bufs=bufs, fast=fast):
+                lbcube.fluid_step_tiled("D3Q19", "TRT", bufs[0], bufs[1], omega, fast=fast)
+                bufs.reverse()
+
+            ms = _events_ms(torch, step, steps, warmup)
+            line[f"rows{rows}"] = n ** 3 / (ms * 1e-3) / 1e6
+            del bufs
+            torch.cuda.empty_cache()
+        key = f"trt_{storage}{'fast' if fast else ''}"
+        out[key] = {k: round(v, 1) for k, v in line.items()}
+        print(f"# tiles 3D {n}^3 {key}: " + ", ".join(f"{k} {v:.0f}" for k, v in line.items()) + " MLUPS",
+              file=sys.stderr, flush=True)
+    del u
+    torch.cuda.empty_cache()
+    return out
+
+
 def shard_rate(lbcube, torch, u, omega, steps=50, warmup=5, gpus=8):
     """The steps at one GPU's share of a z-slab (2D: y-slab) decomposition
     over `gpus` GPUs: D3Q19 TRT at 256 x 256 x (256 / gpus) (the first slabs
@@ -862,7 +897,7 @@ def main():
                     help="coll_storage[fast][_slabs]: _slabs stores the populations slab by slab, [Z][Q][Y][X]")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")
-    ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large,shard",
+    ap.add_argument("--extras", default="pitched,adjoint,reactive_2d,adr_3d,ensemble,lattices,large,tiles_3d,shard",
                     help="which configuration lines to run (comma-separated subset)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -929,6 +964,8 @@ def main():
             extras["lattices"] = lattice_sweep(lbcube, torch)
         if "large" in chosen:
             extras["large"] = large_domain_rate(lbcube, torch)
+        if "tiles_3d" in chosen:
+            extras["tiles_3d"] = tiles_3d_rate(lbcube, torch)
         if "shard" in chosen:
             u = lbinit.taylor_green_3d(n, 0.05, device="cuda", perturbation_rms=0.01 * 0.05, seed=0x5EED1234)
             extras["shard"] = shard_rate(lbcube, torch, u, omega)

@@ -170,15 +170,14 @@ int lbc_relayout(int lattice, int storage, const void* src, uint64_t src_plane_s
                  void* dst, uint64_t dst_plane_stride, uint64_t dst_slab_stride, uint64_t nx, uint64_t ny,
                  uint64_t nz, void* stream);
 
-/* Tiles.  A population buffer in tiles of tile_rows rows holds, tile after
- * tile, the Q planes of the tile's cells back to back: population i of cell
- * (x, y, z) lies at
+/* Tiles.  A population buffer in tiles of tile_rows rows (tile_rows divides
+ * ny) holds, tile after tile, the Q planes of the tile's cells back to back:
+ * population i of cell (x, y, z) lies at
  *   t * Q * T + i * T + ((z * ny + y) * nx + x - t * T)
- * with T = tile_rows * nx the cells of a tile and t the cell's tile:
- *  - 3D: tile_rows = ny, t = z: the slab order [Z][Q][Y][X]
- *    (lbc_fluid_step_strided with plane_stride nx * ny and slab_stride
- *    Q * nx * ny);
- *  - 2D: tile_rows divides ny, t = y / tile_rows: [ny / tile_rows][Q][tile_rows][X].
+ * with T = tile_rows * nx the cells of a tile and t = (z * ny + y) / tile_rows
+ * the cell's tile: [nz][ny / tile_rows][Q][tile_rows][X].  In 3D tile_rows =
+ * ny is the slab order [Z][Q][Y][X] (lbc_fluid_step_strided with
+ * plane_stride nx * ny and slab_stride Q * nx * ny); in 2D nz = 1.
  * tile_rows = 0 selects the reference's planes.  Other values, or buffers of
  * 2^32 elements or more (Q times the cells), -> hipErrorInvalidValue.  The
  * kernels give every cell the result of the reference's planes bit for bit;

@@ -340,8 +340,8 @@ hipError_t plane_layout(int lattice, LaunchShape& s, uint64_t plane_stride, uint
 }
 
 // One side of a layout conversion: population i of cell (x, y, z) at element
-// i * plane + (z ny + y) nx + x + t * zgap, with t = z in 3D and y / rows in
-// 2D (the tile of Cells).
+// i * plane + (z ny + y) nx + x + t * zgap, with t = (z ny + y) / rows (the
+// tile of Cells; rows = ny for the strided layouts, whose tiles are slabs).
 struct LayoutSide {
     uint64_t plane, zgap;
     FastDiv by_rows;
@@ -358,9 +358,8 @@ __global__ __launch_bounds__(256) void relayout_kernel(const T* __restrict__ src
     const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
     const CellCoord p = cell_coord(cell, c);
-    const bool two_d = c.nz == 1;
-    const uint64_t ts = two_d ? fast_div(static_cast<uint32_t>(p.y), from.by_rows) : static_cast<uint64_t>(p.z);
-    const uint64_t td = two_d ? fast_div(static_cast<uint32_t>(p.y), to.by_rows) : static_cast<uint64_t>(p.z);
+    const uint32_t row = static_cast<uint32_t>(p.z) * c.ny + static_cast<uint32_t>(p.y);
+    const uint64_t ts = fast_div(row, from.by_rows), td = fast_div(row, to.by_rows);
     const uint64_t a = cell + ts * from.zgap, b = cell + td * to.zgap;
     for (int i = 0; i < q; ++i)
         dst[static_cast<uint64_t>(i) * to.plane + b] = src[static_cast<uint64_t>(i) * from.plane + a];

@@ -53,13 +53,14 @@ __host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
 // kernels map work-item p of the launch to cell p, so every wave covers 64
 // consecutive cells whatever nx and the block shape are (no idle lanes at row
 // ends beyond the last wave of the domain).
-// Tiles.  A population buffer may be stored in tiles: xy-slabs in 3D, runs of
-// `rows` rows in 2D (rows divides ny).  zgap: elements between the end of one
-// tile of a population plane and the start of the next (0 in the reference's
-// [Q][Z][Y][X] planes; (Q - 1) x the tile's cells when each tile holds its Q
-// planes back to back, e.g. the slab order [Z][Q][Y][X]).  Cell (x, y, z) of
-// tile t (z in 3D, y / rows in 2D) lies at element (z ny + y) nx + x + t zgap
-// of its population plane.  Without tiles rows = ny and tiles = 1.
+// Tiles.  A population buffer may be stored in tiles of `rows` rows (rows
+// divides ny; rows = ny in 3D: whole xy-slabs), `tiles` = ny / rows of them
+// per xy-slab.  zgap: elements between the end of one tile of a population
+// plane and the start of the next (0 in the reference's [Q][Z][Y][X] planes;
+// (Q - 1) x the tile's cells when each tile holds its Q planes back to back,
+// e.g. the slab order [Z][Q][Y][X]).  Cell (x, y, z) of tile
+// t = (z ny + y) / rows lies at element (z ny + y) nx + x + t zgap of its
+// population plane.  Without tiles rows = ny and tiles = 1.
 struct Cells {
     uint32_t nx, ny, nz, count;
     FastDiv by_nx, by_ny;
@@ -80,26 +81,27 @@ __device__ __forceinline__ CellCoord cell_coord(uint32_t p, const Cells& c) {
 
 // The pull sources of destination cell p at coordinates q, for planes whose
 // tiles are spaced `zgap` elements apart (Cells); pull.dst is the cell's
-// element offset within its plane.  3D: the tile is z, and z steps move
-// nx ny + zgap.  2D: the tile is y / rows, and a y step that crosses a tile
-// boundary moves zgap further (the wrap from the last row to the first crosses
-// all tiles).  zgap = 0 and one tile give the reference's offsets.
+// element offset within its plane.  A z step moves nx ny + tiles zgap; a y
+// step that crosses a tile boundary moves zgap further (the wrap from the
+// last row of a slab to its first crosses tiles - 1 of them).  zgap = 0 and
+// one tile per slab give the reference's offsets; the y-tile arithmetic runs
+// only when a slab holds several tiles (a wave-uniform branch).
 template <int D>
 __device__ __forceinline__ Pull locate(uint32_t p, const CellCoord& q, const Cells& c, uint32_t zgap) {
-    if constexpr (D == 3) {
-        return Pull(p + static_cast<uint32_t>(q.z) * zgap, q.x, q.y, q.z, static_cast<int>(c.nx),
-                    static_cast<int>(c.ny), static_cast<int>(c.nz), zgap);
-    } else {
+    const uint32_t slab_gap = c.tiles * zgap;  // the gaps of one xy-slab's tiles
+    Pull pull(p + static_cast<uint32_t>(q.z) * slab_gap, q.x, q.y, q.z, static_cast<int>(c.nx),
+              static_cast<int>(c.ny), static_cast<int>(c.nz), slab_gap);
+    if (c.tiles > 1) {
         const uint32_t y = static_cast<uint32_t>(q.y);
         const uint32_t t = fast_div(y, c.by_rows);
         const uint32_t r = y - t * c.rows;
-        Pull pull(p + t * zgap, q.x, q.y, 0, static_cast<int>(c.nx), static_cast<int>(c.ny), 1);
         const uint32_t row = c.nx;
         const uint32_t last = (c.tiles - 1u) * zgap + row * (c.ny - 1u);  // row 0 -> row ny - 1
+        pull.dst += t * zgap;
         pull.ym = y == 0 ? last : (r == 0 ? 0u - row - zgap : 0u - row);
         pull.yp = y + 1 == c.ny ? 0u - last : (r + 1 == c.rows ? row + zgap : row);
-        return pull;
     }
+    return pull;
 }
 
 struct LaunchShape {
@@ -126,7 +128,7 @@ constexpr uint64_t kMaxBlocks = (1ull << 31) - 1;
 // Elements a lane's offset spans within a population plane: the cell count,
 // or more when the slabs are spaced apart (zgap).
 inline uint64_t lane_extent(const Cells& c) {
-    return static_cast<uint64_t>(c.count) + static_cast<uint64_t>(c.nz - 1 + c.tiles - 1) * c.zgap;
+    return static_cast<uint64_t>(c.count) + (static_cast<uint64_t>(c.nz) * c.tiles - 1) * c.zgap;
 }
 
 // Population planes of more than 4 GiB take 64-bit per-lane addresses (the
@@ -169,20 +171,21 @@ inline hipError_t make_shape(bool two_d, uint64_t nx, uint64_t ny, uint64_t nz, 
 // Dense tiles of `rows` rows for population buffers: 0 = the reference's
 // planes (plane = cells, zgap = 0); otherwise each tile holds the Q planes of
 // its cells back to back, so the plane stride is the tile's cell count and
-// zgap = (Q - 1) x that.  3D: rows = ny only (tiles are xy-slabs: the slab
-// order [Z][Q][Y][X]); 2D: any divisor of ny.  Sets the shape's tile count
-// and returns the plane stride; zgap_of(q) gives each buffer's gap.  The
-// lanes' offsets must stay below 2^32 elements for the largest Q.
+// zgap = (Q - 1) x that.  rows divides ny (rows = ny in 3D: the slab order
+// [Z][Q][Y][X]).  Sets the shape's tiles and the largest Q's gap (a buffer of
+// q planes has gap (q - 1) plane) and returns the plane stride.  The lanes'
+// offsets must stay below 2^32 elements for the largest Q.
 inline hipError_t tile_layout(bool two_d, uint64_t rows, int q_max, LaunchShape& s, uint64_t& plane) {
+    (void)two_d;
     Cells& c = s.cells;
     if (rows == 0) {
         plane = c.count;
         return hipSuccess;
     }
-    if (two_d ? (c.ny % rows != 0) : rows != c.ny) return hipErrorInvalidValue;
+    if (c.ny % rows != 0) return hipErrorInvalidValue;
     plane = static_cast<uint64_t>(c.nx) * rows;
     c.rows = static_cast<uint32_t>(rows);
-    c.tiles = two_d ? static_cast<uint32_t>(c.ny / rows) : 1u;
+    c.tiles = static_cast<uint32_t>(c.ny / rows);
     c.by_rows = make_fastdiv(static_cast<uint32_t>(rows));
     if (static_cast<uint64_t>(q_max) * c.count > 0xFFFFFFFFull) return hipErrorInvalidValue;
     c.zgap = static_cast<uint32_t>((q_max - 1) * plane);

@@ -243,10 +243,15 @@ def from_slabs(lattice, h, stream=None):
 
 def _tiled_extents(lattice, t, name):
     """(nx, ny, nz, rows) of a tiled population tensor: (tiles, Q, rows, nx)
-    in 2D, (nz, Q, ny, nx) in 3D (the slab order: a tile is an xy-slab)."""
+    in 2D; in 3D (nz, Q, ny, nx) for the slab order (a tile is an xy-slab) or
+    (nz, ny / rows, Q, rows, nx) for tiles of fewer rows."""
     _check_storage(t, name)
+    if DIM[lattice] == 3 and t.dim() == 5 and t.shape[2] == Q[lattice]:
+        nz, tiles, _, rows, nx = (int(v) for v in t.shape)
+        return nx, tiles * rows, nz, rows
     if t.dim() != 4 or t.shape[1] != Q[lattice]:
-        raise ValueError(f"{name} must have shape (tiles, {Q[lattice]}, rows, nx)")
+        raise ValueError(f"{name} must have shape (tiles, {Q[lattice]}, rows, nx)"
+                         + (f" or (nz, tiles, {Q[lattice]}, rows, nx)" if DIM[lattice] == 3 else ""))
     tiles, _, rows, nx = (int(v) for v in t.shape)
     if DIM[lattice] == 2:
         return nx, tiles * rows, 1, rows
@@ -254,19 +259,18 @@ def _tiled_extents(lattice, t, name):
 
 
 def to_tiles(lattice, f, rows=None, stream=None):
-    """Populations of shape (Q, [nz,] ny, nx) -> a new tensor in tiles
-    (lbc_relayout_tiled): 2D (ny / rows, Q, rows, nx), each tile holding the Q
-    planes of its rows back to back; 3D (nz, Q, ny, nx), the slab order
-    (rows = ny)."""
+    """Populations of shape (Q, [nz,] ny, nx) -> a new tensor in tiles of
+    `rows` rows (lbc_relayout_tiled), each tile holding the Q planes of its
+    rows back to back: 2D (ny / rows, Q, rows, nx); 3D (nz, Q, ny, nx) for the
+    slab order (rows = ny, the default) or (nz, ny / rows, Q, rows, nx)."""
     _check_pop(lattice, f, "f")
     nx, ny, nz = _extents(lattice, f)
+    rows = ny if (rows is None and DIM[lattice] == 3) else int(rows)
+    if rows <= 0 or ny % rows:
+        raise ValueError(f"rows must divide ny = {ny}")
     if DIM[lattice] == 3:
-        rows = ny if rows is None else int(rows)
-        shape = (nz, Q[lattice], ny, nx)
+        shape = (nz, Q[lattice], ny, nx) if rows == ny else (nz, ny // rows, Q[lattice], rows, nx)
     else:
-        rows = int(rows)
-        if rows <= 0 or ny % rows:
-            raise ValueError(f"rows must divide ny = {ny}")
         shape = (ny // rows, Q[lattice], rows, nx)
     out = torch.empty(shape, dtype=f.dtype, device=f.device)
     check(lib().lbc_relayout_tiled(LATTICES[lattice], STORAGE[f.dtype], f.data_ptr(), 0, out.data_ptr(), rows, nx,

@@ -230,7 +230,7 @@ def test_validation_strided_layouts():
 
 
 def test_validation_tiles():
-    """Tiles: 2D rows must divide ny, 3D rows must be ny (the slab order);
+    """Tiles: rows must divide ny (in 3D, rows = ny is the slab order);
     Q times the cells must stay below 2^32; scalar lattices are no fluid;
     checks come after the reference's pointer / block / extent order."""
     L = lbcube.lib()
@@ -247,17 +247,17 @@ def test_validation_tiles():
     assert fluid(0, 16, 12, 1, 4, a=None) == HIP_INVALID_DEVICE_POINTER
     assert fluid(0, 16, 12, 1, 5) == HIP_INVALID_VALUE  # 5 does not divide 12
     assert fluid(0, 16, 12, 1, 24) == HIP_INVALID_VALUE
-    assert fluid(1, 8, 8, 8, 4) == HIP_INVALID_VALUE  # 3D tiles are whole slabs
+    assert fluid(1, 8, 8, 8, 3) == HIP_INVALID_VALUE  # 3 does not divide ny = 8 (3D)
     assert fluid(3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a scalar lattice
     assert fluid(1, 1 << 12, 1 << 10, 1 << 7, 1 << 10) == HIP_INVALID_VALUE  # 19 x 2^29 elements
     assert fused(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
-    assert fused(1, 4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert fused(1, 4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert fused(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # scalar lattice as the fluid
     assert stats(3, 16, 12, 1, 5) == HIP_INVALID_VALUE
-    assert stats(4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert stats(4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert stats(0, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a fluid lattice
     assert relayout(0, 16, 12, 1, 0, 5) == HIP_INVALID_VALUE
-    assert relayout(4, 8, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert relayout(4, 8, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert relayout(7, 8, 8, 8, 0, 8) == HIP_INVALID_VALUE  # unknown lattice
     assert L.lbc_relayout_tiled(0, 0, FAKE_A, 0, FAKE_A, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # src == dst
     assert L.lbc_relayout_tiled(0, 9, FAKE_A, 0, FAKE_B, 4, 16, 12, 1, None) == HIP_INVALID_VALUE  # storage
@@ -265,7 +265,7 @@ def test_validation_tiles():
         fl, sl, st, 1, a, FAKE_B, 0x3000, 0x4000, 0x5000, 0x6000, nx, ny, nz, rows, 256, 1, 1, None)
     assert pull(0, 3, 16, 12, 1, 4, a=None) == HIP_INVALID_DEVICE_POINTER
     assert pull(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
-    assert pull(1, 4, 8, 8, 8, 4) == HIP_INVALID_VALUE
+    assert pull(1, 4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert pull(3, 3, 16, 12, 1, 0) == HIP_INVALID_VALUE  # scalar lattice as the fluid
     assert pull(1, 3, 8, 8, 8, 0) == HIP_INVALID_VALUE  # mismatched pair
     assert pull(0, 3, 16, 12, 1, 4, st=9) == HIP_INVALID_VALUE  # storage

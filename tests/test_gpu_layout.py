@@ -331,3 +331,47 @@ def test_reactive_pull_stream_is_the_pull_permutation(gpu, pair):
             got = [lbcube.from_tiles(lat, x) if rows else x for lat, x in zip((fl, sl, sl), nxt)]
             for g, w, x in zip(got, want, ids):
                 assert torch.equal(g, w if shift else x), (rows, shift)
+
+
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("lattice", LATTICES3)
+def test_3d_tiles_of_rows_match_oracle(gpu, lattice, storage):
+    """3D tiles of fewer rows than a slab (ny = 6 in tiles of 1, 2 and 3
+    rows): y steps across tile boundaries, z steps across whole slabs of
+    tiles; the reference-order step equals the oracle bit for bit and the
+    deviation form equals its run on the planes."""
+    print(f"case 3d tiles {lattice} {storage}", flush=True)
+    f0 = cases.random_perturbed(lattice, (4, 6, 7), storage, seed=23)
+    want = O.fluid_run(lattice, "TRT", storage, f0, 1.0 / 0.62, 4)
+    f = to_dev(f0, storage, gpu)
+    for rows in (1, 2, 3):
+        got = to_host(run_tiled_fluid(lattice, "TRT", f, 1.0 / 0.62, 4, rows), storage)
+        assert np.array_equal(got, want), rows
+        fast = run_tiled_fluid(lattice, "MRT", f, 1.0 / 0.6, 3, rows, fast=True)
+        assert torch.equal(bits(fast), bits(run_planes(lattice, "MRT", f, 1.0 / 0.6, 3, fast=True))), rows
+
+
+def test_3d_tiles_fused_and_stats_match_planes(gpu):
+    """The fused reactive step and the statistics in 3D tiles of 2 and 3 rows
+    (D3Q19 + two D3Q7) equal their runs on the planes bit for bit."""
+    shape = (4, 6, 7)
+    for storage in ("f32", "f16"):
+        refs = (0.5, 0.25) if storage == "f16" else (0.0, 0.0)
+        f = to_dev(cases.random_perturbed("D3Q19", shape, storage, seed=2), storage, gpu)
+        a0, b0 = _species("D3Q7", shape, storage, seed=8, refs=refs)
+        a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+        for fast in (False, True):
+            want = run_fused("D3Q19", "TRT", "D3Q7", f, a, b, 3, fast, refs)
+            for rows in (2, 3):
+                got = run_fused("D3Q19", "TRT", "D3Q7", f, a, b, 3, fast, refs, rows=rows)
+                for x, y in zip(got, want):
+                    assert torch.equal(bits(x), bits(y)), (storage, fast, rows)
+        want = lbcube.reactive_stats("D3Q7", a, b, 0.03, ref_a=refs[0], ref_b=refs[1])
+        for rows in (2, 3):
+            got = lbcube.reactive_stats_tiled("D3Q7", lbcube.to_tiles("D3Q7", a, rows), lbcube.to_tiles("D3Q7", b, rows),
+                                              0.03, ref_a=refs[0], ref_b=refs[1])
+            assert got == want, (storage, rows)
+        t = lbcube.to_tiles("D3Q19", f, 2)
+        assert t.shape == (4, 3, 19, 2, 7)
+        want_t = f.reshape(19, 4, 3, 2, 7).permute(1, 2, 0, 3, 4).contiguous()
+        assert torch.equal(bits(t), bits(want_t))
