@@ -54,8 +54,8 @@ __host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, FastDiv f) {
 // consecutive cells whatever nx and the block shape are (no idle lanes at row
 // ends beyond the last wave of the domain).
 // Tiles.  A population buffer may be stored in tiles of `rows` rows (rows
-// divides ny; rows = ny in 3D: whole xy-slabs), `tiles` = ny / rows of them
-// per xy-slab.  zgap: elements between the end of one tile of a population
+// divides ny; rows = ny makes each xy-slab one tile), `tiles` = ny / rows of
+// them per xy-slab.  zgap: elements between the end of one tile of a population
 // plane and the start of the next (0 in the reference's [Q][Z][Y][X] planes;
 // (Q - 1) x the tile's cells when each tile holds its Q planes back to back,
 // e.g. the slab order [Z][Q][Y][X]).  Cell (x, y, z) of tile
