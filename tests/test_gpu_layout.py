@@ -197,11 +197,11 @@ def test_relayout_tiled_is_the_tile_permutation(gpu):
                 assert torch.equal(bits(lbcube.from_tiles(lattice, t)), bits(f))
 
 
-def run_tiled_fluid(lattice, coll, f, omega, steps, rows, fast=False, block=None):
+def run_tiled_fluid(lattice, coll, f, omega, steps, rows, fast=False, block=None, mfma=False):
     a = lbcube.to_tiles(lattice, f, rows)
     b = torch.empty_like(a)
     for _ in range(steps):
-        lbcube.fluid_step_tiled(lattice, coll, a, b, omega, fast=fast, block=block)
+        lbcube.fluid_step_tiled(lattice, coll, a, b, omega, fast=fast, block=block, mfma=mfma)
         a, b = b, a
     return lbcube.from_tiles(lattice, a)
 
@@ -349,6 +349,10 @@ def test_3d_tiles_of_rows_match_oracle(gpu, lattice, storage):
         assert np.array_equal(got, want), rows
         fast = run_tiled_fluid(lattice, "MRT", f, 1.0 / 0.6, 3, rows, fast=True)
         assert torch.equal(bits(fast), bits(run_planes(lattice, "MRT", f, 1.0 / 0.6, 3, fast=True))), rows
+        if storage in ("f16", "bf16"):  # the matrix-core MRT through the same tile addressing
+            mf = run_tiled_fluid(lattice, "MRT", f, 1.0 / 0.6, 3, rows, fast=True, mfma=True, block=(64, 1, 1))
+            want_mf = run_planes(lattice, "MRT", f, 1.0 / 0.6, 3, fast=True, mfma=True, block=(64, 1, 1))
+            assert torch.equal(bits(mf), bits(want_mf)), rows
 
 
 def test_3d_tiles_fused_and_stats_match_planes(gpu):
