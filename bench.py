@@ -879,9 +879,15 @@ def cpu_baseline(u_host, omega, n):
     O.fluid_step("D3Q19", "TRT", "f32", f, omega, threads=1)
     t_one = time.perf_counter() - t0
     cells = n ** 3
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": cells / t_all / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"1 step of the D3Q19 TRT FP32 {n}^3 TGV, oracle, {threads} OpenMP threads",
-            "single_thread_value": cells / t_one / 1e6}
+            "single_thread_value": cells / t_one / 1e6, "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
 def main():
