@@ -30,7 +30,7 @@ import torch
 import cases
 import lbcube
 from oracle import oracle as O
-from test_gpu_parity import COLLS, SHAPE, STORES, TORCH_DT, _species, to_dev, to_host
+from test_gpu_parity import COLLS, SHAPE, STORES, _species, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
