@@ -371,11 +371,12 @@ def adjoint_ratio(torch, n=512, steps=4096, interval=64):
                                  "adjoint_over_forward": (t_all - t_fwd) / t_fwd}}
 
 
-def adjoint_3d_ratio(torch, n=128, steps=256, interval=64):
-    """Differentiated 3D reactive run (D3Q19 TRT + two D3Q7, A+B->C), FP32 with
-    the fast forward: a sheared layer with species on either side of a
-    smoothed plane; time of the forward steps (graph replay) against the full
-    gradient (forward with checkpoints, segment recompute, adjoint sweep)."""
+def adjoint_3d_ratio(torch, n=128, steps=256, interval=64, collision="TRT"):
+    """Differentiated 3D reactive run (D3Q19 + two D3Q7, A+B->C; TRT unless
+    `collision` says otherwise), FP32 with the fast forward: a sheared layer
+    with species on either side of a smoothed plane; time of the forward steps
+    (graph replay) against the full gradient (forward with checkpoints,
+    segment recompute, adjoint sweep)."""
     import lbcube
     from lbcube import adjoint as ADJ
     from lbcube import init as lbinit
@@ -393,8 +394,8 @@ def adjoint_3d_ratio(torch, n=128, steps=256, interval=64):
     st = [f0.clone(), torch.empty_like(f0), a0.clone(), torch.empty_like(a0), b0.clone(), torch.empty_like(b0)]
 
     def step():
-        lbcube.reactive_fused_step("D3Q19", "TRT", "D3Q7", st[0], st[1], st[2], st[3], st[4], st[5], omega, omega_c,
-                                   k, fast=True)
+        lbcube.reactive_fused_step("D3Q19", collision, "D3Q7", st[0], st[1], st[2], st[3], st[4], st[5], omega,
+                                   omega_c, k, fast=True)
         st[:] = [st[1], st[0], st[3], st[2], st[5], st[4]]
 
     t_fwd = _graph_ms(torch, step, steps, chunk=64) * steps
@@ -402,17 +403,17 @@ def adjoint_3d_ratio(torch, n=128, steps=256, interval=64):
     block = lbcube.default_block("D3Q19", n, n, n)
     ws = torch.empty(ADJ.workspace_bytes(torch.float32, n, n, steps, interval, block, "D3Q19", n), dtype=torch.uint8,
                      device="cuda")
-    ADJ.reactive_gradient("TRT", f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws, fast=True,
+    ADJ.reactive_gradient(collision, f0, a0, b0, 64, omega, omega_c, k, interval, workspace=ws, fast=True,
                           lattice="D3Q19")
     out = {}
-    t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient("TRT", f0, a0, b0, steps, omega, omega_c, k,
-                                                                     interval, workspace=ws, fast=True,
+    t_grad = _time(torch, lambda: out.update(r=ADJ.reactive_gradient(collision, f0, a0, b0, steps, omega, omega_c,
+                                                                     k, interval, workspace=ws, fast=True,
                                                                      lattice="D3Q19")))
     ws_gb = ws.numel() / 1e9
     del ws, f0, a0, b0
     torch.cuda.empty_cache()
-    return {"workload": f"D3Q19 TRT + 2xD3Q7 A+B->C, {n}^3, {steps} steps, FP32 (fast forward), checkpoint every "
-                        f"{interval}",
+    return {"workload": f"D3Q19 {collision} + 2xD3Q7 A+B->C, {n}^3, {steps} steps, FP32 (fast forward), checkpoint "
+                        f"every {interval}",
             "forward_ms_per_step": t_fwd / steps, "gradient_ms_per_step": t_grad / steps,
             "adjoint_over_forward": (t_grad - t_fwd) / t_fwd, "gradient_over_forward": t_grad / t_fwd,
             "workspace_gb": ws_gb, "loss": out["r"][0]}
@@ -951,10 +952,12 @@ def main():
         if "adjoint" in chosen:
             extras["adjoint"] = adjoint_ratio(torch)
             extras["adjoint_3d"] = adjoint_3d_ratio(torch)
-            a3 = extras["adjoint_3d"]
-            print(f"# adjoint 3D: forward {a3['forward_ms_per_step']:.3f} ms/step, gradient "
-                  f"{a3['gradient_ms_per_step']:.3f} ms/step, adjoint / forward {a3['adjoint_over_forward']:.2f}",
-                  file=sys.stderr, flush=True)
+            extras["adjoint_3d_rlbm"] = adjoint_3d_ratio(torch, collision="RLBM")
+            for name in ("adjoint_3d", "adjoint_3d_rlbm"):
+                a3 = extras[name]
+                print(f"# {name}: forward {a3['forward_ms_per_step']:.3f} ms/step, gradient "
+                      f"{a3['gradient_ms_per_step']:.3f} ms/step, adjoint / forward "
+                      f"{a3['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
             print(f"# adjoint/forward: {extras['adjoint']['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
         if "reactive_2d" in chosen:
             extras["reactive_2d"] = reactive_2d_rate(lbcube, torch)

@@ -278,7 +278,7 @@ int lbc_reactive_fused_step_ex(int fluid_lattice, int collision, int scalar_latt
                                double reference_b, uint32_t block_x, uint32_t block_y, uint32_t block_z,
                                void* stream);
 
-/* Gradient of the 2D reactive loop (D2Q9 BGK, TRT or MRT + two D2Q5 species,
+/* Gradient of the 2D reactive loop (D2Q9 BGK, TRT, MRT or RLBM + two D2Q5 species,
  * A + B -> C; `steps` calls of lbc_reactive_fused_step from f0, a0, b0) by the
  * discrete adjoint.  loss = <C>(T) = <A>(0) - <A>(T) (domain means).  Outputs:
  * *loss, the adjoints dloss/d(f0, a0, b0) of the initial populations (same
@@ -306,7 +306,7 @@ int lbc_reactive_gradient(int collision, int storage, uint32_t flags, const void
                           void* stream);
 
 /* The same for the 3D reactive loop: a D3Q19 or D3Q27 fluid (fluid_lattice
- * LBC_D3Q19 / LBC_D3Q27; BGK, TRT or MRT, else hipErrorInvalidValue) plus two D3Q7
+ * LBC_D3Q19 / LBC_D3Q27; BGK, TRT, MRT or RLBM, else hipErrorInvalidValue) plus two D3Q7
  * species, `steps` calls of lbc_reactive_fused_step(fluid_lattice,
  * collision, LBC_D3Q7, ...) from f0, a0, b0; loss <A>(0) - <A>(T).  The
  * workspace holds the checkpoints and two segment buffers of states of

@@ -1,5 +1,5 @@
 // Discrete adjoint of the reactive loop: a D2Q9, D3Q19 or D3Q27 fluid (BGK,
-// TRT or MRT) plus two D2Q5 / D3Q7 species reacting A + B -> C
+// TRT, MRT or RLBM) plus two D2Q5 / D3Q7 species reacting A + B -> C
 // (the loop of src/simulate_reaction.cpp:516-546, run by
 // lbc_reactive_fused_step).  No counterpart in the reference.
 //
@@ -17,8 +17,8 @@
 //      Jacobian of the exact one-step map, and the velocity coupling into the
 //      fluid adjoint through du'/df' = (c - u') / rho';
 //   4. fluid collision transpose at the pre-collision state (BGK, TRT with
-//      its even/odd split, or MRT through the moment basis), written to nu_f
-//      of step n.
+//      its even/odd split, MRT through the moment basis, or RLBM through the
+//      non-equilibrium stress it reconstructs from), written to nu_f of step n.
 // Long runs use checkpoints every `interval` steps and recompute each
 // segment forward before walking it backward (lbc_reactive_gradient).
 #include <hip/hip_runtime.h>
@@ -111,6 +111,154 @@ __device__ __forceinline__ void feq_transpose(const R* lam, R ux, R uy, R uz, R&
     bx += lam[0] * R(F::w[0]) * (-R(3) * ux);
     by += lam[0] * R(F::w[0]) * (-R(3) * uy);
     if constexpr (F::D == 3) bz += lam[0] * R(F::w[0]) * (-R(3) * uz);
+}
+
+// out = J^T lam of the regularized collision (collide_rlbm,
+// lattice_physics.hpp:422-605) at the pre-collision populations f:
+// f' = feq(rho, u) + (1 - omega) w (9/2) [second(Pi) + third(u, Pi)], where
+// Pi = sum c c (f - feq) = M2 - rho cs2 I - j j / rho (the second moments of
+// the second-order equilibrium are exact on D2Q9, D3Q19 and D3Q27) and the
+// third-order coefficients a_k are bilinear in u and Pi.  With psi = (1 -
+// omega) w (9/2) lam, g_P the derivative of psi . [second + third] with
+// respect to the independent components of Pi and g_u that with respect to u:
+//   J^T lam = (d feq / d f)^T lam + sum_P g_P [(c - u)_a (c - u)_b - cs2 delta_ab]
+//             + g_u . (c - u) / rho,
+// since d Pi_ab / d f_i = (c - u)_a (c - u)_b - cs2 delta_ab and d u / d f_i =
+// (c_i - u) / rho.  H_k = sum psi T_k carries the forward's combination of the
+// third-order Hermite polynomials h; D3Q19's paired projection 3 (h1 + h2)
+// (a1 + a2) + (h1 - h2)(a1 - a2) is (4 h1 + 2 h2) a1 + (2 h1 + 4 h2) a2.
+template <class F, class R>
+__device__ __forceinline__ void rlbm_transpose(const R* f, const R* lam, const Macro<R>& m, R omega, R* out) {
+    constexpr bool D3 = F::D == 3;
+    const R cs2 = R(F::cs2);
+    const R ux = m.ux, uy = m.uy, uz = m.uz;
+    const R uu = vel_sq<F, R>(m);
+    R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const R fn = f[i] - feq<F, R>(i, m, uu);
+        if (F::cx[i] != 0) pxx += fn;
+        if (F::cy[i] != 0) pyy += fn;
+        if (F::cz[i] != 0) pzz += fn;
+        if (F::cx[i] * F::cy[i] > 0) pxy += fn;
+        if (F::cx[i] * F::cy[i] < 0) pxy -= fn;
+        if (F::cx[i] * F::cz[i] > 0) pxz += fn;
+        if (F::cx[i] * F::cz[i] < 0) pxz -= fn;
+        if (F::cy[i] * F::cz[i] > 0) pyz += fn;
+        if (F::cy[i] * F::cz[i] < 0) pyz -= fn;
+    }
+    R gxx = R(0), gyy = R(0), gzz = R(0), gxy = R(0), gxz = R(0), gyz = R(0);
+    R Hxxx = R(0), Hyyy = R(0), Hzzz = R(0), Hxxy = R(0), Hxxz = R(0), Hxyy = R(0), Hyyz = R(0), Hxzz = R(0),
+      Hyzz = R(0), Hxyz = R(0);
+    // coefficients below fold to constants once the loop is unrolled; the
+    // zero ones are skipped rather than multiplied
+    auto add = [](R& sum, R coef, R x) {
+        if (coef != R(0)) sum += coef * x;
+    };
+    const R keep = (R(1) - omega) * R(4.5);
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const R psi = keep * R(F::w[i]) * lam[i];
+        const R cx = R(F::cx[i]), cy = R(F::cy[i]), cz = R(F::cz[i]);
+        add(gxx, cx * cx - cs2, psi);
+        add(gyy, cy * cy - cs2, psi);
+        if constexpr (D3) add(gzz, cz * cz - cs2, psi);
+        add(gxy, R(2) * cx * cy, psi);
+        if constexpr (D3) {
+            add(gxz, R(2) * cx * cz, psi);
+            add(gyz, R(2) * cy * cz, psi);
+        }
+        const R h_xxx = cx * cx * cx - R(3) * cs2 * cx, h_yyy = cy * cy * cy - R(3) * cs2 * cy;
+        const R h_zzz = cz * cz * cz - R(3) * cs2 * cz;
+        const R h_xxy = cx * cx * cy - cs2 * cy, h_xxz = cx * cx * cz - cs2 * cz;
+        const R h_xyy = cx * cy * cy - cs2 * cx, h_yyz = cy * cy * cz - cs2 * cz;
+        const R h_xzz = cx * cz * cz - cs2 * cx, h_yzz = cy * cz * cz - cs2 * cy, h_xyz = cx * cy * cz;
+        if constexpr (F::Q == 9) {
+            add(Hxxx, h_xxx, psi);
+            add(Hyyy, h_yyy, psi);
+            add(Hxxy, R(3) * h_xxy, psi);
+            add(Hxyy, R(3) * h_xyy, psi);
+        } else if constexpr (F::Q == 19) {
+            add(Hxxy, R(4) * h_xxy + R(2) * h_yzz, psi);
+            add(Hyzz, R(2) * h_xxy + R(4) * h_yzz, psi);
+            add(Hxzz, R(4) * h_xzz + R(2) * h_xyy, psi);
+            add(Hxyy, R(2) * h_xzz + R(4) * h_xyy, psi);
+            add(Hyyz, R(4) * h_yyz + R(2) * h_xxz, psi);
+            add(Hxxz, R(2) * h_yyz + R(4) * h_xxz, psi);
+        } else {
+            add(Hxxx, h_xxx, psi);
+            add(Hyyy, h_yyy, psi);
+            add(Hzzz, h_zzz, psi);
+            add(Hxxy, R(3) * h_xxy, psi);
+            add(Hxxz, R(3) * h_xxz, psi);
+            add(Hxyy, R(3) * h_xyy, psi);
+            add(Hyyz, R(3) * h_yyz, psi);
+            add(Hxzz, R(3) * h_xzz, psi);
+            add(Hyzz, R(3) * h_yzz, psi);
+            add(Hxyz, R(6) * h_xyz, psi);
+        }
+    }
+    // a_k = sum coef u_c Pi_ab: d/d Pi_ab gains H_k coef u_c, d/d u_c gains
+    // H_k coef Pi_ab
+    R gux = R(0), guy = R(0), guz = R(0);
+    if constexpr (F::Q != 19) {
+        gxx += R(3) * ux * Hxxx;  // a_xxx = 3 ux pxx
+        gux += R(3) * pxx * Hxxx;
+        gyy += R(3) * uy * Hyyy;  // a_yyy = 3 uy pyy
+        guy += R(3) * pyy * Hyyy;
+    }
+    gxy += R(2) * ux * Hxxy;  // a_xxy = 2 ux pxy + uy pxx
+    gux += R(2) * pxy * Hxxy;
+    gxx += uy * Hxxy;
+    guy += pxx * Hxxy;
+    gxy += R(2) * uy * Hxyy;  // a_xyy = 2 uy pxy + ux pyy
+    guy += R(2) * pxy * Hxyy;
+    gyy += ux * Hxyy;
+    gux += pyy * Hxyy;
+    if constexpr (D3) {
+        gxz += R(2) * ux * Hxxz;  // a_xxz = 2 ux pxz + uz pxx
+        gux += R(2) * pxz * Hxxz;
+        gxx += uz * Hxxz;
+        guz += pxx * Hxxz;
+        gyz += R(2) * uy * Hyyz;  // a_yyz = 2 uy pyz + uz pyy
+        guy += R(2) * pyz * Hyyz;
+        gyy += uz * Hyyz;
+        guz += pyy * Hyyz;
+        gxz += R(2) * uz * Hxzz;  // a_xzz = 2 uz pxz + ux pzz
+        guz += R(2) * pxz * Hxzz;
+        gzz += ux * Hxzz;
+        gux += pzz * Hxzz;
+        gyz += R(2) * uz * Hyzz;  // a_yzz = 2 uz pyz + uy pzz
+        guz += R(2) * pyz * Hyzz;
+        gzz += uy * Hyzz;
+        guy += pzz * Hyzz;
+    }
+    if constexpr (F::Q == 27) {
+        gzz += R(3) * uz * Hzzz;  // a_zzz = 3 uz pzz
+        guz += R(3) * pzz * Hzzz;
+        gyz += ux * Hxyz;  // a_xyz = ux pyz + uy pxz + uz pxy
+        gux += pyz * Hxyz;
+        gxz += uy * Hxyz;
+        guy += pxz * Hxyz;
+        gxy += uz * Hxyz;
+        guz += pxy * Hxyz;
+    }
+    R al, bx, by, bz;
+    feq_transpose<F, R>(lam, ux, uy, uz, al, bx, by, bz);
+    const R irho = R(1) / m.rho;
+    bx += gux * irho;
+    by += guy * irho;
+    if constexpr (D3) bz += guz * irho;
+#pragma unroll
+    for (int i = 0; i < F::Q; ++i) {
+        const R dx = R(F::cx[i]) - ux, dy = R(F::cy[i]) - uy;
+        R t = al + bx * dx + by * dy + gxx * (dx * dx - cs2) + gyy * (dy * dy - cs2) + gxy * (dx * dy);
+        if constexpr (D3) {
+            const R dz = R(F::cz[i]) - uz;
+            t += bz * dz + gzz * (dz * dz - cs2) + gxz * (dx * dz) + gyz * (dy * dz);
+        }
+        out[i] = t;
+    }
 }
 
 // w = M^-1 S M lam for the D3Q27 MRT (collide_mrt_d3q27: f' = f - M^-1 S M
@@ -224,7 +372,7 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
             fl[i] = static_cast<R>(plane_ptr(f, i, plane)[src]);
             lf[i] = plane_ptr(nuf1, i, plane)[rev];
         }
-        // BGK, TRT and MRT conserve rho and j, so the velocity the forward
+        // BGK, TRT, MRT and RLBM conserve rho and j, so the velocity the forward
         // scalar step read from the post-collision state equals that of the
         // pulled state up to rounding: the linearisation point is taken from
         // the pulled populations, without re-colliding them.
@@ -378,6 +526,11 @@ __global__ __launch_bounds__(MAXT) void reactive_adjoint_kernel(
                 put(plane_ptr(nuf0, i, plane) + cell,
                     t + grho + R(D2Q9::cx[i]) * gjx + R(D2Q9::cy[i]) * gjy);
             }
+        } else if constexpr (CT == kRLBM) {
+            R out[QF];
+            rlbm_transpose<F, R>(fl, lf, m, omega, out);
+#pragma unroll
+            for (int i = 0; i < QF; ++i) put(plane_ptr(nuf0, i, plane) + cell, out[i]);
         } else if constexpr (CT == kBGK) {
             R al, bx, by, bz;
             feq_transpose<F, R>(lf, m.ux, m.uy, m.uz, al, bx, by, bz);
@@ -684,10 +837,10 @@ constexpr uint32_t kMaxAdjointCells = 1u << 29;
 namespace lbc {
 
 // The pairs the adjoint covers: D2Q9 + D2Q5, D3Q19 / D3Q27 + D3Q7, each with
-// BGK, TRT or MRT.
+// BGK, TRT, MRT or RLBM.
 inline bool gradient_pair(int fluid, int collision) {
     if (fluid != kD2Q9 && fluid != kD3Q19 && fluid != kD3Q27) return false;
-    return collision == kBGK || collision == kTRT || collision == kMRT;
+    return collision == kBGK || collision == kTRT || collision == kMRT || collision == kRLBM;
 }
 
 inline size_t state_planes(int fluid) {
@@ -746,6 +899,7 @@ hipError_t gradient(int fluid, int collision, int storage, uint32_t flags, const
     using BGK = std::integral_constant<int, kBGK>;
     using TRT = std::integral_constant<int, kTRT>;
     using MRT = std::integral_constant<int, kMRT>;
+    using RLBM = std::integral_constant<int, kRLBM>;
     // storage and adjoint precision: FP64; FP32; FP32 with the FP64 adjoint
     auto by_precision = [&](auto fs, auto ct) {
         if (storage == kF64) return run(fs, 0.0, 0.0, ct);
@@ -755,7 +909,8 @@ hipError_t gradient(int fluid, int collision, int storage, uint32_t flags, const
     auto by_collision = [&](auto fs) {
         if (collision == kBGK) return by_precision(fs, BGK{});
         if (collision == kTRT) return by_precision(fs, TRT{});
-        return by_precision(fs, MRT{});
+        if (collision == kMRT) return by_precision(fs, MRT{});
+        return by_precision(fs, RLBM{});
     };
     if (fluid == kD2Q9) return by_collision(std::pair<D2Q9, D2Q5>{});
     if (fluid == kD3Q19) return by_collision(std::pair<D3Q19, D3Q7>{});

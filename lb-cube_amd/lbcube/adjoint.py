@@ -1,7 +1,7 @@
 """Gradients of the reactive loop through the HIP discrete adjoint.
 
 The loop is D2Q9 + two D2Q5 species, or D3Q19 / D3Q27 + two D3Q7 species (BGK,
-TRT or MRT), with A + B -> C, stepped by
+TRT, MRT or RLBM), with A + B -> C, stepped by
 lbc_reactive_fused_step; the loss is the domain-mean product <C>(T) =
 <A>(0) - <A>(T).  `reactive_gradient` calls lbc_reactive_gradient (2D) or
 lbc_reactive_gradient_3d (forward with checkpoints every `interval` steps,
@@ -22,7 +22,7 @@ from . import (LATTICES, VELOCITIES, WEIGHTS, _check_pop, _check_set, _extents, 
                reactive_fused_step)
 
 _STORE = {torch.float32: 0, torch.float64: 1}
-_COLL = {"BGK": 0, "TRT": 1, "MRT": 2}
+_COLL = {"BGK": 0, "TRT": 1, "MRT": 2, "RLBM": 3}
 SPECIES = {"D2Q9": "D2Q5", "D3Q19": "D3Q7", "D3Q27": "D3Q7"}
 
 

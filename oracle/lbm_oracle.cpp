@@ -1103,7 +1103,7 @@ template <class R> D3q19Rows d3q19_rows() {
 }
 
 // The fluid lattice F (D2Q9, D3Q19, D3Q27; all cs2 = 1/3) and the species
-// lattice S (D2Q5, D3Q7) of one reactive pair; BGK, TRT and MRT.
+// lattice S (D2Q5, D3Q7) of one reactive pair; BGK, TRT, MRT and RLBM.
 template <class F, class S, class R, int CT>
 void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, const R* nua1, const R* nub1, R* nuf0,
                            R* nua0, R* nub0, Grid g, R omega, R omega_c, R k, long double* djdw) {
@@ -1283,6 +1283,118 @@ void reactive_adjoint_step(const R* f, const R* a, const R* b, const R* nuf1, co
                 R al, bd[3];
                 moments(lf, al, bd);
                 for (int i = 0; i < QF; ++i) nuf0[i * N + cell] = (R(1) - omega) * lf[i] + omega * (al + dev(i, bd));
+            } else if constexpr (CT == C_RLBM) {
+                // collide_rlbm (lattice_physics.hpp:422-605): f' = feq(rho, u) +
+                // (1 - omega) w (9/2) [second(Pi) + third(u, Pi)] with
+                // Pi_ab = sum c_a c_b (f - feq) = M2_ab - rho cs2 delta_ab - j_a j_b / rho
+                // (fourth-order isotropy of D2Q9 / D3Q19 / D3Q27) and the
+                // third-order coefficients a_k bilinear in u and Pi, so f'
+                // depends on f only through rho, j and M2.  With psi = (1 - omega)
+                // w (9/2) lam, g_P = d(psi . [second + third]) / d Pi_P over the
+                // independent components P (xx, yy, zz, xy, xz, yz) and g_u the
+                // same with respect to u:
+                //   J^T lam = (d feq/d f)^T lam + sum_P g_P [(c - u)_a (c - u)_b
+                //             - cs2 delta_ab] + g_u . (c - u) / rho.
+                const R c3 = R(1) / R(3);
+                const R u3[3] = {m.u[0], m.u[1], D == 3 ? m.u[2] : R(0)};
+                auto cc = [&](int i, int d) { return d < D ? static_cast<R>(F::c[i * D + d]) : R(0); };
+                // P index of the pair (a, b): 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
+                auto pidx = [](int a, int b) {
+                    if (a > b) std::swap(a, b);
+                    return a == b ? a : (a == 0 ? (b == 1 ? 3 : 4) : 5);
+                };
+                R pi[6] = {0, 0, 0, 0, 0, 0};
+                for (int q = 0; q < QF; ++q) {
+                    R cu = 0;
+                    for (int d = 0; d < D; ++d) cu += cc(q, d) * m.u[d];
+                    const R eq = static_cast<R>(F::w[q]) * m.rho * (R(1) + R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu);
+                    const R fn = fl[q] - eq;
+                    for (int a = 0; a < D; ++a)
+                        for (int b2 = a; b2 < D; ++b2) pi[pidx(a, b2)] += cc(q, a) * cc(q, b2) * fn;
+                }
+                R g[6] = {0, 0, 0, 0, 0, 0}, gu[3] = {0, 0, 0};
+                // H_k = sum psi T_k with the forward's third-order combination
+                R Hk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz
+                for (int i = 0; i < QF; ++i) {
+                    const R psi = (R(1) - omega) * static_cast<R>(F::w[i]) * R(4.5) * lf[i];
+                    const R cx = cc(i, 0), cy = cc(i, 1), cz = cc(i, 2);
+                    g[0] += psi * (cx * cx - c3);
+                    g[1] += psi * (cy * cy - c3);
+                    if (D == 3) g[2] += psi * (cz * cz - c3);
+                    g[3] += R(2) * psi * cx * cy;
+                    g[4] += R(2) * psi * cx * cz;
+                    g[5] += R(2) * psi * cy * cz;
+                    const R hxxx = cx * cx * cx - R(3) * c3 * cx, hyyy = cy * cy * cy - R(3) * c3 * cy;
+                    const R hzzz = cz * cz * cz - R(3) * c3 * cz;
+                    const R hxxy = cx * cx * cy - c3 * cy, hxxz = cx * cx * cz - c3 * cz;
+                    const R hxyy = cx * cy * cy - c3 * cx, hyyz = cy * cy * cz - c3 * cz;
+                    const R hxzz = cx * cz * cz - c3 * cx, hyzz = cy * cz * cz - c3 * cy, hxyz = cx * cy * cz;
+                    if constexpr (QF == 9) {
+                        Hk[0] += psi * hxxx;
+                        Hk[1] += psi * hyyy;
+                        Hk[3] += psi * R(3) * hxxy;
+                        Hk[5] += psi * R(3) * hxyy;
+                    } else if constexpr (QF == 19) {  // the paired projection of the forward
+                        Hk[3] += psi * (R(4) * hxxy + R(2) * hyzz);
+                        Hk[8] += psi * (R(2) * hxxy + R(4) * hyzz);
+                        Hk[7] += psi * (R(4) * hxzz + R(2) * hxyy);
+                        Hk[5] += psi * (R(2) * hxzz + R(4) * hxyy);
+                        Hk[6] += psi * (R(4) * hyyz + R(2) * hxxz);
+                        Hk[4] += psi * (R(2) * hyyz + R(4) * hxxz);
+                    } else {
+                        Hk[0] += psi * hxxx;
+                        Hk[1] += psi * hyyy;
+                        Hk[2] += psi * hzzz;
+                        Hk[3] += psi * R(3) * hxxy;
+                        Hk[4] += psi * R(3) * hxxz;
+                        Hk[5] += psi * R(3) * hxyy;
+                        Hk[6] += psi * R(3) * hyyz;
+                        Hk[7] += psi * R(3) * hxzz;
+                        Hk[8] += psi * R(3) * hyzz;
+                        Hk[9] += psi * R(6) * hxyz;
+                    }
+                }
+                // a_k holds coef u_c Pi_ab: d/dPi_ab += H coef u_c, d/du_c += H coef Pi_ab
+                auto bil = [&](R H, int uc, int a, int b2, R coef) {
+                    g[pidx(a, b2)] += H * coef * u3[uc];
+                    gu[uc] += H * coef * pi[pidx(a, b2)];
+                };
+                if (QF != 19) {
+                    bil(Hk[0], 0, 0, 0, R(3));  // a_xxx = 3 ux pxx
+                    bil(Hk[1], 1, 1, 1, R(3));  // a_yyy = 3 uy pyy
+                }
+                if (QF == 27) bil(Hk[2], 2, 2, 2, R(3));
+                bil(Hk[3], 0, 0, 1, R(2));  // a_xxy = 2 ux pxy + uy pxx
+                bil(Hk[3], 1, 0, 0, R(1));
+                bil(Hk[5], 1, 0, 1, R(2));  // a_xyy = 2 uy pxy + ux pyy
+                bil(Hk[5], 0, 1, 1, R(1));
+                if (D == 3) {
+                    bil(Hk[4], 0, 0, 2, R(2));  // a_xxz = 2 ux pxz + uz pxx
+                    bil(Hk[4], 2, 0, 0, R(1));
+                    bil(Hk[6], 1, 1, 2, R(2));  // a_yyz = 2 uy pyz + uz pyy
+                    bil(Hk[6], 2, 1, 1, R(1));
+                    bil(Hk[7], 2, 0, 2, R(2));  // a_xzz = 2 uz pxz + ux pzz
+                    bil(Hk[7], 0, 2, 2, R(1));
+                    bil(Hk[8], 2, 1, 2, R(2));  // a_yzz = 2 uz pyz + uy pzz
+                    bil(Hk[8], 1, 2, 2, R(1));
+                }
+                if (QF == 27) {  // a_xyz = ux pyz + uy pxz + uz pxy
+                    bil(Hk[9], 0, 1, 2, R(1));
+                    bil(Hk[9], 1, 0, 2, R(1));
+                    bil(Hk[9], 2, 0, 1, R(1));
+                }
+                R al, bd[3];
+                moments(lf, al, bd);
+                for (int i = 0; i < QF; ++i) {
+                    R dc[3];
+                    for (int d = 0; d < 3; ++d) dc[d] = cc(i, d) - u3[d];
+                    R t = al + dev(i, bd);
+                    for (int a = 0; a < D; ++a)
+                        for (int b2 = a; b2 < D; ++b2) t += g[pidx(a, b2)] * (dc[a] * dc[b2] - (a == b2 ? c3 : R(0)));
+                    R tu = 0;
+                    for (int d = 0; d < D; ++d) tu += gu[d] * dc[d];
+                    nuf0[i * N + cell] = t + tu / m.rho;
+                }
             } else {
                 R lp[QF], lm[QF];
                 for (int i = 0; i < QF; ++i) {
@@ -1356,7 +1468,7 @@ void reactive_gradient(const R* f0, const R* a0, const R* b0, Grid g, R omega, R
 }
 
 // Entry helpers of the adjoint: the collisions the adjoint covers (BGK, TRT,
-// MRT) for a fluid/species pair, dispatched at run time.
+// MRT, RLBM) for a fluid/species pair, dispatched at run time.
 template <class F, class S>
 int gradient_entry(int ct, int dt, const void* f0, const void* a0, const void* b0, Grid g, double omega,
                    double omega_c, double k, int steps, double* J, void* gf, void* ga, void* gb, double* djdwc) {
@@ -1376,6 +1488,7 @@ int gradient_entry(int ct, int dt, const void* f0, const void* a0, const void* b
         if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
         if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
         if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
+        if (ct == C_RLBM) return run(tag, std::integral_constant<int, C_RLBM>{});
         return -1;
     };
     if (dt == T_F64) return by_ct(double{});
@@ -1402,6 +1515,7 @@ int adjoint_step_entry(int ct, int dt, const void* f, const void* a, const void*
         if (ct == C_BGK) return run(tag, std::integral_constant<int, C_BGK>{});
         if (ct == C_TRT) return run(tag, std::integral_constant<int, C_TRT>{});
         if (ct == C_MRT) return run(tag, std::integral_constant<int, C_MRT>{});
+        if (ct == C_RLBM) return run(tag, std::integral_constant<int, C_RLBM>{});
         return -1;
     };
     int rc;

@@ -58,7 +58,7 @@ def test_validation_order_fluid():
 
 
 def test_gradient_3d_validation():
-    """The 3D adjoint: D3Q19 / D3Q27 fluids only, BGK, TRT or MRT, whole waves."""
+    """The 3D adjoint: D3Q19 / D3Q27 fluids only, BGK, TRT, MRT or RLBM, whole waves."""
     L = lbcube.lib()
     out = ctypes.c_uint64()
     assert L.lbc_reactive_gradient_workspace_3d(1, 0, 16, 16, 16, 8, 4, 256, 1, 1, ctypes.byref(out)) == 0
@@ -77,9 +77,9 @@ def test_gradient_3d_validation():
                                           ctypes.byref(loss), 0x4000, 0x5000, 0x6000, ctypes.byref(dw), 0x7000, 1 << 40,
                                           *block, None)
 
-    assert grad(1, 3) == HIP_INVALID_VALUE  # RLBM
-    assert grad(2, 3) == HIP_INVALID_VALUE
-    assert grad(2, 7) == HIP_INVALID_VALUE  # unknown collision
+    assert grad(1, 4) == HIP_INVALID_VALUE  # unknown collisions
+    assert grad(2, 7) == HIP_INVALID_VALUE
+    assert grad(1, -1) == HIP_INVALID_VALUE
     assert grad(0, 0) == HIP_INVALID_VALUE  # a 2D fluid
     assert grad(1, 0, block=(96, 1, 1)) == HIP_INVALID_CONFIGURATION
 

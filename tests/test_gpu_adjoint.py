@@ -26,7 +26,7 @@ def dev(x, gpu, dtype=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(x)).to(gpu).to(dtype)
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 @pytest.mark.parametrize("interval", [1, 5, 8, 64])
 def test_adjoint_matches_oracle_fp64(gpu, coll, interval):
     # interval 8 over 17 steps: three segments (8, 8, 1), the recomputes
@@ -44,16 +44,18 @@ def test_adjoint_matches_oracle_fp64(gpu, coll, interval):
     assert abs(dwg - dw) <= 1e-11 * abs(dw)
 
 
-def test_adjoint_fp32_against_oracle(gpu):
+@pytest.mark.parametrize("coll", ["BGK", "RLBM"])
+def test_adjoint_fp32_against_oracle(gpu, coll):
     """The FP32 adjoint is as close to the FP64 gradient as the oracle's own
     FP32 adjoint is (their operation orders differ, so not bit for bit)."""
+    print(f"case adjoint fp32 {coll}", flush=True)
     f, a, b = TA.small_state(nx=12, ny=10, seed=6)
     f, a, b = (x.astype(np.float32) for x in (f, a, b))
     omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 17
-    J64, *g64 = O.reactive_gradient("BGK", "f64", *(x.astype(np.float64) for x in (f, a, b)), omega, omega_c, k,
+    J64, *g64 = O.reactive_gradient(coll, "f64", *(x.astype(np.float64) for x in (f, a, b)), omega, omega_c, k,
                                     steps)
-    J32, *g32 = O.reactive_gradient("BGK", "f32", f, a, b, omega, omega_c, k, steps)
-    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient("BGK", dev(f, gpu, torch.float32), dev(a, gpu, torch.float32),
+    J32, *g32 = O.reactive_gradient(coll, "f32", f, a, b, omega, omega_c, k, steps)
+    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient(coll, dev(f, gpu, torch.float32), dev(a, gpu, torch.float32),
                                                    dev(b, gpu, torch.float32), steps, omega, omega_c, k, interval=4)
     assert abs(Jg - J32) <= 1e-6 and abs(Jg - J64) <= 1e-6
     for got, orc, exact in zip((gfg, gag, gbg), g32[:3], g64[:3]):
@@ -74,7 +76,7 @@ def test_fp64_adjoint_of_fp32_trajectory(gpu):
     f, a, b = TA.small_state(nx=48, ny=40, seed=9)
     args64 = (dev(f, gpu), dev(a, gpu), dev(b, gpu), 30, 1 / 0.7, 1 / 0.9, 0.4)
     args32 = tuple(x.float() for x in args64[:3]) + args64[3:]
-    for coll in ("BGK", "TRT", "MRT"):
+    for coll in ("BGK", "TRT", "MRT", "RLBM"):
         print(f"case fp64 adjoint of fp32 trajectory {coll}", flush=True)
         ref = ADJ.reactive_gradient(coll, *args64, interval=8)
         g32 = ADJ.reactive_gradient(coll, *args32, interval=8)

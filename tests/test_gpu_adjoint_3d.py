@@ -1,5 +1,5 @@
 """GPU: the discrete adjoint of the 3D reactive loop (lbc_reactive_gradient_3d):
-a D3Q19 or D3Q27 fluid (BGK, TRT, MRT) with two D3Q7 species, A + B -> C.
+a D3Q19 or D3Q27 fluid (BGK, TRT, MRT, RLBM) with two D3Q7 species, A + B -> C.
 
   * against the oracle's FP64 CPU adjoint of the same loop (tight tolerance;
     checkpoint intervals 1, 5 and 8 over 17 steps, so the segment recompute
@@ -28,7 +28,7 @@ def dev(x, gpu, dtype=torch.float64):
     return torch.from_numpy(np.ascontiguousarray(x)).to(gpu).to(dtype)
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
 @pytest.mark.parametrize("interval", [1, 5, 8])
 def test_adjoint_3d_matches_oracle_fp64(gpu, lattice, coll, interval):
@@ -89,7 +89,7 @@ def sheared_state(n, lattice, gpu):
     return dev(f, gpu), dev(a, gpu), dev(b, gpu)
 
 
-@pytest.mark.parametrize("coll", ["TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["TRT", "MRT", "RLBM"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
 def test_adjoint_3d_central_differences(gpu, lattice, coll):
     """40^3, 200 steps, checkpoints every 32: directional derivatives along 6
@@ -104,7 +104,11 @@ def test_adjoint_3d_central_differences(gpu, lattice, coll):
     worst = 0.0
     for trial in range(6):
         vf, va, vb = (torch.randn(x.shape, generator=gen, dtype=torch.float64).to(gpu) * 1e-3 for x in (f, a, b))
-        h = 1e-3
+        # h = 1e-2 (1e-5 per population): the rounding of the loss falls as
+        # 1 / h and the truncation shows only from 3e-2 on.  RLBM's loss is the
+        # noisiest (its f - feq cancels): |fd - adjoint| 5e-12 at h = 1e-3,
+        # 2e-13 at 3e-3, below 3e-13 for every collision at 1e-2.
+        h = 1e-2
         jp = ADJ.loss_of(coll, f + h * vf, a + h * va, b + h * vb, steps, omega, omega_c, k, lattice=lattice)
         jm = ADJ.loss_of(coll, f - h * vf, a - h * va, b - h * vb, steps, omega, omega_c, k, lattice=lattice)
         fd = (jp - jm) / (2 * h)
@@ -112,7 +116,13 @@ def test_adjoint_3d_central_differences(gpu, lattice, coll):
         err = abs(fd - ad) / abs(fd)
         worst = max(worst, err)
         print(f"{lattice} {coll} direction {trial}: fd {fd:.10e} adjoint {ad:.10e} ({err:.1e})", flush=True)
-    hw = 1e-5
+    # hw = 1e-3: the difference's error is at most 1.4e-7 relative for every
+    # lattice and collision (truncation ~1.1e-7).  Smaller steps pick up the
+    # rounding of the reaction's closed form near the plane where A = B
+    # (lattice_physics.hpp:712-743), up to 1.3e-6 at 1e-5; the adjoint's
+    # omega_c term itself is pinned on the CPU against an extended-precision
+    # forward.
+    hw = 1e-3
     fdw = (ADJ.loss_of(coll, f, a, b, steps, omega, omega_c + hw, k, lattice=lattice) -
            ADJ.loss_of(coll, f, a, b, steps, omega, omega_c - hw, k, lattice=lattice)) / (2 * hw)
     print(f"{lattice} {coll} omega_c: fd {fdw:.10e} adjoint {dw:.10e}; worst direction {worst:.1e}", flush=True)

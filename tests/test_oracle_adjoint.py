@@ -1,6 +1,8 @@
-"""The oracle's FP64 discrete adjoint of the 2D reactive loop (D2Q9 BGK/TRT/MRT +
-two D2Q5 species, A + B -> C), pinned by central finite differences of the
-oracle's own forward loop.  The reference has no adjoint; this CPU adjoint is
+"""The oracle's FP64 discrete adjoint of the 2D reactive loop (D2Q9
+BGK/TRT/MRT/RLBM + two D2Q5 species, A + B -> C), pinned by central finite
+differences of the oracle's own forward loop (in omega_c also over 400 steps
+against an extended-precision forward) and by the one-step dot-product
+identity.  The reference has no adjoint; this CPU adjoint is
 the checker for the HIP adjoint kernel."""
 from __future__ import annotations
 
@@ -92,7 +94,7 @@ def test_reaction_map_derivatives_against_fd():
     assert np.max(np.abs(da - sa)) < 1e-8 and np.max(np.abs(db - sb)) < 1e-8
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 def test_adjoint_matches_central_differences(coll):
     f, a, b = small_state()
     omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 12
@@ -113,6 +115,31 @@ def test_adjoint_matches_central_differences(coll):
     assert abs(fd - dw) <= 1e-6 * abs(fd), (fd, dw)
 
 
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
+def test_adjoint_omega_c_long_run_extended_precision(coll):
+    """dJ/d omega_c over 400 steps of a smooth 2D shear layer (32^2) against
+    central differences of the forward loop in extended precision (h = 1e-5,
+    exact to its truncation).  The profiles are offset by 0.3 cells so that
+    no cell holds A = B, where the reaction's closed form cancels."""
+    n, steps, omega, omega_c, k = 32, 400, 1 / 0.6, 1 / 0.8, 0.05
+    y, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    u = np.stack([0.04 * np.tanh((y - n / 2 - 0.3) / 3.0), 0.004 * np.sin(2 * np.pi * x / n)])
+    f = O.feq_field("D2Q9", "f64", np.ones((n, n)), u)
+    ca = 0.5 * (1 + np.tanh((x - n / 2 - 0.3) / 2.0))
+    a = O.scalar_feq_field("D2Q5", "f64", ca, u)
+    b = O.scalar_feq_field("D2Q5", "f64", 1.0 - ca, u)
+    _, _, _, _, dw = O.reactive_gradient(coll, "f64", f, a, b, omega, omega_c, k, steps)
+    ext = [v.astype(np.longdouble) for v in (f, a, b)]
+
+    def minus_mean_a(oc):
+        _, a1, _ = O.reactive_run("D2Q9", coll, "D2Q5", "f80", *ext, omega, oc, k, steps)
+        return -a1.sum() / (n * n)
+
+    h = 1e-5
+    fd = float((minus_mean_a(omega_c + h) - minus_mean_a(omega_c - h)) / (2 * h))
+    assert abs(fd - dw) <= 1e-9 * abs(dw), (fd, dw)
+
+
 def to_collision_cell(lam, lattice):
     """Stored-state adjoint Lambda[i][y] -> collision-cell adjoint nu[i][x] = Lambda[i][x - c_i]."""
     nu = np.empty_like(lam)
@@ -129,7 +156,7 @@ def to_stored(nu, lattice):
     return lam
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 def test_adjoint_step_dot_product(coll):
     """<Lambda_1, L v> = <L^T Lambda_1, v> for one step, with random v and a
     random adjoint Lambda_1 on every population of the fluid and both species

@@ -1,7 +1,8 @@
 """The oracle's FP64 discrete adjoint of the 3D reactive loop (D3Q19 or D3Q27
-BGK/TRT/MRT + two D3Q7 species, cs2 = 1/4, A + B -> C), pinned like the 2D one:
-by central finite differences of the oracle's own forward loop along random
-directions (and in omega_c), and by the one-step dot-product identity
+BGK/TRT/MRT/RLBM + two D3Q7 species, cs2 = 1/4, A + B -> C), pinned like the 2D
+one: by central finite differences of the oracle's own forward loop along random
+directions (and in omega_c, also over 200 steps against an extended-precision
+forward), and by the one-step dot-product identity
 <Lambda_1, L v> = <L^T Lambda_1, v> with random adjoint inputs on every
 population.  The reference has no adjoint; this CPU adjoint is the checker for
 the HIP kernel of lbc_reactive_gradient_3d."""
@@ -34,7 +35,7 @@ def J_of(lattice, coll, f, a, b, omega, omega_c, k, steps):
     return a.sum() / cells - wa.sum() / cells
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
 def test_adjoint_3d_matches_central_differences(lattice, coll):
     f, a, b = small_state_3d(lattice)
@@ -60,6 +61,36 @@ def test_adjoint_3d_matches_central_differences(lattice, coll):
     assert abs(fd - dw) <= 1e-6 * abs(fd), (fd, dw)
 
 
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
+@pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
+def test_adjoint_3d_omega_c_long_run_extended_precision(lattice, coll):
+    """dJ/d omega_c over 200 steps of a smooth shear layer (12^3) against
+    central differences of the forward loop in extended precision: at
+    h = 1e-5 the difference is exact to its truncation, -3.4e-10 relative.
+    The profiles are offset by 0.3 cells so that no cell holds A = B: where
+    |B - A| is a few ulps above the switch of the reaction's closed form
+    (lattice_physics.hpp:712-743), that form cancels and its rounding, not the
+    derivative, would dominate the differences."""
+    n, steps, omega, omega_c, k = 12, 200, 1 / 0.6, 1 / 0.8, 0.05
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u = np.stack([0.04 * np.tanh((y - n / 2 - 0.3) / 2.0), 0.004 * np.sin(2 * np.pi * x / n),
+                  0.004 * np.cos(2 * np.pi * z / n)])
+    f = O.feq_field(lattice, "f64", np.ones((n, n, n)), u)
+    ca = 0.5 * (1 + np.tanh((x - n / 2 - 0.3) / 1.5))
+    a = O.scalar_feq_field("D3Q7", "f64", ca, u)
+    b = O.scalar_feq_field("D3Q7", "f64", 1.0 - ca, u)
+    _, _, _, _, dw = O.reactive_gradient(coll, "f64", f, a, b, omega, omega_c, k, steps, lattice=lattice)
+    ext = [v.astype(np.longdouble) for v in (f, a, b)]
+
+    def minus_mean_a(oc):
+        _, a1, _ = O.reactive_run(lattice, coll, "D3Q7", "f80", *ext, omega, oc, k, steps)
+        return -a1.sum() / a[0].size
+
+    h = 1e-5
+    fd = float((minus_mean_a(omega_c + h) - minus_mean_a(omega_c - h)) / (2 * h))
+    assert abs(fd - dw) <= 1e-9 * abs(dw), (fd, dw)
+
+
 def roll3(x, c, sign):
     return np.roll(x, shift=(sign * c[2], sign * c[1], sign * c[0]), axis=(0, 1, 2))
 
@@ -74,7 +105,7 @@ def to_stored(nu, lattice):
     return np.stack([roll3(nu[i], c, -1) for i, c in enumerate(VEL3[lattice])])
 
 
-@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
 def test_adjoint_3d_step_dot_product(lattice, coll):
     """One step: <Lambda_1, L v> (L v the central difference of the forward
@@ -110,8 +141,13 @@ def test_adjoint_3d_step_dot_product(lattice, coll):
     assert abs(lw - djdw) <= 1e-9 * max(abs(lw), 1.0), (lw, djdw)
 
 
-def test_adjoint_3d_collisions_covered():
-    """The 3D adjoint covers BGK, TRT and MRT; RLBM is refused."""
+def test_adjoint_3d_collisions_covered(monkeypatch):
+    """The 3D adjoint covers BGK, TRT, MRT and RLBM; an unknown collision id
+    is refused."""
     f, a, b = small_state_3d("D3Q19", nx=3, ny=3, nz=3)
+    for coll in ("BGK", "TRT", "MRT", "RLBM"):
+        _, gf, ga, gb, _ = O.reactive_gradient(coll, "f64", f, a, b, 1.5, 1.2, 0.1, 1, lattice="D3Q19")
+        assert all(np.all(np.isfinite(x)) for x in (gf, ga, gb))
+    monkeypatch.setitem(O.COLLISIONS, "unknown", 4)
     with pytest.raises(ValueError):
-        O.reactive_gradient("RLBM", "f64", f, a, b, 1.5, 1.2, 0.1, 1, lattice="D3Q19")
+        O.reactive_gradient("unknown", "f64", f, a, b, 1.5, 1.2, 0.1, 1, lattice="D3Q19")
