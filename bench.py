@@ -820,7 +820,8 @@ def plane_copy_by_row_length(lbcube, torch, cells=1 << 22, reps=50):
 def reactive_2d_large(lbcube, torch, n=8192, steps=20):
     """The same fused step on an 8192^2 double shear layer (16x the cells of
     the 2048^2 line, ~0.9 ms per FP16 step against ~0.06): the rate of the
-    kernel when the fixed start and drain of a launch no longer count."""
+    kernel when the fixed start and drain of a launch no longer count; on the
+    reference's planes and in tiles of 4 and 8 rows."""
     from lbcube import init as lbinit
 
     out = {}
@@ -850,7 +851,31 @@ def reactive_2d_large(lbcube, torch, n=8192, steps=20):
                     "bytes_per_cell": bpc}
         print(f"# reactive 2D {key}: {out[key]['mlups']:.0f} MLUPS, {out[key]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
+        planes = state[0::2]
         del state
+        for rows in (4, 8):
+            tiled = [lbcube.to_tiles(lat, x, rows) for lat, x in zip(("D2Q9", "D2Q5", "D2Q5"), planes)]
+            tstate = [tiled[0], torch.empty_like(tiled[0]), tiled[1], torch.empty_like(tiled[1]), tiled[2],
+                      torch.empty_like(tiled[2])]
+            del tiled
+
+            def fused_tiles(tstate=tstate, fast=fast, ref=ref):
+                f, ft, a, at, b, bt = tstate
+                lbcube.reactive_fused_step_tiled("D2Q9", "MRT", "D2Q5", f, ft, a, at, b, bt, 1 / 0.6, 1 / 0.6, k,
+                                                 fast=fast, ref_a=ref, ref_b=ref)
+                tstate[:] = [ft, f, at, a, bt, b]
+
+            for _ in range(2):
+                fused_tiles()
+            ms = _graph_ms(torch, fused_tiles, steps)
+            tkey = f"{key}_rows{rows}"
+            out[tkey] = {"mlups": n * n / (ms * 1e-3) / 1e6, "achieved_gbs": bpc * n * n / (ms * 1e-3) / 1e9,
+                         "bytes_per_cell": bpc}
+            print(f"# reactive 2D {tkey}: {out[tkey]['mlups']:.0f} MLUPS, {out[tkey]['achieved_gbs']:.0f} GB/s",
+                  file=sys.stderr, flush=True)
+            del tstate
+            torch.cuda.empty_cache()
+        del planes
         torch.cuda.empty_cache()
     del u, ca, cb
     torch.cuda.empty_cache()
