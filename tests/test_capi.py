@@ -253,6 +253,9 @@ def test_validation_tiles():
     assert fused(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
     assert fused(1, 4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert fused(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # scalar lattice as the fluid
+    # 9 x 21888^2 = 4,311,760,896 elements: one step past the 2D limit
+    assert fused(0, 3, 21888, 21888, 1, 4) == HIP_INVALID_VALUE
+    assert relayout(0, 21888, 21888, 1, 0, 4) == HIP_INVALID_VALUE
     assert stats(3, 16, 12, 1, 5) == HIP_INVALID_VALUE
     assert stats(4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert stats(0, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a fluid lattice

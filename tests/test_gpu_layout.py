@@ -300,6 +300,36 @@ def test_tiled_fused_step_at_2048_squared(gpu):
     torch.cuda.empty_cache()
 
 
+def test_tiled_fused_step_near_the_element_limit(gpu):
+    """21824^2 cells, FP16 deviation form: the tiled fluid buffer holds
+    9 x 476,286,976 = 4,286,582,784 elements, 0.2% below 2^32 (the tiled
+    limit; 21888^2 is refused), and takes the 64-bit-address kernels.  Two
+    fused MRT steps in tiles of 4 and 64 rows equal the reference's planes
+    bit for bit."""
+    print("case tiles fused 21824^2 f16 fast", flush=True)
+    from lbcube import init as lbinit
+    n = 21824
+    rho, u = lbinit.taylor_green_2d(n, n, 0.04)
+    f = lbinit.equilibrium("D2Q9", rho, u, "f16")
+    ca, cb = lbinit.tanh_half_planes(n, delta=400.0)
+    a = lbinit.scalar_equilibrium("D2Q5", ca, u, "f16", 0.5)
+    b = lbinit.scalar_equilibrium("D2Q5", cb, u, "f16", 0.5)
+    del rho, u, ca, cb
+    torch.cuda.empty_cache()
+    assert 9 * n * n < 1 << 32 < 9 * 21888 * 21888
+    want = run_fused("D2Q9", "MRT", "D2Q5", f, a, b, 2, True, (0.5, 0.5))
+    torch.cuda.empty_cache()
+    for rows in (4, 64):
+        print(f"case tiles fused 21824^2 rows {rows}", flush=True)
+        got = run_fused("D2Q9", "MRT", "D2Q5", f, a, b, 2, True, (0.5, 0.5), rows=rows)
+        for x, y in zip(got, want):
+            assert torch.equal(bits(x), bits(y)), rows
+        del got
+        torch.cuda.empty_cache()
+    del f, a, b, want
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("pair", [("D2Q9", "D2Q5", (12, 13), (0, 4)), ("D3Q19", "D3Q7", (4, 6, 7), (0, 6)),
                                   ("D3Q27", "D3Q7", (4, 6, 7), (0, 6))])
 def test_reactive_pull_stream_is_the_pull_permutation(gpu, pair):
