@@ -250,6 +250,8 @@ def test_validation_tiles():
     assert fluid(1, 8, 8, 8, 3) == HIP_INVALID_VALUE  # 3 does not divide ny = 8 (3D)
     assert fluid(3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a scalar lattice
     assert fluid(1, 1 << 12, 1 << 10, 1 << 7, 1 << 10) == HIP_INVALID_VALUE  # 19 x 2^29 elements
+    assert fluid(1, 610, 610, 610, 610) == HIP_INVALID_VALUE  # 19 x 610^3 > 2^32: slab order
+    assert fluid(1, 610, 610, 610, 61) == HIP_INVALID_VALUE  # and tiles
     assert fused(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
     assert fused(1, 4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert fused(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # scalar lattice as the fluid

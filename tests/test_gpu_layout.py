@@ -179,6 +179,38 @@ def test_slab_order_at_256_cubed(gpu):
     torch.cuda.empty_cache()
 
 
+def test_slab_order_and_tiles_near_the_element_limit(gpu):
+    """608^3 cells, D3Q19 TRT, FP32 deviation form: a buffer in the slab order
+    or in tiles holds 19 x 608^3 = 4,270,358,528 elements, 0.6% below 2^32
+    (the limit of these layouts; 610^3 is refused), and takes the
+    64-bit-address kernels.  Two steps in the slab order and in tiles of 16
+    rows equal the reference's planes bit for bit."""
+    print("case slabs and tiles 608^3", flush=True)
+    from lbcube import init as lbinit
+    n, omega = 608, 1 / 0.51
+    assert 19 * n ** 3 < 1 << 32 < 19 * 610 ** 3
+    u = lbinit.taylor_green_3d(n, 0.05, device=gpu)
+    f = lbinit.equilibrium("D3Q19", None, u, "f32")
+    del u
+    torch.cuda.empty_cache()
+    want = run_planes("D3Q19", "TRT", f, omega, 2, fast=True)
+    torch.cuda.empty_cache()
+    got = run_slabs("D3Q19", "TRT", f, omega, 2, fast=True)
+    assert torch.equal(got, want)
+    del got
+    torch.cuda.empty_cache()
+    print("case tiles of 16 rows 608^3", flush=True)
+    a = lbcube.to_tiles("D3Q19", f, 16)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        lbcube.fluid_step_tiled("D3Q19", "TRT", a, b, omega, fast=True)
+        a, b = b, a
+    del b
+    assert torch.equal(lbcube.from_tiles("D3Q19", a), want)
+    del a, f, want
+    torch.cuda.empty_cache()
+
+
 def test_relayout_tiled_is_the_tile_permutation(gpu):
     """2D tiles of R rows: tile t holds rows tR..tR+R-1 of every plane; 3D
     tiles are the xy-slabs.  Fluid and species lattices, every storage."""
