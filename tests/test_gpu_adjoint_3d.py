@@ -4,6 +4,8 @@ a D3Q19 or D3Q27 fluid (BGK, TRT, MRT, RLBM) with two D3Q7 species, A + B -> C.
   * against the oracle's FP64 CPU adjoint of the same loop (tight tolerance;
     checkpoint intervals 1, 5 and 8 over 17 steps, so the segment recompute
     and its overlap with the adjoint walk are exercised);
+  * blocks of 1024 work-items (the kernels' second instantiation), 2D and 3D,
+    every collision, against the same oracle;
   * the FP32 adjoint, and the FP64 adjoint of the FP32 trajectory, against the
     FP64 gradient;
   * checkpoint-interval invariance, bit for bit;
@@ -17,6 +19,7 @@ import numpy as np
 import pytest
 import torch
 
+import test_oracle_adjoint as TA
 import test_oracle_adjoint_3d as TA3
 from lbcube import adjoint as ADJ
 from oracle import oracle as O
@@ -45,6 +48,33 @@ def test_adjoint_3d_matches_oracle_fp64(gpu, lattice, coll, interval):
         g = got.cpu().numpy()
         assert np.max(np.abs(g - want)) <= 1e-12 * np.max(np.abs(want))
     assert abs(dwg - dw) <= 1e-11 * abs(dw)
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
+@pytest.mark.parametrize("lattice", ["D2Q9", "D3Q19", "D3Q27"])
+def test_adjoint_blocks_of_1024_match_oracle(gpu, lattice, coll):
+    """The kernels' 1024-work-item instantiations (the largest block the
+    reference's launchers accept; they spill registers to scratch) against
+    the oracle's FP64 adjoint, with checkpoints every 5 steps: blocks of
+    1024 x 1 x 1 and, in 3D, 16 x 16 x 4."""
+    print(f"case adjoint blocks of 1024 {lattice} {coll}", flush=True)
+    if lattice == "D2Q9":
+        f, a, b = TA.small_state(nx=40, ny=30, seed=4)
+        blocks = [(1024, 1, 1), (32, 32, 1)]
+        kw = {}
+    else:
+        f, a, b = TA3.small_state_3d(lattice, nx=12, ny=10, nz=9, seed=4)
+        blocks = [(1024, 1, 1), (16, 16, 4)]
+        kw = {"lattice": lattice}
+    omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 11
+    J, gf, ga, gb, dw = O.reactive_gradient(coll, "f64", f, a, b, omega, omega_c, k, steps, **kw)
+    for block in blocks:
+        Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient(coll, dev(f, gpu), dev(a, gpu), dev(b, gpu), steps, omega,
+                                                       omega_c, k, interval=5, block=block, **kw)
+        assert abs(Jg - J) <= 1e-13 * abs(J), block
+        for got, want in ((gfg, gf), (gag, ga), (gbg, gb)):
+            assert np.max(np.abs(got.cpu().numpy() - want)) <= 1e-12 * np.max(np.abs(want)), block
+        assert abs(dwg - dw) <= 1e-11 * abs(dw), block
 
 
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
