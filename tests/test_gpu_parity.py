@@ -175,6 +175,45 @@ def test_reaction_step_parity(gpu, pair, storage, refs):
         assert np.mean(got == want) > 0.5
 
 
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
+def test_reactive_kernels_blocks_of_1024(gpu, pair, storage):
+    """The scalar, reaction and fused steps' 1024-work-item instantiations
+    (blocks of 1024 x 1 x 1 and 32 x 32 x 1 / 16 x 16 x 4, several blocks per
+    launch) give the default launch's results bit for bit, every collision and
+    both arithmetic forms."""
+    fl, sl = pair
+    print(f"case reactive blocks of 1024 {fl} {storage}", flush=True)
+    shape = (70, 90) if fl == "D2Q9" else (12, 14, 18)
+    refs = (0.5, 0.25) if storage in ("f16", "bf16") else (0.0, 0.0)
+    f = to_dev(cases.random_perturbed(fl, shape, storage, seed=2), storage, gpu)
+    a0, b0 = _species(sl, shape, storage, seed=8, refs=refs)
+    a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+    blocks = [(1024, 1, 1), (32, 32, 1) if fl == "D2Q9" else (16, 16, 4)]
+
+    def run(block):
+        out = []
+        s = torch.empty_like(a)
+        lbcube.scalar_step(fl, sl, a, s, f, 1.25, 3e-4, block=block, ref=refs[0])
+        out.append(s)
+        na, nb = torch.empty_like(a), torch.empty_like(b)
+        lbcube.reaction_step(fl, sl, a, na, b, nb, f, 1.1, 0.05, block=block, ref_a=refs[0], ref_b=refs[1])
+        out += [na, nb]
+        for coll in COLLS:
+            for fast in ((False, True) if coll != "RLBM" else (False,)):
+                nf, na, nb = torch.empty_like(f), torch.empty_like(a), torch.empty_like(b)
+                lbcube.reactive_fused_step(fl, coll, sl, f, nf, a, na, b, nb, 1.3, 1.05, 0.02, block=block,
+                                           fast=fast, ref_a=refs[0], ref_b=refs[1])
+                out += [nf, na, nb]
+        torch.cuda.synchronize()
+        return out
+
+    want = run(None)
+    for block in blocks:
+        for x, y in zip(run(block), want):
+            assert torch.equal(x.view(torch.uint8), y.view(torch.uint8)), block
+
+
 @pytest.mark.parametrize("refs", [(0.0, 0.0), (0.5, 0.25)])
 @pytest.mark.parametrize("storage", STORES)
 @pytest.mark.parametrize("coll", COLLS)
