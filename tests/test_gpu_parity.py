@@ -70,6 +70,46 @@ def test_block_shape_does_not_change_results(gpu, block):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("lattice", FLUIDS)
+def test_fluid_kernels_blocks_of_1024(gpu, lattice, storage):
+    """The fluid step's 1024-work-item instantiations, every collision and
+    arithmetic form (and the MFMA form where it applies), on the reference's
+    planes and in tiles: blocks of 1024 x 1 x 1 and 32 x 32 x 1 / 16 x 16 x 4,
+    several per launch, give the default launch's results bit for bit."""
+    print(f"case fluid blocks of 1024 {lattice} {storage}", flush=True)
+    two_d = lattice == "D2Q9"
+    shape = (70, 90) if two_d else (12, 14, 18)
+    f = to_dev(cases.random_perturbed(lattice, shape, storage, seed=6), storage, gpu)
+    blocks = [(1024, 1, 1), (32, 32, 1) if two_d else (16, 16, 4)]
+    rows_set = (7, 70) if two_d else (7, 14)
+
+    def run(block):
+        out = []
+        for coll in COLLS:
+            for fast in ((False, True) if coll != "RLBM" else (False,)):
+                n = torch.empty_like(f)
+                lbcube.fluid_step(lattice, coll, f, n, 1.3, block=block, fast=fast)
+                out.append(n)
+        if not two_d and storage in ("f16", "bf16"):
+            n = torch.empty_like(f)
+            lbcube.fluid_step(lattice, "MRT", f, n, 1.3, block=block, fast=True, mfma=True)
+            out.append(n)
+        for rows in rows_set:
+            t = lbcube.to_tiles(lattice, f, rows)
+            for coll in ("TRT", "MRT"):
+                n = torch.empty_like(t)
+                lbcube.fluid_step_tiled(lattice, coll, t, n, 1.3, block=block, fast=True)
+                out.append(lbcube.from_tiles(lattice, n))
+        torch.cuda.synchronize()
+        return out
+
+    want = run(None)
+    for block in blocks:
+        for x, y in zip(run(block), want):
+            assert torch.equal(x.view(torch.uint8), y.view(torch.uint8)), block
+
+
 @pytest.mark.parametrize("lattice,shape", [("D3Q19", (3, 5, 263)), ("D2Q9", (70001, 3)), ("D3Q27", (2, 3, 1000)),
                                            ("D2Q9", (2, 4099))])
 def test_linear_cell_mapping_odd_extents(gpu, lattice, shape):
