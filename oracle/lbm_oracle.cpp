@@ -45,6 +45,12 @@
 //                                        high-precision ODE integration
 //   statistics ......................... closed forms (uniform, complementary
 //                                        half planes) and long double sums
+//   discrete adjoint (2D and 3D, ....... central differences of this forward,
+//   BGK/TRT/MRT/RLBM; no counterpart     the one-step identity <L1, J v> =
+//   in the reference)                    <J^T L1, v> against an extended-
+//                                        precision tangent, and dJ/d omega_c
+//                                        over 200-400 steps against extended-
+//                                        precision differences
 //   Eigen's summation grouping in dot() is not observable here; dot products
 //   are taken left to right (c_x u_x + c_y u_y) + c_z u_z.
 // ============================================================================
@@ -1030,13 +1036,14 @@ template <class S, int DT> void scalar_feq_field(const double* conc, const doubl
 
 // ===========================================================================
 // Discrete adjoint of the 2D reactive loop (project-written; no counterpart in
-// the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK, TRT or MRT),
+// the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK, TRT, MRT or RLBM),
 // then (a_{n+1}, b_{n+1}) = A+B->C step of (a_n, b_n) advected by u(f_{n+1})
 // at the same cell (the order of src/simulate_reaction.cpp:548-554).
 // Adjoint buffers are kept "at the collision cell": nu[i][x] holds the
 // derivative with respect to the population i gathered by cell x, so the
 // adjoint of the stored state is Lambda[i][y] = nu[i][y + c_i] (reverse pull).
-// Pinned by central finite differences of the forward (tests).
+// Pinned by central finite differences of the forward and the one-step
+// dot-product identity (tests); the 3D loop's adjoint follows below.
 // ===========================================================================
 
 // Derivatives of the exact one-step A+B->C map.  With D = B - A (invariant)
