@@ -1036,7 +1036,8 @@ template <class S, int DT> void scalar_feq_field(const double* conc, const doubl
 
 // ===========================================================================
 // Discrete adjoint of the reactive loop, 2D (D2Q9 + 2 D2Q5) and 3D (D3Q19 /
-// D3Q27 + 2 D3Q7) (project-written; no counterpart in the reference).  Forward step n: f_{n+1} = stream+collide(f_n) (BGK, TRT, MRT or RLBM),
+// D3Q27 + 2 D3Q7) (project-written; no counterpart in the reference).
+// Forward step n: f_{n+1} = stream+collide(f_n) (BGK, TRT, MRT or RLBM),
 // then (a_{n+1}, b_{n+1}) = A+B->C step of (a_n, b_n) advected by u(f_{n+1})
 // at the same cell (the order of src/simulate_reaction.cpp:548-554).
 // Adjoint buffers are kept "at the collision cell": nu[i][x] holds the
