@@ -18,7 +18,8 @@ FAKE_A, FAKE_B = 0x1000, 0x2000  # never dereferenced: validation fails first
 
 
 def declared_entry_points():
-    text = open(os.path.join(ROOT, "include", "lbcube.h")).read()
+    with open(os.path.join(ROOT, "include", "lbcube.h")) as fh:
+        text = fh.read()
     return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(lbc_\w+)\s*\(", text, flags=re.M)))
 
 
