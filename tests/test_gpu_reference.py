@@ -6,7 +6,10 @@ This module collects the same test functions again and runs them with every
 step taken on the device through the C ABI, in FP64 storage (the reference's
 tests use double): the fluid loops through lbc_fluid_step, the passive-scalar
 test through lbc_scalar_step, the reaction tests through
-lbc_reaction_step.  The oracle only builds the initial fields here.  So the
+lbc_reaction_step.  Each test runs twice: in the reference's evaluation order
+(bit-exact to the oracle) and with the fluid step's deviation-form
+arithmetic (LBC_FLAG_FAST; RLBM has none).  The oracle only builds the
+initial fields here.  So the
 kernels meet the reference's analytic expectations (decay rates, closed
 forms, convergence orders) at the reference's tolerances directly, not only
 through their bit-parity with the oracle.  The MRT transform-identity tests
@@ -39,6 +42,7 @@ class _DeviceSteps:
     `launches` counts the device steps a test took."""
 
     launches = 0
+    fast = False
 
     def __getattr__(self, name):
         return getattr(O, name)
@@ -50,7 +54,7 @@ class _DeviceSteps:
         a = _to(f)
         b = torch.empty_like(a)
         for _ in range(steps):
-            lbcube.fluid_step(lattice, collision, a, b, omega)
+            lbcube.fluid_step(lattice, collision, a, b, omega, fast=_DeviceSteps.fast)
             a, b = b, a
         return a.cpu().numpy()
 
@@ -77,11 +81,12 @@ class _DeviceSteps:
         return na.cpu().numpy(), nb.cpu().numpy()
 
 
-@pytest.fixture(autouse=True)
+@pytest.fixture(autouse=True, params=[False, True], ids=["reference_order", "deviation_form"])
 def _steps_on_the_device(gpu, monkeypatch, request):
     print(f"case reference suite on the GPU: {request.node.name}", flush=True)
     monkeypatch.setattr(TR, "O", _DeviceSteps())
     _DeviceSteps.launches = 0
+    _DeviceSteps.fast = request.param
     yield
     assert _DeviceSteps.launches > 0, "the test took no step on the device"
 
