@@ -12,7 +12,8 @@ arithmetic (LBC_FLAG_FAST; RLBM has none).  The oracle only builds the
 initial fields here.  So the
 kernels meet the reference's analytic expectations (decay rates, closed
 forms, convergence orders) at the reference's tolerances directly, not only
-through their bit-parity with the oracle.  The MRT transform-identity tests
+through their bit-parity with the oracle.  Two of the analytic tests are
+also restated on the fused reactive step.  The MRT transform-identity tests
 exercise no kernel and stay in the CPU suite; the three D3Q19 MRT
 expectations that do not hold for the reference's own algorithm remain strict
 xfails.
@@ -97,3 +98,48 @@ from test_oracle_reference import (  # noqa: E402,F401  (collected here, run on 
     test_shear_wave_decay, test_spatial_convergence_2d, test_spatial_convergence_3d, test_taylor_green_2d_total_energy,
     test_taylor_green_3d_velocity, test_temporal_convergence, test_tke_angled_shear_3d, test_tke_shear_wave,
     test_tke_taylor_green_2d)
+
+
+# The fused reactive step (lbc_reactive_fused_step: fluid collision, then both
+# species advected by the new velocity, one pass) against the same analytic
+# tests, in the form the reference's own loops would take it.
+def _fused_run(f, a, b, coll, omega, omega_c, k, steps):
+    _DeviceSteps.launches += steps
+    st = [_to(f), None, _to(a), None, _to(b), None]
+    st[1], st[3], st[5] = (torch.empty_like(x) for x in (st[0], st[2], st[4]))
+    for _ in range(steps):
+        lbcube.reactive_fused_step("D2Q9", coll, "D2Q5", *st, omega, omega_c, k, fast=_DeviceSteps.fast)
+        st = [st[1], st[0], st[3], st[2], st[5], st[4]]
+    return st[0].cpu().numpy(), st[2].cpu().numpy(), st[4].cpu().numpy()
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+def test_fused_passive_advection_diffusion(coll):
+    """ReactiveTransport.PassiveScalarAdvectionDiffusion (test_validation.cpp:
+    1590-1646) through the fused step with k = 0: a uniform flow u_x = 0.05
+    (omega = 1, an equilibrium every collision keeps) carries both species'
+    sine, C = C0 + a sin(k (x - U t)) e^{-D k^2 t}, D = cs2 (tau_c - 1/2)."""
+    nx, ny, steps, ux, tau_c = 64, 8, 200, 0.05, 0.8
+    dc, kx = TR.CS2 * (tau_c - 0.5), 2 * np.pi / nx
+    u = np.stack([np.full((ny, nx), ux), np.zeros((ny, nx))])
+    f = O.feq_field("D2Q9", "f64", np.ones((ny, nx)), u)
+    x = np.arange(nx) * np.ones((ny, 1))
+    g = O.scalar_feq_field("D2Q5", "f64", 1.0 + 0.1 * np.sin(kx * x), u)
+    _, a, b = _fused_run(f, g, g, coll, 1.0, 1 / tau_c, 0.0, steps)
+    want = 1.0 + 0.1 * np.sin(kx * (x - ux * steps)) * np.exp(-dc * kx * kx * steps)
+    for s in (a, b):
+        assert np.max(np.abs(s.sum(0) - want)) < 1e-4
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
+def test_fused_reaction_kinetics_batch(coll):
+    """MathAndTransforms.ReactionKineticsBatch (test_validation.cpp:1655-1702)
+    through the fused step: the fluid at rest (a fixed point of every
+    collision), A = B = 1, k = 0.005, 200 steps: C(t) = C0 / (1 + C0 k t)."""
+    n, steps, c0, kr, tau_c = 16, 200, 1.0, 0.005, 0.8
+    u = np.zeros((2, n, n))
+    f = O.feq_field("D2Q9", "f64", np.ones((n, n)), u)
+    a = O.scalar_feq_field("D2Q5", "f64", np.full((n, n), c0), u)
+    _, a, b = _fused_run(f, a, a, coll, 1 / 0.7, 1 / tau_c, kr, steps)
+    want = c0 / (1.0 + c0 * kr * steps)
+    assert np.max(np.abs(a.sum(0) - want)) < 1e-4 and np.max(np.abs(b.sum(0) - want)) < 1e-4
