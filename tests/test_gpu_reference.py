@@ -143,3 +143,64 @@ def test_fused_reaction_kinetics_batch(coll):
     _, a, b = _fused_run(f, a, a, coll, 1 / 0.7, 1 / tau_c, kr, steps)
     want = c0 / (1.0 + c0 * kr * steps)
     assert np.max(np.abs(a.sum(0) - want)) < 1e-4 and np.max(np.abs(b.sum(0) - want)) < 1e-4
+
+
+def _fused_run_3d(fl, f, a, b, coll, omega, omega_c, k, steps):
+    _DeviceSteps.launches += steps
+    st = [_to(f), None, _to(a), None, _to(b), None]
+    st[1], st[3], st[5] = (torch.empty_like(x) for x in (st[0], st[2], st[4]))
+    for _ in range(steps):
+        lbcube.reactive_fused_step(fl, coll, "D3Q7", *st, omega, omega_c, k, fast=_DeviceSteps.fast)
+        st = [st[1], st[0], st[3], st[2], st[5], st[4]]
+    return st[0].cpu().numpy(), st[2].cpu().numpy(), st[4].cpu().numpy()
+
+
+def _sine_3d(fl):
+    n = 32
+    u0 = np.array([0.04, -0.03, 0.02])
+    kv = 2 * np.pi / n * np.array([1.0, 1.0, 1.0])
+    shape = (n, n, n)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u = np.stack([np.full(shape, v) for v in u0])
+    f = O.feq_field(fl, "f64", np.ones(shape), u)
+    phase = kv[0] * x + kv[1] * y + kv[2] * z
+    g = O.scalar_feq_field("D3Q7", "f64", 1.0 + 0.1 * np.sin(phase), u)
+    return f, g, u0, kv, phase
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT"])
+@pytest.mark.parametrize("fl", ["D3Q19", "D3Q27"])
+def test_d3q7_advected_decaying_sine(fl, coll):
+    """The advected decaying sine (test_validation.cpp:1590-1646) in 3D on
+    D3Q7, cs2 = 1/4: a uniform flow (omega = 1) carries a sine with a wave
+    vector along (1, 1, 1); C = C0 + A sin(k.(x - u t)) e^{-D |k|^2 t},
+    D = cs2 (tau_c - 1/2), within 1e-4 after 200 steps, through the
+    separate fluid and scalar steps and through the fused step (k = 0)."""
+    steps, tau_c = 200, 0.8
+    f, g, u0, kv, phase = _sine_3d(fl)
+    want = 1.0 + 0.1 * np.sin(phase - (u0 @ kv) * steps) * np.exp(-0.25 * (tau_c - 0.5) * (kv @ kv) * steps)
+    fd, gd = _to(f), _to(g)
+    ft, gt = torch.empty_like(fd), torch.empty_like(gd)
+    _DeviceSteps.launches += steps
+    for _ in range(steps):
+        lbcube.fluid_step(fl, coll, fd, ft, 1.0, fast=_DeviceSteps.fast)
+        lbcube.scalar_step(fl, "D3Q7", gd, gt, ft, 1 / tau_c, 0.0)
+        fd, ft, gd, gt = ft, fd, gt, gd
+    assert np.max(np.abs(gd.cpu().numpy().sum(0) - want)) < 1e-4
+    _, a, b = _fused_run_3d(fl, f, g, g, coll, 1.0, 1 / tau_c, 0.0, steps)
+    for s in (a, b):
+        assert np.max(np.abs(s.sum(0) - want)) < 1e-4
+
+
+@pytest.mark.parametrize("coll", ["BGK", "TRT", "MRT", "RLBM"])
+@pytest.mark.parametrize("fl", ["D3Q19", "D3Q27"])
+def test_fused_reaction_kinetics_batch_3d(fl, coll):
+    """The batch reactor (test_validation.cpp:1655-1702) on the 3D fused
+    step: fluid at rest, A = B = 1 on D3Q7, k = 0.005, 200 steps."""
+    n, steps, c0, kr, tau_c = 12, 200, 1.0, 0.005, 0.8
+    u = np.zeros((3, n, n, n))
+    f = O.feq_field(fl, "f64", np.ones((n, n, n)), u)
+    a = O.scalar_feq_field("D3Q7", "f64", np.full((n, n, n), c0), u)
+    _, a, b = _fused_run_3d(fl, f, a, a, coll, 1 / 0.7, 1 / tau_c, kr, steps)
+    want = c0 / (1.0 + c0 * kr * steps)
+    assert np.max(np.abs(a.sum(0) - want)) < 1e-4 and np.max(np.abs(b.sum(0) - want)) < 1e-4
