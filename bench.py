@@ -456,8 +456,8 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
     scalars, FP32): the passive-scalar step (reads the fluid, pulls one
     scalar: (19 + 2 x 7) x 4 = 132 B/cell), the A + B -> C reaction step
     ((19 + 4 x 7) x 4 = 188 B/cell) and the fused TRT + reaction step in both
-    arithmetic forms ((2 x 19 + 4 x 7) x 4 = 264 B/cell).  CUDA-graph replays
-    of 10-step chunks."""
+    arithmetic forms ((2 x 19 + 4 x 7) x 4 = 264 B/cell), on the reference's
+    planes and in the slab order.  CUDA-graph replays of 10-step chunks."""
     from lbcube import init as lbinit
 
     u = lbinit.taylor_green_3d(n, 0.05, device="cuda")
@@ -499,6 +499,16 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
         lbcube.reactive_fused_step_tiled("D3Q19", "TRT", "D3Q7", f, ft, a, at, b, bt, omega, omega_c, k, fast=fast)
         sstate[:] = [ft, f, at, a, bt, b]
 
+    # the reference's separate scalar and reaction launches in the slab order
+    def scalar_slabs():
+        lbcube.scalar_step_tiled("D3Q19", "D3Q7", sstate[2], sstate[3], sstate[0], omega_c, source=1e-6)
+        sstate[2], sstate[3] = sstate[3], sstate[2]
+
+    def reaction_slabs():
+        lbcube.reaction_step_tiled("D3Q19", "D3Q7", sstate[2], sstate[3], sstate[4], sstate[5], sstate[0], omega_c,
+                                   k)
+        sstate[2], sstate[3], sstate[4], sstate[5] = sstate[3], sstate[2], sstate[5], sstate[4]
+
     def pattern(st):
         lbcube.reactive_pull_stream("D3Q19", "D3Q7", *st, tiled=st is sstate)
         st[:] = [st[1], st[0], st[3], st[2], st[5], st[4]]
@@ -507,6 +517,7 @@ def adr_3d_rate(lbcube, torch, n=256, steps=40):
     for name, fn, bpc in (("fused_pattern", lambda: pattern(state), 264),
                           ("fused_pattern_slabs", lambda: pattern(sstate), 264),
                           ("scalar_step", scalar, 132), ("reaction_step", reaction, 188),
+                          ("scalar_step_slabs", scalar_slabs, 132), ("reaction_step_slabs", reaction_slabs, 188),
                           ("fused_trt", lambda: fused(False), 264), ("fused_trt_fast", lambda: fused(True), 264),
                           ("fused_trt_slabs", lambda: fused_slabs(False), 264),
                           ("fused_trt_fast_slabs", lambda: fused_slabs(True), 264)):
@@ -760,6 +771,29 @@ def reactive_2d_tiles(lbcube, torch, u, ca, cb, k, n, steps, row_set=(2, 4, 8, 3
             del f, a, b, state
         del planes
         torch.cuda.empty_cache()
+    # the reference's two launches per step (fluid step, then reaction step:
+    # 188 B/cell in FP32) in tiles of 4 rows
+    planes = [lbinit.equilibrium("D2Q9", None, u, "f32"), lbinit.scalar_equilibrium("D2Q5", ca, u, "f32"),
+              lbinit.scalar_equilibrium("D2Q5", cb, u, "f32")]
+    tiles = [lbcube.to_tiles(lat, x, 4) for lat, x in zip(("D2Q9", "D2Q5", "D2Q5"), planes)]
+    del planes
+    tstate = [tiles[0], torch.empty_like(tiles[0]), tiles[1], torch.empty_like(tiles[1]), tiles[2],
+              torch.empty_like(tiles[2])]
+    del tiles
+
+    def two_launches(st=tstate):
+        f, ft, a, at, b, bt = st
+        lbcube.fluid_step_tiled("D2Q9", "MRT", f, ft, 1 / 0.6)
+        lbcube.reaction_step_tiled("D2Q9", "D2Q5", a, at, b, bt, ft, 1 / 0.6, k)
+        st[:] = [ft, f, at, a, bt, b]
+
+    ms = _graph_ms(torch, two_launches, steps)
+    out["two_launches_f32_rows4"] = {"mlups": n * n / (ms * 1e-3) / 1e6,
+                                     "achieved_gbs": 47 * 4 * n * n / (ms * 1e-3) / 1e9, "bytes_per_cell": 188}
+    print(f"# reactive 2D two launches in tiles of 4 rows (FP32): {out['two_launches_f32_rows4']['mlups']:.0f} MLUPS",
+          file=sys.stderr, flush=True)
+    del tstate
+    torch.cuda.empty_cache()
     # the fused step's pattern ceiling (lbc_reactive_pull_stream: its loads,
     # stores, addressing and block order without arithmetic) on the planes and
     # in tiles of 4 and 8 rows

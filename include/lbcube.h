@@ -204,6 +204,21 @@ int lbc_reactive_fused_step_tiled(int fluid_lattice, int collision, int scalar_l
                                   double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
                                   uint32_t block_z, void* stream);
 
+/* lbc_scalar_step and lbc_reaction_ab_step (the reference's separate scalar
+ * and reaction launches) with the fluid and the species in tiles of the same
+ * rows (each buffer with its own lattice's Q per tile); tile_rows 0 is the
+ * reference's planes.  Validation as those calls, then as
+ * lbc_reactive_fused_step_tiled. */
+int lbc_scalar_step_tiled(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
+                          void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
+                          uint64_t tile_rows, double omega_c, double source_term, double reference, uint32_t block_x,
+                          uint32_t block_y, uint32_t block_z, void* stream);
+int lbc_reaction_ab_step_tiled(int fluid_lattice, int scalar_lattice, int storage, const void* current_a,
+                               void* next_a, const void* current_b, void* next_b, const void* current_fluid,
+                               uint64_t nx, uint64_t ny, uint64_t nz, uint64_t tile_rows, double omega_c,
+                               double k_react, double reference_a, double reference_b, uint32_t block_x,
+                               uint32_t block_y, uint32_t block_z, void* stream);
+
 /* The fused step's memory pattern alone (the ceiling of
  * lbc_reactive_fused_step(_tiled) on the same buffers): every fluid and
  * species population pulled from x - c_i (shift != 0) or read in place

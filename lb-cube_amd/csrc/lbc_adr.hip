@@ -24,33 +24,40 @@ namespace lbc {
 // A + B -> C step (REACT = true: two species, source from the exact one-step
 // reaction increment).  The advecting velocity is the fluid's at the
 // destination cell, unshifted (include/lattice_core.hpp:259-265, :381-387).
-template <class F, class S, int ST, bool REACT, int MAXT, bool WIDE>
+// TILED: the buffers stored in tiles (tile_layout), the fluid's spaced by
+// c.zgap and the species' by zgap_s, each `plane` cells per population and
+// tile; tiled grids keep the dispatch order of their blocks, as the fused
+// step's do.
+template <class F, class S, int ST, bool REACT, int MAXT, bool WIDE, bool TILED>
 __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     const typename Storage<ST>::S* __restrict__ cur_a, typename Storage<ST>::S* __restrict__ nxt_a,
     const typename Storage<ST>::S* __restrict__ cur_b, typename Storage<ST>::S* __restrict__ nxt_b,
-    const typename Storage<ST>::S* __restrict__ cur_f, Cells c, typename Storage<ST>::R omega_c,
-    typename Storage<ST>::R param, typename Storage<ST>::R ref_a, typename Storage<ST>::R ref_b) {
+    const typename Storage<ST>::S* __restrict__ cur_f, Cells c, uint32_t plane, uint32_t zgap_s,
+    typename Storage<ST>::R omega_c, typename Storage<ST>::R param, typename Storage<ST>::R ref_a,
+    typename Storage<ST>::R ref_b) {
     using St = Storage<ST>;
     using R = typename St::R;
-    const uint32_t block = renumber_blocks<S::D, typename Storage<ST>::S>() ? xcd_block() : blockIdx.x;
+    const uint32_t block = (!TILED && renumber_blocks<S::D, typename Storage<ST>::S>()) ? xcd_block() : blockIdx.x;
     const uint32_t cell = block * blockDim.x + threadIdx.x;
     if (cell >= c.count) return;
     const CellCoord q = cell_coord(cell, c);
     const int x = q.x, y = q.y, z = q.z;
     const int nx = static_cast<int>(c.nx), ny = static_cast<int>(c.ny), nz = static_cast<int>(c.nz);
-    const size_t plane = c.count;
+    const size_t pl = TILED ? plane : c.count;
+    // the fluid's element of this cell in its plane
+    const uint32_t fcell = TILED ? locate<S::D>(cell, q, c, c.zgap).dst : cell;
     R fl[F::Q];
 #pragma unroll
     for (int i = 0; i < F::Q; ++i)
-        fl[i] = St::load(at<WIDE>(plane_ptr(cur_f, i, plane), cell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
+        fl[i] = St::load(at<WIDE>(plane_ptr(cur_f, i, pl), fcell), kOffsetStorage<ST> ? R(F::w[i]) : R(0));
     const Macro<R> m = macro_state<F, R>(fl);
-    const Pull pull(cell, x, y, z, nx, ny, nz);
+    const Pull pull = TILED ? locate<S::D>(cell, q, c, zgap_s) : Pull(cell, x, y, z, nx, ny, nz);
     R ga[S::Q], gb[S::Q];
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
         const uint32_t src = pull.src(S::cx[i], S::cy[i], S::cz[i]);
-        ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, plane), src), R(S::w[i]) * ref_a);
-        if constexpr (REACT) gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, plane), src), R(S::w[i]) * ref_b);
+        ga[i] = St::load(at<WIDE>(plane_ptr(cur_a, i, pl), src), R(S::w[i]) * ref_a);
+        if constexpr (REACT) gb[i] = St::load(at<WIDE>(plane_ptr(cur_b, i, pl), src), R(S::w[i]) * ref_b);
     }
     if constexpr (REACT) {
         const R ca = concentration<S, R>(ga);
@@ -61,10 +68,11 @@ __global__ __launch_bounds__(MAXT) void scalar_step_kernel(
     } else {
         collide_scalar<S, R>(ga, m.ux, m.uy, m.uz, omega_c, param);
     }
+    const uint32_t scell = TILED ? pull.dst : cell;
 #pragma unroll
     for (int i = 0; i < S::Q; ++i) {
-        St::store(at<WIDE>(plane_ptr(nxt_a, i, plane), cell), ga[i], R(S::w[i]) * ref_a);
-        if constexpr (REACT) St::store(at<WIDE>(plane_ptr(nxt_b, i, plane), cell), gb[i], R(S::w[i]) * ref_b);
+        St::store(at<WIDE>(plane_ptr(nxt_a, i, pl), scell), ga[i], R(S::w[i]) * ref_a);
+        if constexpr (REACT) St::store(at<WIDE>(plane_ptr(nxt_b, i, pl), scell), gb[i], R(S::w[i]) * ref_b);
     }
 }
 
@@ -164,19 +172,29 @@ hipError_t launch_scalar(const void* ca, void* na, const void* cb, void* nb, con
     using St = Storage<ST>;
     using T = typename St::S;
     using R = typename St::R;
+    // `s` carries the tiles of the call (tile_layout, with the fluid's gap);
+    // one 2D tile holding every row is the reference's planes
+    const bool tiled = s.cells.zgap != 0 && (S::D == 3 || s.cells.tiles > 1);
+    const uint32_t plane = tiled ? s.cells.zgap / (F::Q - 1) : s.cells.count;
+    const uint32_t zgap_s = tiled ? (S::Q - 1) * plane : 0u;
     const uint32_t lds = capped_residency_adr<S::D, ST, false>() ? capped_lds(s.threads()) : 0;
     auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, s.grid, s.block, lds, stream, static_cast<const T*>(ca), static_cast<T*>(na),
-                           static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells,
-                           static_cast<R>(omega_c), static_cast<R>(param), static_cast<R>(ref_a),
+                           static_cast<const T*>(cb), static_cast<T*>(nb), static_cast<const T*>(cf), s.cells, plane,
+                           zgap_s, static_cast<R>(omega_c), static_cast<R>(param), static_cast<R>(ref_a),
                            static_cast<R>(ref_b));
     };
     const bool wide = wide_planes<T>(s);
-    if (s.threads() <= 256)
-        wide ? args(scalar_step_kernel<F, S, ST, REACT, 256, true>) : args(scalar_step_kernel<F, S, ST, REACT, 256, false>);
-    else
-        wide ? args(scalar_step_kernel<F, S, ST, REACT, 1024, true>)
-             : args(scalar_step_kernel<F, S, ST, REACT, 1024, false>);
+    auto pick = [&](auto tl) {
+        constexpr bool TL = decltype(tl)::value;
+        if (s.threads() <= 256)
+            wide ? args(scalar_step_kernel<F, S, ST, REACT, 256, true, TL>)
+                 : args(scalar_step_kernel<F, S, ST, REACT, 256, false, TL>);
+        else
+            wide ? args(scalar_step_kernel<F, S, ST, REACT, 1024, true, TL>)
+                 : args(scalar_step_kernel<F, S, ST, REACT, 1024, false, TL>);
+    };
+    tiled ? pick(std::true_type{}) : pick(std::false_type{});
     return hipGetLastError();
 }
 
@@ -312,19 +330,66 @@ hipError_t reactive_pull_by_storage(int storage, const void* cf, void* nf, const
 
 }  // namespace lbc
 
+namespace lbc {
+
+// The tiles of a call's buffers (tile_rows 0: the reference's planes), with
+// the fluid's gap; the fluid lattice must be a fluid.
+hipError_t scalar_tiles(int fluid_lattice, int scalar_lattice, uint64_t tile_rows, LaunchShape& s) {
+    if (tile_rows == 0) return hipSuccess;
+    const int qf = q_of(fluid_lattice);
+    if (qf == 0 || fluid_lattice == kD2Q5 || fluid_lattice == kD3Q7) return hipErrorInvalidValue;
+    uint64_t plane = 0;
+    return tile_layout(is_2d(scalar_lattice), tile_rows, qf, s, plane);
+}
+
+hipError_t scalar_entry(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
+                        void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
+                        uint64_t tile_rows, double omega_c, double source_term, double reference, uint32_t block_x,
+                        uint32_t block_y, uint32_t block_z, void* stream) {
+    if (current_scalar == nullptr || next_scalar == nullptr || current_fluid == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_scalar == next_scalar || next_scalar == current_fluid) return hipErrorInvalidValue;
+    if ((e = scalar_tiles(fluid_lattice, scalar_lattice, tile_rows, s)) != hipSuccess) return e;
+    return scalar_dispatch<false>(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, nullptr,
+                                  nullptr, current_fluid, s, omega_c, source_term, reference, 0.0, as_stream(stream));
+}
+
+hipError_t reaction_entry(int fluid_lattice, int scalar_lattice, int storage, const void* current_a, void* next_a,
+                          const void* current_b, void* next_b, const void* current_fluid, uint64_t nx, uint64_t ny,
+                          uint64_t nz, uint64_t tile_rows, double omega_c, double k_react, double reference_a,
+                          double reference_b, uint32_t block_x, uint32_t block_y, uint32_t block_z, void* stream) {
+    if (current_a == nullptr || next_a == nullptr || current_b == nullptr || next_b == nullptr ||
+        current_fluid == nullptr)
+        return hipErrorInvalidDevicePointer;
+    LaunchShape s;
+    hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
+    if (e != hipSuccess) return e;
+    if (current_a == next_a || current_b == next_b || next_a == next_b) return hipErrorInvalidValue;
+    if ((e = scalar_tiles(fluid_lattice, scalar_lattice, tile_rows, s)) != hipSuccess) return e;
+    return scalar_dispatch<true>(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
+                                 current_fluid, s, omega_c, k_react, reference_a, reference_b, as_stream(stream));
+}
+
+}  // namespace lbc
+
 extern "C" int lbc_scalar_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
                                void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
                                double omega_c, double source_term, double reference, uint32_t block_x,
                                uint32_t block_y, uint32_t block_z, void* stream) {
-    using namespace lbc;
-    if (current_scalar == nullptr || next_scalar == nullptr || current_fluid == nullptr)
-        return hipErrorInvalidDevicePointer;
-    LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
-    if (e != hipSuccess) return e;
-    if (current_scalar == next_scalar || next_scalar == current_fluid) return hipErrorInvalidValue;
-    return scalar_dispatch<false>(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, nullptr,
-                                  nullptr, current_fluid, s, omega_c, source_term, reference, 0.0, as_stream(stream));
+    return lbc::scalar_entry(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, current_fluid, nx,
+                             ny, nz, 0, omega_c, source_term, reference, block_x, block_y, block_z, stream);
+}
+
+extern "C" int lbc_scalar_step_tiled(int fluid_lattice, int scalar_lattice, int storage, const void* current_scalar,
+                                     void* next_scalar, const void* current_fluid, uint64_t nx, uint64_t ny,
+                                     uint64_t nz, uint64_t tile_rows, double omega_c, double source_term,
+                                     double reference, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                     void* stream) {
+    return lbc::scalar_entry(fluid_lattice, scalar_lattice, storage, current_scalar, next_scalar, current_fluid, nx,
+                             ny, nz, tile_rows, omega_c, source_term, reference, block_x, block_y, block_z, stream);
 }
 
 extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int storage, const void* current_a,
@@ -332,16 +397,20 @@ extern "C" int lbc_reaction_ab_step(int fluid_lattice, int scalar_lattice, int s
                                     uint64_t nx, uint64_t ny, uint64_t nz, double omega_c, double k_react,
                                     double reference_a, double reference_b, uint32_t block_x, uint32_t block_y,
                                     uint32_t block_z, void* stream) {
-    using namespace lbc;
-    if (current_a == nullptr || next_a == nullptr || current_b == nullptr || next_b == nullptr ||
-        current_fluid == nullptr)
-        return hipErrorInvalidDevicePointer;
-    LaunchShape s;
-    const hipError_t e = make_shape(is_2d(scalar_lattice), nx, ny, nz, block_x, block_y, block_z, s);
-    if (e != hipSuccess) return e;
-    if (current_a == next_a || current_b == next_b || next_a == next_b) return hipErrorInvalidValue;
-    return scalar_dispatch<true>(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
-                                 current_fluid, s, omega_c, k_react, reference_a, reference_b, as_stream(stream));
+    return lbc::reaction_entry(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
+                               current_fluid, nx, ny, nz, 0, omega_c, k_react, reference_a, reference_b, block_x,
+                               block_y, block_z, stream);
+}
+
+extern "C" int lbc_reaction_ab_step_tiled(int fluid_lattice, int scalar_lattice, int storage, const void* current_a,
+                                          void* next_a, const void* current_b, void* next_b,
+                                          const void* current_fluid, uint64_t nx, uint64_t ny, uint64_t nz,
+                                          uint64_t tile_rows, double omega_c, double k_react, double reference_a,
+                                          double reference_b, uint32_t block_x, uint32_t block_y, uint32_t block_z,
+                                          void* stream) {
+    return lbc::reaction_entry(fluid_lattice, scalar_lattice, storage, current_a, next_a, current_b, next_b,
+                               current_fluid, nx, ny, nz, tile_rows, omega_c, k_react, reference_a, reference_b,
+                               block_x, block_y, block_z, stream);
 }
 
 namespace lbc {

@@ -318,6 +318,46 @@ def reactive_fused_step_tiled(fluid_lattice, collision, scalar_lattice, cur_f, n
                                               _stream(stream)), "lbc_reactive_fused_step_tiled")
 
 
+def _same_tiles(ref, ref_name, sets):
+    """(nx, ny, nz, rows) of `ref`; every (tensor, name, lattice) of `sets`
+    must have the same tiles and dtype."""
+    ext = _tiled_extents(ref[1], ref[0], ref_name)
+    for t, n, lat in sets:
+        if _tiled_extents(lat, t, n) != ext or t.dtype != ref[0].dtype:
+            raise ValueError(f"{n} must match {ref_name}'s tiles and dtype")
+    return ext
+
+
+def scalar_step_tiled(fluid_lattice, scalar_lattice, cur_s, nxt_s, cur_f, omega_c, source=0.0, block=None,
+                      stream=None, ref=0.0):
+    """scalar_step with the scalar and the fluid in tiles of the same rows
+    (to_tiles): lbc_scalar_step_tiled."""
+    nx, ny, nz, rows = _same_tiles((cur_s, scalar_lattice), "cur_s",
+                                   ((nxt_s, "nxt_s", scalar_lattice), (cur_f, "cur_f", fluid_lattice)))
+    if nxt_s.data_ptr() in (cur_s.data_ptr(), cur_f.data_ptr()):
+        raise ValueError("nxt_s must not alias cur_s or cur_f")
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_scalar_step_tiled(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_s.dtype],
+                                      cur_s.data_ptr(), nxt_s.data_ptr(), cur_f.data_ptr(), nx, ny, nz, rows,
+                                      float(omega_c), float(source), float(ref), bx, by, bz, _stream(stream)),
+          "lbc_scalar_step_tiled")
+
+
+def reaction_step_tiled(fluid_lattice, scalar_lattice, cur_a, nxt_a, cur_b, nxt_b, cur_f, omega_c, k_react,
+                        block=None, stream=None, ref_a=0.0, ref_b=0.0):
+    """reaction_step with both species and the fluid in tiles of the same
+    rows (to_tiles): lbc_reaction_ab_step_tiled."""
+    nx, ny, nz, rows = _same_tiles((cur_a, scalar_lattice), "cur_a",
+                                   ((nxt_a, "nxt_a", scalar_lattice), (cur_b, "cur_b", scalar_lattice),
+                                    (nxt_b, "nxt_b", scalar_lattice), (cur_f, "cur_f", fluid_lattice)))
+    bx, by, bz = _block(scalar_lattice, block, nx, ny, nz)
+    check(lib().lbc_reaction_ab_step_tiled(LATTICES[fluid_lattice], LATTICES[scalar_lattice], STORAGE[cur_a.dtype],
+                                           cur_a.data_ptr(), nxt_a.data_ptr(), cur_b.data_ptr(), nxt_b.data_ptr(),
+                                           cur_f.data_ptr(), nx, ny, nz, rows, float(omega_c), float(k_react),
+                                           float(ref_a), float(ref_b), bx, by, bz, _stream(stream)),
+          "lbc_reaction_ab_step_tiled")
+
+
 def reactive_pull_stream(fluid_lattice, scalar_lattice, cur_f, nxt_f, cur_a, nxt_a, cur_b, nxt_b, tiled=False,
                          shift=True, block=None, stream=None):
     """lbc_reactive_pull_stream: the fused step's access pattern without

@@ -47,6 +47,10 @@ SIGNATURES = {
                          c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reaction_ab_step": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_double,
                               c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_scalar_step_tiled": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64, c_double, c_double,
+                               c_double, c_u32, c_u32, c_u32, c_vp], c_int),
+    "lbc_reaction_ab_step_tiled": ([c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64, c_u64,
+                                    c_double, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_fused_step": ([c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64, c_u64,
                                  c_double, c_double, c_double, c_double, c_double, c_u32, c_u32, c_u32, c_vp], c_int),
     "lbc_reactive_fused_step_ex": ([c_int, c_int, c_int, c_int, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u64,

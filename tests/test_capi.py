@@ -259,6 +259,18 @@ def test_validation_tiles():
     # 9 x 21888^2 = 4,311,760,896 elements: one step past the 2D limit
     assert fused(0, 3, 21888, 21888, 1, 4) == HIP_INVALID_VALUE
     assert relayout(0, 21888, 21888, 1, 0, 4) == HIP_INVALID_VALUE
+    scalar = lambda fl, sl, nx, ny, nz, rows, a=FAKE_A: L.lbc_scalar_step_tiled(  # noqa: E731
+        fl, sl, 0, a, FAKE_B, 0x3000, nx, ny, nz, rows, 1.0, 0.0, 0.0, 256, 1, 1, None)
+    react = lambda fl, sl, nx, ny, nz, rows: L.lbc_reaction_ab_step_tiled(  # noqa: E731
+        fl, sl, 0, FAKE_A, FAKE_B, 0x3000, 0x4000, 0x5000, nx, ny, nz, rows, 1.0, 0.1, 0.0, 0.0, 256, 1, 1, None)
+    assert scalar(0, 3, 16, 12, 1, 4, a=None) == HIP_INVALID_DEVICE_POINTER
+    assert scalar(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE  # 5 does not divide 12
+    assert scalar(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a scalar lattice as the fluid
+    assert scalar(1, 4, 8, 8, 8, 3) == HIP_INVALID_VALUE
+    assert scalar(0, 3, 21888, 21888, 1, 4) == HIP_INVALID_VALUE  # 9 x 21888^2 elements
+    assert react(0, 3, 16, 12, 1, 5) == HIP_INVALID_VALUE
+    assert react(3, 3, 16, 12, 1, 4) == HIP_INVALID_VALUE
+    assert react(1, 4, 610, 610, 610, 610) == HIP_INVALID_VALUE  # 19 x 610^3 elements
     assert stats(3, 16, 12, 1, 5) == HIP_INVALID_VALUE
     assert stats(4, 8, 8, 8, 3) == HIP_INVALID_VALUE
     assert stats(0, 16, 12, 1, 4) == HIP_INVALID_VALUE  # a fluid lattice

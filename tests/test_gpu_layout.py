@@ -296,6 +296,48 @@ def test_tiled_fused_step_matches_planes(gpu, pair, storage):
                     assert torch.equal(bits(x), bits(y)), (coll, fast, rows)
 
 
+@pytest.mark.parametrize("storage", STORES)
+@pytest.mark.parametrize("pair", [("D2Q9", "D2Q5"), ("D3Q19", "D3Q7"), ("D3Q27", "D3Q7")])
+def test_tiled_scalar_and_reaction_steps_match_planes(gpu, pair, storage):
+    """The reference's separate launches in tiles (lbc_scalar_step_tiled,
+    lbc_reaction_ab_step_tiled): three alternating fluid / scalar / reaction
+    steps on tiled buffers equal the same steps on the reference's planes bit
+    for bit, for tiles of several heights (3D: the slab order and tiles of
+    fewer rows)."""
+    fl, sl = pair
+    shape = (12, 13) if fl == "D2Q9" else (4, 6, 7)
+    refs = (0.5, 0.25) if storage in ("f16", "bf16") else (0.0, 0.0)
+    f = to_dev(cases.random_perturbed(fl, shape, storage, seed=2), storage, gpu)
+    a0, b0 = _species(sl, shape, storage, seed=8, refs=refs)
+    a, b = to_dev(a0, storage, gpu), to_dev(b0, storage, gpu)
+    omega, omega_c, k = 1.3, 1.05, 0.02
+
+    def run(rows):
+        tiled = rows is not None
+        cf, ca, cb = ((lbcube.to_tiles(fl, f, rows), lbcube.to_tiles(sl, a, rows), lbcube.to_tiles(sl, b, rows))
+                      if tiled else (f.clone(), a.clone(), b.clone()))
+        nf, na, nb, ns = (torch.empty_like(x) for x in (cf, ca, cb, ca))
+        scal = lbcube.scalar_step_tiled if tiled else lbcube.scalar_step
+        react = lbcube.reaction_step_tiled if tiled else lbcube.reaction_step
+        fluid = lbcube.fluid_step_tiled if tiled else lbcube.fluid_step
+        for _ in range(3):
+            fluid(fl, "TRT", cf, nf, omega)
+            scal(fl, sl, ca, ns, nf, omega_c, 3e-4, ref=refs[0])
+            react(fl, sl, ca, na, cb, nb, nf, omega_c, k, ref_a=refs[0], ref_b=refs[1])
+            cf, nf, ca, na, cb, nb = nf, cf, na, ca, nb, cb
+        out = (cf, ca, cb, ns)
+        if tiled:
+            out = tuple(lbcube.from_tiles(lat, x) for lat, x in zip((fl, sl, sl, sl), out))
+        torch.cuda.synchronize()
+        return out
+
+    want = run(None)
+    for rows in ((1, 4, 12) if fl == "D2Q9" else (6, 3, 2)):
+        print(f"case tiles scalar and reaction {fl} {storage} rows {rows}", flush=True)
+        for x, y in zip(run(rows), want):
+            assert torch.equal(bits(x), bits(y)), rows
+
+
 @pytest.mark.parametrize("pair", [("D2Q5", (12, 13), (1, 4, 12)), ("D3Q7", (4, 6, 7), (6,))])
 def test_tiled_reactive_stats_equal_planes(gpu, pair):
     sl, shape, row_set = pair

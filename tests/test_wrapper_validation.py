@@ -131,3 +131,13 @@ def test_strided_and_tiled_layout_mismatches(cpu_wrappers):
                                     tiled=True, block=(64, 1, 1))
     with pytest.raises(ValueError, match="match a's tiles"):
         lbcube.reactive_stats_tiled("D2Q5", a, torch.zeros(6, 5, 2, 8), 0.1)
+    with pytest.raises(ValueError, match="match cur_s"):  # the fluid in tiles of 2 rows, the scalar in 4
+        lbcube.scalar_step_tiled("D2Q9", "D2Q5", a, a.clone(), torch.zeros(6, 9, 2, 8), 1.0, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="must not alias"):
+        lbcube.scalar_step_tiled("D2Q9", "D2Q5", a, a, f, 1.0, block=(64, 1, 1))
+    with pytest.raises(ValueError, match="match cur_a"):
+        lbcube.reaction_step_tiled("D2Q9", "D2Q5", a, a.clone(), a.clone(), a.to(torch.float16), f, 1.0, 0.1,
+                                   block=(64, 1, 1))
+    with pytest.raises(ValueError, match="must have shape"):  # a fluid lattice's tiles for a species
+        lbcube.reaction_step_tiled("D2Q9", "D2Q5", f, f.clone(), f.clone(), f.clone(), f, 1.0, 0.1,
+                                   block=(64, 1, 1))
