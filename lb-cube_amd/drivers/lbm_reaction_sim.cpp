@@ -14,12 +14,13 @@
 //   -csv FILE              statistics file (default reaction_diagnostics.csv)
 //   -vtk_dir DIR           folder of the reaction_<step>.vtk snapshots (default .)
 //   -no_vtk                no snapshots
-//   -tiles R               the fused step with the three population sets in
-//                          tiles of R rows (R divides ny; lbc_reactive_fused_step_tiled,
-//                          statistics by lbc_reactive_stats_tiled): the same
-//                          results bit for bit as -fused, a faster HBM pattern;
-//                          converted into the idle ping-pong buffers for
-//                          snapshots and -dump
+//   -tiles R               the three population sets in tiles of R rows (R
+//                          divides ny): the two launches as lbc_fluid_step_tiled
+//                          and lbc_reaction_ab_step_tiled, with -fused
+//                          lbc_reactive_fused_step_tiled; statistics by
+//                          lbc_reactive_stats_tiled.  The same results bit for
+//                          bit as the planes, a faster HBM pattern; converted
+//                          into the idle ping-pong buffers for snapshots and -dump
 // Rows: step,mean_A,mean_B,var_A,var_B,covariance,segregation_intensity,
 // true_reaction_rate,mixed_reaction_rate with %.17g (include/lattice_io.hpp:352-368),
 // computed on the device with FP64 accumulation.  Snapshots: C_A, C_B,
@@ -172,7 +173,7 @@ int main(int argc, char** argv) {
         std::cout << "LB-Cube reactive mixing (MI355X)\nGrid: " << c.nx << " x " << c.ny << "\nSteps: " << c.steps
                   << "\nOutput frequency: " << c.output_frequency << "\nOmega: " << omega << "\nOmega_c: " << omega_c
                   << "\nk_react: " << c.k_react << "\nBackend: GPU"
-                  << (c.tiles ? " (fused step, tiles)" : c.fused ? " (fused step)" : "") << '\n'
+                  << (c.fused ? " (fused step)" : "") << (c.tiles ? " (tiles)" : "") << '\n'
                   << std::flush;
 
         Device f(fluid.population_count()), a(species_a.population_count()), b(species_b.population_count());
@@ -233,12 +234,22 @@ int main(int argc, char** argv) {
         log(0);
         const auto start = std::chrono::steady_clock::now();
         for (int s = 1; s <= c.steps; ++s) {
-            if (c.tiles != 0) {
+            if (c.tiles != 0 && c.fused) {
                 check(static_cast<hipError_t>(lbc_reactive_fused_step_tiled(
                           fluid_code, LBC_BGK, scalar_code, LBC_F64, 0u, f.cur, f.next, a.cur, a.next, b.cur, b.next,
                           c.nx, c.ny, 1, c.tiles, omega, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
                       "lbc_reactive_fused_step_tiled");
                 std::swap(f.cur, f.next);
+            } else if (c.tiles != 0) {
+                check(static_cast<hipError_t>(lbc_fluid_step_tiled(fluid_code, LBC_BGK, LBC_F64, 0u, f.cur, f.next,
+                                                                   c.nx, c.ny, 1, c.tiles, omega, 256, 1, 1,
+                                                                   nullptr)),
+                      "lbc_fluid_step_tiled");
+                std::swap(f.cur, f.next);
+                check(static_cast<hipError_t>(lbc_reaction_ab_step_tiled(
+                          fluid_code, scalar_code, LBC_F64, a.cur, a.next, b.cur, b.next, f.cur, c.nx, c.ny, 1,
+                          c.tiles, omega_c, c.k_react, 0.0, 0.0, 256, 1, 1, nullptr)),
+                      "lbc_reaction_ab_step_tiled");
             } else if (c.fused) {
                 check(static_cast<hipError_t>(lbc_reactive_fused_step(
                           lbm::lattice_code<FluidLattice>(), LBC_BGK, lbm::lattice_code<ScalarLattice>(), LBC_F64, f.cur, f.next, a.cur, a.next, b.cur,

@@ -500,24 +500,28 @@ def test_lbm_sim_slab_order_equals_planes(gpu, tmp_path, collision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("tiles", [1, 4, 48])
 def test_reaction_sim_tiles_equal_fused(gpu, tmp_path, tiles):
-    """lbm_reaction_sim -tiles R (the fused step and the statistics on the
-    three population sets in tiles of R rows): every statistics row, every VTK
-    field and the final populations equal the -fused run on the reference's
-    planes byte for byte."""
-    print(f"case lbm_reaction_sim -tiles {tiles}", flush=True)
+    """lbm_reaction_sim -tiles R (the three population sets in tiles of R
+    rows: the reference's two launches as tiled launches, or with -fused the
+    tiled fused step; statistics on the tiles): every statistics row, every
+    VTK field and the final populations equal the -fused run on the
+    reference's planes byte for byte."""
     outs = []
-    for extra in (["-fused"], ["-tiles", tiles]):
-        d = tmp_path / ("tiles" if extra[0] == "-tiles" else "fused")
+    for name, extra in (("fused", ["-fused"]), ("tiles", ["-tiles", tiles]),
+                        ("tiles_fused", ["-tiles", tiles, "-fused"])):
+        print(f"case lbm_reaction_sim {' '.join(map(str, extra))}", flush=True)
+        d = tmp_path / name
         d.mkdir()
         r = run("-nx", 40, "-ny", 48, "-steps", 60, "-out_freq", 20, "-dump", d / "out", "-csv", d / "s.csv",
                 "-vtk_dir", d, *extra, cwd=d, exe=REACT_EXE)
         assert r.returncode == 0, r.stderr
         outs.append(d)
-    fused, tiled = outs
-    for suffix in "fab":
-        assert (fused / f"out.{suffix}").read_bytes() == (tiled / f"out.{suffix}").read_bytes(), suffix
-    assert (fused / "s.csv").read_text() == (tiled / "s.csv").read_text()
+    fused = outs[0]
     vtks = sorted(p.name for p in fused.glob("*.vtk"))
-    assert len(vtks) == 4 and vtks == sorted(p.name for p in tiled.glob("*.vtk"))
-    for name in vtks:
-        assert (fused / name).read_text() == (tiled / name).read_text(), name
+    assert len(vtks) == 4
+    for tiled in outs[1:]:
+        for suffix in "fab":
+            assert (fused / f"out.{suffix}").read_bytes() == (tiled / f"out.{suffix}").read_bytes(), (tiled, suffix)
+        assert (fused / "s.csv").read_text() == (tiled / "s.csv").read_text()
+        assert vtks == sorted(p.name for p in tiled.glob("*.vtk"))
+        for name in vtks:
+            assert (fused / name).read_text() == (tiled / name).read_text(), (tiled, name)
