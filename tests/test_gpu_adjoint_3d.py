@@ -77,28 +77,29 @@ def test_adjoint_blocks_of_1024_match_oracle(gpu, lattice, coll):
         assert abs(dwg - dw) <= 1e-11 * abs(dw), block
 
 
+@pytest.mark.parametrize("coll", ["TRT", "RLBM"])
 @pytest.mark.parametrize("lattice", ["D3Q19", "D3Q27"])
-def test_adjoint_3d_fp32_against_fp64(gpu, lattice):
+def test_adjoint_3d_fp32_against_fp64(gpu, lattice, coll):
     """The FP32 adjoint is as close to the FP64 gradient as the oracle's own
     FP32 adjoint is; the FP64 adjoint of the FP32 trajectory no farther."""
-    print(f"case adjoint 3d fp32 {lattice}", flush=True)
+    print(f"case adjoint 3d fp32 {lattice} {coll}", flush=True)
     f, a, b = TA3.small_state_3d(lattice, nx=7, ny=6, nz=5, seed=6)
     f, a, b = (x.astype(np.float32) for x in (f, a, b))
     omega, omega_c, k, steps = 1 / 0.7, 1 / 0.9, 0.4, 17
-    J64, *g64 = O.reactive_gradient("TRT", "f64", *(x.astype(np.float64) for x in (f, a, b)), omega, omega_c, k,
+    J64, *g64 = O.reactive_gradient(coll, "f64", *(x.astype(np.float64) for x in (f, a, b)), omega, omega_c, k,
                                     steps, lattice=lattice)
-    J32, *g32 = O.reactive_gradient("TRT", "f32", f, a, b, omega, omega_c, k, steps, lattice=lattice)
+    J32, *g32 = O.reactive_gradient(coll, "f32", f, a, b, omega, omega_c, k, steps, lattice=lattice)
     args = (dev(f, gpu, torch.float32), dev(a, gpu, torch.float32), dev(b, gpu, torch.float32), steps, omega,
             omega_c, k)
-    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient("TRT", *args, interval=4, lattice=lattice)
-    mixed = ADJ.reactive_gradient("TRT", *args, interval=4, lattice=lattice, adjoint_f64=True)
+    Jg, gfg, gag, gbg, dwg = ADJ.reactive_gradient(coll, *args, interval=4, lattice=lattice)
+    mixed = ADJ.reactive_gradient(coll, *args, interval=4, lattice=lattice, adjoint_f64=True)
     assert abs(Jg - J32) <= 1e-6 and abs(Jg - J64) <= 1e-6
     for got, gm, orc, exact in zip((gfg, gag, gbg), mixed[1:4], g32[:3], g64[:3]):
         scale = np.max(np.abs(exact))
         e_gpu = np.max(np.abs(got.cpu().numpy() - exact)) / scale
         e_mix = np.max(np.abs(gm.cpu().numpy() - exact)) / scale
         e_orc = np.max(np.abs(orc - exact)) / scale
-        print(f"{lattice} fp32 adjoint: gpu {e_gpu:.2e}, FP64 adjoint of the FP32 run {e_mix:.2e}, oracle FP32 "
+        print(f"{lattice} {coll} fp32 adjoint: gpu {e_gpu:.2e}, FP64 adjoint of the FP32 run {e_mix:.2e}, oracle FP32 "
               f"{e_orc:.2e} (relative to the FP64 gradient)", flush=True)
         assert e_gpu <= max(4 * e_orc, 1e-5)
         assert e_mix <= 1.05 * e_gpu + 1e-7
@@ -170,13 +171,15 @@ def test_adjoint_3d_interval_invariance(gpu):
             assert torch.equal(x, y)
 
 
-def test_adjoint_3d_autograd_gradcheck(gpu):
+@pytest.mark.parametrize("coll", ["TRT", "RLBM"])
+def test_adjoint_3d_autograd_gradcheck(gpu, coll):
+    print(f"case adjoint 3d gradcheck {coll}", flush=True)
     f, a, b = TA3.small_state_3d("D3Q19", nx=4, ny=3, nz=3, seed=8)
     f0, a0, b0 = (dev(x, gpu).requires_grad_(True) for x in (f, a, b))
     tau = torch.tensor(0.9, dtype=torch.float64, device=gpu, requires_grad=True)
 
     def fn(f0, a0, b0, tau):
-        return ADJ.reactive_loss(f0, a0, b0, tau, "TRT", steps=3, omega=1 / 0.7, k_react=0.4, interval=2,
+        return ADJ.reactive_loss(f0, a0, b0, tau, coll, steps=3, omega=1 / 0.7, k_react=0.4, interval=2,
                                  lattice="D3Q19")
 
     assert torch.autograd.gradcheck(fn, (f0, a0, b0, tau), eps=1e-6, atol=1e-9, rtol=1e-6)
