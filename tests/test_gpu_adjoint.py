@@ -139,13 +139,15 @@ def test_overlapped_gradient_on_a_caller_stream(gpu):
         assert torch.equal(x, y)
 
 
-def test_autograd_gradcheck(gpu):
+@pytest.mark.parametrize("coll", ["BGK", "RLBM"])
+def test_autograd_gradcheck(gpu, coll):
+    print(f"case adjoint gradcheck {coll}", flush=True)
     f, a, b = TA.small_state(nx=6, ny=5, seed=8)
     f0, a0, b0 = (dev(x, gpu).requires_grad_(True) for x in (f, a, b))
     tau = torch.tensor(0.9, dtype=torch.float64, device=gpu, requires_grad=True)
 
     def fn(f0, a0, b0, tau):
-        return ADJ.reactive_loss(f0, a0, b0, tau, "BGK", steps=4, omega=1 / 0.7, k_react=0.4, interval=2)
+        return ADJ.reactive_loss(f0, a0, b0, tau, coll, steps=4, omega=1 / 0.7, k_react=0.4, interval=2)
 
     assert torch.autograd.gradcheck(fn, (f0, a0, b0, tau), eps=1e-6, atol=1e-9, rtol=1e-6)
 
