@@ -32,6 +32,23 @@ def test_library_exports_every_declared_symbol():
         assert name in _lib.SIGNATURES, f"{name} missing from the Python binding"
 
 
+@pytest.mark.parametrize("std", ["c99", "c11"])
+def test_header_compiles_as_c(tmp_path, std):
+    """include/lbcube.h is a C header: a C translation unit that includes it
+    and calls an entry point compiles with -pedantic -Werror."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "uses_lbcube.c"
+    src.write_text('#include "lbcube.h"\nint main(void) { return lbc_abi_version() == 2 ? 0 : 1; }\n')
+    r = subprocess.run([cc, f"-std={std}", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_abi_version():
     assert lbcube.lib().lbc_abi_version() == 2
 
