@@ -959,7 +959,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", default="trt_f32,trt_f16fast,trt_f32fast,trt_f16,bgk_f32,bgk_f16fast,mrt_f32,"
                                           "mrt_f16,mrt_f32fast,mrt_f16fast,trt_bf16fast,trt_f32_slabs,"
-                                          "trt_f32fast_slabs,trt_f16fast_slabs,mrt_f16fast_slabs",
+                                          "trt_f32fast_slabs,trt_f16fast_slabs,mrt_f16fast_slabs,rlbm_f32,rlbm_f16",
                     help="coll_storage[fast][_slabs]: _slabs stores the populations slab by slab, [Z][Q][Y][X]")
     ap.add_argument("--block", default=None, help="block shape bx,by,bz (default: the library's)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint, ensemble and 2D reactive lines")

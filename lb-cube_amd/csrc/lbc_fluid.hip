@@ -152,11 +152,13 @@ __global__ __launch_bounds__(MAXT) void pull_stream_kernel(const typename Storag
 // Kernels that run faster with residency capped at kCappedWavesPerCu (measured
 // on MI355X): 3D lattices with FP32 storage or FP16 in the deviation form
 // (the reference-order D3Q27 MRT included since its constant divisions made it
-// memory-bound: +0.9% at 256^3, +2.7% at 384^3).  Left uncapped: BF16 (its
-// software rounding makes the step VALU-heavier: 71.0k against 74.4k MLUPS
-// capped), FP64 (unchanged by the cap) and RLBM (not measured).
+// memory-bound: +0.9% at 256^3, +2.7% at 384^3; RLBM in FP32: D3Q19 38.8-38.9k
+// MLUPS against 37.2-37.3k uncapped at 256^3, D3Q27 26.8k against 26.1-26.3k).
+// Left uncapped: BF16 (its software rounding makes the step VALU-heavier:
+// 71.0k against 74.4k MLUPS capped), FP64 (unchanged by the cap) and 16-bit
+// storage in the reference order (RLBM has no other form).
 template <class L, int CT, int ST, bool FAST> constexpr bool capped_residency() {
-    if (L::D != 3 || ST == kF64 || ST == kBF16 || CT == kRLBM) return false;
+    if (L::D != 3 || ST == kF64 || ST == kBF16) return false;
     if (kOffsetStorage<ST> && !FAST) return false;
     return true;
 }
