@@ -198,7 +198,7 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
     MRT in both forms, and the deviation form with its transforms on the
     matrix cores), with the pull-stream pattern ceilings of FP32 and FP16;
     TRT FP32 and FP16 deviation form again in the slab order [Z][Q][Y][X]
-    (keys *_slabs)."""
+    (keys *_slabs); RLBM in FP32."""
     from lbcube import init as lbinit
 
     out = {}
@@ -210,7 +210,8 @@ def lattice_sweep(lbcube, torch, steps=10, warmup=3):
                       ("MRT", "f32", False, False, "planes"), ("MRT", "f32", True, False, "planes"),
                       ("MRT", "f16", True, False, "planes"), ("MRT", "f16", True, True, "planes"),
                       ("MRT", "bf16", True, False, "planes"), ("MRT", "bf16", True, True, "planes"),
-                      ("TRT", "f32", False, False, "slabs"), ("TRT", "f16", True, False, "slabs")]
+                      ("TRT", "f32", False, False, "slabs"), ("TRT", "f16", True, False, "slabs"),
+                      ("RLBM", "f32", False, False, "planes")]
             for coll, storage, fast, mfma, layout in combos:
                 key = (f"{lattice}_{n}_{coll.lower()}_{storage}{'fast' if fast else ''}{'mfma' if mfma else ''}"
                        f"{'_slabs' if layout == 'slabs' else ''}")
