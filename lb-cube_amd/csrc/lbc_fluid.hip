@@ -169,7 +169,9 @@ template <class L, int CT, int ST, bool FAST> constexpr bool capped_residency() 
 // 74.5-75.4k MLUPS against 70.4-74.6k at 16 (81.5-81.9k against 79.4-79.8k
 // with the planes stored slab by slab) and MRT 75.1-75.8k against 73.8-74.0k
 // (81.3-81.8k against 77.2-77.6k); 28 gave the same, 32 less.  D3Q27 in FP16
-// (50.7-50.8k at 24 against 51.4-51.7k) and every FP32 step stay at 16.
+// (50.7-50.8k at 24 against 51.4-51.7k) and every FP32 step stay at 16 (D3Q27
+// FP32 TRT / MRT / RLBM at 256^3 and 384^3: 12 waves lost about 7%, 20 were
+// within 1% and 24 lost 1-2% against 16).
 template <class L, int CT, int ST, bool FAST> constexpr uint32_t capped_waves() {
     return (L::Q == 19 && ST == kF16 && FAST) ? 24u : kCappedWavesPerCu;
 }
